@@ -1,0 +1,289 @@
+"""ctypes bindings for the checker libraries.
+
+- ``Oracle``: oracle/liboracle.so, the plain-C restatement (test checker).
+- ``RefLib``: oracle/_ref/libhalign_ref.so, the real reference path compiled
+  from its own sources (only present where the reference tree was available
+  at build time).
+
+Both expose the same call shapes so a test can run one input through both.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libhalign_ref.so")
+
+RIGHT = 0x02
+GENERIC = 0x04
+FLAG_RIGHT = GENERIC | RIGHT
+FLAG_LEFT = GENERIC
+
+
+class Ez(C.Structure):
+    _fields_ = [("max", C.c_uint32), ("zdropped", C.c_int32), ("max_q", C.c_int32),
+                ("max_t", C.c_int32), ("mqe", C.c_int32), ("mqe_t", C.c_int32),
+                ("mte", C.c_int32), ("mte_q", C.c_int32), ("score", C.c_int32),
+                ("n_cigar", C.c_int32)]
+
+    def as_tuple(self):
+        return tuple(getattr(self, f) for f, _ in self._fields_)
+
+
+ANCHOR_DTYPE = np.dtype([("hash", "<u8"), ("rid_ref", "<u4"), ("pos_ref", "<u4"),
+                         ("rid_qry", "<u4"), ("pos_qry", "<u4"), ("span", "<u4"),
+                         ("is_rev", "<u4")])
+
+_P = C.c_void_p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def encode_codes(s):
+    """ScoreChar2Idx (include/align.h:72-89) for a bytes/str sequence."""
+    if isinstance(s, str):
+        s = s.encode()
+    lut = np.full(256, 4, np.uint8)
+    for ch, v in ((b"A", 0), (b"C", 1), (b"G", 2), (b"T", 3)):
+        lut[ch[0]] = v
+        lut[ch.lower()[0]] = v
+    return lut[np.frombuffer(s, np.uint8)]
+
+
+def cigar_str(cig):
+    ops = "MIDNSHP=X"
+    return "".join(f"{int(c) >> 4}{ops[int(c) & 15]}" for c in cig)
+
+
+class _Base:
+    def __init__(self, path):
+        self.lib = C.CDLL(path)
+
+
+class Oracle(_Base):
+    def __init__(self, path=ORACLE_SO):
+        super().__init__(path)
+        L = self.lib
+        L.or_ksw_extz2.argtypes = [C.c_int, _P, C.c_int, _P, C.c_int, C.c_int, C.POINTER(Ez), _P,
+                                   C.c_int, _P]
+        L.or_global_align_ksw2.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int, _P,
+                                           C.c_int, _P]
+        L.or_extract_minimizer.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P,
+                                           C.c_int, _P]
+        L.or_collect_anchors.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, C.c_int]
+        L.or_chain_anchors.argtypes = [_P, C.c_int, _P, _P]
+        L.or_global_align_mm2.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _P, C.c_int,
+                                          _P, _P, C.c_int, _P, _P]
+        L.or_ref_global_align.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _P, _P,
+                                          C.c_int, _P, _P, C.c_int, _P, _P, _P]
+        L.or_del_query_to_ref.argtypes = [C.c_char_p, C.c_int, _P, C.c_int, _P, C.c_int]
+        L.or_consensus.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, _P,
+                                   C.c_char_p]
+        L.or_consensus.restype = C.c_int64
+        L.or_auto_band.argtypes = [C.c_int, C.c_int]
+        L.or_fill_log2_lut.argtypes = [_P]
+        self.lut = np.zeros(502, np.float32)
+        L.or_fill_log2_lut(_ptr(self.lut))
+
+    def auto_band(self, rlen, qlen):
+        return self.lib.or_auto_band(rlen, qlen)
+
+    def ksw(self, query_codes, target_codes, w, flag):
+        """Returns (ez tuple, cigar array, cells)."""
+        q = np.ascontiguousarray(query_codes, np.uint8)
+        t = np.ascontiguousarray(target_codes, np.uint8)
+        ez = Ez()
+        cells = np.zeros(1, np.int64)
+        cap = len(q) + len(t) + 4
+        cig = np.zeros(cap, np.uint32)
+        n = self.lib.or_ksw_extz2(len(q), _ptr(q), len(t), _ptr(t), w, flag, C.byref(ez),
+                                  _ptr(cig), cap, _ptr(cells))
+        assert n >= 0
+        return ez.as_tuple(), cig[:n].copy(), int(cells[0])
+
+    def global_align_ksw2(self, ref, qry, flag=FLAG_RIGHT):
+        ref = ref.encode() if isinstance(ref, str) else ref
+        qry = qry.encode() if isinstance(qry, str) else qry
+        cap = len(ref) + len(qry) + 4
+        cig = np.zeros(cap, np.uint32)
+        cells = np.zeros(1, np.int64)
+        n = self.lib.or_global_align_ksw2(ref, len(ref), qry, len(qry), flag, _ptr(cig), cap,
+                                          _ptr(cells))
+        assert n >= 0
+        return cig[:n].copy(), int(cells[0])
+
+    def minimizers(self, seq, k=15, w=10, non_canonical=True):
+        seq = seq.encode() if isinstance(seq, str) else seq
+        cap = len(seq) + 1
+        x = np.zeros(cap, np.uint64)
+        y = np.zeros(cap, np.uint64)
+        ub = np.zeros(1, np.int32)
+        n = self.lib.or_extract_minimizer(seq, len(seq), k, w, int(non_canonical), _ptr(x),
+                                          _ptr(y), cap, _ptr(ub))
+        assert n >= 0
+        return x[:n].copy(), y[:n].copy(), int(ub[0])
+
+    def collect_anchors(self, rx, ry, qx, qy):
+        cap = max(16, len(qx) * 64)
+        out = np.zeros(cap, ANCHOR_DTYPE)
+        n = self.lib.or_collect_anchors(_ptr(rx), _ptr(ry), len(rx), _ptr(qx), _ptr(qy), len(qx),
+                                        _ptr(out), cap)
+        assert n >= 0
+        return out[:n].copy()
+
+    def chain(self, anchors):
+        a = np.ascontiguousarray(anchors.copy())
+        out = np.zeros(max(1, len(a)), ANCHOR_DTYPE)
+        n = self.lib.or_chain_anchors(_ptr(a), len(a), _ptr(self.lut), _ptr(out))
+        return a, out[:n].copy()
+
+    def mm2(self, ref, qry, anchors):
+        ref = ref.encode() if isinstance(ref, str) else ref
+        qry = qry.encode() if isinstance(qry, str) else qry
+        a = np.ascontiguousarray(anchors)
+        cap = len(ref) + len(qry) + 4
+        cig = np.zeros(cap, np.uint32)
+        cells = np.zeros(1, np.int64)
+        st = np.zeros(4, np.int32)
+        n = self.lib.or_global_align_mm2(ref, len(ref), qry, len(qry), _ptr(a), len(a),
+                                         _ptr(self.lut), _ptr(cig), cap, _ptr(cells), _ptr(st))
+        assert n >= 0
+        return cig[:n].copy(), int(cells[0]), st.copy()
+
+    def ref_global_align(self, ref, qry, ref_hits=None):
+        ref = ref.encode() if isinstance(ref, str) else ref
+        qry = qry.encode() if isinstance(qry, str) else qry
+        rx = ry = None
+        nref = 0
+        if ref_hits is not None:
+            rx, ry = ref_hits
+            nref = len(rx)
+        cap = len(ref) + len(qry) + 4
+        cig = np.zeros(cap, np.uint32)
+        cells = np.zeros(1, np.int64)
+        st = np.zeros(4, np.int32)
+        ub = np.zeros(1, np.int32)
+        n = self.lib.or_ref_global_align(ref, len(ref), qry, len(qry), _ptr(rx), _ptr(ry), nref,
+                                         _ptr(self.lut), _ptr(cig), cap, _ptr(cells), _ptr(st),
+                                         _ptr(ub))
+        assert n >= 0
+        return cig[:n].copy(), int(cells[0]), st.copy(), int(ub[0])
+
+    def del_query_to_ref(self, qry, cig):
+        qry = qry.encode() if isinstance(qry, str) else qry
+        c = np.ascontiguousarray(cig, np.uint32)
+        cap = int(sum(int(x) >> 4 for x in c)) + len(qry) + 1
+        out = np.zeros(cap, np.uint8)
+        n = self.lib.or_del_query_to_ref(qry, len(qry), _ptr(c), len(c), _ptr(out), cap)
+        assert n >= 0
+        return out[:n].tobytes()
+
+    def consensus(self, rows, batch=4096, seq_limit=0):
+        """rows: (nrows, aln_len) uint8 array. Returns (counts[aln,7], consensus bytes, n)."""
+        rows = np.ascontiguousarray(rows, np.uint8)
+        nrows, aln = rows.shape
+        counts = np.zeros((aln, 7), np.uint32)
+        cons = C.create_string_buffer(aln)
+        n = self.lib.or_consensus(rows.tobytes(), nrows, aln, batch, seq_limit, _ptr(counts), cons)
+        return counts, cons.raw[:aln], int(n)
+
+
+class RefLib(_Base):
+    """The compiled reference (same call shapes as Oracle)."""
+
+    def __init__(self, path=REF_SO):
+        super().__init__(path)
+        L = self.lib
+        L.ref_ksw_extz2.argtypes = [C.c_int, _P, C.c_int, _P, C.c_int, C.c_int, C.POINTER(Ez), _P,
+                                    C.c_int]
+        L.ref_global_align_ksw2.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int,
+                                            _P, C.c_int]
+        L.ref_extract_minimizer.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _P,
+                                            _P, C.c_int]
+        L.ref_collect_anchors.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, C.c_int]
+        L.ref_chain_anchors.argtypes = [_P, C.c_int, _P]
+        L.ref_global_align_mm2.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _P, C.c_int,
+                                           _P, C.c_int]
+        L.ref_ref_global_align.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _P, C.c_int]
+        L.ref_consensus.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_int,
+                                    C.c_int, C.c_char_p, C.c_int]
+
+    def ksw(self, query_codes, target_codes, w, flag):
+        q = np.ascontiguousarray(query_codes, np.uint8)
+        t = np.ascontiguousarray(target_codes, np.uint8)
+        ez = Ez()
+        cap = len(q) + len(t) + 4
+        cig = np.zeros(cap, np.uint32)
+        n = self.lib.ref_ksw_extz2(len(q), _ptr(q), len(t), _ptr(t), w, flag, C.byref(ez),
+                                   _ptr(cig), cap)
+        assert n >= 0
+        return ez.as_tuple(), cig[:n].copy()
+
+    def global_align_ksw2(self, ref, qry, flag=FLAG_RIGHT):
+        ref = ref.encode() if isinstance(ref, str) else ref
+        qry = qry.encode() if isinstance(qry, str) else qry
+        cap = len(ref) + len(qry) + 4
+        cig = np.zeros(cap, np.uint32)
+        n = self.lib.ref_global_align_ksw2(ref, len(ref), qry, len(qry), flag, _ptr(cig), cap)
+        assert n >= 0
+        return cig[:n].copy()
+
+    def minimizers(self, seq, k=15, w=10, non_canonical=True):
+        seq = seq.encode() if isinstance(seq, str) else seq
+        cap = len(seq) + 1
+        x = np.zeros(cap, np.uint64)
+        y = np.zeros(cap, np.uint64)
+        n = self.lib.ref_extract_minimizer(seq, len(seq), k, w, int(non_canonical), _ptr(x),
+                                           _ptr(y), cap)
+        assert n >= 0
+        return x[:n].copy(), y[:n].copy()
+
+    def collect_anchors(self, rx, ry, qx, qy):
+        cap = max(16, len(qx) * 64)
+        out = np.zeros(cap, ANCHOR_DTYPE)
+        n = self.lib.ref_collect_anchors(_ptr(rx), _ptr(ry), len(rx), _ptr(qx), _ptr(qy),
+                                         len(qx), _ptr(out), cap)
+        assert n >= 0
+        return out[:n].copy()
+
+    def chain(self, anchors):
+        a = np.ascontiguousarray(anchors.copy())
+        out = np.zeros(max(1, len(a)), ANCHOR_DTYPE)
+        n = self.lib.ref_chain_anchors(_ptr(a), len(a), _ptr(out))
+        return a, out[:n].copy()
+
+    def mm2(self, ref, qry, anchors):
+        ref = ref.encode() if isinstance(ref, str) else ref
+        qry = qry.encode() if isinstance(qry, str) else qry
+        a = np.ascontiguousarray(anchors)
+        cap = len(ref) + len(qry) + 4
+        cig = np.zeros(cap, np.uint32)
+        n = self.lib.ref_global_align_mm2(ref, len(ref), qry, len(qry), _ptr(a), len(a),
+                                          _ptr(cig), cap)
+        assert n >= 0
+        return cig[:n].copy()
+
+    def ref_global_align(self, ref, qry):
+        ref = ref.encode() if isinstance(ref, str) else ref
+        qry = qry.encode() if isinstance(qry, str) else qry
+        cap = len(ref) + len(qry) + 4
+        cig = np.zeros(cap, np.uint32)
+        n = self.lib.ref_ref_global_align(ref, len(ref), qry, len(qry), _ptr(cig), cap)
+        assert n >= 0
+        return cig[:n].copy()
+
+    def consensus_file(self, fasta, out_fa, out_json, seq_limit=0, batch=4096, threads=1,
+                       aln_len=1 << 20):
+        cons = C.create_string_buffer(aln_len)
+        n = self.lib.ref_consensus(fasta.encode(), out_fa.encode(), out_json.encode(), seq_limit,
+                                   threads, batch, cons, aln_len)
+        assert n >= 0
+        return cons.raw[:n]
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
