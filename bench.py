@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Kernel-only pairwise DP benchmark on one MI355X.
+
+Workload: 100,000 whole query-vs-center pairs (center 29,903 bp, queries
+29.0-30.0 kb at 0.5% substitutions / 0.1% indels, seeded synthetic data),
+globalAlignKSW2 semantics (auto band ~4,665, RIGHT-aligned gaps, score +
+CIGAR on the device). Inputs are resident in HBM before timing starts.
+
+A step is one batch of --batch pairs (the batch index walks through the
+100,000-pair set). value = batched-persistent GCUPS over the timed steps,
+cells = sum of the exact in-band cells of ksw2 (en0 - st0 + 1 per
+anti-diagonal). The plain one-pair-per-launch form is timed on a few pairs
+beside it, and the reference CPU path (compiled reference if present, else
+the oracle port) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "halign-4_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "GCUPS + seqs/sec, batched vs plain GPU form, bit-exact; p99 latency serving"
+N_PAIRS = 100000
+OPS_PER_CELL = 24                 # algorithmic integer ops per DP cell (recurrence + H + max)
+PEAK_VALU_TOPS = 78.6             # MI355X packed-16-bit integer VALU: 256 CU x 128 lanes x 2.4 GHz
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(center, qbytes, off, sample_pairs, threads):
+    """Time globalAlignKSW2 on `sample_pairs` pairs with `threads` host threads."""
+    from oracle_lib import REF_SO, ORACLE_SO, Oracle, RefLib, FLAG_RIGHT
+    kind = "reference" if os.path.exists(REF_SO) else "port"
+    lib = RefLib() if kind == "reference" else Oracle()
+
+    def one(i):
+        q = qbytes[off[i]:off[i + 1]]
+        lib.global_align_ksw2(center, q, FLAG_RIGHT)
+        return 0
+
+    # cells of the sample (closed form of the band, same as the kernel counts)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(one, range(sample_pairs)))
+    dt = time.perf_counter() - t0
+    return kind, dt
+
+
+def band_cells(qlen, tlen, w):
+    """sum_r (en0 - st0 + 1) of ksw2's band (src/align/ksw2_extz2_sse.c:107-110)."""
+    if w < 0:
+        w = max(qlen, tlen)
+    r = np.arange(qlen + tlen - 1, dtype=np.int64)
+    st = np.maximum.reduce([np.zeros_like(r), r - qlen + 1, (r - w + 1) >> 1])
+    en = np.minimum.reduce([np.full_like(r, tlen - 1), r, (r + w) >> 1])
+    bad = np.nonzero(st > en)[0]
+    if len(bad):
+        st, en = st[:bad[0]], en[:bad[0]]
+    return int((en - st + 1).sum())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--plain-pairs", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="pairs for the CPU baseline (0 = auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+              "the path shards by query ranges with no collective, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import halign
+    from halign import synth
+
+    need = (args.warmup + args.steps) * args.batch
+    n_res = min(N_PAIRS, max(need, args.batch))
+    log(f"generating config-2 workload: center 29903 bp, {n_res} of {N_PAIRS} queries resident")
+    t0 = time.time()
+    center = synth.center(2)
+    qbytes, off = synth.queries(2, 0, n_res)
+    log(f"  generated {len(qbytes) / 1e9:.2f} GB in {time.time() - t0:.1f}s")
+    ctx = halign.Context(0)
+    ps = halign.PairSet(ctx, center, qbytes, off)
+
+    # plain form: one pair per launch
+    plain_cells = 0
+    plain_ms = 0.0
+    for i in range(args.plain_pairs):
+        ms, cells = ps.run(i, 1, plain=True)
+        plain_ms += ms
+        plain_cells += cells
+    plain_gcups = plain_cells / (plain_ms * 1e-3) / 1e9 if plain_ms > 0 else 0.0
+    log(f"plain form: {args.plain_pairs} pairs, {plain_gcups:.2f} GCUPS")
+
+    for s in range(args.warmup):
+        ps.run((s * args.batch) % n_res, min(args.batch, n_res), plain=False)
+    log("warmup done")
+    tot_cells = 0
+    tot_kernel_ms = 0.0
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        first = ((args.warmup + s) * args.batch) % n_res
+        cnt = min(args.batch, n_res - first)
+        ms, cells = ps.run(first, cnt, plain=False)
+        tot_cells += cells
+        tot_kernel_ms += ms
+        log(f"step {s}: {cnt} pairs, kernel {ms:.1f} ms, {cells / (ms * 1e-3) / 1e9:.1f} GCUPS")
+    t_wall = time.perf_counter() - t_start
+    pairs = args.steps * args.batch
+    gcups = tot_cells / t_wall / 1e9
+    kernel_gcups = tot_cells / (tot_kernel_ms * 1e-3) / 1e9
+    achieved_tops = tot_cells * OPS_PER_CELL / (tot_kernel_ms * 1e-3) / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        sample = args.cpu_sample or threads * 2
+        kind, dt = cpu_baseline(center, qbytes, off, sample, threads)
+        w = [halign_auto_band(len(center), int(off[i + 1] - off[i])) for i in range(sample)]
+        cells = sum(band_cells(int(off[i + 1] - off[i]), len(center), w[i]) for i in range(sample))
+        cpu = {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": kind,
+               "sample": f"{sample} config-2 whole pairs (globalAlignKSW2), {dt:.1f}s on {threads} threads"}
+        log(f"cpu baseline ({kind}): {cpu['value']} GCUPS")
+
+    out = {
+        "metric": METRIC,
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic (seeded, csrc/synth.c config 2)",
+        "config": {"workload": "config 2: kernel-only pairwise DP, 100000 pairs, center 29903 bp, "
+                               "queries 29.0-30.0 kb, auto band, batched persistent form",
+                   "pairs_per_step": args.batch, "pairs_timed": pairs},
+        "seqs_per_s": round(pairs / t_wall, 2),
+        "kernel_gcups": round(kernel_gcups, 3),
+        "plain_gcups": round(plain_gcups, 4),
+        "batched_vs_plain": round(kernel_gcups / plain_gcups, 1) if plain_gcups else None,
+        "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": PEAK_VALU_TOPS,
+                     "unit": "TOPS", "frac": round(achieved_tops / PEAK_VALU_TOPS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def halign_auto_band(rlen, qlen):
+    d = abs(rlen - qlen)
+    if d / max(rlen, qlen) > 0.5:
+        return -1
+    return 200 + int(0.1 * (rlen + qlen // 2))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,636 @@
+// dp_kernel.hip -- banded affine-gap global DP with ksw2 extz2 semantics on
+// CDNA4 (gfx950), plus device backtrack to a CIGAR.
+//
+// Restates src/align/ksw2_extz2_sse.c:23-304 (SSE2 branch; production build
+// has no -march) and ksw_backtrack include/ksw2.h:129-161 for the path's
+// fixed parameters: dna5 matrix (+5/-4/0 for N), q=6, e=2, zdrop<0,
+// end_bonus=0, flags GENERIC_SC with or without RIGHT.
+//
+// Work decomposition. One workgroup of W waves owns one DP job. The
+// reference walks anti-diagonals r and, on each, a region of target columns
+// [st, en] rounded out to 16-cell blocks; cells outside the exact band read
+// whatever stale values the persistent per-column arrays hold. Here every lane
+// owns 16-cell blocks: block b lives in slot (b mod R) / (64W) of lane
+// (b mod R) mod (64W), R = 64*W*S, so the active blocks of a diagonal are a
+// contiguous run of the ring and the ring slides with the band without any
+// data movement. A lane's block state (u, v, x, y, score, target, query
+// window) stays in VGPRs for the whole life of the block, which reproduces
+// the reference's persistent arrays, stale margin reads included.
+//
+// Arithmetic. Each 32-bit VGPR holds two cells as 16-bit halves, the int8
+// value in the HIGH byte and 0 in the low byte ("x<<8"). With that layout the
+// packed 16-bit VALU ops give the reference's 8-bit lane semantics exactly:
+// v_pk_add/sub_u16 wrap modulo 256 in the high byte, v_pk_max/min_u16 are the
+// unsigned byte max/min, v_pk_max_i16 the signed one, and a saturating
+// v_pk_sub_i16 puts the signed compare result in bit 15. No masking or
+// sign-extension op is needed anywhere in the recurrence.
+//
+// Traceback: 4 bits per cell (two state-compare bits and the two gap
+// continuation bits), 8 bytes per block per anti-diagonal, rows of
+// n_col_*8 bytes (n_col_ exactly as the reference sizes its matrix).
+#include <hip/hip_runtime.h>
+
+#include "dp_common.h"
+
+namespace hal {
+
+constexpr int32_t kNegInf = -0x40000000;   // KSW_NEG_INF include/ksw2.h:6
+constexpr int kQ = 6, kQE = 8;             // q, q+e of KSW2AlignConfig include/align.h:126-135
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
+}
+__device__ __forceinline__ uint32_t pk_maxu(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_minu(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_maxi(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, a),
+                                                                  __builtin_bit_cast(s16x2, b)));
+}
+// signed saturating a - b per half: bit 15 (31) is set iff a < b (as int8 in the high byte)
+__device__ __forceinline__ uint32_t pk_subsat(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(s16x2, a),
+                                                                      __builtin_bit_cast(s16x2, b)));
+}
+__device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { return (a & m) | (b & ~m); }
+// cells (2j-1, 2j) from pairs hi=(2j,2j+1), lo=(2j-2,2j-1)
+__device__ __forceinline__ uint32_t cat16(uint32_t hi, uint32_t lo) { return __builtin_amdgcn_alignbit(hi, lo, 16); }
+
+struct Bounds { int st0, en0, st, en; };
+
+// Band of anti-diagonal r: src/align/ksw2_extz2_sse.c:107-116
+__device__ __forceinline__ bool diag_bounds(int r, int qlen, int tlen, int w, Bounds &b)
+{
+    int st = 0, en = tlen - 1;
+    if (st < r - qlen + 1) st = r - qlen + 1;
+    if (en > r) en = r;
+    if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
+    if (en > (r + w) >> 1) en = (r + w) >> 1;
+    if (st > en) return false;
+    b.st0 = st; b.en0 = en; b.st = st & ~15; b.en = (en & ~15) + 15;
+    return true;
+}
+
+// Register-array access with a wave-uniform runtime index. Written as mask
+// arithmetic over every element so the compiler keeps the arrays in VGPRs
+// (an "if (idx == i) a[i] = v" pattern becomes an indexed scratch access).
+__device__ __forceinline__ uint32_t sel_mask(int a, int b) { return 0u - (uint32_t)(a == b); }
+__device__ __forceinline__ uint32_t pick8(const uint32_t (&a)[8], int j)
+{
+    uint32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v |= a[i] & sel_mask(j, i);
+    return v;
+}
+__device__ __forceinline__ int32_t pick16(const int32_t (&a)[16], int j)
+{
+    uint32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v |= (uint32_t)a[i] & sel_mask(j, i);
+    return (int32_t)v;
+}
+__device__ __forceinline__ void put16(int32_t (&a)[16], int j, int32_t val)
+{
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t m = sel_mask(j, i);
+        a[i] = (int32_t)(((uint32_t)a[i] & ~m) | ((uint32_t)val & m));
+    }
+}
+// unsigned byte value of cell c (0..15) of a block stored as x<<8 halves
+__device__ __forceinline__ int byte_of(const uint32_t (&a)[8], int c)
+{
+    return (int)((pick8(a, c >> 1) >> ((c & 1) * 16 + 8)) & 0xffu);
+}
+__device__ __forceinline__ void set_half(uint32_t (&a)[8], int c, uint32_t v16)
+{
+    const uint32_t hm = (c & 1) ? 0xffff0000u : 0x0000ffffu;
+    const uint32_t vv = v16 | (v16 << 16);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t m = hm & sel_mask(c >> 1, i);
+        a[i] = (a[i] & ~m) | (vv & m);
+    }
+}
+
+template <int W, int S>
+struct Smem {
+    uint32_t xch[2][W][S][3];
+    int32_t logH[64][2];
+    uint32_t chunk[64][10];
+    int32_t red[64 * W][3];
+    uint8_t logu[64], logv[64];
+    int job;
+};
+
+// One DP job on one workgroup. Every thread keeps the uniform (per-diagonal)
+// state redundantly; all branches on it are uniform.
+template <int W, int S, bool STATS>
+__device__ void dp_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restrict__ out,
+                       Smem<W, S> &sm)
+{
+    constexpr int NL = 64 * W;
+    constexpr int R = NL * S;
+    const int L = threadIdx.x, wave = L >> 6, lane = L & 63;
+    const int qlen = J.qlen, tlen = J.tlen;
+    int w = J.w;
+    if (w < 0) w = tlen > qlen ? tlen : qlen;
+    const bool right = (J.flags & kFlagRight) != 0;
+    const bool hasn = (J.flags & kFlagHasN) != 0;
+    const int ndiag = qlen + tlen - 1;
+    int ncol = qlen < tlen ? qlen : tlen;
+    ncol = ((ncol < w + 1 ? ncol : w + 1) + 15) / 16 + 1;
+    const size_t rowb = (size_t)ncol * 8;
+    if (ncol > R - 1) {
+        if (L == 0) { out->status = kStTooWide; out->n_cigar = 0; }
+        return;
+    }
+
+    uint32_t U[S][8], V[S][8], X[S][8], Y[S][8], Z[S][8], T[S][8], Q[S][8];
+    int32_t HC[STATS ? S : 1][16];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) { U[k][j] = V[k][j] = X[k][j] = Y[k][j] = 0; Z[k][j] = 0x10001000u; T[k][j] = Q[k][j] = 0; }
+        if constexpr (STATS) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) HC[k][c] = kNegInf;
+        }
+    }
+    int32_t bH = 0, bR = -1, bK = 0;   // STATS: best (H, r, key) of this lane, valid iff bH > 0
+
+    // uniform state
+    int last_st = -1, last_en = -1;
+    int32_t He = kNegInf, Hl = kNegInf;
+    int last_en0 = -1;
+    int32_t score = kNegInf, mte = kNegInf, mte_q = -1, mqe = kNegInf, mqe_t = -1;
+    int64_t cells = 0;
+    bool zd = false;
+    int rlast = 0;                        // last computed anti-diagonal
+    int p_st0 = 0, p_en0 = 0, p_en = 0;   // its bounds (its edge log is pending)
+
+    // consume the edge log of anti-diagonal rr (src/align/ksw2_extz2_sse.c:227-269)
+    auto consume = [&](int rr, int st0, int en0, int en) {
+        const int ue = sm.logu[rr & 63], vl = sm.logv[rr & 63];
+        if (rr == 0) {
+            He = vl - 2 * kQE;
+        } else if (en0 == 0) {
+            He = He + vl - kQE;
+        } else {
+            const int32_t Hp = (en0 == last_en0) ? Hl : He;
+            Hl = (en0 - 1 >= st0) ? Hp + vl - kQE : Hp;
+            He = Hp + ue - kQE;
+        }
+        last_en0 = en0;
+        if (en0 == tlen - 1 && He > mte) { mte = He; mte_q = rr - en; }
+        if constexpr (STATS) {
+            if (rr - st0 == qlen - 1 && sm.logH[rr & 63][1] > mqe) { mqe = sm.logH[rr & 63][1]; mqe_t = st0; }
+        }
+        if (rr == ndiag - 1 && en0 == tlen - 1) score = He;
+    };
+
+    // element pointers (as dwords) for 16-cell windows starting at element e
+    auto tgt_ptr = [&](int e) -> const uint32_t * {
+        return (const uint32_t *)((e & 1) ? (J.tgt_odd + (e - 1)) : (J.tgt_even + e));
+    };
+    auto qry_ptr = [&](int e) -> const uint32_t * {
+        return (const uint32_t *)((e & 1) ? (J.qr_odd + (e - 1)) : (J.qr_even + e));
+    };
+
+    // load query windows (and set up new blocks) for anti-diagonal r
+    auto prefetch = [&](int r, const Bounds &bd, int prev_be) {
+        const int bs = bd.st >> 4, be = bd.en >> 4, bsm = bs % R;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            int d = NL * k + L - bsm;
+            if (d < 0) d += R;
+            const int b = bs + d;
+            if (b <= be) {
+                if (b > prev_be) {   // first activation: calloc state, load target block
+                    const uint32_t *tp = tgt_ptr(16 * b);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        U[k][j] = V[k][j] = X[k][j] = Y[k][j] = 0;
+                        Z[k][j] = 0x10001000u;
+                        T[k][j] = tp[j];
+                    }
+                }
+                const uint32_t *qp = qry_ptr(qlen - 1 - r + 16 * b);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) Q[k][j] = qp[j];
+            }
+        }
+    };
+
+    Bounds bd;
+    diag_bounds(0, qlen, tlen, w, bd);
+    prefetch(0, bd, -1);
+
+    for (int r = 0; r < ndiag; ++r) {
+        const int st0 = bd.st0, en0 = bd.en0, st = bd.st, en = bd.en;
+        const int bs = st >> 4, be = en >> 4, bsm = bs % R;
+        cells += en0 - st0 + 1;
+        int blk[S];
+        bool act[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            int d = NL * k + L - bsm;
+            if (d < 0) d += R;
+            blk[k] = bs + d;
+            act[k] = blk[k] <= be;
+        }
+
+        // top boundary y[r]=0, u[r]=q (:123)
+        if (en >= r) {
+            const int rb = r >> 4, rc = r & 15;
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                if (act[k] && blk[k] == rb) {
+                    set_half(U[k], rc, r ? (uint32_t)kQ << 8 : 0u);
+                    set_half(Y[k], rc, 0u);
+                }
+            }
+        }
+
+        // neighbour exchange: x[t-1], v[t-1] (and H) of the previous diagonal
+        uint32_t nx[S], nv[S];
+        int32_t nh[S];
+        {
+            uint32_t rx[S], rv[S];
+            int32_t rh[S];
+            const int src = (lane + 63) & 63;
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                rx[k] = __shfl(X[k][7], src);
+                rv[k] = __shfl(V[k][7], src);
+                if constexpr (STATS) rh[k] = __shfl(HC[k][15], src); else rh[k] = 0;
+            }
+            if constexpr (W > 1) {
+                if (lane == 63) {
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        sm.xch[r & 1][wave][k][0] = X[k][7];
+                        sm.xch[r & 1][wave][k][1] = V[k][7];
+                        if constexpr (STATS) sm.xch[r & 1][wave][k][2] = (uint32_t)HC[k][15];
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < S; ++k) { nx[k] = rx[k]; nv[k] = rv[k]; nh[k] = rh[k]; }
+            if (lane == 0) {
+                if constexpr (W == 1) {
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        const int kp = (k + S - 1) % S;
+                        nx[k] = rx[kp]; nv[k] = rv[kp]; nh[k] = rh[kp];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        const int ws = wave ? wave - 1 : W - 1;
+                        const int ks = wave ? k : (k + S - 1) % S;
+                        nx[k] = sm.xch[r & 1][ws][ks][0];
+                        nv[k] = sm.xch[r & 1][ws][ks][1];
+                        nh[k] = (int32_t)sm.xch[r & 1][ws][ks][2];
+                    }
+                }
+            }
+        }
+        // the log of diagonal r-1 is complete once every wave passed the barrier
+        if (r > 0) consume(r - 1, p_st0, p_en0, p_en);
+
+        // left boundary of the rounded region (:118-122)
+        if (L == bsm % NL) {
+            const int ks = bsm / NL;
+            const bool keep = st > 0 && st - 1 >= last_st && st - 1 <= last_en;
+            const uint32_t v1 = (st == 0 && r) ? (uint32_t)kQ << 24 : 0u;
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const uint32_t m = keep ? 0u : sel_mask(k, ks);
+                nx[k] = nx[k] & ~m;
+                nv[k] = (nv[k] & ~m) | (v1 & m);
+            }
+        }
+
+        // cell recurrence (:26-47, :197-222) over all active blocks
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            if (!act[k]) continue;
+            const int b = blk[k];
+            const int lo = st0 - 16 * b, hi = en0 - 16 * b;
+            // score fill only inside [st0, en0] (:142-143): elsewhere the old value stays
+            {
+                uint32_t zf[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t sel = T[k][j] ^ Q[k][j];
+                    uint32_t z = __builtin_amdgcn_perm(0x10101010u, 0x0C0C0C15u, sel);
+                    if (hasn) z -= ((T[k][j] & Q[k][j] & 0x04000400u) >> 10) * 0x500u;
+                    zf[j] = z;
+                }
+                if (lo <= 0 && hi >= 15) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) Z[k][j] = zf[j];
+                } else {
+                    const int l0 = lo < 0 ? 0 : lo, h0 = hi > 15 ? 15 : hi;
+                    const uint32_t m16 = (h0 >= l0) ? (((2u << h0) - 1u) & ~((1u << l0) - 1u)) : 0u;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const uint32_t m = ((m16 >> (2 * j)) & 1u ? 0x0000ffffu : 0u) |
+                                           ((m16 >> (2 * j + 1)) & 1u ? 0xffff0000u : 0u);
+                        Z[k][j] = bfi(m, zf[j], Z[k][j]);
+                    }
+                }
+            }
+            uint32_t C[4];
+            auto cells_rl = [&](auto right_tag) {
+                constexpr bool RT = decltype(right_tag)::value;
+#pragma unroll
+                for (int m = 3; m >= 0; --m) {
+                    uint32_t fa[2], fb[2], fx[2], fy[2];
+#pragma unroll
+                    for (int h = 1; h >= 0; --h) {
+                        const int j = 2 * m + h;
+                        const uint32_t xlo = j ? X[k][j - 1] : nx[k];
+                        const uint32_t vlo = j ? V[k][j - 1] : nv[k];
+                        const uint32_t xs = cat16(X[k][j], xlo);
+                        const uint32_t vs = cat16(V[k][j], vlo);
+                        const uint32_t a = pk_add(xs, vs);
+                        const uint32_t ut = U[k][j];
+                        const uint32_t bb = pk_add(Y[k][j], ut);
+                        const uint32_t z0 = Z[k][j];
+                        fa[h] = RT ? pk_subsat(a, z0) : pk_subsat(z0, a);
+                        uint32_t z = pk_maxu(z0, a);
+                        fb[h] = RT ? pk_subsat(bb, z) : pk_subsat(z, bb);
+                        z = pk_maxu(z, bb);
+                        z = pk_minu(z, 0x15001500u);
+                        U[k][j] = pk_sub(z, vs);
+                        V[k][j] = pk_sub(z, ut);
+                        z = pk_sub(z, 0x06000600u);
+                        const uint32_t a2 = pk_sub(a, z), b2 = pk_sub(bb, z);
+                        X[k][j] = pk_maxi(a2, 0u);
+                        Y[k][j] = pk_maxi(b2, 0u);
+                        fx[h] = RT ? a2 : pk_subsat(0u, a2);
+                        fy[h] = RT ? b2 : pk_subsat(0u, b2);
+                    }
+                    // gather the sign bits of 4 cells, then merge the 4 flags into the high nibble
+                    const uint32_t ga = __builtin_amdgcn_perm(fa[1], fa[0], 0x07050301u);
+                    const uint32_t gb = __builtin_amdgcn_perm(fb[1], fb[0], 0x07050301u);
+                    const uint32_t gx = __builtin_amdgcn_perm(fx[1], fx[0], 0x07050301u);
+                    const uint32_t gy = __builtin_amdgcn_perm(fy[1], fy[0], 0x07050301u);
+                    uint32_t c = bfi(0x80808080u, ga, gb >> 1);
+                    c = bfi(0xC0C0C0C0u, c, gx >> 2);
+                    C[m] = bfi(0xE0E0E0E0u, c, gy >> 3);
+                }
+            };
+            if (right) cells_rl(std::integral_constant<bool, true>{});
+            else cells_rl(std::integral_constant<bool, false>{});
+            uint2 o;
+            o.x = bfi(0xF0F0F0F0u, C[1], C[0] >> 4);
+            o.y = bfi(0xF0F0F0F0u, C[3], C[2] >> 4);
+            *(uint2 *)(tb + (size_t)r * rowb + (size_t)(b - bs) * 8) = o;
+
+            // H bookkeeping
+            if (b == (en0 >> 4)) sm.logu[r & 63] = (uint8_t)byte_of(U[k], en0 & 15);
+            {
+                const int tv = en0 > 0 ? en0 - 1 : 0;
+                if (b == (tv >> 4)) sm.logv[r & 63] = (uint8_t)byte_of(V[k], tv & 15);
+            }
+            if constexpr (STATS) {
+                // exact H row (:227-261): fix H[en0] from the pre-update H[en0-1], then add
+                // v-qe over [st0, en0)
+                const int ce = en0 - 16 * b;
+                int32_t hfix = 0;
+                const bool own_e = (ce >= 0 && ce < 16);
+                if (own_e) {
+                    if (r == 0) hfix = byte_of(V[k], 0) - 2 * kQE;
+                    else if (en0 == 0) hfix = HC[k][0] + byte_of(V[k], 0) - kQE;
+                    else hfix = (ce > 0 ? pick16(HC[k], ce - 1) : nh[k]) + byte_of(U[k], ce) - kQE;
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int t = 16 * b + c;
+                    if (t >= st0 && t < en0) HC[k][c] += (int32_t)((V[k][c >> 1] >> ((c & 1) * 16 + 8)) & 0xffu) - kQE;
+                }
+                if (own_e) put16(HC[k], ce, hfix);
+                // per-diagonal max with the reference's tie order (:228-260), folded into a
+                // running lexicographic (H desc, r asc, key asc) best
+                const int en1 = st0 + ((en0 - st0) >> 2 << 2);
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int t = 16 * b + c;
+                    if (t >= st0 && t <= en0) {
+                        const int32_t h = HC[k][c];
+                        const int key = (t == en0) ? 0
+                                        : (t < en1) ? 1 + (((t - st0) & 3) << 20) + ((t - st0) >> 2)
+                                                    : 1 + (4 << 20) + (t - en1);
+                        if (h > bH || (h == bH && bR == r && key < bK)) { bH = h; bR = r; bK = key; }
+                    }
+                }
+                const int cs = st0 - 16 * b;
+                if (r - st0 == qlen - 1 && cs >= 0 && cs < 16) sm.logH[r & 63][1] = pick16(HC[k], cs);
+            }
+        }
+
+        p_st0 = st0; p_en0 = en0; p_en = en; rlast = r;
+        last_st = st; last_en = en;
+        if (r + 1 < ndiag) {
+            if (!diag_bounds(r + 1, qlen, tlen, w, bd)) { zd = true; break; }
+            prefetch(r + 1, bd, be);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    consume(rlast, p_st0, p_en0, p_en);
+
+    // STATS: reduce the per-lane best (H desc, r asc, key asc) over the workgroup
+    int32_t emax = 0, emax_t = -1, emax_q = -1;
+    if constexpr (STATS) {
+        sm.red[L][0] = bH; sm.red[L][1] = bR; sm.red[L][2] = bK;
+        __syncthreads();
+        int32_t gH = 0, gR = -1, gK = 0;
+        for (int i = 0; i < NL; ++i) {
+            const int32_t h = sm.red[i][0], rr = sm.red[i][1], kk = sm.red[i][2];
+            if (rr < 0 || h <= 0) continue;
+            if (h > gH || (h == gH && (rr < gR || (rr == gR && kk < gK)))) { gH = h; gR = rr; gK = kk; }
+        }
+        if (gR >= 0 && gH > 0) {
+            Bounds b2;
+            diag_bounds(gR, qlen, tlen, w, b2);
+            const int en1 = b2.st0 + ((b2.en0 - b2.st0) >> 2 << 2);
+            int t;
+            if (gK == 0) t = b2.en0;
+            else if (gK < 1 + (4 << 20)) t = b2.st0 + 4 * ((gK - 1) & 0xfffff) + ((gK - 1) >> 20);
+            else t = en1 + (gK - 1 - (4 << 20));
+            emax = gH; emax_t = t; emax_q = gR - t;
+        }
+        __syncthreads();
+    }
+
+    // backtrack (include/ksw2.h:129-161) on wave 0, traceback rows staged in LDS
+    int i0 = -1, j0 = -1;
+    if (!zd) { i0 = tlen - 1; j0 = qlen - 1; }
+    else if (STATS && emax_t >= 0 && emax_q >= 0) { i0 = emax_t; j0 = emax_q; }
+    int nout = 0;
+    int status = kStOk;
+    if (zd && !STATS) status = kStNeedStats;
+    if (wave == 0 && i0 >= 0 && status == kStOk) {
+        volatile uint32_t(*chunk)[10] = sm.chunk;
+        int i = i0, j = j0, state = 0;
+        int rc = -1, bb0 = 0;
+        int cur_op = -1, cur_len = 0;
+        auto flush = [&]() {
+            if (cur_len > 0) {
+                if (lane == 0 && nout < J.cig_cap) J.cigar[nout] = (uint32_t)cur_len << 4 | (uint32_t)cur_op;
+                ++nout;
+            }
+        };
+        auto push = [&](int op, int len) {
+            if (op == cur_op) cur_len += len;
+            else { flush(); cur_op = op; cur_len = len; }
+        };
+        while (i >= 0 && j >= 0) {
+            const int r = i + j;
+            if (rc < 0 || r < rc - 63) {
+                rc = r;
+                bb0 = (i - 63 > 0 ? i - 63 : 0) >> 4;
+                const int rr = rc - lane;
+                uint32_t wv[10];
+#pragma unroll
+                for (int q2 = 0; q2 < 10; ++q2) wv[q2] = 0;
+                Bounds br;
+                if (rr >= 0 && diag_bounds(rr, qlen, tlen, w, br)) {
+                    const int sb = br.st >> 4, eb = br.en >> 4;
+                    const uint8_t *row = tb + (size_t)rr * rowb;
+#pragma unroll
+                    for (int q2 = 0; q2 < 5; ++q2) {
+                        const int bb = bb0 + q2;
+                        if (bb >= sb && bb <= eb) {
+                            const uint64_t v = __hip_atomic_load((const uint64_t *)(row + (size_t)(bb - sb) * 8),
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            wv[2 * q2] = (uint32_t)v;
+                            wv[2 * q2 + 1] = (uint32_t)(v >> 32);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q2 = 0; q2 < 10; ++q2) chunk[lane][q2] = wv[q2];
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+            Bounds b3;
+            diag_bounds(r, qlen, tlen, w, b3);
+            int force = -1, d = 0;
+            if (i < b3.st) force = 2;
+            if (i > b3.en) force = 1;
+            if (force < 0) {
+                const int c = i & 15;
+                const uint32_t word = chunk[rc - r][2 * ((i >> 4) - bb0) + (c >> 3)];
+                const uint32_t nib = (word >> ((c & 3) * 8 + ((c >> 2) & 1) * 4)) & 15u;
+                uint32_t fA = (nib >> 3) & 1, fB = (nib >> 2) & 1, fX = (nib >> 1) & 1, fY = nib & 1;
+                if (right) { fA ^= 1; fB ^= 1; fX ^= 1; fY ^= 1; }
+                d = (int)((fB ? 2u : fA) | (fX << 3) | (fY << 4));
+            }
+            if (state == 0) state = d & 7;
+            else if (!((d >> (state + 2)) & 1)) state = 0;
+            if (state == 0) state = d & 7;
+            if (force >= 0) state = force;
+            if (state == 0) { push(0, 1); --i; --j; }
+            else if (state == 1) { push(2, 1); --i; }
+            else { push(1, 1); --j; }
+        }
+        if (i >= 0) push(2, i + 1);
+        if (j >= 0) push(1, j + 1);
+        flush();
+        if (nout > J.cig_cap) status = kStCigarOverflow;
+    }
+    if (L == 0) {
+        out->score = score; out->n_cigar = nout; out->zdropped = zd ? 1 : 0; out->status = status;
+        out->max = (uint32_t)emax; out->max_t = emax_t; out->max_q = emax_q;
+        out->mqe = STATS ? mqe : kNegInf; out->mqe_t = STATS ? mqe_t : -1;
+        out->mte = mte; out->mte_q = mte_q; out->pad_ = 0; out->cells = cells;
+    }
+    __syncthreads();
+}
+
+// Persistent kernel: each workgroup pulls jobs from a queue and reuses its own
+// traceback slot. With njobs == 1 and one workgroup it is the plain
+// one-pair-per-launch form.
+template <int W, int S, bool STATS>
+__global__ __launch_bounds__(64 * W) void dp_persistent(const DpJob *__restrict__ jobs, int njobs,
+                                                        int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
+                                                        size_t tb_slot_bytes, DpOut *__restrict__ outs)
+{
+    __shared__ Smem<W, S> sm;
+    uint8_t *tb = tb_pool + (size_t)blockIdx.x * tb_slot_bytes;
+    for (;;) {
+        if (threadIdx.x == 0) sm.job = atomicAdd(counter, 1);
+        __syncthreads();
+        const int jid = sm.job;
+        __syncthreads();
+        if (jid >= njobs) break;
+        const DpJob J = jobs[jid];
+        dp_job<W, S, STATS>(J, tb, &outs[jid], sm);
+    }
+}
+
+// (W, S) shapes: ring of R = 64*W*S blocks serves bands with n_col_ <= R-1.
+struct Shape { int W, S; };
+static const Shape kShapes[] = {{1, 1}, {1, 2}, {1, 3}, {2, 2}, {2, 3}, {4, 2}, {4, 3}, {8, 3}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+template <int W, int S, bool STATS>
+static const void *kernel_ptr() { return (const void *)&dp_persistent<W, S, STATS>; }
+
+static const void *shape_kernel(int idx, bool stats)
+{
+#define HAL_SHAPE(i, w, s) case i: return stats ? kernel_ptr<w, s, true>() : kernel_ptr<w, s, false>();
+    switch (idx) {
+        HAL_SHAPE(0, 1, 1) HAL_SHAPE(1, 1, 2) HAL_SHAPE(2, 1, 3) HAL_SHAPE(3, 2, 2)
+        HAL_SHAPE(4, 2, 3) HAL_SHAPE(5, 4, 2) HAL_SHAPE(6, 4, 3) HAL_SHAPE(7, 8, 3)
+    }
+#undef HAL_SHAPE
+    return nullptr;
+}
+
+int dp_num_shapes() { return kNumShapes; }
+int dp_shape_ring(int idx) { return 64 * kShapes[idx].W * kShapes[idx].S; }
+int dp_shape_threads(int idx) { return 64 * kShapes[idx].W; }
+
+int dp_shape_max_groups(int idx, bool stats, int num_cu)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, shape_kernel(idx, stats),
+                                                     64 * kShapes[idx].W, 0) != hipSuccess)
+        return 0;
+    return per_cu * num_cu;
+}
+
+hipError_t dp_launch(int idx, bool stats, int groups, const DpJob *jobs, int njobs, int *counter,
+                     uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream)
+{
+    void *args[] = {(void *)&jobs, (void *)&njobs, (void *)&counter, (void *)&tb_pool,
+                    (void *)&tb_slot_bytes, (void *)&outs};
+    return hipLaunchKernel(shape_kernel(idx, stats), dim3(groups), dim3(64 * kShapes[idx].W), args, 0,
+                           stream);
+}
+
+}  // namespace hal

@@ -1,0 +1,670 @@
+// halign_capi.hip -- host side of libhalign_hip.so: the C ABI declared in
+// include/halign_hip.h, sequence preparation, job bucketing by ring shape,
+// traceback-pool sizing and kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/halign_hip.h"
+#include "dp_common.h"
+
+namespace hal {
+int dp_num_shapes();
+int dp_shape_ring(int idx);
+int dp_shape_threads(int idx);
+int dp_shape_max_groups(int idx, bool stats, int num_cu);
+hipError_t dp_launch(int idx, bool stats, int groups, const DpJob *jobs, int njobs, int *counter,
+                     uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
+}  // namespace hal
+
+using namespace hal;
+
+namespace {
+
+// ScoreChar2Idx (reference include/align.h:72-89)
+inline uint8_t score_code(unsigned char c)
+{
+    switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 4;
+    }
+}
+
+// align::auto_band(qlen=ref_len, tlen=query_len) (reference include/align.h:159-172)
+inline int auto_band(int ref_len, int query_len)
+{
+    const int d = std::abs(ref_len - query_len);
+    const int mx = std::max(ref_len, query_len);
+    if ((double)d / (double)mx > 0.5) return -1;
+    return 200 + (int)(0.1 * (double)(ref_len + query_len / 2));
+}
+
+inline int eff_w(int w, int qlen, int tlen) { return w < 0 ? std::max(qlen, tlen) : w; }
+
+// ksw2's n_col_ (reference src/align/ksw2_extz2_sse.c:75-76)
+inline int n_col(int qlen, int tlen, int w)
+{
+    int n = std::min(qlen, tlen);
+    n = std::min(n, w + 1);
+    return (n + 15) / 16 + 1;
+}
+
+// Band overflow ("zdropped" break, src/align/ksw2_extz2_sse.c:111-114) happens
+// iff some anti-diagonal has st > en. Both violated constraints grow
+// monotonically with r, so testing the last anti-diagonal decides it.
+inline bool predict_zdrop(int qlen, int tlen, int w)
+{
+    const int r = qlen + tlen - 2;
+    if (r - qlen + 1 > (r + w) >> 1) return true;
+    if ((r - w + 1) >> 1 > tlen - 1) return true;
+    return false;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n)
+    {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+}  // namespace
+
+struct halign_ctx {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+    std::string err;
+    DevBuf seq, jobs, outs, counters, tb, cig;
+    size_t tb_budget = 0;
+};
+
+namespace {
+
+int fail(halign_ctx *ctx, int code, const char *what, hipError_t e = hipSuccess)
+{
+    char buf[512];
+    if (e != hipSuccess) snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    else snprintf(buf, sizeof buf, "%s", what);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define HCHECK(expr, what)                                           \
+    do {                                                             \
+        hipError_t e_ = (expr);                                      \
+        if (e_ != hipSuccess) return fail(ctx, HALIGN_E_HIP, what, e_); \
+    } while (0)
+
+// Host-side description of one DP job before its device pointers are known.
+struct Spec {
+    int64_t tgt_elem, qry_elem;   // element offsets of the even copies (pad included before)
+    int64_t tgt_odd_elem, qry_odd_elem;
+    int32_t tlen, qlen, w, flags;
+    int64_t cig_off;              // uint32 offset into the device cigar buffer
+    int32_t cig_cap;
+    int shape;
+    bool stats;
+    size_t tb_bytes;
+};
+
+void spread_target(const uint8_t *codes, int len, uint16_t *even, uint16_t *odd)
+{
+    // even[-kSeqPad .. len+kSeqPad), odd[k] = even[k+1]
+    const int n = len + 2 * kSeqPad;
+    for (int k = 0; k < n; ++k) {
+        const int t = k - kSeqPad;
+        even[k] = (t >= 0 && t < len) ? (uint16_t)((codes[t] << 8) | 0x0C) : (uint16_t)0x000C;
+    }
+    for (int k = 0; k < n; ++k) odd[k] = (k + 1 < n) ? even[k + 1] : (uint16_t)0x000C;
+}
+
+void spread_query_rev(const uint8_t *codes, int len, uint16_t *even, uint16_t *odd)
+{
+    const int n = len + 2 * kSeqPad;
+    for (int k = 0; k < n; ++k) {
+        const int t = k - kSeqPad;   // reversed index
+        even[k] = (t >= 0 && t < len) ? (uint16_t)(codes[len - 1 - t] << 8) : (uint16_t)0;
+    }
+    for (int k = 0; k < n; ++k) odd[k] = (k + 1 < n) ? even[k + 1] : (uint16_t)0;
+}
+
+int pick_shape(int ncol)
+{
+    const int ns = dp_num_shapes();
+    for (int i = 0; i < ns; ++i)
+        if (ncol <= dp_shape_ring(i) - 1) return i;
+    return -1;
+}
+
+// Run a list of jobs whose sequences already sit in ctx->seq (spread u16) and
+// whose CIGARs go to ctx->cig. outs receives one DpOut per job, host side.
+int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector<DpOut> &outs,
+              double *kernel_ms)
+{
+    const size_t n = specs.size();
+    outs.assign(n, DpOut{});
+    if (kernel_ms) *kernel_ms = 0.0;
+    if (n == 0) return HALIGN_OK;
+    const uint16_t *seq = ctx->seq.as<uint16_t>();
+    uint32_t *cig = ctx->cig.as<uint32_t>();
+
+    // bucket by (shape, stats)
+    const int ns = dp_num_shapes();
+    std::vector<std::vector<int>> buckets(2 * ns);
+    for (size_t i = 0; i < n; ++i) buckets[specs[i].shape * 2 + (specs[i].stats ? 1 : 0)].push_back((int)i);
+
+    HCHECK(ctx->jobs.ensure(n * sizeof(DpJob)), "alloc jobs");
+    HCHECK(ctx->outs.ensure(n * sizeof(DpOut)), "alloc outs");
+    HCHECK(ctx->counters.ensure((size_t)(plain ? n : 2 * ns) * sizeof(int)), "alloc counters");
+    std::vector<DpJob> hjobs(n);
+    std::vector<int> order;
+    order.reserve(n);
+    for (auto &b : buckets)
+        for (int i : b) order.push_back(i);
+    for (size_t k = 0; k < n; ++k) {
+        const Spec &s = specs[order[k]];
+        DpJob &j = hjobs[k];
+        j.tgt_even = seq + s.tgt_elem;
+        j.tgt_odd = seq + s.tgt_odd_elem;
+        j.qr_even = seq + s.qry_elem;
+        j.qr_odd = seq + s.qry_odd_elem;
+        j.tlen = s.tlen; j.qlen = s.qlen; j.w = s.w; j.flags = s.flags;
+        j.cigar = cig + s.cig_off;
+        j.cig_cap = s.cig_cap;
+        j.pad_ = 0;
+    }
+    HCHECK(hipMemcpyAsync(ctx->jobs.p, hjobs.data(), n * sizeof(DpJob), hipMemcpyHostToDevice, ctx->stream),
+           "upload jobs");
+    HCHECK(hipMemsetAsync(ctx->counters.p, 0, ctx->counters.cap, ctx->stream), "clear counters");
+
+    // traceback pool: one slot per resident workgroup
+    size_t need_slot = 0;
+    for (auto &s : specs) need_slot = std::max(need_slot, s.tb_bytes);
+    if (need_slot > ctx->tb_budget) return fail(ctx, HALIGN_E_NOMEM, "traceback of one job exceeds the device budget");
+
+    HCHECK(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    DpJob *djobs = ctx->jobs.as<DpJob>();
+    DpOut *douts = ctx->outs.as<DpOut>();
+    int *ctr = ctx->counters.as<int>();
+    size_t pos = 0;
+    for (int bi = 0; bi < 2 * ns; ++bi) {
+        auto &b = buckets[bi];
+        if (b.empty()) continue;
+        const int shape = bi / 2;
+        const bool stats = bi & 1;
+        size_t slot = 0;
+        for (int i : b) slot = std::max(slot, specs[i].tb_bytes);
+        slot = (slot + 255) & ~(size_t)255;
+        if (plain) {
+            HCHECK(ctx->tb.ensure(slot), "alloc traceback");
+            for (size_t k = 0; k < b.size(); ++k) {
+                HCHECK(dp_launch(shape, stats, 1, djobs + pos + k, 1, ctr + pos + k, ctx->tb.as<uint8_t>(), slot,
+                                 douts + pos + k, ctx->stream),
+                       "launch dp (plain)");
+            }
+        } else {
+            int groups = dp_shape_max_groups(shape, stats, ctx->num_cu);
+            if (groups <= 0) return fail(ctx, HALIGN_E_HIP, "occupancy query failed");
+            groups = std::min<size_t>(groups, b.size());
+            groups = (int)std::min<size_t>(groups, std::max<size_t>(1, ctx->tb_budget / slot));
+            HCHECK(ctx->tb.ensure(slot * groups), "alloc traceback pool");
+            HCHECK(dp_launch(shape, stats, groups, djobs + pos, (int)b.size(), ctr + bi, ctx->tb.as<uint8_t>(), slot,
+                             douts + pos, ctx->stream),
+                   "launch dp");
+        }
+        pos += b.size();
+    }
+    HCHECK(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    std::vector<DpOut> tmp(n);
+    HCHECK(hipMemcpyAsync(tmp.data(), ctx->outs.p, n * sizeof(DpOut), hipMemcpyDeviceToHost, ctx->stream),
+           "download outs");
+    HCHECK(hipStreamSynchronize(ctx->stream), "dp kernels");
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
+    if (kernel_ms) *kernel_ms = ms;
+    for (size_t k = 0; k < n; ++k) outs[order[k]] = tmp[k];
+    for (size_t i = 0; i < n; ++i) {
+        const int st = outs[i].status;
+        if (st != kStOk) {
+            char msg[128];
+            snprintf(msg, sizeof msg, "dp job %zu returned status %d", i, st);
+            return fail(ctx, HALIGN_E_KERNEL, msg);
+        }
+    }
+    return HALIGN_OK;
+}
+
+bool fill_spec_shape(Spec &s, bool full_stats)
+{
+    const int w = eff_w(s.w, s.qlen, s.tlen);
+    const int nc = n_col(s.qlen, s.tlen, w);
+    s.shape = pick_shape(nc);
+    s.stats = full_stats || predict_zdrop(s.qlen, s.tlen, w);
+    s.tb_bytes = (size_t)(s.qlen + s.tlen - 1) * (size_t)nc * 8;
+    return s.shape >= 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *halign_version(void) { return "halign-mi355x 0.1 (gfx950)"; }
+
+int halign_init(int device, halign_ctx **out)
+{
+    if (!out) return HALIGN_E_ARG;
+    *out = nullptr;
+    halign_ctx *ctx = new (std::nothrow) halign_ctx();
+    if (!ctx) return HALIGN_E_NOMEM;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) { delete ctx; return HALIGN_E_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; return HALIGN_E_HIP; }
+    ctx->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return HALIGN_E_HIP;
+    }
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    ctx->tb_budget = (size_t)((double)free_b * 0.45);
+    *out = ctx;
+    return HALIGN_OK;
+}
+
+void halign_free(halign_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->seq.release(); ctx->jobs.release(); ctx->outs.release();
+    ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *halign_last_error(const halign_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int halign_dp_batch(halign_ctx *ctx, int32_t n, const halign_dp_task *tasks, const uint8_t *query_codes,
+                    const uint8_t *target_codes, int32_t full_stats, int32_t plain,
+                    halign_dp_result *results, uint32_t *cigar_out, int64_t cigar_cap, double *kernel_ms)
+{
+    if (!ctx || n < 0 || (n > 0 && (!tasks || !results))) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    // spread sequences: per task, target even/odd then query even/odd
+    std::vector<Spec> specs(n);
+    int64_t elems = 0, cig_total = 0;
+    for (int i = 0; i < n; ++i) {
+        const halign_dp_task &t = tasks[i];
+        if (t.qlen < 0 || t.tlen < 0) return fail(ctx, HALIGN_E_ARG, "negative length");
+        if (!(t.flag & 0x04) || (t.flag & ~0x06)) return fail(ctx, HALIGN_E_UNSUPPORTED, "flag must be GENERIC_SC [| RIGHT]");
+        const int64_t tn = t.tlen + 2 * kSeqPad, qn = t.qlen + 2 * kSeqPad;
+        Spec &s = specs[i];
+        s.tgt_elem = elems + kSeqPad; s.tgt_odd_elem = elems + tn + kSeqPad;
+        s.qry_elem = elems + 2 * tn + kSeqPad; s.qry_odd_elem = elems + 2 * tn + qn + kSeqPad;
+        elems += 2 * tn + 2 * qn;
+        elems = (elems + 1) & ~(int64_t)1;   // keep every copy 4-byte aligned
+        s.tlen = t.tlen; s.qlen = t.qlen; s.w = t.w;
+        s.flags = (t.flag & 0x02) ? kFlagRight : 0;
+        for (int k = 0; k < t.qlen; ++k) if (query_codes[t.query_off + k] > 3) { s.flags |= kFlagHasN; break; }
+        if (!(s.flags & kFlagHasN))
+            for (int k = 0; k < t.tlen; ++k) if (target_codes[t.target_off + k] > 3) { s.flags |= kFlagHasN; break; }
+        s.cig_off = cig_total;
+        s.cig_cap = t.qlen + t.tlen + 4;
+        cig_total += s.cig_cap;
+        if (t.qlen > 0 && t.tlen > 0 && !fill_spec_shape(s, full_stats != 0))
+            return fail(ctx, HALIGN_E_UNSUPPORTED, "band wider than the largest ring shape");
+    }
+    std::vector<uint16_t> host((size_t)elems + 2, 0);
+    for (int i = 0; i < n; ++i) {
+        const halign_dp_task &t = tasks[i];
+        const Spec &s = specs[i];
+        spread_target(target_codes + t.target_off, t.tlen, host.data() + s.tgt_elem - kSeqPad,
+                      host.data() + s.tgt_odd_elem - kSeqPad);
+        spread_query_rev(query_codes + t.query_off, t.qlen, host.data() + s.qry_elem - kSeqPad,
+                         host.data() + s.qry_odd_elem - kSeqPad);
+    }
+    HCHECK(ctx->seq.ensure(host.size() * 2), "alloc sequences");
+    HCHECK(ctx->cig.ensure((size_t)std::max<int64_t>(cig_total, 1) * 4), "alloc cigars");
+    HCHECK(hipMemcpyAsync(ctx->seq.p, host.data(), host.size() * 2, hipMemcpyHostToDevice, ctx->stream),
+           "upload sequences");
+    // empty sequences: ksw2 returns immediately with a reset ez (:56-57)
+    std::vector<Spec> run;
+    std::vector<int> run_idx;
+    for (int i = 0; i < n; ++i)
+        if (tasks[i].qlen > 0 && tasks[i].tlen > 0) { run.push_back(specs[i]); run_idx.push_back(i); }
+    std::vector<DpOut> outs;
+    int rc = run_specs(ctx, run, plain != 0, outs, kernel_ms);
+    if (rc != HALIGN_OK) return rc;
+    std::vector<uint32_t> hcig((size_t)std::max<int64_t>(cig_total, 1));
+    HCHECK(hipMemcpy(hcig.data(), ctx->cig.p, (size_t)cig_total * 4, hipMemcpyDeviceToHost), "download cigars");
+    int64_t out_pos = 0;
+    for (int i = 0; i < n; ++i) {
+        halign_dp_result &r = results[i];
+        r.max = 0; r.zdropped = 0; r.max_q = r.max_t = r.mqe_t = r.mte_q = -1;
+        r.mqe = r.mte = r.score = -0x40000000; r.n_cigar = 0; r.status = 0; r.pad_ = 0; r.cells = 0;
+        r.cigar_off = out_pos;
+    }
+    for (size_t k = 0; k < run.size(); ++k) {
+        const int i = run_idx[k];
+        const DpOut &o = outs[k];
+        halign_dp_result &r = results[i];
+        r.max = o.max; r.zdropped = o.zdropped; r.max_q = o.max_q; r.max_t = o.max_t;
+        r.mqe = o.mqe; r.mqe_t = o.mqe_t; r.mte = o.mte; r.mte_q = o.mte_q; r.score = o.score;
+        r.n_cigar = o.n_cigar; r.status = o.status; r.cells = o.cells;
+    }
+    for (int i = 0; i < n; ++i) {
+        halign_dp_result &r = results[i];
+        r.cigar_off = out_pos;
+        if (out_pos + r.n_cigar > cigar_cap) return fail(ctx, HALIGN_E_CAPACITY, "cigar_out too small");
+        const uint32_t *src = hcig.data() + specs[i].cig_off;
+        for (int k = 0; k < r.n_cigar; ++k) cigar_out[out_pos + k] = src[r.n_cigar - 1 - k];
+        out_pos += r.n_cigar;
+    }
+    return HALIGN_OK;
+}
+
+int halign_ksw_extz2(halign_ctx *ctx, void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                     int8_t m, const int8_t *mat, int8_t q, int8_t e, int w, int zdrop, int end_bonus, int flag,
+                     halign_extz_t *ez)
+{
+    (void)km;
+    if (!ctx || !ez) return HALIGN_E_ARG;
+    static const int8_t dna5[25] = {5, -4, -4, -4, 0, -4, 5, -4, -4, 0, -4, -4, 5, -4, 0,
+                                    -4, -4, -4, 5, 0, 0, 0, 0, 0, 0};
+    // ksw_reset_extz (include/ksw2.h:184-189)
+    ez->max_q = ez->max_t = ez->mqe_t = ez->mte_q = -1;
+    ez->max = 0; ez->score = ez->mqe = ez->mte = -0x40000000;
+    ez->n_cigar = 0; ez->zdropped = 0; ez->reach_end = 0;
+    if (m <= 0 || qlen <= 0 || tlen <= 0) return HALIGN_OK;
+    if (m != 5 || !mat || memcmp(mat, dna5, 25) != 0 || q != 6 || e != 2 || zdrop >= 0 || end_bonus != 0 ||
+        !(flag & 0x04) || (flag & ~0x06))
+        return fail(ctx, HALIGN_E_UNSUPPORTED, "ksw_extz2: only the center-star path configuration is supported");
+    halign_dp_task t{qlen, tlen, w, flag, 0, 0};
+    halign_dp_result r;
+    std::vector<uint32_t> cig((size_t)qlen + tlen + 4);
+    int rc = halign_dp_batch(ctx, 1, &t, query, target, 1, 1, &r, cig.data(), (int64_t)cig.size(), nullptr);
+    if (rc != HALIGN_OK) return rc;
+    ez->max = r.max; ez->zdropped = r.zdropped; ez->max_q = r.max_q; ez->max_t = r.max_t;
+    ez->mqe = r.mqe; ez->mqe_t = r.mqe_t; ez->mte = r.mte; ez->mte_q = r.mte_q; ez->score = r.score;
+    if (r.n_cigar > ez->m_cigar) {
+        uint32_t *nc = (uint32_t *)realloc(ez->cigar, (size_t)r.n_cigar * 4);
+        if (!nc) return fail(ctx, HALIGN_E_NOMEM, "cigar realloc");
+        ez->cigar = nc;
+        ez->m_cigar = r.n_cigar;
+    }
+    memcpy(ez->cigar, cig.data(), (size_t)r.n_cigar * 4);
+    ez->n_cigar = r.n_cigar;
+    return HALIGN_OK;
+}
+
+int halign_global_align_batch(halign_ctx *ctx, int32_t n, const char *const *refs, const int32_t *ref_lens,
+                              const char *const *queries, const int32_t *query_lens, const int32_t *flags,
+                              uint32_t *cigar_out, int64_t cigar_cap, int64_t *offsets, int32_t *scores)
+{
+    if (!ctx || n < 0 || (n > 0 && (!refs || !ref_lens || !queries || !query_lens || !flags || !offsets)))
+        return HALIGN_E_ARG;
+    // encode (ScoreChar2Idx) and build tasks for the non-empty pairs
+    std::vector<uint8_t> qc, tc;
+    std::vector<halign_dp_task> tasks;
+    std::vector<int> idx;
+    for (int i = 0; i < n; ++i) {
+        const int rl = ref_lens[i], ql = query_lens[i];
+        if (rl < 0 || ql < 0) return fail(ctx, HALIGN_E_ARG, "negative length");
+        if (rl == 0 || ql == 0) continue;
+        halign_dp_task t;
+        t.qlen = ql; t.tlen = rl; t.w = auto_band(rl, ql); t.flag = flags[i];
+        t.query_off = (int64_t)qc.size(); t.target_off = (int64_t)tc.size();
+        for (int k = 0; k < ql; ++k) qc.push_back(score_code((unsigned char)queries[i][k]));
+        for (int k = 0; k < rl; ++k) tc.push_back(score_code((unsigned char)refs[i][k]));
+        tasks.push_back(t);
+        idx.push_back(i);
+    }
+    std::vector<halign_dp_result> res(tasks.size());
+    int64_t tot = 0;
+    for (auto &t : tasks) tot += t.qlen + t.tlen + 4;
+    std::vector<uint32_t> cig((size_t)std::max<int64_t>(tot, 1));
+    if (!tasks.empty()) {
+        int rc = halign_dp_batch(ctx, (int32_t)tasks.size(), tasks.data(), qc.data(), tc.data(), 0, 0, res.data(),
+                                 cig.data(), (int64_t)cig.size(), nullptr);
+        if (rc != HALIGN_OK) return rc;
+    }
+    // assemble in input order; empty-side pairs are pure I or D (src/align/align.cpp:26-34)
+    int64_t pos = 0;
+    size_t k = 0;
+    for (int i = 0; i < n; ++i) {
+        offsets[i] = pos;
+        const int rl = ref_lens[i], ql = query_lens[i];
+        if (rl == 0 || ql == 0) {
+            if (scores) scores[i] = -0x40000000;
+            if (rl == 0 && ql > 0) {
+                if (pos + 1 > cigar_cap) return fail(ctx, HALIGN_E_CAPACITY, "cigar_out too small");
+                cigar_out[pos++] = (uint32_t)ql << 4 | 1u;
+            } else if (ql == 0 && rl > 0) {
+                if (pos + 1 > cigar_cap) return fail(ctx, HALIGN_E_CAPACITY, "cigar_out too small");
+                cigar_out[pos++] = (uint32_t)rl << 4 | 2u;
+            }
+            continue;
+        }
+        const halign_dp_result &r = res[k++];
+        if (scores) scores[i] = r.score;
+        if (pos + r.n_cigar > cigar_cap) return fail(ctx, HALIGN_E_CAPACITY, "cigar_out too small");
+        memcpy(cigar_out + pos, cig.data() + r.cigar_off, (size_t)r.n_cigar * 4);
+        pos += r.n_cigar;
+    }
+    offsets[n] = pos;
+    return HALIGN_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device-resident pair set (kernel-only DP workload and serving)
+// ---------------------------------------------------------------------------
+namespace {
+
+// encode ASCII queries [first, first+count) into reversed spread even/odd copies
+__global__ void prep_queries(const char *__restrict__ raw, const int64_t *__restrict__ qoff,
+                             const int64_t *__restrict__ region, int first, int count,
+                             uint16_t *__restrict__ seq)
+{
+    const int qi = blockIdx.y;
+    if (qi >= count) return;
+    const int64_t beg = qoff[first + qi], len = qoff[first + qi + 1] - beg;
+    const int64_t n = len + 2 * kSeqPad;
+    uint16_t *even = seq + region[qi];
+    uint16_t *odd = even + n;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        auto code = [&](int64_t kk) -> uint16_t {
+            const int64_t t = kk - kSeqPad;
+            if (t < 0 || t >= len) return 0;
+            const unsigned char c = (unsigned char)raw[beg + len - 1 - t];
+            const uint16_t v = (c == 'A' || c == 'a') ? 0 : (c == 'C' || c == 'c') ? 1
+                             : (c == 'G' || c == 'g') ? 2 : (c == 'T' || c == 't') ? 3 : 4;
+            return (uint16_t)(v << 8);
+        };
+        even[k] = code(k);
+        odd[k] = (k + 1 < n) ? code(k + 1) : (uint16_t)0;
+    }
+}
+
+}  // namespace
+
+struct halign_pairset {
+    halign_ctx *ctx = nullptr;
+    int32_t clen = 0, n = 0;
+    DevBuf center;     // spread even/odd center
+    DevBuf raw, qoff;  // ASCII queries and their offsets
+    std::vector<int64_t> hoff;
+    std::vector<uint8_t> hasn;
+    bool center_hasn = false;
+    DevBuf work, cig, region;
+    int32_t last_first = 0, last_count = 0;
+    std::vector<DpOut> outs;
+    std::vector<int64_t> cig_off;
+};
+
+extern "C" {
+
+int halign_pairset_create(halign_ctx *ctx, const char *center, int32_t clen, int32_t n, const char *queries,
+                          const int64_t *query_offsets, halign_pairset **out)
+{
+    if (!ctx || !center || clen <= 0 || n < 0 || !queries || !query_offsets || !out) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    halign_pairset *ps = new (std::nothrow) halign_pairset();
+    if (!ps) return HALIGN_E_NOMEM;
+    ps->ctx = ctx; ps->clen = clen; ps->n = n;
+    std::vector<uint8_t> cc(clen);
+    for (int i = 0; i < clen; ++i) {
+        cc[i] = score_code((unsigned char)center[i]);
+        if (cc[i] > 3) ps->center_hasn = true;
+    }
+    const int64_t cn = clen + 2 * kSeqPad;
+    std::vector<uint16_t> hc((size_t)cn * 2);
+    spread_target(cc.data(), clen, hc.data(), hc.data() + cn);
+    const int64_t total = query_offsets[n] - query_offsets[0];
+    ps->hoff.assign(query_offsets, query_offsets + n + 1);
+    for (auto &o : ps->hoff) o -= query_offsets[0];
+    ps->hasn.assign(n, 0);
+    for (int i = 0; i < n; ++i)
+        for (int64_t k = ps->hoff[i]; k < ps->hoff[i + 1]; ++k)
+            if (score_code((unsigned char)queries[query_offsets[0] + k]) > 3) { ps->hasn[i] = 1; break; }
+    hipError_t e;
+    if ((e = ps->center.ensure(hc.size() * 2)) != hipSuccess ||
+        (e = ps->raw.ensure((size_t)std::max<int64_t>(total, 1))) != hipSuccess ||
+        (e = ps->qoff.ensure((size_t)(n + 1) * 8)) != hipSuccess) {
+        delete ps;
+        return fail(ctx, HALIGN_E_NOMEM, "pairset alloc", e);
+    }
+    HCHECK(hipMemcpy(ps->center.p, hc.data(), hc.size() * 2, hipMemcpyHostToDevice), "upload center");
+    HCHECK(hipMemcpy(ps->raw.p, queries + query_offsets[0], (size_t)total, hipMemcpyHostToDevice), "upload queries");
+    HCHECK(hipMemcpy(ps->qoff.p, ps->hoff.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice), "upload offsets");
+    *out = ps;
+    return HALIGN_OK;
+}
+
+int halign_pairset_run(halign_pairset *ps, int32_t first, int32_t count, int32_t plain, double *kernel_ms,
+                       int64_t *cells)
+{
+    if (!ps || first < 0 || count < 0 || first + count > ps->n) return HALIGN_E_ARG;
+    halign_ctx *ctx = ps->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    // The pair set's sequences live in its own buffers; run_specs reads from
+    // ctx->seq, so lay the center and the prepared queries out there.
+    const int64_t cn = ps->clen + 2 * kSeqPad;
+    std::vector<int64_t> region(count);
+    int64_t elems = 2 * cn;
+    for (int i = 0; i < count; ++i) {
+        region[i] = elems;
+        const int64_t ql = ps->hoff[first + i + 1] - ps->hoff[first + i];
+        elems += 2 * (ql + 2 * kSeqPad);
+        elems = (elems + 1) & ~(int64_t)1;
+    }
+    HCHECK(ctx->seq.ensure((size_t)elems * 2 + 16), "alloc sequences");
+    HCHECK(ps->region.ensure((size_t)std::max(count, 1) * 8), "alloc regions");
+    HCHECK(hipMemcpyAsync(ctx->seq.p, ps->center.p, (size_t)cn * 2 * 2, hipMemcpyDeviceToDevice, ctx->stream),
+           "copy center");
+    HCHECK(hipMemcpyAsync(ps->region.p, region.data(), (size_t)count * 8, hipMemcpyHostToDevice, ctx->stream),
+           "upload regions");
+    if (count > 0) {
+        dim3 grid(64, count);
+        hipLaunchKernelGGL(prep_queries, grid, dim3(256), 0, ctx->stream, ps->raw.as<const char>(),
+                           ps->qoff.as<const int64_t>(), ps->region.as<const int64_t>(), first, count,
+                           ctx->seq.as<uint16_t>());
+        HCHECK(hipGetLastError(), "launch prep");
+    }
+    std::vector<Spec> specs(count);
+    int64_t cig_total = 0;
+    ps->cig_off.assign(count, 0);
+    for (int i = 0; i < count; ++i) {
+        Spec &s = specs[i];
+        const int32_t ql = (int32_t)(ps->hoff[first + i + 1] - ps->hoff[first + i]);
+        s.tgt_elem = kSeqPad; s.tgt_odd_elem = cn + kSeqPad;
+        s.qry_elem = region[i] + kSeqPad;
+        s.qry_odd_elem = region[i] + (ql + 2 * kSeqPad) + kSeqPad;
+        s.tlen = ps->clen; s.qlen = ql;
+        s.w = auto_band(ps->clen, ql);
+        s.flags = kFlagRight | ((ps->center_hasn || ps->hasn[first + i]) ? kFlagHasN : 0);
+        s.cig_off = cig_total;
+        s.cig_cap = ql + ps->clen + 4;
+        ps->cig_off[i] = cig_total;
+        cig_total += s.cig_cap;
+        if (ql == 0) return fail(ctx, HALIGN_E_ARG, "empty query in pair set");
+        if (!fill_spec_shape(s, false)) return fail(ctx, HALIGN_E_UNSUPPORTED, "band too wide");
+    }
+    HCHECK(ctx->cig.ensure((size_t)std::max<int64_t>(cig_total, 1) * 4), "alloc cigars");
+    int rc = run_specs(ctx, specs, plain != 0, ps->outs, kernel_ms);
+    if (rc != HALIGN_OK) return rc;
+    if (cells) {
+        int64_t c = 0;
+        for (auto &o : ps->outs) c += o.cells;
+        *cells = c;
+    }
+    ps->last_first = first;
+    ps->last_count = count;
+    return HALIGN_OK;
+}
+
+int halign_pairset_result(halign_pairset *ps, int32_t idx, int32_t *score, uint32_t *cigar, int32_t cigar_cap,
+                          int32_t *n_cigar)
+{
+    if (!ps || idx < ps->last_first || idx >= ps->last_first + ps->last_count) return HALIGN_E_ARG;
+    halign_ctx *ctx = ps->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const int k = idx - ps->last_first;
+    const DpOut &o = ps->outs[k];
+    if (score) *score = o.score;
+    if (n_cigar) *n_cigar = o.n_cigar;
+    if (cigar) {
+        if (o.n_cigar > cigar_cap) return fail(ctx, HALIGN_E_CAPACITY, "cigar buffer too small");
+        std::vector<uint32_t> tmp((size_t)std::max(o.n_cigar, 1));
+        HCHECK(hipMemcpy(tmp.data(), ctx->cig.as<uint32_t>() + ps->cig_off[k], (size_t)o.n_cigar * 4,
+                         hipMemcpyDeviceToHost),
+               "download cigar");
+        for (int i = 0; i < o.n_cigar; ++i) cigar[i] = tmp[o.n_cigar - 1 - i];
+    }
+    return HALIGN_OK;
+}
+
+void halign_pairset_free(halign_pairset *ps)
+{
+    if (!ps) return;
+    ps->center.release(); ps->raw.release(); ps->qoff.release(); ps->work.release();
+    ps->cig.release(); ps->region.release();
+    delete ps;
+}
+
+}  // extern "C"

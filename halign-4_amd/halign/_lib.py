@@ -1,0 +1,188 @@
+"""ctypes binding of libhalign_hip.so (C ABI in include/halign_hip.h).
+
+The product path runs only through this library; if it is missing the import
+fails loudly (there is no CPU fallback).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libhalign_hip.so")
+
+HALIGN_OK = 0
+ERRORS = {-1: "invalid argument", -2: "unsupported configuration", -3: "HIP runtime error",
+          -4: "out of memory", -5: "output buffer too small", -6: "kernel status"}
+
+_P = C.c_void_p
+
+
+class DpTask(C.Structure):
+    _fields_ = [("qlen", C.c_int32), ("tlen", C.c_int32), ("w", C.c_int32), ("flag", C.c_int32),
+                ("query_off", C.c_int64), ("target_off", C.c_int64)]
+
+
+DP_RESULT_DTYPE = np.dtype([("max", "<u4"), ("zdropped", "<i4"), ("max_q", "<i4"),
+                            ("max_t", "<i4"), ("mqe", "<i4"), ("mqe_t", "<i4"), ("mte", "<i4"),
+                            ("mte_q", "<i4"), ("score", "<i4"), ("n_cigar", "<i4"),
+                            ("status", "<i4"), ("pad_", "<i4"), ("cells", "<i8"),
+                            ("cigar_off", "<i8")])
+DP_TASK_DTYPE = np.dtype([("qlen", "<i4"), ("tlen", "<i4"), ("w", "<i4"), ("flag", "<i4"),
+                          ("query_off", "<i8"), ("target_off", "<i8")])
+
+
+class HalignError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(LIB_PATH)
+    lib.halign_init.argtypes = [C.c_int, C.POINTER(_P)]
+    lib.halign_free.argtypes = [_P]
+    lib.halign_free.restype = None
+    lib.halign_last_error.argtypes = [_P]
+    lib.halign_last_error.restype = C.c_char_p
+    lib.halign_version.restype = C.c_char_p
+    lib.halign_dp_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, _P,
+                                    C.c_int64, C.POINTER(C.c_double)]
+    lib.halign_global_align_batch.argtypes = [_P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int64,
+                                              _P, _P]
+    lib.halign_pairset_create.argtypes = [_P, C.c_char_p, C.c_int32, C.c_int32, C.c_char_p, _P,
+                                          C.POINTER(_P)]
+    lib.halign_pairset_run.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    lib.halign_pairset_result.argtypes = [_P, C.c_int32, C.POINTER(C.c_int32), _P, C.c_int32,
+                                          C.POINTER(C.c_int32)]
+    lib.halign_pairset_free.argtypes = [_P]
+    lib.halign_pairset_free.restype = None
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One HIP device + stream (halign_init)."""
+
+    def __init__(self, device=0):
+        self.lib = lib()
+        h = _P()
+        rc = self.lib.halign_init(device, C.byref(h))
+        if rc != HALIGN_OK:
+            raise HalignError(f"halign_init failed: {ERRORS.get(rc, rc)}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.halign_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != HALIGN_OK:
+            msg = self.lib.halign_last_error(self.h).decode()
+            raise HalignError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def dp_batch(self, tasks, query_codes, target_codes, full_stats=True, plain=False):
+        """tasks: structured array DP_TASK_DTYPE. Returns (results, cigars list, kernel_ms)."""
+        tasks = np.ascontiguousarray(tasks, DP_TASK_DTYPE)
+        n = len(tasks)
+        qc = np.ascontiguousarray(query_codes, np.uint8)
+        tc = np.ascontiguousarray(target_codes, np.uint8)
+        res = np.zeros(n, DP_RESULT_DTYPE)
+        cap = int((tasks["qlen"].astype(np.int64) + tasks["tlen"] + 4).sum()) + 1
+        cig = np.zeros(cap, np.uint32)
+        ms = C.c_double(0)
+        rc = self.lib.halign_dp_batch(self.h, n, _ptr(tasks), _ptr(qc) if len(qc) else None,
+                                      _ptr(tc) if len(tc) else None, int(full_stats), int(plain),
+                                      _ptr(res), _ptr(cig), cap, C.byref(ms))
+        self._check(rc, "halign_dp_batch")
+        cigs = [cig[r["cigar_off"]:r["cigar_off"] + r["n_cigar"]].copy() for r in res]
+        return res, cigs, ms.value
+
+    def global_align_batch(self, refs, queries, flags):
+        """globalAlignKSW2 over ASCII pairs. Returns (list of CIGAR arrays, scores)."""
+        n = len(refs)
+        rb = [r.encode() if isinstance(r, str) else bytes(r) for r in refs]
+        qb = [q.encode() if isinstance(q, str) else bytes(q) for q in queries]
+        rbufs = [C.create_string_buffer(x, len(x) + 1) for x in rb]
+        qbufs = [C.create_string_buffer(x, len(x) + 1) for x in qb]
+        rptr = (C.c_char_p * n)(*[C.cast(b, C.c_char_p) for b in rbufs])
+        qptr = (C.c_char_p * n)(*[C.cast(b, C.c_char_p) for b in qbufs])
+        rl = np.array([len(x) for x in rb], np.int32)
+        ql = np.array([len(x) for x in qb], np.int32)
+        fl = np.ascontiguousarray(flags, np.int32)
+        cap = int(rl.astype(np.int64).sum() + ql.sum() + 4 * n + 4)
+        cig = np.zeros(cap, np.uint32)
+        off = np.zeros(n + 1, np.int64)
+        sc = np.zeros(n, np.int32)
+        rc = self.lib.halign_global_align_batch(self.h, n, C.cast(rptr, _P), _ptr(rl),
+                                                C.cast(qptr, _P), _ptr(ql), _ptr(fl), _ptr(cig),
+                                                cap, _ptr(off), _ptr(sc))
+        self._check(rc, "halign_global_align_batch")
+        return [cig[off[i]:off[i + 1]].copy() for i in range(n)], sc
+
+
+class PairSet:
+    """Center + queries resident in HBM (halign_pairset_*)."""
+
+    def __init__(self, ctx, center, queries_concat, offsets):
+        self.ctx = ctx
+        center = center.encode() if isinstance(center, str) else bytes(center)
+        qc = queries_concat if isinstance(queries_concat, (bytes, bytearray)) else bytes(queries_concat)
+        off = np.ascontiguousarray(offsets, np.int64)
+        h = _P()
+        rc = ctx.lib.halign_pairset_create(ctx.h, center, len(center), len(off) - 1, qc, _ptr(off),
+                                           C.byref(h))
+        ctx._check(rc, "halign_pairset_create")
+        self.h = h
+        self.n = len(off) - 1
+
+    def run(self, first, count, plain=False):
+        ms = C.c_double(0)
+        cells = C.c_int64(0)
+        rc = self.ctx.lib.halign_pairset_run(self.h, first, count, int(plain), C.byref(ms),
+                                             C.byref(cells))
+        self.ctx._check(rc, "halign_pairset_run")
+        return ms.value, cells.value
+
+    def result(self, idx):
+        sc = C.c_int32(0)
+        nc = C.c_int32(0)
+        self.ctx._check(self.ctx.lib.halign_pairset_result(self.h, idx, C.byref(sc), None, 0,
+                                                            C.byref(nc)), "pairset_result")
+        cig = np.zeros(max(1, nc.value), np.uint32)
+        self.ctx._check(self.ctx.lib.halign_pairset_result(self.h, idx, C.byref(sc), _ptr(cig),
+                                                            len(cig), C.byref(nc)),
+                        "pairset_result")
+        return sc.value, cig[:nc.value].copy()
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.halign_pairset_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,119 @@
+/*
+ * halign_hip.h -- C ABI of libhalign_hip.so, the MI355X (gfx950) implementation
+ * of HAlign-4's per-query center-star alignment path.
+ *
+ * Plain C: pointers and sizes only. Every entry point returns an int status
+ * (HALIGN_OK == 0, negative on error); no exception crosses this boundary.
+ * A context owns one HIP device and stream and is safe to share between
+ * threads (calls on one context are serialised).
+ */
+#ifndef HALIGN_HIP_H
+#define HALIGN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HALIGN_OK 0
+#define HALIGN_E_ARG (-1)          /* invalid argument */
+#define HALIGN_E_UNSUPPORTED (-2)  /* parameters outside the path's fixed configuration */
+#define HALIGN_E_HIP (-3)          /* HIP runtime error (see halign_last_error) */
+#define HALIGN_E_NOMEM (-4)        /* device or host allocation failed */
+#define HALIGN_E_CAPACITY (-5)     /* caller-provided output buffer too small */
+#define HALIGN_E_KERNEL (-6)       /* a kernel reported an internal status */
+
+typedef struct halign_ctx halign_ctx;
+
+/* ksw_extz_t field layout (reference include/ksw2.h:33-42). cigar is
+ * malloc()ed by halign_ksw_extz2 and must be released with free(), exactly as
+ * the reference caller does (src/align/align.cpp:58-63). */
+typedef struct {
+    uint32_t max : 31, zdropped : 1;
+    int max_q, max_t;
+    int mqe, mqe_t;
+    int mte, mte_q;
+    int score;
+    int m_cigar, n_cigar;
+    int reach_end;
+    uint32_t *cigar;
+} halign_extz_t;
+
+/* One DP task for halign_dp_batch: sequences as 0..4 codes (A,C,G,T,other),
+ * offsets into the query/target code arrays passed alongside. */
+typedef struct {
+    int32_t qlen, tlen;
+    int32_t w;      /* band width; < 0 = unbanded (ksw2 semantics) */
+    int32_t flag;   /* KSW_EZ_* bits; GENERIC_SC (0x04) required, RIGHT (0x02) optional */
+    int64_t query_off, target_off;
+} halign_dp_task;
+
+typedef struct {
+    uint32_t max;
+    int32_t zdropped, max_q, max_t, mqe, mqe_t, mte, mte_q, score, n_cigar;
+    int32_t status, pad_;
+    int64_t cells;      /* sum over computed anti-diagonals of the exact band width */
+    int64_t cigar_off;  /* offset of this task's CIGAR in the caller's cigar buffer */
+} halign_dp_result;
+
+/* Context ------------------------------------------------------------------ */
+int halign_init(int device, halign_ctx **out);
+void halign_free(halign_ctx *ctx);
+const char *halign_last_error(const halign_ctx *ctx);
+/* library version string */
+const char *halign_version(void);
+
+/* Drop-in for ksw_extz2_sse (reference include/ksw2.h:64-65, called at
+ * src/align/align.cpp:49-55): the plain one-pair-per-launch GPU form with every
+ * ksw_extz_t field exact. Supports the path's configuration only: m=5, the
+ * dna5 matrix (include/align.h:109-116), q=6, e=2, zdrop<0, end_bonus=0, flag
+ * GENERIC_SC with or without RIGHT; anything else -> HALIGN_E_UNSUPPORTED.
+ * km is ignored (kalloc pool in the reference). */
+int halign_ksw_extz2(halign_ctx *ctx, void *km, int qlen, const uint8_t *query, int tlen,
+                     const uint8_t *target, int8_t m, const int8_t *mat, int8_t q, int8_t e,
+                     int w, int zdrop, int end_bonus, int flag, halign_extz_t *ez);
+
+/* Many ksw_extz2_sse calls in one submission (batched persistent form; or the
+ * plain form, one launch per task, when plain != 0). full_stats != 0 makes
+ * max/max_t/max_q/mqe/mqe_t exact for every task (otherwise they are only
+ * computed where the band overflows, which is where the reference uses them).
+ * CIGARs are written back to back into cigar_out (capacity cigar_cap uint32),
+ * each result holds its offset and length. kernel_ms (optional) receives the
+ * summed DP kernel time measured with HIP events. */
+int halign_dp_batch(halign_ctx *ctx, int32_t n, const halign_dp_task *tasks,
+                    const uint8_t *query_codes, const uint8_t *target_codes, int32_t full_stats,
+                    int32_t plain, halign_dp_result *results, uint32_t *cigar_out,
+                    int64_t cigar_cap, double *kernel_ms);
+
+/* align::globalAlignKSW2(ref, query, cfg) for n ASCII pairs (reference
+ * src/align/align.cpp:22-65, cfg.flag = flags[i]): empty-side handling,
+ * ScoreChar2Idx encoding, auto_band, DP, CIGAR. Pair i's CIGAR is written at
+ * cigar_out[offsets[i] .. offsets[i+1]); offsets has n+1 entries. */
+int halign_global_align_batch(halign_ctx *ctx, int32_t n, const char *const *refs,
+                              const int32_t *ref_lens, const char *const *queries,
+                              const int32_t *query_lens, const int32_t *flags,
+                              uint32_t *cigar_out, int64_t cigar_cap, int64_t *offsets,
+                              int32_t *scores);
+
+/* Device-resident workload: one center (target) and n queries kept in HBM,
+ * aligned pairwise as whole pairs (globalAlignKSW2 semantics). Used for the
+ * kernel-only DP throughput measurement and by the serving loop. */
+typedef struct halign_pairset halign_pairset;
+int halign_pairset_create(halign_ctx *ctx, const char *center, int32_t center_len, int32_t n,
+                          const char *queries, const int64_t *query_offsets,
+                          halign_pairset **out);
+/* Align queries [first, first+count). plain != 0: one pair per launch.
+ * kernel_ms: summed time of the DP kernels (HIP events on the library's
+ * stream); cells: algorithmic DP cells of the range. */
+int halign_pairset_run(halign_pairset *ps, int32_t first, int32_t count, int32_t plain,
+                       double *kernel_ms, int64_t *cells);
+/* Score and CIGAR of query idx from the last run that covered it. */
+int halign_pairset_result(halign_pairset *ps, int32_t idx, int32_t *score, uint32_t *cigar,
+                          int32_t cigar_cap, int32_t *n_cigar);
+void halign_pairset_free(halign_pairset *ps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,167 @@
+"""GPU parity of the DP kernel (ksw_extz2_sse semantics) against the oracle.
+
+Every ksw_extz_t field and the CIGAR must be bit-identical to the oracle
+(oracle/halign_oracle.c), which is itself pinned to the compiled reference
+and the golden fixtures by tests/test_oracle.py.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle_lib import FLAG_LEFT, FLAG_RIGHT, Oracle, cigar_str
+from seqgen import codes_to_ascii, mutate_codes, rand_codes
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import halign
+    c = halign.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return Oracle()
+
+
+def _tasks(cases):
+    import halign
+    qc, tc = [], []
+    tasks = np.zeros(len(cases), halign.DP_TASK_DTYPE)
+    qo = to = 0
+    for i, (q, t, w, flag) in enumerate(cases):
+        tasks[i] = (len(q), len(t), w, flag, qo, to)
+        qc.append(q)
+        tc.append(t)
+        qo += len(q)
+        to += len(t)
+    return tasks, np.concatenate(qc) if qc else np.zeros(0, np.uint8), \
+        np.concatenate(tc) if tc else np.zeros(0, np.uint8)
+
+
+def _compare(oracle, cases, res, cigs, full=True, label=""):
+    bad = 0
+    for i, (q, t, w, flag) in enumerate(cases):
+        ez, cig, cells = oracle.ksw(q, t, w, flag)
+        got = res[i]
+        names = FIELDS if full else ("zdropped", "score", "mte", "mte_q")
+        exp = dict(zip(("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q",
+                        "score", "n_cigar"), ez))
+        ok = all(int(got[f]) == int(exp[f]) for f in names) and np.array_equal(cigs[i], cig)
+        ok = ok and int(got["cells"]) == cells
+        if not ok:
+            bad += 1
+            if bad <= 3:
+                print(f"[{label}] case {i} qlen={len(q)} tlen={len(t)} w={w} flag={flag}",
+                      {f: (int(got[f]), int(exp[f])) for f in names}, "cells", int(got["cells"]),
+                      cells, "\n gpu:", cigar_str(cigs[i])[:120], "\n ora:", cigar_str(cig)[:120],
+                      file=sys.stderr, flush=True)
+    return bad
+
+
+def _random_cases(rng, n):
+    cases = []
+    for it in range(n):
+        kind = it % 6
+        if kind == 0:
+            t = rand_codes(rng, int(rng.integers(1, 80)))
+            q = rand_codes(rng, int(rng.integers(1, 80)))
+        elif kind == 1:
+            t = rand_codes(rng, int(rng.integers(1, 300)), p_n=0.05)
+            q = mutate_codes(rng, t, 0.1, 0.05)
+        elif kind == 2:
+            t = rand_codes(rng, int(rng.integers(50, 900)))
+            q = mutate_codes(rng, t, 0.05, 0.02)
+        elif kind == 3:
+            t = rand_codes(rng, int(rng.integers(1, 200)), alphabet=2)
+            q = rand_codes(rng, int(rng.integers(1, 200)), alphabet=2)
+        elif kind == 4:
+            t = rand_codes(rng, int(rng.integers(100, 1500)))
+            q = mutate_codes(rng, t, 0.2, 0.05)
+        else:
+            t = rand_codes(rng, int(rng.integers(1, 40)))
+            q = rand_codes(rng, int(rng.integers(1, 40)))
+        if len(q) == 0:
+            q = rand_codes(rng, 1)
+        flag = FLAG_RIGHT if rng.random() < 0.6 else FLAG_LEFT
+        r = rng.random()
+        if r < 0.3:
+            w = -1
+        elif r < 0.6:
+            w = int(rng.integers(0, 48))
+        else:
+            from oracle_lib import Oracle as _O  # auto band as globalAlignKSW2 computes it
+            w = _O().auto_band(len(t), len(q))
+        cases.append((q, t, w, flag))
+    return cases
+
+
+def test_dp_random_batched_full_stats(ctx, oracle):
+    rng = np.random.default_rng(1234)
+    cases = _random_cases(rng, 1200)
+    tasks, qc, tc = _tasks(cases)
+    print("dp batched full-stats: 1200 random cases", file=sys.stderr, flush=True)
+    res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=True, plain=False)
+    assert _compare(oracle, cases, res, cigs, True, "batched-stats") == 0
+
+
+def test_dp_random_batched_cigar_mode(ctx, oracle):
+    rng = np.random.default_rng(99)
+    cases = _random_cases(rng, 1200)
+    tasks, qc, tc = _tasks(cases)
+    print("dp batched cigar-mode: 1200 random cases", file=sys.stderr, flush=True)
+    res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=False, plain=False)
+    # in CIGAR mode max/mqe are only exact where the band overflows
+    assert _compare(oracle, cases, res, cigs, False, "batched-cigar") == 0
+
+
+def test_dp_plain_form(ctx, oracle):
+    rng = np.random.default_rng(7)
+    cases = _random_cases(rng, 60)
+    tasks, qc, tc = _tasks(cases)
+    print("dp plain: 60 random cases", file=sys.stderr, flush=True)
+    res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=True, plain=True)
+    assert _compare(oracle, cases, res, cigs, True, "plain") == 0
+
+
+def test_dp_long_banded_pairs(ctx, oracle):
+    """Whole-pair sized DPs with auto band: several ring shapes."""
+    rng = np.random.default_rng(5)
+    cases = []
+    for n in (3000, 8000, 16569):
+        t = rand_codes(rng, n)
+        q = mutate_codes(rng, t, 0.01, 0.002)
+        cases.append((q, t, oracle.auto_band(len(t), len(q)), FLAG_RIGHT))
+    t = rand_codes(rng, 5000)
+    q = mutate_codes(rng, t[:2600], 0.01, 0.002)
+    cases.append((q, t, oracle.auto_band(len(t), len(q)), FLAG_RIGHT))   # band overflow
+    tasks, qc, tc = _tasks(cases)
+    print("dp long pairs", [(len(c[0]), len(c[1]), c[2]) for c in cases], file=sys.stderr,
+          flush=True)
+    res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=True, plain=False)
+    assert _compare(oracle, cases, res, cigs, True, "long") == 0
+
+
+def test_global_align_batch(ctx, oracle):
+    rng = np.random.default_rng(11)
+    refs, qrys, flags = [], [], []
+    for i in range(300):
+        t = rand_codes(rng, int(rng.integers(0, 400)), p_n=0.01)
+        q = mutate_codes(rng, t, 0.05, 0.02) if i % 4 else rand_codes(rng, int(rng.integers(0, 50)))
+        refs.append(codes_to_ascii(t))
+        qrys.append(codes_to_ascii(q).lower() if i % 7 == 0 else codes_to_ascii(q))
+        flags.append(FLAG_RIGHT if i % 3 else FLAG_LEFT)
+    print("global_align_batch: 300 pairs", file=sys.stderr, flush=True)
+    cigs, scores = ctx.global_align_batch(refs, qrys, flags)
+    bad = 0
+    for i in range(300):
+        exp, _ = oracle.global_align_ksw2(refs[i], qrys[i], flags[i])
+        if not np.array_equal(exp, cigs[i]):
+            bad += 1
+    assert bad == 0
