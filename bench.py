@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=0, help="pairs for the CPU baseline (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--policy", type=int, default=0,
+                    help="kernel family: 0 automatic (single-wave where the band fits), 1 multi-wave only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -94,6 +96,7 @@ def main():
     qbytes, off = synth.queries(2, 0, n_res)
     log(f"  generated {len(qbytes) / 1e9:.2f} GB in {time.time() - t0:.1f}s")
     ctx = halign.Context(0)
+    ctx.set_kernel_policy(args.policy)
     ps = halign.PairSet(ctx, center, qbytes, off)
 
     # plain form: one pair per launch
@@ -128,7 +131,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        sample = args.cpu_sample or threads * 2
+        sample = args.cpu_sample or threads * 24
         kind, dt = cpu_baseline(center, qbytes, off, sample, threads)
         w = [halign_auto_band(len(center), int(off[i + 1] - off[i])) for i in range(sample)]
         cells = sum(band_cells(int(off[i + 1] - off[i]), len(center), w[i]) for i in range(sample))

@@ -8,6 +8,9 @@ namespace hal {
 // Padding (in 16-bit elements) around every spread sequence so that the
 // 16-cell windows the kernels read at band margins stay inside the buffer.
 constexpr int kSeqPad = 32;
+// Padding (bytes) around the 0..4 byte-code copies: a 16-byte window is read as
+// five aligned dwords, i.e. up to 19 bytes past its start and 3 before.
+constexpr int kBytePad = 64;
 
 // Job flags
 constexpr int kFlagRight = 0x02;   // KSW_EZ_RIGHT (include/ksw2.h:9)
@@ -30,6 +33,8 @@ struct DpJob {
     const uint16_t *tgt_odd;    // tgt_odd[k]  = target element k+1
     const uint16_t *qr_even;    // reversed query: qr_even[k] = query[qlen-1-k]
     const uint16_t *qr_odd;     // qr_odd[k] = qr_even[k+1]
+    const uint8_t *tgt_b;       // target as 0..4 byte codes, tgt_b[k] for k in [-kBytePad, tlen+kBytePad)
+    const uint8_t *qr_b;        // reversed query as byte codes, same padding
     int32_t tlen, qlen, w, flags;
     uint32_t *cigar;            // output, ops in backtrack (reversed) order
     int32_t cig_cap;

@@ -1,0 +1,435 @@
+// dp_wave.hip -- single-wave form of the extz2 DP (CIGAR + score mode) for
+// bands up to 64*S-1 blocks. Same semantics as dp_kernel.hip (which restates
+// src/align/ksw2_extz2_sse.c:23-304, SSE2 branch), organised so that nothing
+// on the per-anti-diagonal critical path waits on memory, LDS or a barrier:
+//
+//  * one wave owns the job; block b lives in slot (b mod 64S)/64 of lane
+//    b mod 64, neighbour values move with DPP row/wave rotates;
+//  * the query window of a block is a 16-byte register window that slides one
+//    base per anti-diagonal (v_alignbyte), fed from the left lane's window; only
+//    the first block of a band, whose left neighbour is idle, takes its new base
+//    from a scalar load issued a diagonal ahead;
+//  * target and query are 0..4 byte codes, four cells per VGPR;
+//  * no per-block score array is kept: inside the band the score is recomputed
+//    every diagonal, right of the band it is the never-written 0 of the
+//    reference's calloc'd array, and left of the band (first block only) the
+//    frozen last in-band value is kept in one 8-VGPR set per lane;
+//  * the H-row edge (score/mte) is tracked in scalars fed by v_readlane.
+#include "dp_device.h"
+
+namespace hal {
+
+// lane i receives lane i-1 (lane 0 receives lane 63): DPP wave_ror:1
+__device__ __forceinline__ uint32_t wave_ror1(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x13C, 0xF, 0xF, false);
+}
+
+// 16 bytes starting at an arbitrary byte address (aligned dword loads + v_alignbyte)
+__device__ __forceinline__ void load16(const uint8_t *p, uint32_t (&o)[4])
+{
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t *d = (const uint32_t *)(a & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(a & 3);
+    uint32_t wv[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) wv[i] = d[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = __builtin_amdgcn_alignbyte(wv[i + 1], wv[i], sh);
+}
+
+__device__ __forceinline__ uint32_t load_byte_uniform(const uint8_t *p)
+{
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t wd = *(const uint32_t *)(a & ~(uintptr_t)3);
+    return (wd >> (8 * (uint32_t)(a & 3))) & 0xffu;
+}
+
+// Mask of the cells c < k of a block for each register j (cells j and j+8 in
+// the low and high halves), k = 0..16: wave-uniform band-edge masks.
+struct HalfMaskTable { uint32_t m[17][8]; };
+static constexpr HalfMaskTable make_half_masks()
+{
+    HalfMaskTable t{};
+    for (int k = 0; k <= 16; ++k)
+        for (int j = 0; j < 8; ++j)
+            t.m[k][j] = (j < k ? 0x0000ffffu : 0u) | (j + 8 < k ? 0xffff0000u : 0u);
+    return t;
+}
+__constant__ HalfMaskTable kHalfMask = make_half_masks();
+
+// cell c of a block held as registers j = c & 7, half c >> 3 (x<<8 form)
+__device__ __forceinline__ int cell_byte(const uint32_t (&a)[8], int c)
+{
+    return (int)((pick8(a, c & 7) >> ((c >> 3) * 16 + 8)) & 0xffu);
+}
+__device__ __forceinline__ void set_cell(uint32_t (&a)[8], int c, uint32_t v16)
+{
+    const uint32_t hm = (c >> 3) ? 0xffff0000u : 0x0000ffffu;
+    const uint32_t vv = v16 | (v16 << 16);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t m = hm & sel_mask(c & 7, i);
+        a[i] = (a[i] & ~m) | (vv & m);
+    }
+}
+
+template <int S>
+__device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restrict__ out,
+                            uint32_t (*chunk)[10])
+{
+    constexpr int R = 64 * S;
+    const int lane = threadIdx.x & 63;
+    const int qlen = J.qlen, tlen = J.tlen;
+    int w = J.w;
+    if (w < 0) w = tlen > qlen ? tlen : qlen;
+    const bool right = (J.flags & kFlagRight) != 0;
+    const bool hasn = (J.flags & kFlagHasN) != 0;
+    const int ndiag = qlen + tlen - 1;
+    int ncol = qlen < tlen ? qlen : tlen;
+    ncol = ((ncol < w + 1 ? ncol : w + 1) + 15) / 16 + 1;
+    const size_t rowb = (size_t)ncol * 8;
+    // one spare ring block: a block entering two diagonals ahead must not reuse the
+    // slot of the left neighbour of the next diagonal's first block
+    if (ncol > R - 2) {
+        if (lane == 0) { out->status = kStTooWide; out->n_cigar = 0; }
+        return;
+    }
+    const uint8_t *__restrict__ tgt = J.tgt_b;
+    const uint8_t *__restrict__ qr = J.qr_b;
+
+    // Block state: register j holds cells j (low half) and j+8 (high half),
+    // int8 values in the high byte of each half. T4/Q4: byte codes, cell c in
+    // byte c&3 of dword c>>2.
+    uint32_t U[S][8], V[S][8], X[S][8], Y[S][8], T4[S][4], Q4[S][4];
+    uint32_t FZ[8];   // frozen scores of the first block (left of the band)
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) U[k][j] = V[k][j] = X[k][j] = Y[k][j] = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) T4[k][j] = Q4[k][j] = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) FZ[j] = 0x10001000u;
+
+    // block 0 for anti-diagonal 0 lives in lane 0, slot 0
+    if (lane == 0) {
+        load16(tgt, T4[0]);
+        load16(qr + (qlen - 1), Q4[0]);
+    }
+
+    int last_st = -1, last_en = -1, last_bs = 0;
+    int32_t He = kNegInf, Hl = kNegInf;
+    int last_en0 = -1;
+    int32_t score = kNegInf, mte = kNegInf, mte_q = -1;
+    int64_t cells = 0;
+    bool zd = false;
+
+    // Bounds of r and r+1 are known at the top of diagonal r. Loads for a
+    // diagonal (new block windows, the first block's new query base) are
+    // issued two diagonals ahead so their latency is hidden by a full diagonal.
+    Bounds bd, bd1;
+    diag_bounds(0, qlen, tlen, w, bd);
+    bool ok1 = ndiag > 1 && diag_bounds(1, qlen, tlen, w, bd1);
+    uint32_t fe = 0, fe_next = 0;   // first block's new query base for r and r+1
+    if (ok1) {
+        const int nbe = bd1.en >> 4;
+        if (nbe > (bd.en >> 4)) {
+            const int nm = nbe % R;
+            if (lane == nm % 64) {
+                const int kn = nm / 64;
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    if (k == kn) {
+                        load16(tgt + 16 * nbe, T4[k]);
+                        load16(qr + (qlen - 2 + 16 * nbe), Q4[k]);
+                    }
+                }
+            }
+        }
+        if ((bd1.st >> 4) == (bd.st >> 4)) fe_next = load_byte_uniform(qr + (qlen - 2 + (bd1.st & ~15)));
+    }
+
+    for (int r = 0; r < ndiag; ++r) {
+        const int st0 = bd.st0, en0 = bd.en0, st = bd.st, en = bd.en;
+        const int bs = st >> 4, be = en >> 4, bsm = bs % R;
+        cells += en0 - st0 + 1;
+        // band-edge masks (uniform): cells < st0 of the first block, cells <= en0 of the last
+        uint32_t leftm[8], lastm[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            leftm[j] = kHalfMask.m[st0 & 15][j];
+            lastm[j] = kHalfMask.m[(en0 & 15) + 1][j];
+        }
+        int blk[S];
+        bool act[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            int d = 64 * k + lane - bsm;
+            if (d < 0) d += R;
+            blk[k] = bs + d;
+            act[k] = blk[k] <= be;
+        }
+
+        // neighbour values of the previous anti-diagonal: x, v of cell 15 of the left
+        // block (high half of register 7) and byte 15 of its query window
+        uint32_t nx[S], nv[S], nq[S];
+        {
+            uint32_t rx[S], rv[S], rq[S];
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                rx[k] = wave_ror1(X[k][7]);
+                rv[k] = wave_ror1(V[k][7]);
+                rq[k] = wave_ror1(Q4[k][3]);
+            }
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const int kp = (k + S - 1) % S;
+                nx[k] = lane ? rx[k] : rx[kp];
+                nv[k] = lane ? rv[k] : rv[kp];
+                nq[k] = lane ? rq[k] : rq[kp];
+            }
+        }
+        // slide the query windows one base for blocks that were already active
+        if (r > 0) {
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const int b = blk[k];
+                if (act[k] && b >= last_bs && b <= (last_en >> 4)) {
+                    const uint32_t in = (b == last_bs) ? (fe << 24) : nq[k];
+                    Q4[k][3] = __builtin_amdgcn_alignbyte(Q4[k][3], Q4[k][2], 3);
+                    Q4[k][2] = __builtin_amdgcn_alignbyte(Q4[k][2], Q4[k][1], 3);
+                    Q4[k][1] = __builtin_amdgcn_alignbyte(Q4[k][1], Q4[k][0], 3);
+                    Q4[k][0] = __builtin_amdgcn_alignbyte(Q4[k][0], in, 3);
+                }
+            }
+        }
+        // left boundary of the rounded region (:118-122)
+        if (lane == bsm % 64) {
+            const int ks = bsm / 64;
+            const bool keep = st > 0 && st - 1 >= last_st && st - 1 <= last_en;
+            const uint32_t v1 = (st == 0 && r) ? (uint32_t)kQ << 24 : 0u;
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const uint32_t m = keep ? 0u : sel_mask(k, ks);
+                nx[k] = nx[k] & ~m;
+                nv[k] = (nv[k] & ~m) | (v1 & m);
+            }
+        }
+        // top boundary y[r]=0, u[r]=q (:123)
+        if (en >= r) {
+            const int rb = r >> 4, rc = r & 15;
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                if (act[k] && blk[k] == rb) {
+                    set_cell(U[k], rc, r ? (uint32_t)kQ << 8 : 0u);
+                    set_cell(Y[k], rc, 0u);
+                }
+            }
+        }
+
+        const int ke = ((en0 >> 4) % R) / 64, le = ((en0 >> 4) % R) % 64;
+        const int tv = en0 > 0 ? en0 - 1 : 0;
+        const int kv = ((tv >> 4) % R) / 64, lv = ((tv >> 4) % R) % 64;
+        uint32_t ue_v = 0, vl_v = 0;
+
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            if (!act[k]) continue;
+            const int b = blk[k];
+            uint32_t z[8];
+            {
+                // fresh s+2(q+e) per byte: table[t^q] = [21,12,12,12,16,16,16,16]
+                uint32_t zb[4];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    zb[d] = __builtin_amdgcn_perm(0x10101010u, 0x0C0C0C15u, T4[k][d] ^ Q4[k][d]);
+                    if (hasn) zb[d] -= ((T4[k][d] & Q4[k][d] & 0x04040404u) >> 2) * 5u;   // N:N scores 0
+                }
+                // spread: register j <- cells j and j+8, high byte of each half
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    z[j] = __builtin_amdgcn_perm(zb[(j >> 2) + 2], zb[j >> 2],
+                                                 0x040C000Cu + (uint32_t)(j & 3) * 0x01000100u);
+                // right of the band: the never-written score 0 (s+16 = 16)
+                if (b == be) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) z[j] = bfi(lastm[j], z[j], 0x10001000u);
+                }
+                // left of the band (first block): the frozen last in-band score
+                if (b == bs) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        z[j] = bfi(leftm[j], FZ[j], z[j]);
+                        FZ[j] = z[j];
+                    }
+                }
+            }
+            uint32_t C[4];
+            auto cells_rl = [&](auto right_tag) {
+                constexpr bool RT = decltype(right_tag)::value;
+                const uint32_t x7o = X[k][7], v7o = V[k][7];
+#pragma unroll
+                for (int m = 3; m >= 0; --m) {
+                    uint32_t fa[2], fb[2], fx[2], fy[2];
+#pragma unroll
+                    for (int h = 1; h >= 0; --h) {
+                        const int j = 2 * m + h;   // descending: X[j-1], V[j-1] are still old
+                        // previous-diagonal x, v of cells (j-1, j+7)
+                        const uint32_t xs = j ? X[k][j - 1] : cat16(x7o, nx[k]);
+                        const uint32_t vs = j ? V[k][j - 1] : cat16(v7o, nv[k]);
+                        const uint32_t a = pk_add(xs, vs);
+                        const uint32_t ut = U[k][j];
+                        const uint32_t bb = pk_add(Y[k][j], ut);
+                        const uint32_t z0 = z[j];
+                        fa[h] = RT ? pk_subsat(a, z0) : pk_subsat(z0, a);
+                        uint32_t zz = pk_maxu(z0, a);
+                        fb[h] = RT ? pk_subsat(bb, zz) : pk_subsat(zz, bb);
+                        zz = pk_maxu(zz, bb);
+                        zz = pk_minu(zz, 0x15001500u);
+                        U[k][j] = pk_sub(zz, vs);
+                        V[k][j] = pk_sub(zz, ut);
+                        zz = pk_sub(zz, 0x06000600u);
+                        const uint32_t a2 = pk_sub(a, zz), b2 = pk_sub(bb, zz);
+                        X[k][j] = pk_maxi(a2, 0u);
+                        Y[k][j] = pk_maxi(b2, 0u);
+                        fx[h] = RT ? a2 : pk_subsat(0u, a2);
+                        fy[h] = RT ? b2 : pk_subsat(0u, b2);
+                    }
+                    // sign bytes of cells (2m, 2m+8, 2m+1, 2m+9), then 4 flags per byte
+                    const uint32_t ga = __builtin_amdgcn_perm(fa[1], fa[0], 0x07050301u);
+                    const uint32_t gb = __builtin_amdgcn_perm(fb[1], fb[0], 0x07050301u);
+                    const uint32_t gx = __builtin_amdgcn_perm(fx[1], fx[0], 0x07050301u);
+                    const uint32_t gy = __builtin_amdgcn_perm(fy[1], fy[0], 0x07050301u);
+                    uint32_t c = bfi(0x80808080u, ga, gb >> 1);
+                    c = bfi(0xC0C0C0C0u, c, gx >> 2);
+                    C[m] = bfi(0xE0E0E0E0u, c, gy >> 3);
+                }
+            };
+            if (right) cells_rl(std::integral_constant<bool, true>{});
+            else cells_rl(std::integral_constant<bool, false>{});
+            uint2 o;
+            o.x = bfi(0xF0F0F0F0u, C[1], C[0] >> 4);
+            o.y = bfi(0xF0F0F0F0u, C[3], C[2] >> 4);
+            *(uint2 *)(tb + (size_t)r * rowb + (size_t)(b - bs) * 8) = o;
+            if (k == ke) ue_v = (uint32_t)cell_byte(U[k], en0 & 15);
+            if (k == kv) vl_v = (uint32_t)cell_byte(V[k], tv & 15);
+        }
+
+        // H-row edge (:227-269): H[en0] = H[en0-1] + u - qe, H[en0-1] += v - qe
+        {
+            const int ue = (int)__builtin_amdgcn_readlane(ue_v, le);
+            const int vl = (int)__builtin_amdgcn_readlane(vl_v, lv);
+            if (r == 0) {
+                He = vl - 2 * kQE;
+            } else if (en0 == 0) {
+                He = He + vl - kQE;
+            } else {
+                const int32_t Hp = (en0 == last_en0) ? Hl : He;
+                Hl = (en0 - 1 >= st0) ? Hp + vl - kQE : Hp;
+                He = Hp + ue - kQE;
+            }
+            last_en0 = en0;
+            if (en0 == tlen - 1 && He > mte) { mte = He; mte_q = r - en; }
+            if (r == ndiag - 1 && en0 == tlen - 1) score = He;
+        }
+
+        last_st = st; last_en = en; last_bs = bs;
+        if (r + 1 < ndiag) {
+            if (!ok1) { zd = true; break; }
+            bd = bd1;
+            fe = fe_next;
+            fe_next = 0;
+            ok1 = r + 2 < ndiag && diag_bounds(r + 2, qlen, tlen, w, bd1);
+            if (ok1) {
+                // prefetch for r+2 (bd = bounds of r+1, bd1 = bounds of r+2)
+                const int nbs = bd1.st >> 4, nbe = bd1.en >> 4;
+                if (nbe > (bd.en >> 4)) {   // a block entering the band at r+2
+                    const int nm = nbe % R;
+                    if (lane == nm % 64) {
+                        const int kn = nm / 64;
+#pragma unroll
+                        for (int k = 0; k < S; ++k) {
+                            if (k == kn) {
+#pragma unroll
+                                for (int j = 0; j < 8; ++j) U[k][j] = V[k][j] = X[k][j] = Y[k][j] = 0;
+                                load16(tgt + 16 * nbe, T4[k]);
+                                load16(qr + (qlen - 1 - (r + 2) + 16 * nbe), Q4[k]);
+                            }
+                        }
+                    }
+                }
+                // the first block keeps its place at r+2: its window's new base from memory
+                if (nbs == (bd.st >> 4)) fe_next = load_byte_uniform(qr + (qlen - 1 - (r + 2) + 16 * nbs));
+            }
+        }
+    }
+
+    int nout = 0, status = kStOk;
+    if (zd) status = kStNeedStats;
+    else {
+        nout = backtrack_to_cigar(J, tb, rowb, w, right, tlen - 1, qlen - 1, chunk, 1);
+        if (nout > J.cig_cap) status = kStCigarOverflow;
+    }
+    if (lane == 0) {
+        out->score = score; out->n_cigar = nout; out->zdropped = zd ? 1 : 0; out->status = status;
+        out->max = 0; out->max_t = -1; out->max_q = -1;
+        out->mqe = kNegInf; out->mqe_t = -1;
+        out->mte = mte; out->mte_q = mte_q; out->pad_ = 0; out->cells = cells;
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void dp_wave_persistent(const DpJob *__restrict__ jobs, int njobs,
+                                                         int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
+                                                         size_t tb_slot_bytes, DpOut *__restrict__ outs)
+{
+    __shared__ uint32_t chunk[64][10];
+    __shared__ int job_sh;
+    uint8_t *tb = tb_pool + (size_t)blockIdx.x * tb_slot_bytes;
+    for (;;) {
+        // Job index through LDS between two barriers: every lane reads the same
+        // value and the loop stays uniform. (A lane-0 atomic broadcast with
+        // readfirstlane lets the compiler structurize this as a divergent loop in
+        // which the other lanes re-read index 0 forever.)
+        if (threadIdx.x == 0) job_sh = atomicAdd(counter, 1);
+        __syncthreads();
+        const int jid = job_sh;
+        __syncthreads();
+        if (jid >= njobs) break;
+        const DpJob J = jobs[jid];
+        dp_wave_job<S>(J, tb, &outs[jid], chunk);
+    }
+}
+
+static const void *wave_kernel(int S)
+{
+    switch (S) {
+    case 1: return (const void *)&dp_wave_persistent<1>;
+    case 2: return (const void *)&dp_wave_persistent<2>;
+    case 3: return (const void *)&dp_wave_persistent<3>;
+    case 4: return (const void *)&dp_wave_persistent<4>;
+    case 5: return (const void *)&dp_wave_persistent<5>;
+    }
+    return nullptr;
+}
+
+int dp_wave_max_slots() { return 5; }
+
+int dp_wave_max_groups(int S, int num_cu)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(S), 64, 0) != hipSuccess) return 0;
+    return per_cu * num_cu;
+}
+
+hipError_t dp_wave_launch(int S, int groups, const DpJob *jobs, int njobs, int *counter, uint8_t *tb_pool,
+                          size_t tb_slot_bytes, DpOut *outs, hipStream_t stream)
+{
+    void *args[] = {(void *)&jobs, (void *)&njobs, (void *)&counter, (void *)&tb_pool,
+                    (void *)&tb_slot_bytes, (void *)&outs};
+    return hipLaunchKernel(wave_kernel(S), dim3(groups), dim3(64), args, 0, stream);
+}
+
+}  // namespace hal

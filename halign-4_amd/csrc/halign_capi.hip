@@ -21,6 +21,10 @@ int dp_shape_threads(int idx);
 int dp_shape_max_groups(int idx, bool stats, int num_cu);
 hipError_t dp_launch(int idx, bool stats, int groups, const DpJob *jobs, int njobs, int *counter,
                      uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
+int dp_wave_max_slots();
+int dp_wave_max_groups(int S, int num_cu);
+hipError_t dp_wave_launch(int S, int groups, const DpJob *jobs, int njobs, int *counter, uint8_t *tb_pool,
+                          size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
 }  // namespace hal
 
 using namespace hal;
@@ -100,8 +104,9 @@ struct halign_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::mutex mu;
     std::string err;
-    DevBuf seq, jobs, outs, counters, tb, cig;
+    DevBuf seq, seqb, jobs, outs, counters, tb, cig;
     size_t tb_budget = 0;
+    int policy = 0;   // 0: auto kernel choice, 1: multi-wave kernels only
 };
 
 namespace {
@@ -125,6 +130,7 @@ int fail(halign_ctx *ctx, int code, const char *what, hipError_t e = hipSuccess)
 struct Spec {
     int64_t tgt_elem, qry_elem;   // element offsets of the even copies (pad included before)
     int64_t tgt_odd_elem, qry_odd_elem;
+    int64_t tgt_b, qry_b;         // byte offsets of the 0..4 code copies (pad included before)
     int32_t tlen, qlen, w, flags;
     int64_t cig_off;              // uint32 offset into the device cigar buffer
     int32_t cig_cap;
@@ -154,8 +160,33 @@ void spread_query_rev(const uint8_t *codes, int len, uint16_t *even, uint16_t *o
     for (int k = 0; k < n; ++k) odd[k] = (k + 1 < n) ? even[k + 1] : (uint16_t)0;
 }
 
-int pick_shape(int ncol)
+void bytes_target(const uint8_t *codes, int len, uint8_t *out)
 {
+    for (int k = 0; k < len + 2 * kBytePad; ++k) {
+        const int t = k - kBytePad;
+        out[k] = (t >= 0 && t < len) ? codes[t] : 0;
+    }
+}
+
+void bytes_query_rev(const uint8_t *codes, int len, uint8_t *out)
+{
+    for (int k = 0; k < len + 2 * kBytePad; ++k) {
+        const int t = k - kBytePad;
+        out[k] = (t >= 0 && t < len) ? codes[len - 1 - t] : 0;
+    }
+}
+
+// Shape ids: [0, dp_num_shapes()) multi-wave kernels (any mode), then
+// kWaveBase + S - 1 for the single-wave CIGAR-mode kernel with S slots.
+constexpr int kWaveBase = 16;
+constexpr int kNumShapeIds = kWaveBase + 8;
+
+int pick_shape(int ncol, bool stats, int policy)
+{
+    if (!stats && policy == 0) {
+        for (int S = 1; S <= dp_wave_max_slots(); ++S)
+            if (ncol <= 64 * S - 2) return kWaveBase + S - 1;
+    }
     const int ns = dp_num_shapes();
     for (int i = 0; i < ns; ++i)
         if (ncol <= dp_shape_ring(i) - 1) return i;
@@ -172,10 +203,11 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     if (kernel_ms) *kernel_ms = 0.0;
     if (n == 0) return HALIGN_OK;
     const uint16_t *seq = ctx->seq.as<uint16_t>();
+    const uint8_t *seqb = ctx->seqb.as<uint8_t>();
     uint32_t *cig = ctx->cig.as<uint32_t>();
 
     // bucket by (shape, stats)
-    const int ns = dp_num_shapes();
+    const int ns = kNumShapeIds;
     std::vector<std::vector<int>> buckets(2 * ns);
     for (size_t i = 0; i < n; ++i) buckets[specs[i].shape * 2 + (specs[i].stats ? 1 : 0)].push_back((int)i);
 
@@ -194,6 +226,8 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         j.tgt_odd = seq + s.tgt_odd_elem;
         j.qr_even = seq + s.qry_elem;
         j.qr_odd = seq + s.qry_odd_elem;
+        j.tgt_b = seqb + s.tgt_b;
+        j.qr_b = seqb + s.qry_b;
         j.tlen = s.tlen; j.qlen = s.qlen; j.w = s.w; j.flags = s.flags;
         j.cigar = cig + s.cig_off;
         j.cig_cap = s.cig_cap;
@@ -221,22 +255,28 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         size_t slot = 0;
         for (int i : b) slot = std::max(slot, specs[i].tb_bytes);
         slot = (slot + 255) & ~(size_t)255;
+        const bool wave = shape >= kWaveBase;
+        const int S = shape - kWaveBase + 1;
         if (plain) {
             HCHECK(ctx->tb.ensure(slot), "alloc traceback");
             for (size_t k = 0; k < b.size(); ++k) {
-                HCHECK(dp_launch(shape, stats, 1, djobs + pos + k, 1, ctr + pos + k, ctx->tb.as<uint8_t>(), slot,
-                                 douts + pos + k, ctx->stream),
-                       "launch dp (plain)");
+                hipError_t e = wave ? dp_wave_launch(S, 1, djobs + pos + k, 1, ctr + pos + k, ctx->tb.as<uint8_t>(),
+                                                     slot, douts + pos + k, ctx->stream)
+                                    : dp_launch(shape, stats, 1, djobs + pos + k, 1, ctr + pos + k,
+                                                ctx->tb.as<uint8_t>(), slot, douts + pos + k, ctx->stream);
+                HCHECK(e, "launch dp (plain)");
             }
         } else {
-            int groups = dp_shape_max_groups(shape, stats, ctx->num_cu);
+            int groups = wave ? dp_wave_max_groups(S, ctx->num_cu) : dp_shape_max_groups(shape, stats, ctx->num_cu);
             if (groups <= 0) return fail(ctx, HALIGN_E_HIP, "occupancy query failed");
             groups = std::min<size_t>(groups, b.size());
             groups = (int)std::min<size_t>(groups, std::max<size_t>(1, ctx->tb_budget / slot));
             HCHECK(ctx->tb.ensure(slot * groups), "alloc traceback pool");
-            HCHECK(dp_launch(shape, stats, groups, djobs + pos, (int)b.size(), ctr + bi, ctx->tb.as<uint8_t>(), slot,
-                             douts + pos, ctx->stream),
-                   "launch dp");
+            hipError_t e = wave ? dp_wave_launch(S, groups, djobs + pos, (int)b.size(), ctr + bi,
+                                                 ctx->tb.as<uint8_t>(), slot, douts + pos, ctx->stream)
+                                : dp_launch(shape, stats, groups, djobs + pos, (int)b.size(), ctr + bi,
+                                            ctx->tb.as<uint8_t>(), slot, douts + pos, ctx->stream);
+            HCHECK(e, "launch dp");
         }
         pos += b.size();
     }
@@ -260,12 +300,12 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     return HALIGN_OK;
 }
 
-bool fill_spec_shape(Spec &s, bool full_stats)
+bool fill_spec_shape(Spec &s, bool full_stats, int policy)
 {
     const int w = eff_w(s.w, s.qlen, s.tlen);
     const int nc = n_col(s.qlen, s.tlen, w);
-    s.shape = pick_shape(nc);
     s.stats = full_stats || predict_zdrop(s.qlen, s.tlen, w);
+    s.shape = pick_shape(nc, s.stats, policy);
     s.tb_bytes = (size_t)(s.qlen + s.tlen - 1) * (size_t)nc * 8;
     return s.shape >= 0;
 }
@@ -305,7 +345,7 @@ void halign_free(halign_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    ctx->seq.release(); ctx->jobs.release(); ctx->outs.release();
+    ctx->seq.release(); ctx->seqb.release(); ctx->jobs.release(); ctx->outs.release();
     ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
@@ -314,6 +354,14 @@ void halign_free(halign_ctx *ctx)
 }
 
 const char *halign_last_error(const halign_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int halign_set_kernel_policy(halign_ctx *ctx, int policy)
+{
+    if (!ctx || policy < 0 || policy > 1) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->policy = policy;
+    return HALIGN_OK;
+}
 
 int halign_dp_batch(halign_ctx *ctx, int32_t n, const halign_dp_task *tasks, const uint8_t *query_codes,
                     const uint8_t *target_codes, int32_t full_stats, int32_t plain,
@@ -324,7 +372,7 @@ int halign_dp_batch(halign_ctx *ctx, int32_t n, const halign_dp_task *tasks, con
     HCHECK(hipSetDevice(ctx->device), "set device");
     // spread sequences: per task, target even/odd then query even/odd
     std::vector<Spec> specs(n);
-    int64_t elems = 0, cig_total = 0;
+    int64_t elems = 0, cig_total = 0, nbytes = 0;
     for (int i = 0; i < n; ++i) {
         const halign_dp_task &t = tasks[i];
         if (t.qlen < 0 || t.tlen < 0) return fail(ctx, HALIGN_E_ARG, "negative length");
@@ -335,6 +383,10 @@ int halign_dp_batch(halign_ctx *ctx, int32_t n, const halign_dp_task *tasks, con
         s.qry_elem = elems + 2 * tn + kSeqPad; s.qry_odd_elem = elems + 2 * tn + qn + kSeqPad;
         elems += 2 * tn + 2 * qn;
         elems = (elems + 1) & ~(int64_t)1;   // keep every copy 4-byte aligned
+        s.tgt_b = nbytes + kBytePad;
+        nbytes += t.tlen + 2 * kBytePad;
+        s.qry_b = nbytes + kBytePad;
+        nbytes += t.qlen + 2 * kBytePad;
         s.tlen = t.tlen; s.qlen = t.qlen; s.w = t.w;
         s.flags = (t.flag & 0x02) ? kFlagRight : 0;
         for (int k = 0; k < t.qlen; ++k) if (query_codes[t.query_off + k] > 3) { s.flags |= kFlagHasN; break; }
@@ -343,10 +395,11 @@ int halign_dp_batch(halign_ctx *ctx, int32_t n, const halign_dp_task *tasks, con
         s.cig_off = cig_total;
         s.cig_cap = t.qlen + t.tlen + 4;
         cig_total += s.cig_cap;
-        if (t.qlen > 0 && t.tlen > 0 && !fill_spec_shape(s, full_stats != 0))
+        if (t.qlen > 0 && t.tlen > 0 && !fill_spec_shape(s, full_stats != 0, ctx->policy))
             return fail(ctx, HALIGN_E_UNSUPPORTED, "band wider than the largest ring shape");
     }
     std::vector<uint16_t> host((size_t)elems + 2, 0);
+    std::vector<uint8_t> hostb((size_t)nbytes + 16, 0);
     for (int i = 0; i < n; ++i) {
         const halign_dp_task &t = tasks[i];
         const Spec &s = specs[i];
@@ -354,8 +407,13 @@ int halign_dp_batch(halign_ctx *ctx, int32_t n, const halign_dp_task *tasks, con
                       host.data() + s.tgt_odd_elem - kSeqPad);
         spread_query_rev(query_codes + t.query_off, t.qlen, host.data() + s.qry_elem - kSeqPad,
                          host.data() + s.qry_odd_elem - kSeqPad);
+        bytes_target(target_codes + t.target_off, t.tlen, hostb.data() + s.tgt_b - kBytePad);
+        bytes_query_rev(query_codes + t.query_off, t.qlen, hostb.data() + s.qry_b - kBytePad);
     }
     HCHECK(ctx->seq.ensure(host.size() * 2), "alloc sequences");
+    HCHECK(ctx->seqb.ensure(hostb.size()), "alloc byte sequences");
+    HCHECK(hipMemcpyAsync(ctx->seqb.p, hostb.data(), hostb.size(), hipMemcpyHostToDevice, ctx->stream),
+           "upload byte sequences");
     HCHECK(ctx->cig.ensure((size_t)std::max<int64_t>(cig_total, 1) * 4), "alloc cigars");
     HCHECK(hipMemcpyAsync(ctx->seq.p, host.data(), host.size() * 2, hipMemcpyHostToDevice, ctx->stream),
            "upload sequences");
@@ -495,9 +553,15 @@ int halign_global_align_batch(halign_ctx *ctx, int32_t n, const char *const *ref
 namespace {
 
 // encode ASCII queries [first, first+count) into reversed spread even/odd copies
+__device__ inline uint8_t ascii_code(unsigned char c)
+{
+    return (c == 'A' || c == 'a') ? 0 : (c == 'C' || c == 'c') ? 1 : (c == 'G' || c == 'g') ? 2
+         : (c == 'T' || c == 't') ? 3 : 4;
+}
+
 __global__ void prep_queries(const char *__restrict__ raw, const int64_t *__restrict__ qoff,
-                             const int64_t *__restrict__ region, int first, int count,
-                             uint16_t *__restrict__ seq)
+                             const int64_t *__restrict__ region, const int64_t *__restrict__ bregion,
+                             int first, int count, uint16_t *__restrict__ seq, uint8_t *__restrict__ seqb)
 {
     const int qi = blockIdx.y;
     if (qi >= count) return;
@@ -505,18 +569,18 @@ __global__ void prep_queries(const char *__restrict__ raw, const int64_t *__rest
     const int64_t n = len + 2 * kSeqPad;
     uint16_t *even = seq + region[qi];
     uint16_t *odd = even + n;
+    auto code = [&](int64_t t) -> uint8_t {   // reversed query code at index t, 0 outside
+        if (t < 0 || t >= len) return 0;
+        return ascii_code((unsigned char)raw[beg + len - 1 - t]);
+    };
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        auto code = [&](int64_t kk) -> uint16_t {
-            const int64_t t = kk - kSeqPad;
-            if (t < 0 || t >= len) return 0;
-            const unsigned char c = (unsigned char)raw[beg + len - 1 - t];
-            const uint16_t v = (c == 'A' || c == 'a') ? 0 : (c == 'C' || c == 'c') ? 1
-                             : (c == 'G' || c == 'g') ? 2 : (c == 'T' || c == 't') ? 3 : 4;
-            return (uint16_t)(v << 8);
-        };
-        even[k] = code(k);
-        odd[k] = (k + 1 < n) ? code(k + 1) : (uint16_t)0;
+        even[k] = (uint16_t)(code(k - kSeqPad) << 8);
+        odd[k] = (k + 1 < n) ? (uint16_t)(code(k + 1 - kSeqPad) << 8) : (uint16_t)0;
     }
+    uint8_t *bq = seqb + bregion[qi];
+    const int64_t nb = len + 2 * kBytePad;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x)
+        bq[k] = code(k - kBytePad);
 }
 
 }  // namespace
@@ -525,11 +589,12 @@ struct halign_pairset {
     halign_ctx *ctx = nullptr;
     int32_t clen = 0, n = 0;
     DevBuf center;     // spread even/odd center
+    DevBuf centerb;    // center as padded byte codes
     DevBuf raw, qoff;  // ASCII queries and their offsets
     std::vector<int64_t> hoff;
     std::vector<uint8_t> hasn;
     bool center_hasn = false;
-    DevBuf work, cig, region;
+    DevBuf work, cig, region, bregion;
     int32_t last_first = 0, last_count = 0;
     std::vector<DpOut> outs;
     std::vector<int64_t> cig_off;
@@ -554,6 +619,8 @@ int halign_pairset_create(halign_ctx *ctx, const char *center, int32_t clen, int
     const int64_t cn = clen + 2 * kSeqPad;
     std::vector<uint16_t> hc((size_t)cn * 2);
     spread_target(cc.data(), clen, hc.data(), hc.data() + cn);
+    std::vector<uint8_t> hcb((size_t)clen + 2 * kBytePad);
+    bytes_target(cc.data(), clen, hcb.data());
     const int64_t total = query_offsets[n] - query_offsets[0];
     ps->hoff.assign(query_offsets, query_offsets + n + 1);
     for (auto &o : ps->hoff) o -= query_offsets[0];
@@ -562,13 +629,14 @@ int halign_pairset_create(halign_ctx *ctx, const char *center, int32_t clen, int
         for (int64_t k = ps->hoff[i]; k < ps->hoff[i + 1]; ++k)
             if (score_code((unsigned char)queries[query_offsets[0] + k]) > 3) { ps->hasn[i] = 1; break; }
     hipError_t e;
-    if ((e = ps->center.ensure(hc.size() * 2)) != hipSuccess ||
+    if ((e = ps->center.ensure(hc.size() * 2)) != hipSuccess || (e = ps->centerb.ensure(hcb.size())) != hipSuccess ||
         (e = ps->raw.ensure((size_t)std::max<int64_t>(total, 1))) != hipSuccess ||
         (e = ps->qoff.ensure((size_t)(n + 1) * 8)) != hipSuccess) {
         delete ps;
         return fail(ctx, HALIGN_E_NOMEM, "pairset alloc", e);
     }
     HCHECK(hipMemcpy(ps->center.p, hc.data(), hc.size() * 2, hipMemcpyHostToDevice), "upload center");
+    HCHECK(hipMemcpy(ps->centerb.p, hcb.data(), hcb.size(), hipMemcpyHostToDevice), "upload center bytes");
     HCHECK(hipMemcpy(ps->raw.p, queries + query_offsets[0], (size_t)total, hipMemcpyHostToDevice), "upload queries");
     HCHECK(hipMemcpy(ps->qoff.p, ps->hoff.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice), "upload offsets");
     *out = ps;
@@ -585,25 +653,36 @@ int halign_pairset_run(halign_pairset *ps, int32_t first, int32_t count, int32_t
     // The pair set's sequences live in its own buffers; run_specs reads from
     // ctx->seq, so lay the center and the prepared queries out there.
     const int64_t cn = ps->clen + 2 * kSeqPad;
-    std::vector<int64_t> region(count);
+    std::vector<int64_t> region(count), bregion(count);
     int64_t elems = 2 * cn;
+    const int64_t cbn = ps->clen + 2 * kBytePad;
+    int64_t nbytes = cbn;
     for (int i = 0; i < count; ++i) {
         region[i] = elems;
+        bregion[i] = nbytes;
         const int64_t ql = ps->hoff[first + i + 1] - ps->hoff[first + i];
         elems += 2 * (ql + 2 * kSeqPad);
         elems = (elems + 1) & ~(int64_t)1;
+        nbytes += ql + 2 * kBytePad;
     }
     HCHECK(ctx->seq.ensure((size_t)elems * 2 + 16), "alloc sequences");
+    HCHECK(ctx->seqb.ensure((size_t)nbytes + 16), "alloc byte sequences");
     HCHECK(ps->region.ensure((size_t)std::max(count, 1) * 8), "alloc regions");
+    HCHECK(ps->bregion.ensure((size_t)std::max(count, 1) * 8), "alloc regions");
     HCHECK(hipMemcpyAsync(ctx->seq.p, ps->center.p, (size_t)cn * 2 * 2, hipMemcpyDeviceToDevice, ctx->stream),
            "copy center");
+    HCHECK(hipMemcpyAsync(ctx->seqb.p, ps->centerb.p, (size_t)cbn, hipMemcpyDeviceToDevice, ctx->stream),
+           "copy center bytes");
     HCHECK(hipMemcpyAsync(ps->region.p, region.data(), (size_t)count * 8, hipMemcpyHostToDevice, ctx->stream),
+           "upload regions");
+    HCHECK(hipMemcpyAsync(ps->bregion.p, bregion.data(), (size_t)count * 8, hipMemcpyHostToDevice, ctx->stream),
            "upload regions");
     if (count > 0) {
         dim3 grid(64, count);
         hipLaunchKernelGGL(prep_queries, grid, dim3(256), 0, ctx->stream, ps->raw.as<const char>(),
-                           ps->qoff.as<const int64_t>(), ps->region.as<const int64_t>(), first, count,
-                           ctx->seq.as<uint16_t>());
+                           ps->qoff.as<const int64_t>(), ps->region.as<const int64_t>(),
+                           ps->bregion.as<const int64_t>(), first, count, ctx->seq.as<uint16_t>(),
+                           ctx->seqb.as<uint8_t>());
         HCHECK(hipGetLastError(), "launch prep");
     }
     std::vector<Spec> specs(count);
@@ -615,6 +694,8 @@ int halign_pairset_run(halign_pairset *ps, int32_t first, int32_t count, int32_t
         s.tgt_elem = kSeqPad; s.tgt_odd_elem = cn + kSeqPad;
         s.qry_elem = region[i] + kSeqPad;
         s.qry_odd_elem = region[i] + (ql + 2 * kSeqPad) + kSeqPad;
+        s.tgt_b = kBytePad;
+        s.qry_b = bregion[i] + kBytePad;
         s.tlen = ps->clen; s.qlen = ql;
         s.w = auto_band(ps->clen, ql);
         s.flags = kFlagRight | ((ps->center_hasn || ps->hasn[first + i]) ? kFlagHasN : 0);
@@ -623,7 +704,7 @@ int halign_pairset_run(halign_pairset *ps, int32_t first, int32_t count, int32_t
         ps->cig_off[i] = cig_total;
         cig_total += s.cig_cap;
         if (ql == 0) return fail(ctx, HALIGN_E_ARG, "empty query in pair set");
-        if (!fill_spec_shape(s, false)) return fail(ctx, HALIGN_E_UNSUPPORTED, "band too wide");
+        if (!fill_spec_shape(s, false, ctx->policy)) return fail(ctx, HALIGN_E_UNSUPPORTED, "band too wide");
     }
     HCHECK(ctx->cig.ensure((size_t)std::max<int64_t>(cig_total, 1) * 4), "alloc cigars");
     int rc = run_specs(ctx, specs, plain != 0, ps->outs, kernel_ms);
@@ -662,7 +743,7 @@ int halign_pairset_result(halign_pairset *ps, int32_t idx, int32_t *score, uint3
 void halign_pairset_free(halign_pairset *ps)
 {
     if (!ps) return;
-    ps->center.release(); ps->raw.release(); ps->qoff.release(); ps->work.release();
+    ps->center.release(); ps->centerb.release(); ps->raw.release(); ps->bregion.release(); ps->qoff.release(); ps->work.release();
     ps->cig.release(); ps->region.release();
     delete ps;
 }

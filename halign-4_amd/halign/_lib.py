@@ -46,6 +46,7 @@ def _load():
     lib.halign_last_error.argtypes = [_P]
     lib.halign_last_error.restype = C.c_char_p
     lib.halign_version.restype = C.c_char_p
+    lib.halign_set_kernel_policy.argtypes = [_P, C.c_int]
     lib.halign_dp_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, _P,
                                     C.c_int64, C.POINTER(C.c_double)]
     lib.halign_global_align_batch.argtypes = [_P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int64,
@@ -101,6 +102,10 @@ class Context:
         if rc != HALIGN_OK:
             msg = self.lib.halign_last_error(self.h).decode()
             raise HalignError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def set_kernel_policy(self, policy):
+        """0 = automatic kernel family, 1 = multi-wave kernels only."""
+        self._check(self.lib.halign_set_kernel_policy(self.h, int(policy)), "set_kernel_policy")
 
     def dp_batch(self, tasks, query_codes, target_codes, full_stats=True, plain=False):
         """tasks: structured array DP_TASK_DTYPE. Returns (results, cigars list, kernel_ms)."""

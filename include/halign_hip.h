@@ -63,6 +63,10 @@ void halign_free(halign_ctx *ctx);
 const char *halign_last_error(const halign_ctx *ctx);
 /* library version string */
 const char *halign_version(void);
+/* Kernel family choice: 0 = automatic (single-wave kernel where the band fits,
+ * multi-wave otherwise), 1 = multi-wave kernels only. Results are identical;
+ * used to test both families and by the shape autotuner. */
+int halign_set_kernel_policy(halign_ctx *ctx, int policy);
 
 /* Drop-in for ksw_extz2_sse (reference include/ksw2.h:64-65, called at
  * src/align/align.cpp:49-55): the plain one-pair-per-launch GPU form with every

@@ -17,10 +17,12 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score")
 
 
-@pytest.fixture(scope="module")
-def ctx():
+@pytest.fixture(scope="module", params=[0, 1], ids=["auto", "multiwave"])
+def ctx(request):
     import halign
     c = halign.Context(0)
+    c.set_kernel_policy(request.param)
+    print(f"kernel policy {request.param}", file=sys.stderr, flush=True)
     yield c
     c.close()
 
@@ -165,3 +167,20 @@ def test_global_align_batch(ctx, oracle):
         if not np.array_equal(exp, cigs[i]):
             bad += 1
     assert bad == 0
+
+
+def test_pairset_whole_pairs(ctx, oracle):
+    """Resident center + queries (config-2 sized pairs), device-side query prep."""
+    import halign
+    from halign import synth
+    center = synth.center(2)
+    qb, off = synth.queries(2, 0, 3)
+    print("pairset: 3 whole pairs of ~29.5 kb", file=sys.stderr, flush=True)
+    ps = halign.PairSet(ctx, center, qb, off)
+    ms, cells = ps.run(0, 3)
+    for i in range(3):
+        q = qb[off[i]:off[i + 1]]
+        exp, ecells = oracle.global_align_ksw2(center, q)
+        sc, cig = ps.result(i)
+        assert np.array_equal(cig, exp), i
+    ps.close()
