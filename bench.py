@@ -77,13 +77,26 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=0, help="pairs for the CPU baseline (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--policy", type=int, default=0,
-                    help="kernel family: 0 automatic (single-wave where the band fits), 1 multi-wave only")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 times), to scale HBM counters")
+    ap.add_argument("--kernel", default="auto",
+                    help="auto | multiwave | WxS (ring kernel with W waves and S slots per lane)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "the path shards by query ranges with no collective, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+        return
 
     import halign
     from halign import synth
@@ -96,7 +109,14 @@ def main():
     qbytes, off = synth.queries(2, 0, n_res)
     log(f"  generated {len(qbytes) / 1e9:.2f} GB in {time.time() - t0:.1f}s")
     ctx = halign.Context(0)
-    ctx.set_kernel_policy(args.policy)
+    if args.kernel == "auto":
+        policy = 0
+    elif args.kernel == "multiwave":
+        policy = 1
+    else:
+        wv, sl = (int(x) for x in args.kernel.lower().split("x"))
+        policy = halign.ring_policy(wv, sl)
+    ctx.set_kernel_policy(policy)
     ps = halign.PairSet(ctx, center, qbytes, off)
 
     # plain form: one pair per launch
@@ -154,17 +174,30 @@ def main():
         "data": "synthetic (seeded, csrc/synth.c config 2)",
         "config": {"workload": "config 2: kernel-only pairwise DP, 100000 pairs, center 29903 bp, "
                                "queries 29.0-30.0 kb, auto band, batched persistent form",
-                   "pairs_per_step": args.batch, "pairs_timed": pairs},
+                   "pairs_per_step": args.batch, "pairs_timed": pairs, "kernel": args.kernel},
         "seqs_per_s": round(pairs / t_wall, 2),
         "kernel_gcups": round(kernel_gcups, 3),
         "plain_gcups": round(plain_gcups, 4),
         "batched_vs_plain": round(kernel_gcups / plain_gcups, 1) if plain_gcups else None,
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3), "peak": PEAK_VALU_TOPS,
                      "unit": "TOPS", "frac": round(achieved_tops / PEAK_VALU_TOPS, 4),
-                     "traffic": None},
+                     "traffic": traffic_per_launch(tot_cells / args.steps)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def traffic_per_launch(cells_per_launch):
+    """HBM bytes of one DP launch (one step): bytes per cell from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), times the cells
+    of a step. None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "dp_ring_config2_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_cell:"):
+            return int(float(line.split(":")[1]) * cells_per_launch)
+    return None
 
 
 def halign_auto_band(rlen, qlen):

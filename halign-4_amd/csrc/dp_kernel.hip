@@ -50,7 +50,8 @@ __device__ void dp_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restri
 {
     constexpr int NL = 64 * W;
     constexpr int R = NL * S;
-    const int L = threadIdx.x, wave = L >> 6, lane = L & 63;
+    const int L = threadIdx.x, lane = L & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(L >> 6);
     const int qlen = J.qlen, tlen = J.tlen;
     int w = J.w;
     if (w < 0) w = tlen > qlen ? tlen : qlen;
@@ -61,7 +62,7 @@ __device__ void dp_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restri
     ncol = ((ncol < w + 1 ? ncol : w + 1) + 15) / 16 + 1;
     const size_t rowb = (size_t)ncol * 8;
     if (ncol > R - 1) {
-        if (L == 0) { out->status = kStTooWide; out->n_cigar = 0; }
+        if (wave == 0) { out->status = kStTooWide; out->n_cigar = 0; }   // same value from every lane
         return;
     }
 
@@ -398,7 +399,9 @@ __device__ void dp_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restri
         nout = backtrack_to_cigar(J, tb, rowb, w, right, i0, j0, sm.chunk, 0);
         if (nout > J.cig_cap) status = kStCigarOverflow;
     }
-    if (L == 0) {
+    // every lane of wave 0 stores the same (uniform) values: no lane-0 branch
+    // next to the job loop's back edge
+    if (wave == 0) {
         out->score = score; out->n_cigar = nout; out->zdropped = zd ? 1 : 0; out->status = status;
         out->max = (uint32_t)emax; out->max_t = emax_t; out->max_q = emax_q;
         out->mqe = STATS ? mqe : kNegInf; out->mqe_t = STATS ? mqe_t : -1;
@@ -417,10 +420,14 @@ __global__ __launch_bounds__(64 * W) void dp_persistent(const DpJob *__restrict_
 {
     __shared__ Smem<W, S> sm;
     uint8_t *tb = tb_pool + (size_t)blockIdx.x * tb_slot_bytes;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
-        if (threadIdx.x == 0) sm.job = atomicAdd(counter, 1);
+        // Job ticket from an atomic that every lane of wave 0 executes (lane 0
+        // adds 1), shared through LDS and read back as a scalar: no lane-0
+        // branch at the loop's back edge, and the loop exit is uniform.
+        if (wave == 0) sm.job = __builtin_amdgcn_readfirstlane(atomicAdd(counter, threadIdx.x == 0 ? 1 : 0));
         __syncthreads();
-        const int jid = sm.job;
+        const int jid = __builtin_amdgcn_readfirstlane(sm.job);
         __syncthreads();
         if (jid >= njobs) break;
         const DpJob J = jobs[jid];

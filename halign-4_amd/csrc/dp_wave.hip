@@ -1,10 +1,12 @@
-// dp_wave.hip -- single-wave form of the extz2 DP (CIGAR + score mode) for
-// bands up to 64*S-1 blocks. Same semantics as dp_kernel.hip (which restates
-// src/align/ksw2_extz2_sse.c:23-304, SSE2 branch), organised so that nothing
-// on the per-anti-diagonal critical path waits on memory, LDS or a barrier:
+// dp_wave.hip -- ring form of the extz2 DP (CIGAR + score mode) for bands up
+// to 64*W*S-2 blocks. Same semantics as dp_kernel.hip (which restates
+// src/align/ksw2_extz2_sse.c:23-304, SSE2 branch), organised so that as little
+// as possible on the per-anti-diagonal critical path waits on memory or LDS:
 //
-//  * one wave owns the job; block b lives in slot (b mod 64S)/64 of lane
-//    b mod 64, neighbour values move with DPP row/wave rotates;
+//  * a workgroup of W waves owns the job; block b lives in slot (b mod R)/NL of
+//    lane (b mod R) mod NL (NL = 64*W lanes, R = NL*S blocks); neighbour values
+//    move with a DPP wave rotate, and only the first lane of each wave takes
+//    its neighbour from LDS (one barrier per anti-diagonal when W > 1);
 //  * the query window of a block is a 16-byte register window that slides one
 //    base per anti-diagonal (v_alignbyte), fed from the left lane's window; only
 //    the first block of a band, whose left neighbour is idle, takes its new base
@@ -14,7 +16,10 @@
 //    every diagonal, right of the band it is the never-written 0 of the
 //    reference's calloc'd array, and left of the band (first block only) the
 //    frozen last in-band value is kept in one 8-VGPR set per lane;
-//  * the H-row edge (score/mte) is tracked in scalars fed by v_readlane.
+//  * the H-row edge (score/mte) is tracked in scalars fed by v_readlane (one
+//    wave) or by two LDS bytes read one diagonal later (several waves);
+//  * the gap direction (RIGHT/LEFT) is a template parameter, so the cell loop
+//    has no runtime branch and no register state merges at a join.
 #include "dp_device.h"
 
 namespace hal {
@@ -74,16 +79,25 @@ __device__ __forceinline__ void set_cell(uint32_t (&a)[8], int c, uint32_t v16)
     }
 }
 
-template <int S>
+template <int W, int S>
+struct WaveSmem {
+    uint32_t xch[2][W][S][3];   // last lane of each wave: x, v of cell 15 and query window dword 3
+    uint32_t chunk[64][10];     // backtrack staging
+    uint8_t logu[2], logv[2];   // H-edge bytes of the previous anti-diagonal (W > 1)
+    int job;
+};
+
+template <int W, int S, bool RT>
 __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restrict__ out,
-                            uint32_t (*chunk)[10])
+                            WaveSmem<W, S> &sm)
 {
-    constexpr int R = 64 * S;
-    const int lane = threadIdx.x & 63;
+    constexpr int NL = 64 * W;   // lanes of the workgroup
+    constexpr int R = NL * S;    // ring size in blocks
+    const int L = threadIdx.x, lane = L & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(L >> 6);
     const int qlen = J.qlen, tlen = J.tlen;
     int w = J.w;
     if (w < 0) w = tlen > qlen ? tlen : qlen;
-    const bool right = (J.flags & kFlagRight) != 0;
     const bool hasn = (J.flags & kFlagHasN) != 0;
     const int ndiag = qlen + tlen - 1;
     int ncol = qlen < tlen ? qlen : tlen;
@@ -92,7 +106,7 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
     // one spare ring block: a block entering two diagonals ahead must not reuse the
     // slot of the left neighbour of the next diagonal's first block
     if (ncol > R - 2) {
-        if (lane == 0) { out->status = kStTooWide; out->n_cigar = 0; }
+        if (wave == 0) { out->status = kStTooWide; out->n_cigar = 0; }   // same value from every lane
         return;
     }
     const uint8_t *__restrict__ tgt = J.tgt_b;
@@ -113,10 +127,17 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
 #pragma unroll
     for (int j = 0; j < 8; ++j) FZ[j] = 0x10001000u;
 
-    // block 0 for anti-diagonal 0 lives in lane 0, slot 0
-    if (lane == 0) {
-        load16(tgt, T4[0]);
-        load16(qr + (qlen - 1), Q4[0]);
+    // block 0 for anti-diagonal 0 lives in lane 0, slot 0 (every lane loads the
+    // same words and keeps them on lane 0: no lane-0 branch near the loop head)
+    {
+        uint32_t t0[4], q0[4];
+        load16(tgt, t0);
+        load16(qr + (qlen - 1), q0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T4[0][j] = L == 0 ? t0[j] : 0u;
+            Q4[0][j] = L == 0 ? q0[j] : 0u;
+        }
     }
 
     int last_st = -1, last_en = -1, last_bs = 0;
@@ -125,6 +146,24 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
     int32_t score = kNegInf, mte = kNegInf, mte_q = -1;
     int64_t cells = 0;
     bool zd = false;
+
+    // H-row edge (:227-269): H[en0] = H[en0-1] + u - qe, H[en0-1] += v - qe, fed
+    // with u at en0 and v at en0-1 (v at 0 when en0 == 0) of anti-diagonal rr.
+    auto edge = [&](int rr, int st0, int en0, int en, int ue, int vl) {
+        if (rr == 0) {
+            He = vl - 2 * kQE;
+        } else if (en0 == 0) {
+            He = He + vl - kQE;
+        } else {
+            const int32_t Hp = (en0 == last_en0) ? Hl : He;
+            Hl = (en0 - 1 >= st0) ? Hp + vl - kQE : Hp;
+            He = Hp + ue - kQE;
+        }
+        last_en0 = en0;
+        if (en0 == tlen - 1 && He > mte) { mte = He; mte_q = rr - en; }
+        if (rr == ndiag - 1 && en0 == tlen - 1) score = He;
+    };
+    int p_st0 = 0, p_en0 = 0, p_en = 0, rlast = 0;   // bounds of the diagonal whose edge is pending
 
     // Bounds of r and r+1 are known at the top of diagonal r. Loads for a
     // diagonal (new block windows, the first block's new query base) are
@@ -137,8 +176,8 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
         const int nbe = bd1.en >> 4;
         if (nbe > (bd.en >> 4)) {
             const int nm = nbe % R;
-            if (lane == nm % 64) {
-                const int kn = nm / 64;
+            if (L == nm % NL) {
+                const int kn = nm / NL;
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     if (k == kn) {
@@ -166,7 +205,7 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
         bool act[S];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
-            int d = 64 * k + lane - bsm;
+            int d = NL * k + L - bsm;
             if (d < 0) d += R;
             blk[k] = bs + d;
             act[k] = blk[k] <= be;
@@ -183,12 +222,39 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 rv[k] = wave_ror1(V[k][7]);
                 rq[k] = wave_ror1(Q4[k][3]);
             }
+            if constexpr (W > 1) {
+                const int p = r & 1;
+                if (lane == 63) {
 #pragma unroll
-            for (int k = 0; k < S; ++k) {
-                const int kp = (k + S - 1) % S;
-                nx[k] = lane ? rx[k] : rx[kp];
-                nv[k] = lane ? rv[k] : rv[kp];
-                nq[k] = lane ? rq[k] : rq[kp];
+                    for (int k = 0; k < S; ++k) {
+                        sm.xch[p][wave][k][0] = X[k][7];
+                        sm.xch[p][wave][k][1] = V[k][7];
+                        sm.xch[p][wave][k][2] = Q4[k][3];
+                    }
+                }
+                __syncthreads();
+                // lane 0 of wave i > 0: last lane of wave i-1, same slot; lane 0 of
+                // wave 0: last lane of wave W-1, previous slot
+                const int ws = wave ? wave - 1 : W - 1;
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    const int kp = (k + S - 1) % S;
+                    const uint32_t *e0 = sm.xch[p][ws][k], *e1 = sm.xch[p][ws][kp];
+                    const uint32_t *e = wave ? e0 : e1;
+                    nx[k] = lane ? rx[k] : e[0];
+                    nv[k] = lane ? rv[k] : e[1];
+                    nq[k] = lane ? rq[k] : e[2];
+                }
+                // the edge bytes of diagonal r-1 were stored before this barrier
+                if (r > 0) edge(r - 1, p_st0, p_en0, p_en, sm.logu[p ^ 1], sm.logv[p ^ 1]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    const int kp = (k + S - 1) % S;
+                    nx[k] = lane ? rx[k] : rx[kp];
+                    nv[k] = lane ? rv[k] : rv[kp];
+                    nq[k] = lane ? rq[k] : rq[kp];
+                }
             }
         }
         // slide the query windows one base for blocks that were already active
@@ -206,8 +272,8 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
             }
         }
         // left boundary of the rounded region (:118-122)
-        if (lane == bsm % 64) {
-            const int ks = bsm / 64;
+        if (L == bsm % NL) {
+            const int ks = bsm / NL;
             const bool keep = st > 0 && st - 1 >= last_st && st - 1 <= last_en;
             const uint32_t v1 = (st == 0 && r) ? (uint32_t)kQ << 24 : 0u;
 #pragma unroll
@@ -229,9 +295,9 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
             }
         }
 
-        const int ke = ((en0 >> 4) % R) / 64, le = ((en0 >> 4) % R) % 64;
+        const int ke = ((en0 >> 4) % R) / NL, le = ((en0 >> 4) % R) % NL;
         const int tv = en0 > 0 ? en0 - 1 : 0;
-        const int kv = ((tv >> 4) % R) / 64, lv = ((tv >> 4) % R) % 64;
+        const int kv = ((tv >> 4) % R) / NL, lv = ((tv >> 4) % R) % NL;
         uint32_t ue_v = 0, vl_v = 0;
 
 #pragma unroll
@@ -260,55 +326,49 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 // left of the band (first block): the frozen last in-band score
                 if (b == bs) {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        z[j] = bfi(leftm[j], FZ[j], z[j]);
-                        FZ[j] = z[j];
-                    }
+                    for (int j = 0; j < 8; ++j) z[j] = bfi(leftm[j], FZ[j], z[j]);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) FZ[j] = z[j];
                 }
             }
             uint32_t C[4];
-            auto cells_rl = [&](auto right_tag) {
-                constexpr bool RT = decltype(right_tag)::value;
-                const uint32_t x7o = X[k][7], v7o = V[k][7];
+            const uint32_t x7o = X[k][7], v7o = V[k][7];
 #pragma unroll
-                for (int m = 3; m >= 0; --m) {
-                    uint32_t fa[2], fb[2], fx[2], fy[2];
+            for (int m = 3; m >= 0; --m) {
+                uint32_t fa[2], fb[2], fx[2], fy[2];
 #pragma unroll
-                    for (int h = 1; h >= 0; --h) {
-                        const int j = 2 * m + h;   // descending: X[j-1], V[j-1] are still old
-                        // previous-diagonal x, v of cells (j-1, j+7)
-                        const uint32_t xs = j ? X[k][j - 1] : cat16(x7o, nx[k]);
-                        const uint32_t vs = j ? V[k][j - 1] : cat16(v7o, nv[k]);
-                        const uint32_t a = pk_add(xs, vs);
-                        const uint32_t ut = U[k][j];
-                        const uint32_t bb = pk_add(Y[k][j], ut);
-                        const uint32_t z0 = z[j];
-                        fa[h] = RT ? pk_subsat(a, z0) : pk_subsat(z0, a);
-                        uint32_t zz = pk_maxu(z0, a);
-                        fb[h] = RT ? pk_subsat(bb, zz) : pk_subsat(zz, bb);
-                        zz = pk_maxu(zz, bb);
-                        zz = pk_minu(zz, 0x15001500u);
-                        U[k][j] = pk_sub(zz, vs);
-                        V[k][j] = pk_sub(zz, ut);
-                        zz = pk_sub(zz, 0x06000600u);
-                        const uint32_t a2 = pk_sub(a, zz), b2 = pk_sub(bb, zz);
-                        X[k][j] = pk_maxi(a2, 0u);
-                        Y[k][j] = pk_maxi(b2, 0u);
-                        fx[h] = RT ? a2 : pk_subsat(0u, a2);
-                        fy[h] = RT ? b2 : pk_subsat(0u, b2);
-                    }
-                    // sign bytes of cells (2m, 2m+8, 2m+1, 2m+9), then 4 flags per byte
-                    const uint32_t ga = __builtin_amdgcn_perm(fa[1], fa[0], 0x07050301u);
-                    const uint32_t gb = __builtin_amdgcn_perm(fb[1], fb[0], 0x07050301u);
-                    const uint32_t gx = __builtin_amdgcn_perm(fx[1], fx[0], 0x07050301u);
-                    const uint32_t gy = __builtin_amdgcn_perm(fy[1], fy[0], 0x07050301u);
-                    uint32_t c = bfi(0x80808080u, ga, gb >> 1);
-                    c = bfi(0xC0C0C0C0u, c, gx >> 2);
-                    C[m] = bfi(0xE0E0E0E0u, c, gy >> 3);
+                for (int h = 1; h >= 0; --h) {
+                    const int j = 2 * m + h;   // descending: X[j-1], V[j-1] are still old
+                    // previous-diagonal x, v of cells (j-1, j+7)
+                    const uint32_t xs = j ? X[k][j - 1] : cat16(x7o, nx[k]);
+                    const uint32_t vs = j ? V[k][j - 1] : cat16(v7o, nv[k]);
+                    const uint32_t a = pk_add(xs, vs);
+                    const uint32_t ut = U[k][j];
+                    const uint32_t bb = pk_add(Y[k][j], ut);
+                    const uint32_t z0 = z[j];
+                    fa[h] = RT ? pk_subsat(a, z0) : pk_subsat(z0, a);
+                    uint32_t zz = pk_maxu(z0, a);
+                    fb[h] = RT ? pk_subsat(bb, zz) : pk_subsat(zz, bb);
+                    zz = pk_maxu(zz, bb);
+                    zz = pk_minu(zz, 0x15001500u);
+                    U[k][j] = pk_sub(zz, vs);
+                    V[k][j] = pk_sub(zz, ut);
+                    zz = pk_sub(zz, 0x06000600u);
+                    const uint32_t a2 = pk_sub(a, zz), b2 = pk_sub(bb, zz);
+                    X[k][j] = pk_maxi(a2, 0u);
+                    Y[k][j] = pk_maxi(b2, 0u);
+                    fx[h] = RT ? a2 : pk_subsat(0u, a2);
+                    fy[h] = RT ? b2 : pk_subsat(0u, b2);
                 }
-            };
-            if (right) cells_rl(std::integral_constant<bool, true>{});
-            else cells_rl(std::integral_constant<bool, false>{});
+                // sign bytes of cells (2m, 2m+8, 2m+1, 2m+9), then 4 flags per byte
+                const uint32_t ga = __builtin_amdgcn_perm(fa[1], fa[0], 0x07050301u);
+                const uint32_t gb = __builtin_amdgcn_perm(fb[1], fb[0], 0x07050301u);
+                const uint32_t gx = __builtin_amdgcn_perm(fx[1], fx[0], 0x07050301u);
+                const uint32_t gy = __builtin_amdgcn_perm(fy[1], fy[0], 0x07050301u);
+                uint32_t c = bfi(0x80808080u, ga, gb >> 1);
+                c = bfi(0xC0C0C0C0u, c, gx >> 2);
+                C[m] = bfi(0xE0E0E0E0u, c, gy >> 3);
+            }
             uint2 o;
             o.x = bfi(0xF0F0F0F0u, C[1], C[0] >> 4);
             o.y = bfi(0xF0F0F0F0u, C[3], C[2] >> 4);
@@ -317,23 +377,16 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
             if (k == kv) vl_v = (uint32_t)cell_byte(V[k], tv & 15);
         }
 
-        // H-row edge (:227-269): H[en0] = H[en0-1] + u - qe, H[en0-1] += v - qe
-        {
-            const int ue = (int)__builtin_amdgcn_readlane(ue_v, le);
-            const int vl = (int)__builtin_amdgcn_readlane(vl_v, lv);
-            if (r == 0) {
-                He = vl - 2 * kQE;
-            } else if (en0 == 0) {
-                He = He + vl - kQE;
-            } else {
-                const int32_t Hp = (en0 == last_en0) ? Hl : He;
-                Hl = (en0 - 1 >= st0) ? Hp + vl - kQE : Hp;
-                He = Hp + ue - kQE;
-            }
-            last_en0 = en0;
-            if (en0 == tlen - 1 && He > mte) { mte = He; mte_q = r - en; }
-            if (r == ndiag - 1 && en0 == tlen - 1) score = He;
+        if constexpr (W == 1) {
+            edge(r, st0, en0, en, (int)__builtin_amdgcn_readlane(ue_v, le),
+                 (int)__builtin_amdgcn_readlane(vl_v, lv));
+        } else {
+            // read by every wave after the next barrier
+            if (L == le) sm.logu[r & 1] = (uint8_t)ue_v;
+            if (L == lv) sm.logv[r & 1] = (uint8_t)vl_v;
+            p_st0 = st0; p_en0 = en0; p_en = en;
         }
+        rlast = r;
 
         last_st = st; last_en = en; last_bs = bs;
         if (r + 1 < ndiag) {
@@ -347,8 +400,8 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 const int nbs = bd1.st >> 4, nbe = bd1.en >> 4;
                 if (nbe > (bd.en >> 4)) {   // a block entering the band at r+2
                     const int nm = nbe % R;
-                    if (lane == nm % 64) {
-                        const int kn = nm / 64;
+                    if (L == nm % NL) {
+                        const int kn = nm / NL;
 #pragma unroll
                         for (int k = 0; k < S; ++k) {
                             if (k == kn) {
@@ -366,13 +419,21 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
         }
     }
 
+    if constexpr (W > 1) {
+        // traceback rows of every wave and the last edge bytes become visible
+        __threadfence();
+        __syncthreads();
+        edge(rlast, p_st0, p_en0, p_en, sm.logu[rlast & 1], sm.logv[rlast & 1]);
+    }
+
     int nout = 0, status = kStOk;
     if (zd) status = kStNeedStats;
-    else {
-        nout = backtrack_to_cigar(J, tb, rowb, w, right, tlen - 1, qlen - 1, chunk, 1);
+    else if (wave == 0) {
+        nout = backtrack_to_cigar(J, tb, rowb, w, RT, tlen - 1, qlen - 1, sm.chunk, 1);
         if (nout > J.cig_cap) status = kStCigarOverflow;
     }
-    if (lane == 0) {
+    // every lane of wave 0 stores the same (uniform) values: no lane-0 branch here
+    if (wave == 0) {
         out->score = score; out->n_cigar = nout; out->zdropped = zd ? 1 : 0; out->status = status;
         out->max = 0; out->max_t = -1; out->max_q = -1;
         out->mqe = kNegInf; out->mqe_t = -1;
@@ -380,56 +441,82 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
     }
 }
 
-template <int S>
-__global__ __launch_bounds__(64) void dp_wave_persistent(const DpJob *__restrict__ jobs, int njobs,
-                                                         int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
-                                                         size_t tb_slot_bytes, DpOut *__restrict__ outs)
+// Persistent kernel: each workgroup pulls jobs from a queue and reuses its own
+// traceback slot; all jobs of one launch share the gap direction.
+template <int W, int S, bool RT>
+__global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__restrict__ jobs, int njobs,
+                                                             int *__restrict__ counter,
+                                                             uint8_t *__restrict__ tb_pool, size_t tb_slot_bytes,
+                                                             DpOut *__restrict__ outs)
 {
-    __shared__ uint32_t chunk[64][10];
-    __shared__ int job_sh;
+    __shared__ WaveSmem<W, S> sm;
     uint8_t *tb = tb_pool + (size_t)blockIdx.x * tb_slot_bytes;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
-        // Job index through LDS between two barriers: every lane reads the same
-        // value and the loop stays uniform. (A lane-0 atomic broadcast with
-        // readfirstlane lets the compiler structurize this as a divergent loop in
-        // which the other lanes re-read index 0 forever.)
-        if (threadIdx.x == 0) job_sh = atomicAdd(counter, 1);
-        __syncthreads();
-        const int jid = job_sh;
-        __syncthreads();
+        // Job ticket: every lane of wave 0 executes the atomic (lane 0 adds 1,
+        // the others 0) and readfirstlane keeps lane 0's ticket, so no lane-0
+        // branch sits at the loop's back edge. Behind a lane-0 branch the
+        // compiler may fuse the previous job's lane-0 output writes with the
+        // next lane-0 atomic into a path that only lane 0 leaves the job loop
+        // by, while the other lanes re-run the old job forever (with one wave
+        // per workgroup the barrier is only a wave barrier and does not stop
+        // this). Several waves share the ticket through LDS; those barriers
+        // also keep the next job's traceback writes behind wave 0's backtrack.
+        int jid;
+        if constexpr (W == 1) {
+            jid = __builtin_amdgcn_readfirstlane(atomicAdd(counter, threadIdx.x == 0 ? 1 : 0));
+        } else {
+            if (wave == 0) sm.job = __builtin_amdgcn_readfirstlane(atomicAdd(counter, threadIdx.x == 0 ? 1 : 0));
+            __syncthreads();
+            jid = __builtin_amdgcn_readfirstlane(sm.job);
+            __syncthreads();
+        }
         if (jid >= njobs) break;
         const DpJob J = jobs[jid];
-        dp_wave_job<S>(J, tb, &outs[jid], chunk);
+        dp_wave_job<W, S, RT>(J, tb, &outs[jid], sm);
     }
 }
 
-static const void *wave_kernel(int S)
+// (W waves, S slots per lane); a shape serves n_col_ <= 64*W*S - 2
+#define HAL_WAVE_SHAPES(X) \
+    X(1, 1) X(1, 2) X(2, 1) X(1, 3) X(3, 1) X(2, 2) X(4, 1) X(5, 1) X(3, 2) X(6, 1) X(4, 2) X(8, 1) X(8, 2) X(16, 1)
+
+static const int kWaveShapes[][2] = {
+#define X(w, s) {w, s},
+    HAL_WAVE_SHAPES(X)
+#undef X
+};
+constexpr int kNumWaveShapes = sizeof(kWaveShapes) / sizeof(kWaveShapes[0]);
+
+static const void *wave_kernel(int idx, bool rt)
 {
-    switch (S) {
-    case 1: return (const void *)&dp_wave_persistent<1>;
-    case 2: return (const void *)&dp_wave_persistent<2>;
-    case 3: return (const void *)&dp_wave_persistent<3>;
-    case 4: return (const void *)&dp_wave_persistent<4>;
-    case 5: return (const void *)&dp_wave_persistent<5>;
-    }
-    return nullptr;
+    static const void *const tab[][2] = {
+#define X(w, s) {(const void *)&dp_wave_persistent<w, s, false>, (const void *)&dp_wave_persistent<w, s, true>},
+        HAL_WAVE_SHAPES(X)
+#undef X
+    };
+    return tab[idx][rt ? 1 : 0];
 }
 
-int dp_wave_max_slots() { return 5; }
+int dp_wave_num_shapes() { return kNumWaveShapes; }
+int dp_wave_shape_waves(int idx) { return kWaveShapes[idx][0]; }
+int dp_wave_shape_ring(int idx) { return 64 * kWaveShapes[idx][0] * kWaveShapes[idx][1]; }
 
-int dp_wave_max_groups(int S, int num_cu)
+int dp_wave_max_groups(int idx, bool rt, int num_cu)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(S), 64, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(idx, rt), 64 * kWaveShapes[idx][0], 0) !=
+        hipSuccess)
+        return 0;
     return per_cu * num_cu;
 }
 
-hipError_t dp_wave_launch(int S, int groups, const DpJob *jobs, int njobs, int *counter, uint8_t *tb_pool,
-                          size_t tb_slot_bytes, DpOut *outs, hipStream_t stream)
+hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
+                          uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream)
 {
     void *args[] = {(void *)&jobs, (void *)&njobs, (void *)&counter, (void *)&tb_pool,
                     (void *)&tb_slot_bytes, (void *)&outs};
-    return hipLaunchKernel(wave_kernel(S), dim3(groups), dim3(64), args, 0, stream);
+    return hipLaunchKernel(wave_kernel(idx, rt), dim3(groups), dim3(64 * kWaveShapes[idx][0]), args, 0, stream);
 }
 
 }  // namespace hal

@@ -21,10 +21,12 @@ int dp_shape_threads(int idx);
 int dp_shape_max_groups(int idx, bool stats, int num_cu);
 hipError_t dp_launch(int idx, bool stats, int groups, const DpJob *jobs, int njobs, int *counter,
                      uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
-int dp_wave_max_slots();
-int dp_wave_max_groups(int S, int num_cu);
-hipError_t dp_wave_launch(int S, int groups, const DpJob *jobs, int njobs, int *counter, uint8_t *tb_pool,
-                          size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
+int dp_wave_num_shapes();
+int dp_wave_shape_waves(int idx);
+int dp_wave_shape_ring(int idx);
+int dp_wave_max_groups(int idx, bool rt, int num_cu);
+hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
+                          uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
 }  // namespace hal
 
 using namespace hal;
@@ -63,10 +65,14 @@ inline int n_col(int qlen, int tlen, int w)
 }
 
 // Band overflow ("zdropped" break, src/align/ksw2_extz2_sse.c:111-114) happens
-// iff some anti-diagonal has st > en. Both violated constraints grow
-// monotonically with r, so testing the last anti-diagonal decides it.
+// iff some anti-diagonal has st > en. Of the ways st > en can hold, two grow
+// monotonically with r (tested on the last anti-diagonal) and one needs w = 0:
+// (r-w+1)>>1 > (r+w)>>1 at every odd r, so from r = 1 on. The others cannot
+// hold for qlen, tlen >= 1 and w >= 0 (checked exhaustively over small sizes
+// by tests/test_host_logic.py).
 inline bool predict_zdrop(int qlen, int tlen, int w)
 {
+    if (w == 0 && qlen + tlen - 1 >= 2) return true;
     const int r = qlen + tlen - 2;
     if (r - qlen + 1 > (r + w) >> 1) return true;
     if ((r - w + 1) >> 1 > tlen - 1) return true;
@@ -106,7 +112,7 @@ struct halign_ctx {
     std::string err;
     DevBuf seq, seqb, jobs, outs, counters, tb, cig;
     size_t tb_budget = 0;
-    int policy = 0;   // 0: auto kernel choice, 1: multi-wave kernels only
+    int policy = 0;   // 0: auto kernel choice, 1: multi-wave kernels only, 2+i: ring shape i
 };
 
 namespace {
@@ -177,15 +183,30 @@ void bytes_query_rev(const uint8_t *codes, int len, uint8_t *out)
 }
 
 // Shape ids: [0, dp_num_shapes()) multi-wave kernels (any mode), then
-// kWaveBase + S - 1 for the single-wave CIGAR-mode kernel with S slots.
+// kWaveBase + i for ring shape i (CIGAR mode).
 constexpr int kWaveBase = 16;
-constexpr int kNumShapeIds = kWaveBase + 8;
+constexpr int kNumShapeIds = kWaveBase + 32;
 
 int pick_shape(int ncol, bool stats, int policy)
 {
-    if (!stats && policy == 0) {
-        for (int S = 1; S <= dp_wave_max_slots(); ++S)
-            if (ncol <= 64 * S - 2) return kWaveBase + S - 1;
+    if (!stats && policy != 1) {
+        const int nw = dp_wave_num_shapes();
+        if (policy >= 2 && policy - 2 < nw && ncol <= dp_wave_shape_ring(policy - 2) - 2)
+            return kWaveBase + policy - 2;
+        // Measured preference (config-2 band, n_col_ = 293, kernel GCUPS: 3x2 909,
+        // 4x2 913, 5x1 861, 8x1 802, 6x1 786, 16x1 508): two slots per lane pay
+        // for the per-diagonal work better than a tighter ring, one slot only for
+        // bands of a wave's width. First shape of this list that holds the band.
+        static const int kAuto[][2] = {{1, 1}, {1, 2}, {2, 2}, {3, 2}, {4, 2}, {8, 2}};
+        int best = -1;
+        for (const auto &ws : kAuto) {
+            for (int i = 0; i < nw && best < 0; ++i)
+                if (dp_wave_shape_waves(i) == ws[0] && dp_wave_shape_ring(i) == 64 * ws[0] * ws[1] &&
+                    ncol <= dp_wave_shape_ring(i) - 2)
+                    best = i;
+            if (best >= 0) break;
+        }
+        if (best >= 0) return kWaveBase + best;
     }
     const int ns = dp_num_shapes();
     for (int i = 0; i < ns; ++i)
@@ -206,14 +227,17 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     const uint8_t *seqb = ctx->seqb.as<uint8_t>();
     uint32_t *cig = ctx->cig.as<uint32_t>();
 
-    // bucket by (shape, stats)
+    // bucket by (shape, stats, gap direction)
     const int ns = kNumShapeIds;
-    std::vector<std::vector<int>> buckets(2 * ns);
-    for (size_t i = 0; i < n; ++i) buckets[specs[i].shape * 2 + (specs[i].stats ? 1 : 0)].push_back((int)i);
+    const int nb = 4 * ns;
+    std::vector<std::vector<int>> buckets(nb);
+    for (size_t i = 0; i < n; ++i)
+        buckets[specs[i].shape * 4 + (specs[i].stats ? 2 : 0) + ((specs[i].flags & kFlagRight) ? 1 : 0)]
+            .push_back((int)i);
 
     HCHECK(ctx->jobs.ensure(n * sizeof(DpJob)), "alloc jobs");
     HCHECK(ctx->outs.ensure(n * sizeof(DpOut)), "alloc outs");
-    HCHECK(ctx->counters.ensure((size_t)(plain ? n : 2 * ns) * sizeof(int)), "alloc counters");
+    HCHECK(ctx->counters.ensure((size_t)(plain ? n : nb) * sizeof(int)), "alloc counters");
     std::vector<DpJob> hjobs(n);
     std::vector<int> order;
     order.reserve(n);
@@ -247,32 +271,34 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     DpOut *douts = ctx->outs.as<DpOut>();
     int *ctr = ctx->counters.as<int>();
     size_t pos = 0;
-    for (int bi = 0; bi < 2 * ns; ++bi) {
+    for (int bi = 0; bi < nb; ++bi) {
         auto &b = buckets[bi];
         if (b.empty()) continue;
-        const int shape = bi / 2;
-        const bool stats = bi & 1;
+        const int shape = bi / 4;
+        const bool stats = (bi & 2) != 0;
+        const bool rt = (bi & 1) != 0;
         size_t slot = 0;
         for (int i : b) slot = std::max(slot, specs[i].tb_bytes);
         slot = (slot + 255) & ~(size_t)255;
         const bool wave = shape >= kWaveBase;
-        const int S = shape - kWaveBase + 1;
+        const int wi = shape - kWaveBase;
         if (plain) {
             HCHECK(ctx->tb.ensure(slot), "alloc traceback");
             for (size_t k = 0; k < b.size(); ++k) {
-                hipError_t e = wave ? dp_wave_launch(S, 1, djobs + pos + k, 1, ctr + pos + k, ctx->tb.as<uint8_t>(),
-                                                     slot, douts + pos + k, ctx->stream)
+                hipError_t e = wave ? dp_wave_launch(wi, rt, 1, djobs + pos + k, 1, ctr + pos + k,
+                                                     ctx->tb.as<uint8_t>(), slot, douts + pos + k, ctx->stream)
                                     : dp_launch(shape, stats, 1, djobs + pos + k, 1, ctr + pos + k,
                                                 ctx->tb.as<uint8_t>(), slot, douts + pos + k, ctx->stream);
                 HCHECK(e, "launch dp (plain)");
             }
         } else {
-            int groups = wave ? dp_wave_max_groups(S, ctx->num_cu) : dp_shape_max_groups(shape, stats, ctx->num_cu);
+            int groups = wave ? dp_wave_max_groups(wi, rt, ctx->num_cu)
+                              : dp_shape_max_groups(shape, stats, ctx->num_cu);
             if (groups <= 0) return fail(ctx, HALIGN_E_HIP, "occupancy query failed");
             groups = std::min<size_t>(groups, b.size());
             groups = (int)std::min<size_t>(groups, std::max<size_t>(1, ctx->tb_budget / slot));
             HCHECK(ctx->tb.ensure(slot * groups), "alloc traceback pool");
-            hipError_t e = wave ? dp_wave_launch(S, groups, djobs + pos, (int)b.size(), ctr + bi,
+            hipError_t e = wave ? dp_wave_launch(wi, rt, groups, djobs + pos, (int)b.size(), ctr + bi,
                                                  ctx->tb.as<uint8_t>(), slot, douts + pos, ctx->stream)
                                 : dp_launch(shape, stats, groups, djobs + pos, (int)b.size(), ctr + bi,
                                             ctx->tb.as<uint8_t>(), slot, douts + pos, ctx->stream);
@@ -355,9 +381,40 @@ void halign_free(halign_ctx *ctx)
 
 const char *halign_last_error(const halign_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
+int halign_band_info(int32_t qlen, int32_t tlen, int32_t w, int32_t *ncol, int32_t *overflow, int64_t *cells)
+{
+    if (qlen <= 0 || tlen <= 0) return HALIGN_E_ARG;
+    const int we = eff_w(w, qlen, tlen);
+    if (ncol) *ncol = n_col(qlen, tlen, we);
+    if (overflow) *overflow = predict_zdrop(qlen, tlen, we) ? 1 : 0;
+    if (cells) {
+        // sum of en0 - st0 + 1 over the anti-diagonals computed before any overflow
+        // (src/align/ksw2_extz2_sse.c:107-114)
+        int64_t c = 0;
+        for (int r = 0; r < qlen + tlen - 1; ++r) {
+            const int st0 = std::max(std::max(0, r - qlen + 1), (r - we + 1) >> 1);
+            const int en0 = std::min(std::min(tlen - 1, r), (r + we) >> 1);
+            if (st0 > en0) break;
+            c += en0 - st0 + 1;
+        }
+        *cells = c;
+    }
+    return HALIGN_OK;
+}
+
+int halign_ring_shapes(int32_t *waves, int32_t *ring, int32_t cap)
+{
+    const int n = dp_wave_num_shapes();
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (waves) waves[i] = dp_wave_shape_waves(i);
+        if (ring) ring[i] = dp_wave_shape_ring(i);
+    }
+    return n;
+}
+
 int halign_set_kernel_policy(halign_ctx *ctx, int policy)
 {
-    if (!ctx || policy < 0 || policy > 1) return HALIGN_E_ARG;
+    if (!ctx || policy < 0 || policy >= 2 + dp_wave_num_shapes()) return HALIGN_E_ARG;
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->policy = policy;
     return HALIGN_OK;

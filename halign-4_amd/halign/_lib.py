@@ -47,6 +47,7 @@ def _load():
     lib.halign_last_error.restype = C.c_char_p
     lib.halign_version.restype = C.c_char_p
     lib.halign_set_kernel_policy.argtypes = [_P, C.c_int]
+    lib.halign_ring_shapes.argtypes = [_P, _P, C.c_int32]
     lib.halign_dp_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, _P,
                                     C.c_int64, C.POINTER(C.c_double)]
     lib.halign_global_align_batch.argtypes = [_P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int64,
@@ -74,6 +75,23 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def ring_shapes():
+    """[(waves per workgroup, ring size in blocks)] of the ring-form kernel, by shape index."""
+    n = lib().halign_ring_shapes(None, None, 0)
+    w = np.zeros(n, np.int32)
+    r = np.zeros(n, np.int32)
+    lib().halign_ring_shapes(_ptr(w), _ptr(r), n)
+    return [(int(a), int(b)) for a, b in zip(w, r)]
+
+
+def ring_policy(waves, slots):
+    """Kernel policy that forces the ring shape (waves, slots per lane)."""
+    for i, (w, r) in enumerate(ring_shapes()):
+        if w == waves and r == 64 * waves * slots:
+            return 2 + i
+    raise ValueError(f"no ring shape with {waves} waves and {slots} slots")
 
 
 class Context:
@@ -104,7 +122,7 @@ class Context:
             raise HalignError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
 
     def set_kernel_policy(self, policy):
-        """0 = automatic kernel family, 1 = multi-wave kernels only."""
+        """0 = automatic, 1 = multi-wave kernels only, 2 + i = ring shape i (see ring_policy)."""
         self._check(self.lib.halign_set_kernel_policy(self.h, int(policy)), "set_kernel_policy")
 
     def dp_batch(self, tasks, query_codes, target_codes, full_stats=True, plain=False):

@@ -63,10 +63,23 @@ void halign_free(halign_ctx *ctx);
 const char *halign_last_error(const halign_ctx *ctx);
 /* library version string */
 const char *halign_version(void);
-/* Kernel family choice: 0 = automatic (single-wave kernel where the band fits,
- * multi-wave otherwise), 1 = multi-wave kernels only. Results are identical;
- * used to test both families and by the shape autotuner. */
+/* Kernel family choice: 0 = automatic (ring-form kernel where the band fits,
+ * multi-wave otherwise), 1 = multi-wave kernels only, 2 + i = ring shape i
+ * wherever the band fits it (automatic otherwise). Results are identical;
+ * used to test every family and to tune the shape choice. No reference
+ * counterpart (the reference has one CPU kernel). */
 int halign_set_kernel_policy(halign_ctx *ctx, int policy);
+/* Band geometry of ksw_extz2_sse for (qlen, tlen, w) (reference
+ * src/align/ksw2_extz2_sse.c:75-76 n_col_, :107-114 band and overflow): the
+ * 16-cell column count n_col_, whether the band overflows ("zdropped"), and
+ * the in-band cells computed. Host-only, no GPU needed. No reference
+ * counterpart (the reference computes these inline). */
+int halign_band_info(int32_t qlen, int32_t tlen, int32_t w, int32_t *ncol, int32_t *overflow,
+                     int64_t *cells);
+/* Ring shapes of the ring-form kernel: returns their number and fills up to
+ * cap entries of waves per workgroup and ring size in 16-cell blocks (a shape
+ * serves bands of n_col_ <= ring - 2 blocks). No reference counterpart. */
+int halign_ring_shapes(int32_t *waves, int32_t *ring, int32_t cap);
 
 /* Drop-in for ksw_extz2_sse (reference include/ksw2.h:64-65, called at
  * src/align/align.cpp:49-55): the plain one-pair-per-launch GPU form with every

@@ -424,6 +424,15 @@ int or_collect_anchors(const uint64_t *ref_x, const uint64_t *ref_y, int nref,
                        const uint64_t *qry_x, const uint64_t *qry_y, int nqry,
                        or_anchor_t *out, int cap)
 {
+    /* SeedFilterParams defaults, include/anchor.h:65-71 */
+    const or_seed_params_t p = {2e-4, 10, 1000000, 0.01, 500};
+    return or_collect_anchors_ex(ref_x, ref_y, nref, qry_x, qry_y, nqry, &p, out, cap);
+}
+
+int or_collect_anchors_ex(const uint64_t *ref_x, const uint64_t *ref_y, int nref,
+                          const uint64_t *qry_x, const uint64_t *qry_y, int nqry,
+                          const or_seed_params_t *p, or_anchor_t *out, int cap)
+{
     if (nref == 0 || nqry == 0) return 0;
     hit_t *sr = (hit_t *)malloc((size_t)nref * sizeof(hit_t));
     for (int i = 0; i < nref; ++i) sr[i] = unpack_hit(ref_x[i], ref_y[i]);
@@ -439,11 +448,12 @@ int or_collect_anchors(const uint64_t *ref_x, const uint64_t *ref_y, int nref,
         dh[nd] = sr[i].h; dstart[nd] = (uint64_t)i; docc[nd] = (uint64_t)(j - i); ++nd;
         i = j;
     }
-    /* compute_ref_occ_threshold src/seed/anchor.cpp:73-117, defaults
-     * f=2e-4, U=[10, 1e6] */
+    /* compute_ref_occ_threshold src/seed/anchor.cpp:73-117 */
     uint64_t f_cutoff = UINT64_MAX;
-    {
-        size_t n_skip = (size_t)floor(2e-4 * (double)nd);
+    if (p->f_top_frac >= 1.0) {
+        f_cutoff = nd > 0 ? 1 : UINT64_MAX;
+    } else if (p->f_top_frac > 0.0) {
+        size_t n_skip = (size_t)floor(p->f_top_frac * (double)nd);
         if (n_skip > 0) {
             uint64_t *tmp = (uint64_t *)malloc((size_t)nd * 8);
             memcpy(tmp, docc, (size_t)nd * 8);
@@ -452,13 +462,13 @@ int or_collect_anchors(const uint64_t *ref_x, const uint64_t *ref_y, int nref,
             free(tmp);
         }
     }
-    uint64_t capped = f_cutoff < 1000000ULL ? f_cutoff : 1000000ULL;
-    uint64_t thr = capped > 10ULL ? capped : 10ULL;
+    uint64_t capped = f_cutoff < p->u_ceil ? f_cutoff : p->u_ceil;
+    uint64_t thr = capped > p->u_floor ? capped : p->u_floor;
     /* query hash occurrence (:280-284) */
     uint64_t *qh = (uint64_t *)malloc((size_t)nqry * 8);
     for (int i = 0; i < nqry; ++i) qh[i] = qry_x[i] >> 8;
     qsort(qh, (size_t)nqry, 8, cmp_u64);
-    const double q_occ_limit = 0.01 * (double)nqry;
+    const double q_occ_limit = p->q_occ_frac * (double)nqry;   /* used only when q_occ_frac > 0 */
     int cnt = 0, ok = 1;
     for (int iq = 0; iq < nqry && ok; ++iq) {
         hit_t qhit = unpack_hit(qry_x[iq], qry_y[iq]);
@@ -476,10 +486,11 @@ int or_collect_anchors(const uint64_t *ref_x, const uint64_t *ref_y, int nref,
         size_t c0 = a; b = (size_t)nqry;
         while (a < b) { size_t m = (a + b) >> 1; if (qh[m] <= qhit.h) a = m + 1; else b = m; }
         uint64_t qocc = (uint64_t)(a - c0);
-        if ((double)qocc > q_occ_limit) continue;
+        if (p->q_occ_frac > 0.0 && (double)qocc > q_occ_limit) continue;
         uint64_t ref_occ = docc[found];
         if (ref_occ > thr) {
-            if (qhit.pos % 500u != 0) continue;
+            if (p->sample_every_bp == 0) continue;
+            if (qhit.pos % p->sample_every_bp != 0) continue;
         }
         for (uint64_t i = dstart[found]; i < dstart[found] + ref_occ; ++i) {
             if (cnt >= cap) { ok = 0; break; }
@@ -517,22 +528,25 @@ static int cmp_anchor_chain(const void *pa, const void *pb)
 
 /* chainScoreSimple src/seed/anchor.cpp:254-301 with ChainParams defaults
  * include/anchor.h:116-126. Float ops in program order, no contraction. */
-static int32_t chain_score(const or_anchor_t *ai, const or_anchor_t *aj, const float *lut)
+static int32_t chain_score(const or_anchor_t *ai, const or_anchor_t *aj, const or_chain_params_t *cp,
+                           const float *lut)
 {
     if (ai->rid_ref != aj->rid_ref) return INT32_MIN;
     if (ai->is_rev != aj->is_rev) return INT32_MIN;
     const int32_t dq = (int32_t)ai->pos_qry - (int32_t)aj->pos_qry;
-    if (dq <= 0 || dq > 5000) return INT32_MIN;
+    if (dq <= 0 || dq > cp->max_dist_x) return INT32_MIN;
     const int32_t dr = (int32_t)ai->pos_ref - (int32_t)aj->pos_ref;
-    if (dr <= 0 || dr > 5000) return INT32_MIN;
+    if (dr <= 0 || dr > cp->max_dist_y) return INT32_MIN;
     const int32_t dd = dr > dq ? dr - dq : dq - dr;
-    if (dd > 500) return INT32_MIN;
+    if (dd > cp->bw) return INT32_MIN;
     const int32_t dg = dr < dq ? dr : dq;
     const int32_t q_span = (int32_t)aj->span;
     int32_t sc = q_span < dg ? q_span : dg;
     if (dd > 0 || dg > q_span) {
-        volatile float t1 = 0.01f * (float)dd;
-        volatile float t2 = 0.01f * (float)dg;
+        /* each float operation rounded separately, no contraction (the
+         * reference's production build has no FMA) */
+        volatile float t1 = cp->gap_penalty * (float)dd;
+        volatile float t2 = cp->skip_penalty * (float)dg;
         volatile float lin_pen = t1 + t2;
         const float log_pen = dd >= 1 ? lut[dd + 1] : 0.0f;
         volatile float half = 0.5f * log_pen;
@@ -544,7 +558,16 @@ static int32_t chain_score(const or_anchor_t *ai, const or_anchor_t *aj, const f
 
 int or_chain_anchors(or_anchor_t *anchors, int n, const float *lut, or_anchor_t *out)
 {
+    /* ChainParams defaults, include/anchor.h:116-126 */
+    const or_chain_params_t cp = {5000, 5000, 500, 25, 5000, 3, 40, 0.01f, 0.01f};
+    return or_chain_anchors_ex(anchors, n, &cp, lut, out);
+}
+
+int or_chain_anchors_ex(or_anchor_t *anchors, int n, const or_chain_params_t *cp, const float *lut,
+                        or_anchor_t *out)
+{
     if (n == 0) return 0;
+    if (cp->bw > 500) return -1;   /* lut covers dd + 1 <= 501 */
     qsort(anchors, (size_t)n, sizeof(or_anchor_t), cmp_anchor_chain);
     int32_t *f = (int32_t *)malloc((size_t)n * 4);
     int64_t *p = (int64_t *)calloc((size_t)n, 8);
@@ -559,19 +582,19 @@ int or_chain_anchors(or_anchor_t *anchors, int n, const float *lut, or_anchor_t 
         while (st < i) {
             const or_anchor_t *as = &anchors[st];
             if (as->rid_ref != ai->rid_ref || as->is_rev != ai->is_rev ||
-                (int32_t)(ai->pos_ref - as->pos_ref) > 5000) ++st;
+                (int32_t)(ai->pos_ref - as->pos_ref) > cp->max_dist_x) ++st;
             else break;
         }
-        int64_t iter_start = (i - st > 5000) ? (i - 5000) : st;
+        int64_t iter_start = (i - st > cp->max_iter) ? (i - cp->max_iter) : st;
         for (int64_t j = i - 1; j >= iter_start; --j) {
-            const int32_t sc = chain_score(ai, &anchors[j], lut);
+            const int32_t sc = chain_score(ai, &anchors[j], cp, lut);
             if (sc == INT32_MIN) continue;
             const int32_t total = f[j] + sc;
             if (total > max_f) {
                 max_f = total; max_j = j;
                 if (n_skip > 0) --n_skip;
             } else if (t[j] == (int32_t)i) {
-                if (++n_skip > 25) break;
+                if (++n_skip > cp->max_skip) break;
             }
             if (p[j] >= 0) t[p[j]] = (int32_t)i;
         }
@@ -579,10 +602,10 @@ int or_chain_anchors(or_anchor_t *anchors, int n, const float *lut, or_anchor_t 
         if (max_f > max_f_global) { max_f_global = max_f; best_end = i; }
     }
     int cnt = 0;
-    if (!(max_f_global < 40 || best_end < 0)) {
+    if (!(max_f_global < cp->min_score || best_end < 0)) {
         int64_t cur = best_end;
         while (cur >= 0) { ++cnt; cur = p[cur]; }
-        if (cnt < 3) cnt = 0;
+        if (cnt < cp->min_cnt) cnt = 0;
         else {
             cur = best_end;
             for (int k = cnt - 1; k >= 0; --k) { out[k] = anchors[cur]; cur = p[cur]; }

@@ -71,17 +71,37 @@ int or_global_align_ksw2(const char *ref, int rlen, const char *qry, int qlen,
 int or_extract_minimizer(const char *seq, int n, int k, int w, int non_canonical,
                          uint64_t *x_out, uint64_t *y_out, int cap, int *ub);
 
+/* SeedFilterParams (include/anchor.h:65-71) */
+typedef struct {
+    double f_top_frac;
+    uint64_t u_floor, u_ceil;
+    double q_occ_frac;
+    uint64_t sample_every_bp;
+} or_seed_params_t;
+
 /* minimizer::collect_anchors src/seed/minimizer.cpp:220-356 with
  * default_mm2_params() (include/anchor.h:65-75). Returns count or -1. */
 int or_collect_anchors(const uint64_t *ref_x, const uint64_t *ref_y, int nref,
                        const uint64_t *qry_x, const uint64_t *qry_y, int nqry,
                        or_anchor_t *out, int cap);
+/* the same with explicit SeedFilterParams (the reference tests' form) */
+int or_collect_anchors_ex(const uint64_t *ref_x, const uint64_t *ref_y, int nref,
+                          const uint64_t *qry_x, const uint64_t *qry_y, int nqry,
+                          const or_seed_params_t *p, or_anchor_t *out, int cap);
 
 /* anchor::chainAnchors src/seed/anchor.cpp:332-461 (sorts `anchors` in place).
  * log2_lut[d] must hold log2f((float)d) for d in [0, 501]. Returns chain
  * length (0 = no chain). */
 int or_chain_anchors(or_anchor_t *anchors, int n, const float *log2_lut,
                      or_anchor_t *out);
+/* ChainParams (include/anchor.h:116-126) */
+typedef struct {
+    int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_score;
+    float gap_penalty, skip_penalty;
+} or_chain_params_t;
+/* the same with explicit ChainParams (the reference tests' form); bw <= 500 */
+int or_chain_anchors_ex(or_anchor_t *anchors, int n, const or_chain_params_t *cp, const float *log2_lut,
+                        or_anchor_t *out);
 
 /* align::globalAlignMM2 src/align/align.cpp:189-305. Returns n_cigar or -1.
  * stats (optional, 4 ints): segments run, forced I+D segments, whole-pair

@@ -32,6 +32,30 @@ class Ez(C.Structure):
         return tuple(getattr(self, f) for f, _ in self._fields_)
 
 
+class SeedParams(C.Structure):
+    _fields_ = [("f_top_frac", C.c_double), ("u_floor", C.c_uint64), ("u_ceil", C.c_uint64),
+                ("q_occ_frac", C.c_double), ("sample_every_bp", C.c_uint64)]
+
+
+class ChainParams(C.Structure):
+    _fields_ = [("max_dist_x", C.c_int32), ("max_dist_y", C.c_int32), ("bw", C.c_int32),
+                ("max_skip", C.c_int32), ("max_iter", C.c_int32), ("min_cnt", C.c_int32),
+                ("min_score", C.c_int32), ("gap_penalty", C.c_float),
+                ("skip_penalty", C.c_float)]
+
+
+def pack_hit(h56, pos, rid=0, strand=False, span=15):
+    """MinimizerHit (include/seed.h:206-265): x = h56<<8 | span, y = (rid | strand<<31)<<32 | pos."""
+    x = (int(h56) << 8) | (int(span) & 0xff)
+    y = ((int(rid) | (int(bool(strand)) << 31)) << 32) | int(pos)
+    return x, y
+
+
+def pack_hits(hits):
+    xs, ys = zip(*[pack_hit(*h) for h in hits]) if hits else ((), ())
+    return np.array(xs, np.uint64), np.array(ys, np.uint64)
+
+
 ANCHOR_DTYPE = np.dtype([("hash", "<u8"), ("rid_ref", "<u4"), ("pos_ref", "<u4"),
                          ("rid_qry", "<u4"), ("pos_qry", "<u4"), ("span", "<u4"),
                          ("is_rev", "<u4")])
@@ -76,6 +100,8 @@ class Oracle(_Base):
                                            C.c_int, _P]
         L.or_collect_anchors.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, C.c_int]
         L.or_chain_anchors.argtypes = [_P, C.c_int, _P, _P]
+        L.or_collect_anchors_ex.argtypes = [_P, _P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_int]
+        L.or_chain_anchors_ex.argtypes = [_P, C.c_int, _P, _P, _P]
         L.or_global_align_mm2.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _P, C.c_int,
                                           _P, _P, C.c_int, _P, _P]
         L.or_ref_global_align.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _P, _P,
@@ -139,6 +165,30 @@ class Oracle(_Base):
         a = np.ascontiguousarray(anchors.copy())
         out = np.zeros(max(1, len(a)), ANCHOR_DTYPE)
         n = self.lib.or_chain_anchors(_ptr(a), len(a), _ptr(self.lut), _ptr(out))
+        return a, out[:n].copy()
+
+    def collect_anchors_ex(self, rx, ry, qx, qy, f_top_frac=2e-4, u_floor=10, u_ceil=1000000,
+                           q_occ_frac=0.01, sample_every_bp=500):
+        """collect_anchors with explicit SeedFilterParams (include/anchor.h:65-71)."""
+        p = SeedParams(f_top_frac, u_floor, u_ceil, q_occ_frac, sample_every_bp)
+        cap = max(16, len(qx) * max(1, len(rx)))
+        out = np.zeros(cap, ANCHOR_DTYPE)
+        n = self.lib.or_collect_anchors_ex(_ptr(rx), _ptr(ry), len(rx), _ptr(qx), _ptr(qy),
+                                           len(qx), C.cast(C.pointer(p), _P), _ptr(out), cap)
+        assert n >= 0
+        return out[:n].copy()
+
+    def chain_ex(self, anchors, **kw):
+        """chainAnchors with explicit ChainParams (include/anchor.h:116-126)."""
+        prm = dict(max_dist_x=5000, max_dist_y=5000, bw=500, max_skip=25, max_iter=5000,
+                   min_cnt=3, min_score=40, gap_penalty=0.01, skip_penalty=0.01)
+        prm.update(kw)
+        p = ChainParams(**prm)
+        a = np.ascontiguousarray(anchors.copy())
+        out = np.zeros(max(1, len(a)), ANCHOR_DTYPE)
+        n = self.lib.or_chain_anchors_ex(_ptr(a), len(a), C.cast(C.pointer(p), _P),
+                                         _ptr(self.lut), _ptr(out))
+        assert n >= 0
         return a, out[:n].copy()
 
     def mm2(self, ref, qry, anchors):

@@ -17,12 +17,19 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score")
 
 
-@pytest.fixture(scope="module", params=[0, 1], ids=["auto", "multiwave"])
+# auto, multi-wave only, and forced ring shapes (waves, slots per lane) so that
+# small cases also run through the multi-wave LDS exchange of the ring kernel
+POLICIES = ["auto", "multiwave", (2, 1), (3, 2), (5, 1)]
+
+
+@pytest.fixture(scope="module", params=POLICIES, ids=lambda p: p if isinstance(p, str) else f"ring{p[0]}x{p[1]}")
 def ctx(request):
     import halign
     c = halign.Context(0)
-    c.set_kernel_policy(request.param)
-    print(f"kernel policy {request.param}", file=sys.stderr, flush=True)
+    p = request.param
+    pol = {"auto": 0, "multiwave": 1}[p] if isinstance(p, str) else halign.ring_policy(*p)
+    c.set_kernel_policy(pol)
+    print(f"kernel policy {p} -> {pol}", file=sys.stderr, flush=True)
     yield c
     c.close()
 
@@ -183,4 +190,57 @@ def test_pairset_whole_pairs(ctx, oracle):
         exp, ecells = oracle.global_align_ksw2(center, q)
         sc, cig = ps.result(i)
         assert np.array_equal(cig, exp), i
+    ps.close()
+
+
+def _golden(name):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_dp_golden_fixtures(ctx):
+    """The reference's own outputs (tests/golden/ksw_cases.npz, written from
+    the compiled reference): every ksw_extz_t field in full-stats mode, score,
+    mte and CIGAR in CIGAR mode."""
+    import halign
+    g = _golden("ksw_cases.npz")
+    n = len(g["w"])
+    tasks = np.zeros(n, halign.DP_TASK_DTYPE)
+    tasks["qlen"] = np.diff(g["q_off"])
+    tasks["tlen"] = np.diff(g["t_off"])
+    tasks["w"] = g["w"]
+    tasks["flag"] = g["flag"]
+    tasks["query_off"] = g["q_off"][:-1]
+    tasks["target_off"] = g["t_off"][:-1]
+    print(f"golden ksw cases: {n}", file=sys.stderr, flush=True)
+    for full in (True, False):
+        res, cigs, ms = ctx.dp_batch(tasks, g["q"], g["t"], full_stats=full, plain=False)
+        names = FIELDS if full else ("zdropped", "score", "mte", "mte_q")
+        bad = 0
+        for i in range(n):
+            exp = dict(zip(g["ez_fields"], g["ez"][i]))
+            ok = all(int(res[i][f]) == int(exp[f]) for f in names)
+            ok = ok and np.array_equal(cigs[i], g["cigar"][g["cigar_off"][i]:g["cigar_off"][i + 1]])
+            bad += not ok
+        assert bad == 0, (full, bad)
+
+
+def test_pairset_golden_whole_pairs(ctx):
+    """20 whole config-2 pairs against the reference's CIGARs
+    (tests/golden/ksw_whole_pairs.npz)."""
+    import halign
+    from halign import synth
+    g = _golden("ksw_whole_pairs.npz")
+    n = int(g["count"])
+    center = synth.center(int(g["config"]))
+    qb, off = synth.queries(int(g["config"]), int(g["first"]), n)
+    assert np.array_equal(off, g["query_off"])
+    print(f"golden whole pairs: {n}", file=sys.stderr, flush=True)
+    ps = halign.PairSet(ctx, center, qb, off)
+    ps.run(0, n)
+    for i in range(n):
+        sc, cig = ps.result(i)
+        assert np.array_equal(cig, g["cigar"][g["cigar_off"][i]:g["cigar_off"][i + 1]]), i
     ps.close()
