@@ -27,6 +27,8 @@ int dp_wave_shape_ring(int idx);
 int dp_wave_max_groups(int idx, bool rt, int num_cu);
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
                           uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
+hipError_t consensus_launch(const uint8_t *rows, int64_t row_stride, int64_t aln_len, int64_t row_end, int64_t batch,
+                            int num_cu, uint32_t *counts, char *cons, hipStream_t stream);
 }  // namespace hal
 
 using namespace hal;
@@ -111,6 +113,7 @@ struct halign_ctx {
     std::mutex mu;
     std::string err;
     DevBuf seq, seqb, jobs, outs, counters, tb, cig;
+    DevBuf crows, ccounts, ccons;   // consensus rows (host input), counts, characters
     size_t tb_budget = 0;
     int policy = 0;   // 0: auto kernel choice, 1: multi-wave kernels only, 2+i: ring shape i
 };
@@ -361,7 +364,10 @@ int halign_init(int device, halign_ctx **out)
     }
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
-    ctx->tb_budget = (size_t)((double)free_b * 0.45);
+    // Traceback pool: one slot per resident workgroup. A config-2 pair needs a
+    // 139 MB slot and the ring kernels keep ~1,024 workgroups resident, so the
+    // pool may take 70% of free HBM (sequences, jobs and CIGARs use the rest).
+    ctx->tb_budget = (size_t)((double)free_b * 0.70);
     *out = ctx;
     return HALIGN_OK;
 }
@@ -373,6 +379,7 @@ void halign_free(halign_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     ctx->seq.release(); ctx->seqb.release(); ctx->jobs.release(); ctx->outs.release();
     ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
+    ctx->crows.release(); ctx->ccounts.release(); ctx->ccons.release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -658,6 +665,63 @@ struct halign_pairset {
 };
 
 extern "C" {
+
+int halign_consensus(halign_ctx *ctx, const char *rows, int64_t nrows, int64_t aln_len, int64_t row_stride,
+                     int32_t rows_on_device, int64_t batch, uint64_t seq_limit, uint32_t *counts_out,
+                     char *cons_out, uint64_t *num_seqs, double *kernel_ms)
+{
+    if (!ctx || !rows || nrows <= 0 || aln_len <= 0 || row_stride < aln_len || batch <= 0 || !counts_out ||
+        !cons_out)
+        return fail(ctx, HALIGN_E_ARG, "consensus: empty or inconsistent input");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    // rows counted: r % (batch + 1) != batch, the first seq_limit of them
+    const int64_t B = batch;
+    const int64_t avail = (nrows / (B + 1)) * B + std::min<int64_t>(nrows % (B + 1), B);
+    int64_t counted = avail, row_end = nrows;
+    if (seq_limit > 0 && (uint64_t)avail > seq_limit) {
+        counted = (int64_t)seq_limit;
+        const int64_t m = counted - 1;   // index among counted rows of the last one
+        row_end = (m / B) * (B + 1) + (m % B) + 1;
+    }
+    if (counted < 1) counted = 1;        // the first record is always counted
+    const int64_t stride4 = (aln_len + 3) & ~(int64_t)3;
+    const uint8_t *drows;
+    int64_t dstride;
+    if (rows_on_device) {
+        if (row_stride % 4 != 0 || row_stride < stride4 || ((uintptr_t)rows & 3))
+            return fail(ctx, HALIGN_E_ARG, "consensus: device rows need a 4-byte aligned stride >= aln_len rounded to 4");
+        drows = (const uint8_t *)rows;
+        dstride = row_stride;
+    } else {
+        HCHECK(ctx->crows.ensure((size_t)(row_end * stride4)), "alloc consensus rows");
+        HCHECK(hipMemcpy2DAsync(ctx->crows.p, (size_t)stride4, rows, (size_t)row_stride, (size_t)aln_len,
+                                (size_t)row_end, hipMemcpyHostToDevice, ctx->stream),
+               "upload consensus rows");
+        drows = ctx->crows.as<uint8_t>();
+        dstride = stride4;
+    }
+    HCHECK(ctx->ccounts.ensure((size_t)aln_len * 7 * 4), "alloc counts");
+    HCHECK(ctx->ccons.ensure((size_t)aln_len), "alloc consensus");
+    HCHECK(hipMemsetAsync(ctx->ccounts.p, 0, (size_t)aln_len * 7 * 4, ctx->stream), "clear counts");
+    HCHECK(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    HCHECK(consensus_launch(drows, dstride, aln_len, row_end, B, ctx->num_cu, ctx->ccounts.as<uint32_t>(),
+                            ctx->ccons.as<char>(), ctx->stream),
+           "launch consensus");
+    HCHECK(hipEventRecord(ctx->ev1, ctx->stream), "event");
+    HCHECK(hipMemcpyAsync(counts_out, ctx->ccounts.p, (size_t)aln_len * 7 * 4, hipMemcpyDeviceToHost, ctx->stream),
+           "download counts");
+    HCHECK(hipMemcpyAsync(cons_out, ctx->ccons.p, (size_t)aln_len, hipMemcpyDeviceToHost, ctx->stream),
+           "download consensus");
+    HCHECK(hipStreamSynchronize(ctx->stream), "consensus kernels");
+    if (kernel_ms) {
+        float ms = 0.f;
+        HCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
+        *kernel_ms = ms;
+    }
+    if (num_seqs) *num_seqs = (uint64_t)counted;
+    return HALIGN_OK;
+}
 
 int halign_pairset_create(halign_ctx *ctx, const char *center, int32_t clen, int32_t n, const char *queries,
                           const int64_t *query_offsets, halign_pairset **out)

@@ -58,6 +58,9 @@ def _load():
                                        C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.halign_pairset_result.argtypes = [_P, C.c_int32, C.POINTER(C.c_int32), _P, C.c_int32,
                                           C.POINTER(C.c_int32)]
+    lib.halign_consensus.argtypes = [_P, _P, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int64,
+                                     C.c_uint64, _P, _P, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_double)]
     lib.halign_pairset_free.argtypes = [_P]
     lib.halign_pairset_free.restype = None
     return lib
@@ -163,6 +166,28 @@ class Context:
                                                 cap, _ptr(off), _ptr(sc))
         self._check(rc, "halign_global_align_batch")
         return [cig[off[i]:off[i + 1]].copy() for i in range(n)], sc
+
+
+    def consensus(self, rows, batch=4096, seq_limit=0, device_ptr=None, row_stride=None):
+        """Column counts (aln_len x 7: A C G T U N -), consensus bytes and the
+        number of rows counted (generateConsensusSequence's counting core).
+        rows: (nrows, aln_len) uint8 host array, or its shape when device_ptr
+        (an integer device address) holds the rows with row_stride bytes."""
+        if device_ptr is None:
+            rows = np.ascontiguousarray(rows, np.uint8)
+            nrows, aln = rows.shape
+            ptr, stride, on_dev = _ptr(rows), aln, 0
+        else:
+            nrows, aln = rows
+            ptr, stride, on_dev = C.c_void_p(device_ptr), row_stride, 1
+        counts = np.zeros((aln, 7), np.uint32)
+        cons = np.zeros(aln, np.uint8)
+        n = C.c_uint64(0)
+        ms = C.c_double(0)
+        rc = self.lib.halign_consensus(self.h, ptr, nrows, aln, stride, on_dev, batch, seq_limit,
+                                       _ptr(counts), _ptr(cons), C.byref(n), C.byref(ms))
+        self._check(rc, "halign_consensus")
+        return counts, cons.tobytes(), n.value, ms.value
 
 
 class PairSet:

@@ -113,6 +113,21 @@ int halign_global_align_batch(halign_ctx *ctx, int32_t n, const char *const *ref
                               uint32_t *cigar_out, int64_t cigar_cap, int64_t *offsets,
                               int32_t *scores);
 
+/* Column counts and consensus of an alignment (the counting core of
+ * consensus::generateConsensusSequence, reference src/consensus/consensus.cpp:
+ * 611-703, pickConsensusChar :18-35, SiteCount include/consensus.h:78-99).
+ * rows: nrows rows of aln_len characters, row_stride bytes apart, on the host
+ * (rows_on_device = 0) or on the device (rows_on_device != 0: 4-byte aligned,
+ * row_stride a multiple of 4 and >= aln_len rounded up to 4). Rows are taken
+ * as the reference reads them: in batches of `batch` records (4096 there) with
+ * the record after each full batch dropped, and at most seq_limit counted
+ * rows (0 = all). counts_out: aln_len x 7 uint32 (A C G T U N -), cons_out:
+ * aln_len characters (no terminator), num_seqs: rows counted. nrows <= 0 is
+ * HALIGN_E_ARG (the reference throws on empty input). */
+int halign_consensus(halign_ctx *ctx, const char *rows, int64_t nrows, int64_t aln_len, int64_t row_stride,
+                     int32_t rows_on_device, int64_t batch, uint64_t seq_limit, uint32_t *counts_out,
+                     char *cons_out, uint64_t *num_seqs, double *kernel_ms);
+
 /* Device-resident workload: one center (target) and n queries kept in HBM,
  * aligned pairwise as whole pairs (globalAlignKSW2 semantics). Used for the
  * kernel-only DP throughput measurement and by the serving loop. */
