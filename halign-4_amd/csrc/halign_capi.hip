@@ -13,6 +13,7 @@
 
 #include "../../include/halign_hip.h"
 #include "dp_common.h"
+#include "host_common.h"
 
 namespace hal {
 int dp_num_shapes();
@@ -81,59 +82,12 @@ inline bool predict_zdrop(int qlen, int tlen, int w)
     return false;
 }
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t n)
-    {
-        if (n <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) cap = n;
-        return e;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T> T *as() const { return (T *)p; }
-};
 
 }  // namespace
 
-struct halign_ctx {
-    int device = 0;
-    int num_cu = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::mutex mu;
-    std::string err;
-    DevBuf seq, seqb, jobs, outs, counters, tb, cig;
-    DevBuf crows, ccounts, ccons;   // consensus rows (host input), counts, characters
-    size_t tb_budget = 0;
-    int policy = 0;   // 0: auto kernel choice, 1: multi-wave kernels only, 2+i: ring shape i
-};
 
 namespace {
 
-int fail(halign_ctx *ctx, int code, const char *what, hipError_t e = hipSuccess)
-{
-    char buf[512];
-    if (e != hipSuccess) snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
-    else snprintf(buf, sizeof buf, "%s", what);
-    if (ctx) ctx->err = buf;
-    return code;
-}
-
-#define HCHECK(expr, what)                                           \
-    do {                                                             \
-        hipError_t e_ = (expr);                                      \
-        if (e_ != hipSuccess) return fail(ctx, HALIGN_E_HIP, what, e_); \
-    } while (0)
 
 // Host-side description of one DP job before its device pointers are known.
 struct Spec {
@@ -380,6 +334,10 @@ void halign_free(halign_ctx *ctx)
     ctx->seq.release(); ctx->seqb.release(); ctx->jobs.release(); ctx->outs.release();
     ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
     ctx->crows.release(); ctx->ccounts.release(); ctx->ccons.release();
+    for (hal::DevBuf *b : {&ctx->s_seqs, &ctx->s_off, &ctx->s_woff, &ctx->s_moff, &ctx->s_codes, &ctx->s_nmask,
+                           &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list, &ctx->s_wsh,
+                           &ctx->s_wsp})
+        b->release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);

@@ -1,0 +1,72 @@
+// host_common.h -- host-side pieces shared by the C-ABI implementation files:
+// the context (device, stream, events, reusable device buffers), the error
+// helper and the HIP status check.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <mutex>
+#include <string>
+
+#include "../../include/halign_hip.h"
+
+namespace hal {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n)
+    {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+}  // namespace hal
+
+struct halign_ctx {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+    std::string err;
+    hal::DevBuf seq, seqb, jobs, outs, counters, tb, cig;
+    hal::DevBuf crows, ccounts, ccons;   // consensus rows (host input), counts, characters
+    // seeding: sequences, offsets, packed codes and N masks, hits, counts, status, workspace
+    hal::DevBuf s_seqs, s_off, s_woff, s_moff, s_codes, s_nmask, s_hoff, s_hx, s_hy, s_cnt, s_st, s_list, s_wsh, s_wsp;
+    size_t tb_budget = 0;
+    int policy = 0;        // 0: auto kernel choice, 1: multi-wave kernels only, 2+i: ring shape i
+    int seed_policy = 0;   // 0: wave form where it applies, 1: sequential form only
+};
+
+namespace hal {
+
+inline int fail(halign_ctx *ctx, int code, const char *what, hipError_t e = hipSuccess)
+{
+    char buf[512];
+    if (e != hipSuccess) snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    else snprintf(buf, sizeof buf, "%s", what);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define HCHECK(expr, what)                                           \
+    do {                                                             \
+        hipError_t e_ = (expr);                                      \
+        if (e_ != hipSuccess) return fail(ctx, HALIGN_E_HIP, what, e_); \
+    } while (0)
+
+}  // namespace hal

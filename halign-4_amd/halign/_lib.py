@@ -61,6 +61,9 @@ def _load():
     lib.halign_consensus.argtypes = [_P, _P, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int64,
                                      C.c_uint64, _P, _P, C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_double)]
+    lib.halign_set_seed_policy.argtypes = [_P, C.c_int]
+    lib.halign_extract_minimizers.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.c_int32, C.c_int32,
+                                              C.c_int32, _P, _P, C.c_int64, _P, _P]
     lib.halign_pairset_free.argtypes = [_P]
     lib.halign_pairset_free.restype = None
     return lib
@@ -188,6 +191,29 @@ class Context:
                                        _ptr(counts), _ptr(cons), C.byref(n), C.byref(ms))
         self._check(rc, "halign_consensus")
         return counts, cons.tobytes(), n.value, ms.value
+
+
+    def set_seed_policy(self, policy):
+        """0 = minimizer wave form where it applies, 1 = sequential form only."""
+        self._check(self.lib.halign_set_seed_policy(self.h, int(policy)), "set_seed_policy")
+        self._seed_policy = int(policy)
+
+    def extract_minimizers(self, seqs, k=15, w=10, non_canonical=True):
+        """extractMinimizer over a list of byte strings. Returns a list of
+        (x, y) uint64 arrays (MinimizerHit records) and the per-sequence status."""
+        n = len(seqs)
+        blob = b"".join(seqs)
+        off = np.zeros(n + 1, np.int64)
+        off[1:] = np.cumsum([len(x) for x in seqs])
+        cap = max(1, int(sum(max(len(x) - k + 1, 0) for x in seqs)))
+        hx = np.zeros(cap, np.uint64)
+        hy = np.zeros(cap, np.uint64)
+        hoff = np.zeros(n + 1, np.int64)
+        st = np.zeros(max(n, 1), np.int32)
+        rc = self.lib.halign_extract_minimizers(self.h, n, blob, _ptr(off), k, w, int(non_canonical),
+                                                _ptr(hx), _ptr(hy), cap, _ptr(hoff), _ptr(st))
+        self._check(rc, "halign_extract_minimizers")
+        return [(hx[hoff[i]:hoff[i + 1]].copy(), hy[hoff[i]:hoff[i + 1]].copy()) for i in range(n)], st[:n]
 
 
 class PairSet:

@@ -113,6 +113,27 @@ int halign_global_align_batch(halign_ctx *ctx, int32_t n, const char *const *ref
                               uint32_t *cigar_out, int64_t cigar_cap, int64_t *offsets,
                               int32_t *scores);
 
+/* Minimizer kernel choice: 0 = the wave form wherever it applies (the
+ * sequential form takes the rest), 1 = the sequential form only. Results are
+ * identical; used to test both forms. No reference counterpart. */
+int halign_set_seed_policy(halign_ctx *ctx, int policy);
+
+/* minimizer::extractMinimizer (reference src/seed/minimizer.cpp:103-177,
+ * declared include/seed.h:283-286) for n sequences: sequence s is
+ * seqs[offsets[s] .. offsets[s+1]) (ASCII). Hits are MinimizerHit records
+ * (include/seed.h:206-265): x = hash56 << 8 | span, y = (rid | strand<<31) << 32
+ * | pos, rid 0; sequence s's hits are x_out/y_out[hit_offsets[s] ..
+ * hit_offsets[s+1]) (hit_offsets has n+1 entries; cap = capacity of the
+ * outputs). status_out (optional, n entries): bit 0 = computed by the exact
+ * sequential form (the reference's queue overwrote its front slot, canonical
+ * k-mers, or w > 32), bit 1 = the reference indexes past its ring buffer on
+ * this input (undefined behaviour there; the output follows a larger buffer),
+ * bit 2 = even the larger buffer overflowed (output truncated). k, w guards as
+ * the reference: k = 0, w = 0, len < k, k > 31 or w >= 256 give no hits. */
+int halign_extract_minimizers(halign_ctx *ctx, int32_t n, const char *seqs, const int64_t *offsets, int32_t k,
+                              int32_t w, int32_t non_canonical, uint64_t *x_out, uint64_t *y_out, int64_t cap,
+                              int64_t *hit_offsets, int32_t *status_out);
+
 /* Column counts and consensus of an alignment (the counting core of
  * consensus::generateConsensusSequence, reference src/consensus/consensus.cpp:
  * 611-703, pickConsensusChar :18-35, SiteCount include/consensus.h:78-99).
