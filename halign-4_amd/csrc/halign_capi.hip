@@ -13,6 +13,7 @@
 
 #include "../../include/halign_hip.h"
 #include "dp_common.h"
+#include "dp_host.h"
 #include "host_common.h"
 
 namespace hal {
@@ -34,73 +35,9 @@ hipError_t consensus_launch(const uint8_t *rows, int64_t row_stride, int64_t aln
 
 using namespace hal;
 
-namespace {
-
-// ScoreChar2Idx (reference include/align.h:72-89)
-inline uint8_t score_code(unsigned char c)
-{
-    switch (c) {
-    case 'A': case 'a': return 0;
-    case 'C': case 'c': return 1;
-    case 'G': case 'g': return 2;
-    case 'T': case 't': return 3;
-    default: return 4;
-    }
-}
-
-// align::auto_band(qlen=ref_len, tlen=query_len) (reference include/align.h:159-172)
-inline int auto_band(int ref_len, int query_len)
-{
-    const int d = std::abs(ref_len - query_len);
-    const int mx = std::max(ref_len, query_len);
-    if ((double)d / (double)mx > 0.5) return -1;
-    return 200 + (int)(0.1 * (double)(ref_len + query_len / 2));
-}
-
-inline int eff_w(int w, int qlen, int tlen) { return w < 0 ? std::max(qlen, tlen) : w; }
-
-// ksw2's n_col_ (reference src/align/ksw2_extz2_sse.c:75-76)
-inline int n_col(int qlen, int tlen, int w)
-{
-    int n = std::min(qlen, tlen);
-    n = std::min(n, w + 1);
-    return (n + 15) / 16 + 1;
-}
-
-// Band overflow ("zdropped" break, src/align/ksw2_extz2_sse.c:111-114) happens
-// iff some anti-diagonal has st > en. Of the ways st > en can hold, two grow
-// monotonically with r (tested on the last anti-diagonal) and one needs w = 0:
-// (r-w+1)>>1 > (r+w)>>1 at every odd r, so from r = 1 on. The others cannot
-// hold for qlen, tlen >= 1 and w >= 0 (checked exhaustively over small sizes
-// by tests/test_host_logic.py).
-inline bool predict_zdrop(int qlen, int tlen, int w)
-{
-    if (w == 0 && qlen + tlen - 1 >= 2) return true;
-    const int r = qlen + tlen - 2;
-    if (r - qlen + 1 > (r + w) >> 1) return true;
-    if ((r - w + 1) >> 1 > tlen - 1) return true;
-    return false;
-}
 
 
-}  // namespace
-
-
-namespace {
-
-
-// Host-side description of one DP job before its device pointers are known.
-struct Spec {
-    int64_t tgt_elem, qry_elem;   // element offsets of the even copies (pad included before)
-    int64_t tgt_odd_elem, qry_odd_elem;
-    int64_t tgt_b, qry_b;         // byte offsets of the 0..4 code copies (pad included before)
-    int32_t tlen, qlen, w, flags;
-    int64_t cig_off;              // uint32 offset into the device cigar buffer
-    int32_t cig_cap;
-    int shape;
-    bool stats;
-    size_t tb_bytes;
-};
+namespace hal {
 
 void spread_target(const uint8_t *codes, int len, uint16_t *even, uint16_t *odd)
 {
@@ -293,7 +230,7 @@ bool fill_spec_shape(Spec &s, bool full_stats, int policy)
     return s.shape >= 0;
 }
 
-}  // namespace
+}  // namespace hal
 
 extern "C" {
 
@@ -606,6 +543,16 @@ __global__ void prep_queries(const char *__restrict__ raw, const int64_t *__rest
 }
 
 }  // namespace
+
+hipError_t hal::prep_queries_launch(const char *raw, const int64_t *qoff, const int64_t *region,
+                                    const int64_t *bregion, int first, int count, uint16_t *seq, uint8_t *seqb,
+                                    hipStream_t stream)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(prep_queries, dim3(64, count), dim3(256), 0, stream, raw, qoff, region, bregion, first, count,
+                       seq, seqb);
+    return hipGetLastError();
+}
 
 struct halign_pairset {
     halign_ctx *ctx = nullptr;

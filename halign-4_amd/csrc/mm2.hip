@@ -1,0 +1,426 @@
+// mm2.hip -- align::globalAlignMM2 (src/align/align.cpp:189-305) on the
+// device, for many queries against one center:
+//
+//  * mm2_segments: one lane per query turns its chain into the 2n+1 segments
+//    of the reference (first segment LEFT-aligned, the others RIGHT; bounds
+//    clipped to the sequences; each segment starts where the previous one
+//    ended) and classifies them: empty, one side empty, small DP (lane DP),
+//    large DP or band overflow (ring / full-statistics kernels, launched by the
+//    host). A query without a chain gets one whole-pair segment whose CIGAR is
+//    taken as it is (align.cpp:204-206).
+//  * lane_dp: one lane per small segment: globalAlignKSW2 (align.cpp:22-65) =
+//    ksw_extz2_sse (src/align/ksw2_extz2_sse.c:23-304, SSE2 semantics incl.
+//    the 16-rounded margins) and ksw_backtrack (include/ksw2.h:129-161),
+//    scalar, with byte arrays in a per-lane workspace.
+//  * mm2_assemble: one lane per query appends the segment CIGARs with
+//    appendCigar's merge (src/align/cigar.cpp:302-329; the ring and
+//    full-statistics kernels store a CIGAR last op first), replacing a segment
+//    whose CIGAR does not consume it exactly by I(query) D(ref)
+//    (align.cpp:243-261).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dp_common.h"
+#include "pipeline_common.h"
+
+namespace hal {
+
+// align::auto_band(qlen = ref length, tlen = query length) (include/align.h:159-172)
+__device__ __forceinline__ int auto_band_dev(int ref_len, int query_len)
+{
+    const int d = ref_len > query_len ? ref_len - query_len : query_len - ref_len;
+    const int mx = ref_len > query_len ? ref_len : query_len;
+    if ((double)d / (double)mx > 0.5) return -1;
+    return 200 + (int)(0.1 * (double)(ref_len + query_len / 2));
+}
+
+// band overflow ("zdropped") before the last anti-diagonal
+__device__ __forceinline__ bool overflows(int qlen, int tlen, int w)
+{
+    if (w == 0 && qlen + tlen - 1 >= 2) return true;
+    const int r = qlen + tlen - 2;
+    return (r - qlen + 1 > ((r + w) >> 1)) || (((r - w + 1) >> 1) > tlen - 1);
+}
+
+__device__ __forceinline__ uint32_t classify(uint32_t srl, uint32_t sql)
+{
+    if (srl == 0 && sql == 0) return kSegEmpty;
+    if (srl == 0 || sql == 0) return kSegPure;
+    int w = auto_band_dev((int)srl, (int)sql);
+    if (w < 0) w = srl > sql ? (int)srl : (int)sql;
+    if (overflows((int)sql, (int)srl, w)) return kSegHost;
+    int ncol = (int)(srl < sql ? srl : sql);
+    ncol = ((ncol < w + 1 ? ncol : w + 1) + 15) / 16 + 1;
+    const int64_t tb = (int64_t)(srl + sql - 1) * ncol * 16;
+    if (srl <= (uint32_t)kLaneMaxT && tb <= kLaneMaxTb) return kSegLane;
+    return kSegHost;
+}
+
+__device__ __forceinline__ uint32_t seg_cig_cap(uint32_t cls, uint32_t srl, uint32_t sql)
+{
+    if (cls == kSegEmpty) return 0;
+    if (cls == kSegPure) return 1;
+    return srl + sql + 1;
+}
+
+// One lane per query. chain_idx/chain_cnt from chain_queries; segments of
+// query q go to segs[soff[q] ..]; nseg[q] receives their count; caps[s] the
+// CIGAR capacity of segment s.
+__global__ void mm2_segments(const AnchorRec *__restrict__ anchors, const uint32_t *__restrict__ chain_idx,
+                             const int64_t *__restrict__ aoff, const int32_t *__restrict__ chain_cnt,
+                             const int64_t *__restrict__ qlens, uint32_t rlen, const int64_t *__restrict__ soff,
+                             int nq, SegRec *__restrict__ segs, int32_t *__restrict__ nseg, int64_t *__restrict__ caps)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const uint32_t qlen = (uint32_t)qlens[q];
+    const int n = chain_cnt[q];
+    SegRec *out = segs + soff[q];
+    int64_t *cap = caps + soff[q];
+    int k = 0;
+    auto emit = [&](uint32_t rs, uint32_t re, uint32_t qs, uint32_t qe, uint32_t flags) {
+        SegRec s;
+        s.rs = rs; s.re = re; s.qs = qs; s.qe = qe;
+        s.flags = flags; s.query = (uint32_t)q; s.pad = 0; s.cig_off = 0;
+        s.cls = classify(re - rs, qe - qs);
+        out[k] = s;
+        cap[k] = seg_cig_cap(s.cls, re - rs, qe - qs);
+        ++k;
+    };
+    if (n == 0) {
+        emit(0, rlen, 0, qlen, kSegRight | kSegRaw);
+        nseg[q] = k;
+        return;
+    }
+    uint32_t ref_pos = 0, qry_pos = 0;
+    auto seg = [&](uint64_t rs, uint64_t re, uint64_t qs, uint64_t qe, uint32_t flags) {
+        if (rs > rlen) rs = rlen;
+        if (re > rlen) re = rlen;
+        if (qs > qlen) qs = qlen;
+        if (qe > qlen) qe = qlen;
+        if (re < rs) re = rs;
+        if (qe < qs) qe = qs;
+        emit((uint32_t)rs, (uint32_t)re, (uint32_t)qs, (uint32_t)qe, flags);
+        ref_pos = (uint32_t)re;
+        qry_pos = (uint32_t)qe;
+    };
+    const uint32_t *ch = chain_idx + aoff[q];
+    {
+        const AnchorRec a0 = anchors[ch[0]];
+        seg(ref_pos, a0.pos_ref, qry_pos, a0.pos_qry, 0);   // first segment: LEFT (align.cpp:194-195)
+    }
+    for (int i = 0; i < n; ++i) {
+        const AnchorRec a = anchors[ch[i]];
+        seg(ref_pos, (uint64_t)a.pos_ref + a.span, qry_pos, (uint64_t)a.pos_qry + a.span, kSegRight);
+        if (i + 1 < n) {
+            const AnchorRec b = anchors[ch[i + 1]];
+            seg(ref_pos, b.pos_ref, qry_pos, b.pos_qry, kSegRight);
+        }
+    }
+    seg(ref_pos, rlen, qry_pos, qlen, kSegRight);
+    nseg[q] = k;
+}
+
+__global__ void seg_set_offsets(SegRec *__restrict__ segs, const int64_t *__restrict__ cig_off, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) segs[i].cig_off = cig_off[i];
+}
+
+// indices of the segments the host aligns (kSegHost), appended with an atomic
+__global__ void seg_collect_host(const SegRec *__restrict__ segs, int64_t n, int64_t *__restrict__ list,
+                                 unsigned long long *__restrict__ count)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && segs[i].cls == kSegHost) {
+        const unsigned long long at = atomicAdd(count, 1ull);
+        list[at] = i;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// lane DP: ksw_extz2_sse, CIGAR mode (no overflow: those segments go to the
+// host kernels), dna5 matrix, q = 6, e = 2.
+
+__device__ __forceinline__ int s8(uint32_t v) { return (int)(int8_t)(uint8_t)v; }
+
+__device__ __forceinline__ int8_t dna5(uint32_t a, uint32_t b)
+{
+    if (a > 3 || b > 3) return 0;
+    return a == b ? 5 : -4;
+}
+
+struct LaneCig {
+    uint32_t *c;
+    int n, cap;
+    __device__ void push(uint32_t op, int len)
+    {
+        if (n > 0 && (c[n - 1] & 0xf) == op) {
+            c[n - 1] += (uint32_t)len << 4;
+        } else {
+            if (n < cap) c[n] = (uint32_t)len << 4 | op;
+            ++n;
+        }
+    }
+};
+
+// tgt[t]: target codes; qrv(j): query code j read from the reversed copy
+// (qrev[-j]); ws: workspace of 5*16*ceil(tlen/16) + (qlen+tlen-1)*n_col_*16 bytes.
+__device__ int lane_extz2(const uint8_t *__restrict__ tgt, int tlen, const uint8_t *__restrict__ qrev, int qlen, int w,
+                          bool right, uint8_t *__restrict__ ws, uint32_t *__restrict__ cig, int cap)
+{
+    constexpr int q = 6, e = 2, qe = q + e;   // KSW2AlignConfig q, e (include/align.h:126-135)
+    const uint32_t qe2 = 2 * qe, max_sc = 5 + 2 * qe;
+    if (w < 0) w = tlen > qlen ? tlen : qlen;
+    const int tl16 = (tlen + 15) / 16 * 16;
+    int n_col_ = qlen < tlen ? qlen : tlen;
+    n_col_ = ((n_col_ < w + 1 ? n_col_ : w + 1) + 15) / 16 + 1;
+    const int ncol = n_col_ * 16;
+    const int ndiag = qlen + tlen - 1;
+    uint8_t *u = ws, *v = u + tl16, *x = v + tl16, *y = x + tl16, *s = y + tl16;
+    uint8_t *p = s + tl16;
+    for (int t = 0; t < 5 * tl16; ++t) ws[t] = 0;
+    int last_st = -1, last_en = -1;
+    for (int r = 0; r < ndiag; ++r) {
+        int st = 0, en = tlen - 1;
+        if (st < r - qlen + 1) st = r - qlen + 1;
+        if (en > r) en = r;
+        if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
+        if (en > (r + w) >> 1) en = (r + w) >> 1;
+        const int st0 = st, en0 = en;   // st <= en: overflowing segments never come here
+        st = st / 16 * 16;
+        en = (en + 16) / 16 * 16 - 1;
+        uint32_t x1, v1;
+        if (st > 0) {
+            if (st - 1 >= last_st && st - 1 <= last_en) { x1 = x[st - 1]; v1 = v[st - 1]; }
+            else x1 = v1 = 0;
+        } else {
+            x1 = 0;
+            v1 = r ? (uint32_t)q : 0;
+        }
+        if (en >= r) { y[r] = 0; u[r] = r ? (uint8_t)q : 0; }
+        for (int t = st0; t <= en0; ++t) s[t] = (uint8_t)dna5(tgt[t], qrev[-(r - t)]);
+        uint8_t *pr = p + (size_t)r * ncol;
+        uint32_t xprev = x1, vprev = v1;
+        for (int t = st; t <= en; ++t) {
+            const uint32_t xt1 = xprev, vt1 = vprev;
+            xprev = x[t];
+            vprev = v[t];
+            uint32_t z = (s[t] + qe2) & 0xff;
+            uint32_t a = (xt1 + vt1) & 0xff;
+            const uint32_t ut = u[t];
+            uint32_t b = (y[t] + ut) & 0xff;
+            uint32_t d;
+            if (right) {
+                d = s8(z) > s8(a) ? 0 : 1;
+                if (!(s8(z) > 0)) z = 0;
+                z = z > a ? z : a;
+                d = s8(z) > s8(b) ? d : 2;
+            } else {
+                d = s8(a) > s8(z) ? 1 : 0;
+                if (!(s8(z) > 0)) z = 0;
+                z = z > a ? z : a;
+                d = s8(b) > s8(z) ? 2 : d;
+            }
+            z = z > b ? z : b;
+            z = z < max_sc ? z : max_sc;
+            u[t] = (uint8_t)(z - vt1);
+            v[t] = (uint8_t)(z - ut);
+            z = (z - q) & 0xff;
+            a = (a - z) & 0xff;
+            b = (b - z) & 0xff;
+            if (right) {
+                x[t] = s8(a) < 0 ? 0 : (uint8_t)a;
+                d |= s8(a) < 0 ? 0 : 0x08;
+                y[t] = s8(b) < 0 ? 0 : (uint8_t)b;
+                d |= s8(b) < 0 ? 0 : 0x10;
+            } else {
+                x[t] = s8(a) > 0 ? (uint8_t)a : 0;
+                d |= s8(a) > 0 ? 0x08 : 0;
+                y[t] = s8(b) > 0 ? (uint8_t)b : 0;
+                d |= s8(b) > 0 ? 0x10 : 0;
+            }
+            pr[t - st] = (uint8_t)d;
+        }
+        last_st = st;
+        last_en = en;
+    }
+    // backtrack from (tlen-1, qlen-1), is_rot = 1 (include/ksw2.h:129-161)
+    LaneCig cb{cig, 0, cap};
+    int i = tlen - 1, j = qlen - 1, state = 0;
+    while (i >= 0 && j >= 0) {
+        const int r = i + j;
+        int st0 = 0, en0 = tlen - 1;
+        if (st0 < r - qlen + 1) st0 = r - qlen + 1;
+        if (en0 > r) en0 = r;
+        if (st0 < (r - w + 1) >> 1) st0 = (r - w + 1) >> 1;
+        if (en0 > (r + w) >> 1) en0 = (r + w) >> 1;
+        const int off = st0 / 16 * 16, off_end = (en0 + 16) / 16 * 16 - 1;
+        int force = -1;
+        if (i < off) force = 2;
+        if (i > off_end) force = 1;
+        const uint32_t tmp = force < 0 ? p[(size_t)r * ncol + (i - off)] : 0;
+        if (state == 0) state = tmp & 7;
+        else if (!((tmp >> (state + 2)) & 1)) state = 0;
+        if (state == 0) state = tmp & 7;
+        if (force >= 0) state = force;
+        if (state == 0) { cb.push(0, 1); --i; --j; }
+        else if (state == 1) { cb.push(2, 1); --i; }
+        else { cb.push(1, 1); --j; }
+    }
+    if (i >= 0) cb.push(2, i + 1);
+    if (j >= 0) cb.push(1, j + 1);
+    const int n = cb.n < cap ? cb.n : cap;
+    for (int k = 0; k < n >> 1; ++k) {
+        const uint32_t tt = cig[k];
+        cig[k] = cig[n - 1 - k];
+        cig[n - 1 - k] = tt;
+    }
+    return cb.n;
+}
+
+// Lanes walk the segment list with a grid stride and align the kLaneDP ones.
+// center: center codes (index 0 = base 0); qrev_base[query] points at the
+// reversed code copy of the query, so query base j is qrev_base[len-1-j].
+__global__ void lane_dp(const SegRec *__restrict__ segs, int64_t n, const uint8_t *__restrict__ center,
+                        const int64_t *__restrict__ qrev_off, const int64_t *__restrict__ qlens,
+                        const uint8_t *__restrict__ qrev, uint8_t *__restrict__ wsall, int64_t ws_lane,
+                        uint32_t *__restrict__ cigs, int32_t *__restrict__ ncig)
+{
+    const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    uint8_t *ws = wsall + lane * ws_lane;
+    for (int64_t i = lane; i < n; i += stride) {
+        const SegRec sg = segs[i];
+        if (sg.cls != kSegLane) continue;
+        const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
+        const int64_t L = qlens[sg.query];
+        // query base qs + j sits at reversed index L-1-qs-j
+        const uint8_t *qr = qrev + qrev_off[sg.query] + (L - 1 - sg.qs);
+        const int w = auto_band_dev(srl, sql);
+        ncig[i] = lane_extz2(center + sg.rs, srl, qr, sql, w, (sg.flags & kSegRight) != 0, ws, cigs + sg.cig_off,
+                             srl + sql + 1);
+    }
+}
+
+// CIGAR consumption (src/align/cigar.cpp:332-365): ref = M D N = X, query = M I S = X
+__device__ __forceinline__ void consumed(const uint32_t *c, int n, int64_t &rl, int64_t &ql)
+{
+    rl = ql = 0;
+    for (int i = 0; i < n; ++i) {
+        const uint32_t op = c[i] & 0xf, len = c[i] >> 4;
+        if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rl += len;
+        if (op == 0 || op == 1 || op == 4 || op == 7 || op == 8) ql += len;
+    }
+}
+
+struct AppendCig {
+    uint32_t *c;
+    int64_t n, cap;
+    __device__ void add(uint32_t v)   // appendCigar: skip empty ops, merge equal neighbours
+    {
+        if ((v >> 4) == 0) return;
+        if (n > 0 && (c[n - 1] & 0xf) == (v & 0xf)) c[n - 1] += v & ~0xfu;
+        else {
+            if (n < cap) c[n] = v;
+            ++n;
+        }
+    }
+};
+
+// One lane per query: the final CIGAR at out[ooff[q] ..], its length in
+// nout[q] (-1 when the total does not consume both sequences: the caller then
+// aligns the whole pair, align.cpp:296-302).
+__global__ void mm2_assemble(const SegRec *__restrict__ segs, const int64_t *__restrict__ soff,
+                             const int32_t *__restrict__ nseg, const uint32_t *__restrict__ cigs,
+                             const int32_t *__restrict__ ncig, const int64_t *__restrict__ qlens, uint32_t rlen,
+                             const int64_t *__restrict__ ooff, int nq, uint32_t *__restrict__ out,
+                             int64_t *__restrict__ nout)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int64_t qlen = qlens[q];
+    AppendCig ac{out + ooff[q], 0, (int64_t)rlen + qlen + 1};
+    const SegRec *sg = segs + soff[q];
+    const int ns = nseg[q];
+    for (int k = 0; k < ns; ++k) {
+        const SegRec s = sg[k];
+        const uint32_t srl = s.re - s.rs, sql = s.qe - s.qs;
+        if (s.cls == kSegEmpty) continue;
+        if (s.cls == kSegPure) {   // globalAlignKSW2 on an empty side (align.cpp:26-34)
+            ac.add(srl == 0 ? (sql << 4 | 1u) : (srl << 4 | 2u));
+            continue;
+        }
+        const int nc = ncig[soff[q] + k];
+        const uint32_t *c = cigs + s.cig_off;
+        if (!(s.flags & kSegRaw)) {
+            int64_t rl, ql;
+            consumed(c, nc, rl, ql);
+            if (rl != srl || ql != sql) {   // forced I(query) D(ref)
+                if (sql > 0) ac.add(sql << 4 | 1u);
+                if (srl > 0) ac.add(srl << 4 | 2u);
+                continue;
+            }
+        }
+        if (s.cls == kSegHost) {   // the ring / full-statistics kernels store the ops last to first
+            for (int i = nc - 1; i >= 0; --i) ac.add(c[i]);
+        } else {
+            for (int i = 0; i < nc; ++i) ac.add(c[i]);
+        }
+    }
+    int64_t rl = 0, ql = 0;
+    const uint32_t *o = out + ooff[q];
+    consumed(o, (int)(ac.n < ac.cap ? ac.n : ac.cap), rl, ql);
+    const bool raw = ns == 1 && (sg[0].flags & kSegRaw);
+    nout[q] = (!raw && (rl != (int64_t)rlen || ql != qlen)) ? -1 : ac.n;
+}
+
+hipError_t mm2_segments_launch(const AnchorRec *anchors, const uint32_t *chain_idx, const int64_t *aoff,
+                               const int32_t *chain_cnt, const int64_t *qlens, uint32_t rlen, const int64_t *soff,
+                               int nq, SegRec *segs, int32_t *nseg, int64_t *caps, hipStream_t stream)
+{
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mm2_segments, dim3((nq + 63) / 64), dim3(64), 0, stream, anchors, chain_idx, aoff, chain_cnt,
+                       qlens, rlen, soff, nq, segs, nseg, caps);
+    return hipGetLastError();
+}
+
+hipError_t seg_offsets_launch(SegRec *segs, const int64_t *cig_off, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(seg_set_offsets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, segs, cig_off, n);
+    return hipGetLastError();
+}
+
+hipError_t seg_collect_launch(const SegRec *segs, int64_t n, int64_t *list, unsigned long long *count,
+                              hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(seg_collect_host, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, segs, n, list,
+                       count);
+    return hipGetLastError();
+}
+
+hipError_t lane_dp_launch(const SegRec *segs, int64_t n, const uint8_t *center, const int64_t *qrev_off,
+                          const int64_t *qlens, const uint8_t *qrev, uint8_t *ws, int64_t ws_lane, int64_t lanes,
+                          uint32_t *cigs, int32_t *ncig, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (lanes + 63) / 64;
+    hipLaunchKernelGGL(lane_dp, dim3((unsigned)blocks), dim3(64), 0, stream, segs, n, center, qrev_off, qlens, qrev,
+                       ws, ws_lane, cigs, ncig);
+    return hipGetLastError();
+}
+
+hipError_t mm2_assemble_launch(const SegRec *segs, const int64_t *soff, const int32_t *nseg, const uint32_t *cigs,
+                               const int32_t *ncig, const int64_t *qlens, uint32_t rlen, const int64_t *ooff, int nq,
+                               uint32_t *out, int64_t *nout, hipStream_t stream)
+{
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mm2_assemble, dim3((nq + 63) / 64), dim3(64), 0, stream, segs, soff, nseg, cigs, ncig, qlens,
+                       rlen, ooff, nq, out, nout);
+    return hipGetLastError();
+}
+
+}  // namespace hal

@@ -1,0 +1,577 @@
+// pipeline_api.hip -- host side of the per-query path against a resident
+// center (RefAligner::globalAlign, src/align/ref_aligner.cpp:91-125, with one
+// center): minimizers -> collect_anchors -> chainAnchors -> globalAlignMM2,
+// all on the device, and the C ABI halign_center_*.
+//
+// Host round trips per batch are small: per-query anchor totals, per-query
+// chain lengths, the size of the segment CIGAR buffer, the records of the
+// large segments (aligned by the ring / full-statistics kernels), and the
+// result.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+#include <string>
+#include <vector>
+
+#include "dp_host.h"
+#include "pipeline_common.h"
+
+namespace hal {
+
+int seed_device(halign_ctx *ctx, int n, const std::vector<int64_t> &off, int k, int w, int non_canonical,
+                std::vector<int64_t> &hit_off, std::vector<int64_t> &counts, std::vector<int32_t> &status);
+hipError_t anchors_launch_count(const CenterIndexDev &ci, const uint64_t *qx, const uint64_t *qy,
+                                const int64_t *hit_off, const int64_t *hit_cnt, int nq, const SeedFilter &f,
+                                int groups, uint32_t *scratch, int32_t *hidx, uint32_t *nanch, int64_t *totals,
+                                hipStream_t stream);
+hipError_t anchors_launch_write(const CenterIndexDev &ci, const uint64_t *qx, const uint64_t *qy,
+                                const int64_t *hit_off, const int64_t *hit_cnt, int nq, int groups,
+                                const int32_t *hidx, const uint32_t *nanch, const int64_t *aoff, AnchorRec *out,
+                                hipStream_t stream);
+hipError_t anchor_keys_launch(const AnchorRec *anchors, int64_t n, uint64_t *keys, uint32_t *vals,
+                              hipStream_t stream);
+hipError_t chain_launch(const AnchorRec *anchors, const uint32_t *order, const int64_t *aoff, const int64_t *na,
+                        int nq, const ChainParamsDev &cp, const float *lut, int32_t *f, int32_t *p, int32_t *t,
+                        uint32_t *chain_idx, int32_t *chain_cnt, hipStream_t stream);
+hipError_t mm2_segments_launch(const AnchorRec *anchors, const uint32_t *chain_idx, const int64_t *aoff,
+                               const int32_t *chain_cnt, const int64_t *qlens, uint32_t rlen, const int64_t *soff,
+                               int nq, SegRec *segs, int32_t *nseg, int64_t *caps, hipStream_t stream);
+hipError_t seg_offsets_launch(SegRec *segs, const int64_t *cig_off, int64_t n, hipStream_t stream);
+hipError_t seg_collect_launch(const SegRec *segs, int64_t n, int64_t *list, unsigned long long *count,
+                              hipStream_t stream);
+hipError_t lane_dp_launch(const SegRec *segs, int64_t n, const uint8_t *center, const int64_t *qrev_off,
+                          const int64_t *qlens, const uint8_t *qrev, uint8_t *ws, int64_t ws_lane, int64_t lanes,
+                          uint32_t *cigs, int32_t *ncig, hipStream_t stream);
+hipError_t mm2_assemble_launch(const SegRec *segs, const int64_t *soff, const int32_t *nseg, const uint32_t *cigs,
+                               const int32_t *ncig, const int64_t *qlens, uint32_t rlen, const int64_t *ooff, int nq,
+                               uint32_t *out, int64_t *nout, hipStream_t stream);
+
+}  // namespace hal
+
+using namespace hal;
+
+struct halign_center {
+    halign_ctx *ctx = nullptr;
+    int32_t rlen = 0;
+    bool hasn = false;
+    std::vector<uint16_t> spread;           // center even/odd spread codes (kSeqPad padding)
+    std::vector<uint8_t> bytes;             // center byte codes (kBytePad padding)
+    std::vector<uint64_t> hx, hy;           // the center's minimizer hits, in the reference's order
+    DevBuf sx, sy, dh, dstart, docc, lut;   // sorted hits, distinct-hash table, log2f table
+    int nd = 0;
+    uint64_t thr = 0;
+    size_t scratch_groups = 0;
+    // per-batch buffers
+    DevBuf hcnt, scratch, hidx, nanch, totals, acount, aoff, anchors, keys, keys2, vals, vals2, sort_tmp;
+    DevBuf fw, pw, tw, chain_idx, chain_cnt, qlens, soff, segs, nseg, caps, cig_off, scan_tmp, region, bregion;
+    DevBuf qrev_off, lane_ws, ncig, hlist, hcount, hsegs, hvals, ooff, out, nout;
+};
+
+namespace {
+
+__global__ void gather_segs(const SegRec *__restrict__ segs, const int64_t *__restrict__ list, int64_t n,
+                            SegRec *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = segs[list[i]];
+}
+
+__global__ void scatter_ncig(const int64_t *__restrict__ list, const int32_t *__restrict__ vals, int64_t n,
+                             int32_t *__restrict__ ncig)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ncig[list[i]] = vals[i];
+}
+
+constexpr int kK = 15, kW = 10;   // RefAligner's k-mer size and window (include/config.hpp defaults)
+constexpr int64_t kLaneWs = 5 * kLaneMaxT + kLaneMaxTb;
+
+template <class T>
+hipError_t up(DevBuf &b, const std::vector<T> &v, hipStream_t st)
+{
+    hipError_t e = b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+template <class T>
+hipError_t down(std::vector<T> &v, const void *src, size_t n, hipStream_t st)
+{
+    v.resize(n);
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
+// SeedHitBase order (include/seed.h:68-74): hash, rid, pos, strand
+bool hit_less(uint64_t ax, uint64_t ay, uint64_t bx, uint64_t by)
+{
+    const uint64_t ha = ax >> 8, hb = bx >> 8;
+    if (ha != hb) return ha < hb;
+    const uint32_t ra = (uint32_t)(ay >> 32) & 0x7fffffffu, rb = (uint32_t)(by >> 32) & 0x7fffffffu;
+    if (ra != rb) return ra < rb;
+    const uint32_t pa = (uint32_t)ay, pb = (uint32_t)by;
+    if (pa != pb) return pa < pb;
+    return (ay >> 63) < (by >> 63);
+}
+
+struct Batch {
+    int n = 0;
+    std::vector<int64_t> off, qlen, hit_off, hit_cnt, aoff, na;
+    std::vector<int32_t> seed_status, chain_cnt, nseg;
+    std::vector<int64_t> soff, ooff, nout;
+    int64_t anchors = 0, segs = 0, host_segs = 0, lane_segs = 0, cells_host = 0;
+    double ms = 0;
+};
+
+// Stages: 1 = anchors, 2 = chains, 3 = CIGARs (everything).
+int run_batch(halign_center *c, int n, const char *queries, const int64_t *offsets, int stop, Batch &b)
+{
+    halign_ctx *ctx = c->ctx;
+    hipStream_t st = ctx->stream;
+    b.n = n;
+    b.off.assign(offsets, offsets + n + 1);
+    for (auto &o : b.off) o -= offsets[0];
+    b.qlen.resize(n);
+    for (int i = 0; i < n; ++i) b.qlen[i] = b.off[i + 1] - b.off[i];
+    const int64_t total = b.off[n];
+    HCHECK(ctx->s_seqs.ensure((size_t)std::max<int64_t>(total, 1)), "alloc queries");
+    HCHECK(ctx->s_off.ensure((size_t)(n + 1) * 8), "alloc offsets");
+    if (total)
+        HCHECK(hipMemcpyAsync(ctx->s_seqs.p, queries + offsets[0], (size_t)total, hipMemcpyHostToDevice, st),
+               "upload queries");
+    HCHECK(hipMemcpyAsync(ctx->s_off.p, b.off.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, st),
+           "upload offsets");
+
+    // minimizers: extractMinimizer(query, 15, 10, non-canonical)
+    int rc = seed_device(ctx, n, b.off, kK, kW, 1, b.hit_off, b.hit_cnt, b.seed_status);
+    if (rc != HALIGN_OK) return rc;
+    HCHECK(up(c->hcnt, b.hit_cnt, st), "upload hit counts");
+
+    // anchors: count per query, then write in the reference's order
+    const int64_t nhits = b.hit_off[n];
+    const int groups = std::max(1, std::min(n, ctx->num_cu * 4));
+    if ((size_t)groups > c->scratch_groups) {
+        const size_t sz = (size_t)groups * (size_t)std::max(c->nd, 1) * 4;
+        HCHECK(c->scratch.ensure(sz), "alloc anchor scratch");
+        HCHECK(hipMemsetAsync(c->scratch.p, 0, sz, st), "clear anchor scratch");
+        c->scratch_groups = groups;
+    }
+    HCHECK(c->hidx.ensure((size_t)std::max<int64_t>(nhits, 1) * 4), "alloc");
+    HCHECK(c->nanch.ensure((size_t)std::max<int64_t>(nhits, 1) * 4), "alloc");
+    HCHECK(c->acount.ensure((size_t)std::max(n, 1) * 8), "alloc");
+    const CenterIndexDev ci{c->sx.as<uint64_t>(),     c->sy.as<uint64_t>(),   c->dh.as<uint64_t>(),
+                            c->dstart.as<uint32_t>(), c->docc.as<uint32_t>(), c->nd, c->thr};
+    const SeedFilter f{0.01, 500};   // SeedFilterParams defaults (include/anchor.h:65-71)
+    HCHECK(anchors_launch_count(ci, ctx->s_hx.as<uint64_t>(), ctx->s_hy.as<uint64_t>(), ctx->s_hoff.as<int64_t>(),
+                                c->hcnt.as<int64_t>(), n, f, groups, c->scratch.as<uint32_t>(), c->hidx.as<int32_t>(),
+                                c->nanch.as<uint32_t>(), c->acount.as<int64_t>(), st),
+           "launch anchors_count");
+    HCHECK(down(b.na, c->acount.p, n, st), "download anchor counts");
+    b.aoff.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) b.aoff[i + 1] = b.aoff[i] + b.na[i];
+    b.anchors = b.aoff[n];
+    const int64_t A = std::max<int64_t>(b.anchors, 1);
+    HCHECK(up(c->aoff, b.aoff, st), "upload anchor offsets");
+    HCHECK(c->anchors.ensure((size_t)A * sizeof(AnchorRec)), "alloc anchors");
+    HCHECK(anchors_launch_write(ci, ctx->s_hx.as<uint64_t>(), ctx->s_hy.as<uint64_t>(), ctx->s_hoff.as<int64_t>(),
+                                c->hcnt.as<int64_t>(), n, groups, c->hidx.as<int32_t>(), c->nanch.as<uint32_t>(),
+                                c->aoff.as<int64_t>(), c->anchors.as<AnchorRec>(), st),
+           "launch anchors_write");
+    if (stop == 1) {
+        HCHECK(hipStreamSynchronize(st), "anchors");
+        return HALIGN_OK;
+    }
+
+    // chains: each query's anchors sorted by (pos_ref, pos_qry), then the DP
+    HCHECK(c->keys.ensure((size_t)A * 8), "alloc");
+    HCHECK(c->keys2.ensure((size_t)A * 8), "alloc");
+    HCHECK(c->vals.ensure((size_t)A * 4), "alloc");
+    HCHECK(c->vals2.ensure((size_t)A * 4), "alloc");
+    HCHECK(anchor_keys_launch(c->anchors.as<AnchorRec>(), b.anchors, c->keys.as<uint64_t>(), c->vals.as<uint32_t>(),
+                              st),
+           "launch anchor_keys");
+    if (b.anchors > 0) {
+        size_t tmp = 0;
+        HCHECK(rocprim::segmented_radix_sort_pairs(nullptr, tmp, c->keys.as<uint64_t>(), c->keys2.as<uint64_t>(),
+                                                   c->vals.as<uint32_t>(), c->vals2.as<uint32_t>(),
+                                                   (unsigned)b.anchors, (unsigned)n, c->aoff.as<int64_t>(),
+                                                   c->aoff.as<int64_t>() + 1, 0u, 64u, st),
+               "sort storage size");
+        HCHECK(c->sort_tmp.ensure(std::max<size_t>(tmp, 1)), "alloc sort storage");
+        HCHECK(rocprim::segmented_radix_sort_pairs(c->sort_tmp.p, tmp, c->keys.as<uint64_t>(),
+                                                   c->keys2.as<uint64_t>(), c->vals.as<uint32_t>(),
+                                                   c->vals2.as<uint32_t>(), (unsigned)b.anchors, (unsigned)n,
+                                                   c->aoff.as<int64_t>(), c->aoff.as<int64_t>() + 1, 0u, 64u, st),
+               "segmented sort");
+    }
+    HCHECK(c->fw.ensure((size_t)A * 4), "alloc");
+    HCHECK(c->pw.ensure((size_t)A * 4), "alloc");
+    HCHECK(c->tw.ensure((size_t)A * 4), "alloc");
+    HCHECK(c->chain_idx.ensure((size_t)A * 4), "alloc");
+    HCHECK(c->chain_cnt.ensure((size_t)std::max(n, 1) * 4), "alloc");
+    const ChainParamsDev cp{5000, 5000, 500, 25, 5000, 3, 40, 0.01f, 0.01f};   // ChainParams defaults
+    HCHECK(chain_launch(c->anchors.as<AnchorRec>(), c->vals2.as<uint32_t>(), c->aoff.as<int64_t>(),
+                        c->acount.as<int64_t>(), n, cp, c->lut.as<float>(), c->fw.as<int32_t>(), c->pw.as<int32_t>(),
+                        c->tw.as<int32_t>(), c->chain_idx.as<uint32_t>(), c->chain_cnt.as<int32_t>(), st),
+           "launch chain");
+    HCHECK(down(b.chain_cnt, c->chain_cnt.p, n, st), "download chain lengths");
+    if (stop == 2) return HALIGN_OK;
+
+    // MM2 segments
+    b.soff.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) b.soff[i + 1] = b.soff[i] + (b.chain_cnt[i] ? 2 * b.chain_cnt[i] + 1 : 1);
+    b.segs = b.soff[n];
+    const int64_t S = std::max<int64_t>(b.segs, 1);
+    HCHECK(up(c->soff, b.soff, st), "upload segment offsets");
+    HCHECK(up(c->qlens, b.qlen, st), "upload query lengths");
+    HCHECK(c->segs.ensure((size_t)S * sizeof(SegRec)), "alloc segments");
+    HCHECK(c->nseg.ensure((size_t)std::max(n, 1) * 4), "alloc");
+    HCHECK(c->caps.ensure((size_t)S * 8), "alloc");
+    HCHECK(c->cig_off.ensure((size_t)S * 8), "alloc");
+    HCHECK(mm2_segments_launch(c->anchors.as<AnchorRec>(), c->chain_idx.as<uint32_t>(), c->aoff.as<int64_t>(),
+                               c->chain_cnt.as<int32_t>(), c->qlens.as<int64_t>(), (uint32_t)c->rlen,
+                               c->soff.as<int64_t>(), n, c->segs.as<SegRec>(), c->nseg.as<int32_t>(),
+                               c->caps.as<int64_t>(), st),
+           "launch mm2_segments");
+    {
+        size_t tmp = 0;
+        HCHECK(rocprim::exclusive_scan(nullptr, tmp, c->caps.as<int64_t>(), c->cig_off.as<int64_t>(), (int64_t)0,
+                                       (size_t)b.segs, rocprim::plus<int64_t>(), st),
+               "scan storage size");
+        HCHECK(c->scan_tmp.ensure(std::max<size_t>(tmp, 1)), "alloc scan storage");
+        HCHECK(rocprim::exclusive_scan(c->scan_tmp.p, tmp, c->caps.as<int64_t>(), c->cig_off.as<int64_t>(),
+                                       (int64_t)0, (size_t)b.segs, rocprim::plus<int64_t>(), st),
+               "scan segment CIGAR offsets");
+    }
+    int64_t last[2] = {0, 0};
+    HCHECK(hipMemcpyAsync(&last[0], c->cig_off.as<int64_t>() + (b.segs - 1), 8, hipMemcpyDeviceToHost, st), "down");
+    HCHECK(hipMemcpyAsync(&last[1], c->caps.as<int64_t>() + (b.segs - 1), 8, hipMemcpyDeviceToHost, st), "down");
+    HCHECK(hipStreamSynchronize(st), "segments");
+    const int64_t cig_total = std::max<int64_t>(last[0] + last[1], 1);
+    HCHECK(seg_offsets_launch(c->segs.as<SegRec>(), c->cig_off.as<int64_t>(), b.segs, st), "launch seg offsets");
+
+    // sequences in ctx->seq / ctx->seqb as the DP runner expects: center first,
+    // then each query's reversed copies
+    const int64_t cn = c->rlen + 2 * kSeqPad, cbn = c->rlen + 2 * kBytePad;
+    std::vector<int64_t> region(n), bregion(n), qrev_off(n);
+    int64_t elems = 2 * cn, nbytes = cbn;
+    for (int i = 0; i < n; ++i) {
+        region[i] = elems;
+        bregion[i] = nbytes;
+        qrev_off[i] = nbytes + kBytePad;
+        elems += 2 * (b.qlen[i] + 2 * kSeqPad);
+        elems = (elems + 1) & ~(int64_t)1;
+        nbytes += b.qlen[i] + 2 * kBytePad;
+    }
+    HCHECK(ctx->seq.ensure((size_t)elems * 2 + 16), "alloc sequences");
+    HCHECK(ctx->seqb.ensure((size_t)nbytes + 16), "alloc byte sequences");
+    HCHECK(hipMemcpyAsync(ctx->seq.p, c->spread.data(), c->spread.size() * 2, hipMemcpyHostToDevice, st),
+           "upload center");
+    HCHECK(hipMemcpyAsync(ctx->seqb.p, c->bytes.data(), c->bytes.size(), hipMemcpyHostToDevice, st),
+           "upload center bytes");
+    HCHECK(up(c->region, region, st), "upload regions");
+    HCHECK(up(c->bregion, bregion, st), "upload regions");
+    HCHECK(up(c->qrev_off, qrev_off, st), "upload regions");
+    HCHECK(prep_queries_launch(ctx->s_seqs.as<char>(), ctx->s_off.as<int64_t>(), c->region.as<int64_t>(),
+                               c->bregion.as<int64_t>(), 0, n, ctx->seq.as<uint16_t>(), ctx->seqb.as<uint8_t>(), st),
+           "launch prep_queries");
+
+    // small segments: one lane each
+    HCHECK(ctx->cig.ensure((size_t)cig_total * 4), "alloc segment CIGARs");
+    HCHECK(c->ncig.ensure((size_t)S * 4), "alloc");
+    const int64_t lanes = (int64_t)ctx->num_cu * 256;
+    HCHECK(c->lane_ws.ensure((size_t)lanes * kLaneWs), "alloc lane DP workspace");
+    HCHECK(lane_dp_launch(c->segs.as<SegRec>(), b.segs, ctx->seqb.as<uint8_t>() + kBytePad,
+                          c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(), ctx->seqb.as<uint8_t>(),
+                          c->lane_ws.as<uint8_t>(), kLaneWs, lanes, ctx->cig.as<uint32_t>(), c->ncig.as<int32_t>(),
+                          st),
+           "launch lane_dp");
+
+    // large segments and band overflows: the ring / full-statistics kernels
+    HCHECK(c->hlist.ensure((size_t)S * 8), "alloc");
+    HCHECK(c->hcount.ensure(8), "alloc");
+    HCHECK(hipMemsetAsync(c->hcount.p, 0, 8, st), "clear");
+    HCHECK(seg_collect_launch(c->segs.as<SegRec>(), b.segs, c->hlist.as<int64_t>(),
+                              c->hcount.as<unsigned long long>(), st),
+           "launch collect");
+    unsigned long long nh = 0;
+    HCHECK(hipMemcpyAsync(&nh, c->hcount.p, 8, hipMemcpyDeviceToHost, st), "down");
+    HCHECK(hipStreamSynchronize(st), "collect");
+    b.host_segs = (int64_t)nh;
+    if (nh > 0) {
+        HCHECK(c->hsegs.ensure((size_t)nh * sizeof(SegRec)), "alloc");
+        hipLaunchKernelGGL(gather_segs, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, c->segs.as<SegRec>(),
+                           c->hlist.as<int64_t>(), (int64_t)nh, c->hsegs.as<SegRec>());
+        HCHECK(hipGetLastError(), "launch gather");
+        std::vector<SegRec> hs;
+        std::vector<int64_t> hl;
+        HCHECK(down(hs, c->hsegs.p, nh, st), "download large segments");
+        HCHECK(down(hl, c->hlist.p, nh, st), "download large segments");
+        // keep the device list order (atomic order) paired with the records
+        std::vector<Spec> specs(nh);
+        for (size_t k = 0; k < nh; ++k) {
+            const SegRec &sg = hs[k];
+            Spec &s = specs[k];
+            const int64_t L = b.qlen[sg.query];
+            const int64_t rq = L - sg.qe;   // reversed index of the segment's last query base
+            s.tgt_elem = kSeqPad + sg.rs;
+            s.tgt_odd_elem = cn + kSeqPad + sg.rs;
+            s.qry_elem = region[sg.query] + kSeqPad + rq;
+            s.qry_odd_elem = region[sg.query] + (L + 2 * kSeqPad) + kSeqPad + rq;
+            s.tgt_b = kBytePad + sg.rs;
+            s.qry_b = bregion[sg.query] + kBytePad + rq;
+            s.tlen = (int32_t)(sg.re - sg.rs);
+            s.qlen = (int32_t)(sg.qe - sg.qs);
+            s.w = auto_band(s.tlen, s.qlen);
+            s.flags = ((sg.flags & kSegRight) ? kFlagRight : 0) | kFlagHasN;
+            s.cig_off = sg.cig_off;
+            s.cig_cap = s.tlen + s.qlen + 1;
+            if (!fill_spec_shape(s, false, ctx->policy)) return fail(ctx, HALIGN_E_UNSUPPORTED, "segment band too wide");
+        }
+        std::vector<DpOut> outs;
+        double ms = 0;
+        rc = run_specs(ctx, specs, false, outs, &ms);
+        if (rc != HALIGN_OK) return rc;
+        std::vector<int32_t> nv(nh);
+        for (size_t k = 0; k < nh; ++k) {
+            nv[k] = outs[k].n_cigar;
+            b.cells_host += outs[k].cells;
+        }
+        HCHECK(up(c->hvals, nv, st), "upload CIGAR lengths");
+        hipLaunchKernelGGL(scatter_ncig, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, c->hlist.as<int64_t>(),
+                           c->hvals.as<int32_t>(), (int64_t)nh, c->ncig.as<int32_t>());
+        HCHECK(hipGetLastError(), "launch scatter");
+    }
+
+    // assembly: appendCigar over the segments of each query
+    b.ooff.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) b.ooff[i + 1] = b.ooff[i] + b.qlen[i] + c->rlen + 1;
+    HCHECK(up(c->ooff, b.ooff, st), "upload output offsets");
+    HCHECK(c->out.ensure((size_t)std::max<int64_t>(b.ooff[n], 1) * 4), "alloc output");
+    HCHECK(c->nout.ensure((size_t)std::max(n, 1) * 8), "alloc");
+    HCHECK(mm2_assemble_launch(c->segs.as<SegRec>(), c->soff.as<int64_t>(), c->nseg.as<int32_t>(),
+                               ctx->cig.as<uint32_t>(), c->ncig.as<int32_t>(), c->qlens.as<int64_t>(),
+                               (uint32_t)c->rlen, c->ooff.as<int64_t>(), n, c->out.as<uint32_t>(),
+                               c->nout.as<int64_t>(), st),
+           "launch assemble");
+    HCHECK(down(b.nout, c->nout.p, n, st), "download CIGAR lengths");
+    return HALIGN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int halign_center_create(halign_ctx *ctx, const char *center, int32_t len, halign_center **out)
+{
+    if (!ctx || !center || len <= 0 || !out) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    halign_center *c = new (std::nothrow) halign_center();
+    if (!c) return HALIGN_E_NOMEM;
+    c->ctx = ctx;
+    c->rlen = len;
+    std::vector<uint8_t> codes(len);
+    for (int i = 0; i < len; ++i) {
+        codes[i] = score_code((unsigned char)center[i]);
+        if (codes[i] > 3) c->hasn = true;
+    }
+    const int64_t cn = len + 2 * kSeqPad;
+    c->spread.resize((size_t)cn * 2);
+    spread_target(codes.data(), len, c->spread.data(), c->spread.data() + cn);
+    c->bytes.resize((size_t)len + 2 * kBytePad);
+    bytes_target(codes.data(), len, c->bytes.data());
+    // the center's minimizers on the device (same kernels as the queries)
+    std::vector<int64_t> off{0, len}, hoff, cnt;
+    std::vector<int32_t> sst;
+    int rc;
+    if (ctx->s_seqs.ensure((size_t)len) != hipSuccess || ctx->s_off.ensure(16) != hipSuccess) {
+        delete c;
+        return fail(ctx, HALIGN_E_NOMEM, "alloc center");
+    }
+    (void)hipMemcpy(ctx->s_seqs.p, center, (size_t)len, hipMemcpyHostToDevice);
+    (void)hipMemcpy(ctx->s_off.p, off.data(), 16, hipMemcpyHostToDevice);
+    rc = seed_device(ctx, 1, off, kK, kW, 1, hoff, cnt, sst);
+    if (rc != HALIGN_OK) {
+        delete c;
+        return rc;
+    }
+    c->hx.resize(cnt[0]);
+    c->hy.resize(cnt[0]);
+    if (cnt[0]) {
+        (void)hipMemcpy(c->hx.data(), ctx->s_hx.p, (size_t)cnt[0] * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(c->hy.data(), ctx->s_hy.p, (size_t)cnt[0] * 8, hipMemcpyDeviceToHost);
+    }
+    // index: hits sorted by (hash, rid, pos, strand), distinct hashes, threshold
+    const size_t nh = c->hx.size();
+    std::vector<size_t> ord(nh);
+    for (size_t i = 0; i < nh; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(),
+              [&](size_t a, size_t b) { return hit_less(c->hx[a], c->hy[a], c->hx[b], c->hy[b]); });
+    std::vector<uint64_t> sx(nh), sy(nh), dh;
+    std::vector<uint32_t> dstart, docc;
+    for (size_t i = 0; i < nh; ++i) { sx[i] = c->hx[ord[i]]; sy[i] = c->hy[ord[i]]; }
+    for (size_t i = 0; i < nh;) {
+        size_t j = i;
+        while (j < nh && (sx[j] >> 8) == (sx[i] >> 8)) ++j;
+        dh.push_back(sx[i] >> 8);
+        dstart.push_back((uint32_t)i);
+        docc.push_back((uint32_t)(j - i));
+        i = j;
+    }
+    c->nd = (int)dh.size();
+    // compute_ref_occ_threshold (src/seed/anchor.cpp:73-117), defaults f = 2e-4, U = [10, 1e6]
+    uint64_t f_cutoff = UINT64_MAX;
+    const size_t n_skip = (size_t)std::floor(2e-4 * (double)dh.size());
+    if (n_skip > 0) {
+        std::vector<uint32_t> tmp(docc);
+        std::nth_element(tmp.begin(), tmp.begin() + (n_skip - 1), tmp.end(), std::greater<uint32_t>());
+        f_cutoff = tmp[n_skip - 1];
+    }
+    const uint64_t capped = std::min<uint64_t>(1000000, f_cutoff);
+    c->thr = std::max<uint64_t>(10, capped);
+    // log2f of the C library for the chain penalty (mg_log2, src/seed/anchor.cpp:227-230)
+    std::vector<float> lut(502);
+    lut[0] = 0.0f;
+    for (int d = 1; d <= 501; ++d) lut[d] = log2f((float)d);
+    hipStream_t s = ctx->stream;
+    if (up(c->sx, sx, s) != hipSuccess || up(c->sy, sy, s) != hipSuccess || up(c->dh, dh, s) != hipSuccess ||
+        up(c->dstart, dstart, s) != hipSuccess || up(c->docc, docc, s) != hipSuccess ||
+        up(c->lut, lut, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        delete c;
+        return fail(ctx, HALIGN_E_HIP, "upload center index");
+    }
+    *out = c;
+    return HALIGN_OK;
+}
+
+void halign_center_free(halign_center *c)
+{
+    if (!c) return;
+    std::lock_guard<std::mutex> lock(c->ctx->mu);
+    (void)hipSetDevice(c->ctx->device);
+    (void)hipStreamSynchronize(c->ctx->stream);
+    for (DevBuf *b : {&c->sx, &c->sy, &c->dh, &c->dstart, &c->docc, &c->lut, &c->hcnt, &c->scratch, &c->hidx,
+                      &c->nanch, &c->totals, &c->acount, &c->aoff, &c->anchors, &c->keys, &c->keys2, &c->vals,
+                      &c->vals2, &c->sort_tmp, &c->fw, &c->pw, &c->tw, &c->chain_idx, &c->chain_cnt, &c->qlens,
+                      &c->soff, &c->segs, &c->nseg, &c->caps, &c->cig_off, &c->scan_tmp, &c->region, &c->bregion,
+                      &c->qrev_off, &c->lane_ws, &c->ncig, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
+                      &c->out, &c->nout})
+        b->release();
+    delete c;
+}
+
+int halign_center_anchors(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                          void *anchors_out, int64_t cap, int64_t *anchor_offsets)
+{
+    if (!c || n < 0 || (n > 0 && (!queries || !offsets || !anchor_offsets))) return HALIGN_E_ARG;
+    halign_ctx *ctx = c->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    anchor_offsets[0] = 0;
+    if (n == 0) return HALIGN_OK;
+    Batch b;
+    const int rc = run_batch(c, n, queries, offsets, 1, b);
+    if (rc != HALIGN_OK) return rc;
+    if (b.anchors > cap) return fail(ctx, HALIGN_E_CAPACITY, "anchor output too small");
+    for (int i = 0; i <= n; ++i) anchor_offsets[i] = b.aoff[i];
+    if (b.anchors && anchors_out)
+        HCHECK(hipMemcpy(anchors_out, c->anchors.p, (size_t)b.anchors * sizeof(AnchorRec), hipMemcpyDeviceToHost),
+               "download anchors");
+    return HALIGN_OK;
+}
+
+int halign_center_chains(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                         void *chains_out, int64_t cap, int64_t *chain_offsets)
+{
+    if (!c || n < 0 || (n > 0 && (!queries || !offsets || !chain_offsets))) return HALIGN_E_ARG;
+    halign_ctx *ctx = c->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    chain_offsets[0] = 0;
+    if (n == 0) return HALIGN_OK;
+    Batch b;
+    const int rc = run_batch(c, n, queries, offsets, 2, b);
+    if (rc != HALIGN_OK) return rc;
+    std::vector<AnchorRec> an;
+    std::vector<uint32_t> ci;
+    HCHECK(down(an, c->anchors.p, (size_t)b.anchors, ctx->stream), "download anchors");
+    HCHECK(down(ci, c->chain_idx.p, (size_t)b.anchors, ctx->stream), "download chains");
+    int64_t o = 0;
+    for (int i = 0; i < n; ++i) {
+        chain_offsets[i] = o;
+        if (o + b.chain_cnt[i] > cap) return fail(ctx, HALIGN_E_CAPACITY, "chain output too small");
+        for (int k = 0; k < b.chain_cnt[i]; ++k)
+            if (chains_out) ((AnchorRec *)chains_out)[o + k] = an[ci[b.aoff[i] + k]];
+        o += b.chain_cnt[i];
+    }
+    chain_offsets[n] = o;
+    return HALIGN_OK;
+}
+
+int halign_center_align(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                        uint32_t *cigar_out, int64_t cap, int64_t *cigar_offsets, int32_t *stats, double *ms)
+{
+    if (!c || n < 0 || (n > 0 && (!queries || !offsets || !cigar_offsets))) return HALIGN_E_ARG;
+    halign_ctx *ctx = c->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    cigar_offsets[0] = 0;
+    if (n == 0) return HALIGN_OK;
+    hipEvent_t e0, e1;
+    HCHECK(hipEventCreate(&e0), "event");
+    HCHECK(hipEventCreate(&e1), "event");
+    HCHECK(hipEventRecord(e0, ctx->stream), "event");
+    Batch b;
+    const int rc = run_batch(c, n, queries, offsets, 3, b);
+    if (rc != HALIGN_OK) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    }
+    HCHECK(hipEventRecord(e1, ctx->stream), "event");
+    int64_t o = 0;
+    std::vector<uint32_t> buf;
+    for (int i = 0; i < n; ++i) {
+        if (b.nout[i] < 0) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return fail(ctx, HALIGN_E_KERNEL, "MM2 CIGAR does not consume the pair");
+        }
+        o += b.nout[i];
+    }
+    if (o > cap) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return fail(ctx, HALIGN_E_CAPACITY, "CIGAR output too small");
+    }
+    HCHECK(down(buf, c->out.p, (size_t)b.ooff[n], ctx->stream), "download CIGARs");
+    o = 0;
+    for (int i = 0; i < n; ++i) {
+        cigar_offsets[i] = o;
+        if (cigar_out && b.nout[i]) memcpy(cigar_out + o, buf.data() + b.ooff[i], (size_t)b.nout[i] * 4);
+        o += b.nout[i];
+        if (stats) {
+            stats[4 * i + 0] = b.chain_cnt[i] ? 2 * b.chain_cnt[i] + 1 : 1;
+            stats[4 * i + 1] = b.chain_cnt[i];
+            stats[4 * i + 2] = b.seed_status[i];
+            stats[4 * i + 3] = (int32_t)b.na[i];
+        }
+    }
+    cigar_offsets[n] = o;
+    if (ms) {
+        float t = 0;
+        HCHECK(hipEventElapsedTime(&t, e0, e1), "event time");
+        *ms = t;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return HALIGN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// pipeline_common.h -- records shared by the per-query path's kernels and
+// their host side: the center index, filter and chain parameters, anchors
+// and MM2 segments.
+#pragma once
+#include <cstdint>
+
+namespace hal {
+
+// anchor::Anchor (include/anchor.h:39-50), 32 bytes as in the reference's
+// layout: hash, rid_ref, pos_ref, rid_qry, pos_qry, span, is_rev.
+struct AnchorRec {
+    uint64_t hash;
+    uint32_t rid_ref, pos_ref, rid_qry, pos_qry, span, is_rev;
+};
+static_assert(sizeof(AnchorRec) == 32, "anchor record");
+
+// A center's minimizer index on the device: hits sorted by
+// (hash, rid, pos, strand) (include/seed.h:68-74), the distinct hashes with
+// their first sorted hit and occurrence count, and the reference-side
+// occurrence threshold max(u_floor, min(u_ceil, f-cutoff)).
+struct CenterIndexDev {
+    const uint64_t *sx, *sy;      // sorted hits (MinimizerHit x, y)
+    const uint64_t *dh;           // distinct hashes, ascending
+    const uint32_t *dstart, *docc;
+    int nd;
+    uint64_t thr;
+};
+
+// SeedFilterParams (include/anchor.h:65-71), the query-side part
+struct SeedFilter {
+    double q_occ_frac;
+    uint64_t sample_every_bp;
+};
+
+// ChainParams (include/anchor.h:116-126)
+struct ChainParamsDev {
+    int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_cnt, min_score;
+    float gap_penalty, skip_penalty;
+};
+
+// One globalAlignMM2 segment (src/align/align.cpp:221-267): ref [rs, re),
+// query [qs, qe) after clipping; flags: gap direction and how it is aligned.
+struct SegRec {
+    uint32_t rs, re, qs, qe;
+    uint32_t flags;   // bit 1: RIGHT (ksw2 flag bit); bit 8: whole pair without a length check
+    uint32_t query;   // query index in the batch
+    uint32_t cls;     // kSegEmpty / kSegPure / kSegLane / kSegHost
+    uint32_t pad;
+    int64_t cig_off;  // offset of its CIGAR in the segment CIGAR buffer
+};
+static_assert(sizeof(SegRec) == 40, "segment record");
+
+constexpr uint32_t kSegRight = 0x02;
+constexpr uint32_t kSegRaw = 0x100;   // empty-chain whole pair: CIGAR taken as is
+
+constexpr uint32_t kSegEmpty = 0;     // both sides empty: no CIGAR
+constexpr uint32_t kSegPure = 1;      // one side empty: a single I or D
+constexpr uint32_t kSegLane = 2;      // small DP, one lane per segment
+constexpr uint32_t kSegHost = 3;      // large DP or band overflow: ring / full-statistics kernels
+
+// lane DP limits: target length and traceback bytes per lane
+constexpr int kLaneMaxT = 512;
+constexpr int kLaneMaxTb = 16384;
+
+}  // namespace hal

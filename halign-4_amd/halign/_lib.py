@@ -64,6 +64,13 @@ def _load():
     lib.halign_set_seed_policy.argtypes = [_P, C.c_int]
     lib.halign_extract_minimizers.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.c_int32, C.c_int32,
                                               C.c_int32, _P, _P, C.c_int64, _P, _P]
+    lib.halign_center_create.argtypes = [_P, C.c_char_p, C.c_int32, C.POINTER(_P)]
+    lib.halign_center_free.argtypes = [_P]
+    lib.halign_center_free.restype = None
+    lib.halign_center_anchors.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P]
+    lib.halign_center_chains.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P]
+    lib.halign_center_align.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P, _P,
+                                        C.POINTER(C.c_double)]
     lib.halign_pairset_free.argtypes = [_P]
     lib.halign_pairset_free.restype = None
     return lib
@@ -110,9 +117,13 @@ class Context:
         if rc != HALIGN_OK:
             raise HalignError(f"halign_init failed: {ERRORS.get(rc, rc)}")
         self.h = h
+        self._children = []   # centers and pair sets, freed before the context
 
     def close(self):
         if self.h:
+            for ch in self._children:
+                ch.close()
+            self._children = []
             self.lib.halign_free(self.h)
             self.h = None
 
@@ -216,6 +227,77 @@ class Context:
         return [(hx[hoff[i]:hoff[i + 1]].copy(), hy[hoff[i]:hoff[i + 1]].copy()) for i in range(n)], st[:n]
 
 
+ANCHOR_DTYPE = np.dtype([("hash", "<u8"), ("rid_ref", "<u4"), ("pos_ref", "<u4"),
+                         ("rid_qry", "<u4"), ("pos_qry", "<u4"), ("span", "<u4"), ("is_rev", "<u4")])
+
+
+def _blob(queries):
+    qs = [q.encode() if isinstance(q, str) else bytes(q) for q in queries]
+    off = np.zeros(len(qs) + 1, np.int64)
+    off[1:] = np.cumsum([len(q) for q in qs])
+    return b"".join(qs), off
+
+
+class Center:
+    """A resident center for the per-query path (halign_center_*)."""
+
+    def __init__(self, ctx, center):
+        self.ctx = ctx
+        center = center.encode() if isinstance(center, str) else bytes(center)
+        self.rlen = len(center)
+        h = _P()
+        ctx._check(ctx.lib.halign_center_create(ctx.h, center, len(center), C.byref(h)),
+                   "halign_center_create")
+        self.h = h
+        ctx._children.append(self)
+
+    def close(self):
+        if self.h:
+            if self.ctx.h:
+                self.ctx.lib.halign_center_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def anchors(self, queries):
+        blob, off = _blob(queries)
+        cap = max(16, 64 * (len(blob) + 1))
+        out = np.zeros(cap, ANCHOR_DTYPE)
+        aoff = np.zeros(len(off), np.int64)
+        self.ctx._check(self.ctx.lib.halign_center_anchors(self.h, len(off) - 1, blob, _ptr(off),
+                                                           _ptr(out), cap, _ptr(aoff)),
+                        "halign_center_anchors")
+        return [out[aoff[i]:aoff[i + 1]].copy() for i in range(len(off) - 1)]
+
+    def chains(self, queries):
+        blob, off = _blob(queries)
+        cap = max(16, len(blob) + 1)
+        out = np.zeros(cap, ANCHOR_DTYPE)
+        coff = np.zeros(len(off), np.int64)
+        self.ctx._check(self.ctx.lib.halign_center_chains(self.h, len(off) - 1, blob, _ptr(off),
+                                                          _ptr(out), cap, _ptr(coff)),
+                        "halign_center_chains")
+        return [out[coff[i]:coff[i + 1]].copy() for i in range(len(off) - 1)]
+
+    def align(self, queries):
+        """RefAligner::globalAlign per query: (list of CIGAR arrays, stats[n,4], ms)."""
+        blob, off = _blob(queries)
+        n = len(off) - 1
+        cap = int(off[-1]) + n * (self.rlen + 1) + 1
+        cig = np.zeros(cap, np.uint32)
+        coff = np.zeros(n + 1, np.int64)
+        st = np.zeros((max(n, 1), 4), np.int32)
+        ms = C.c_double(0)
+        self.ctx._check(self.ctx.lib.halign_center_align(self.h, n, blob, _ptr(off), _ptr(cig), cap,
+                                                         _ptr(coff), _ptr(st), C.byref(ms)),
+                        "halign_center_align")
+        return [cig[coff[i]:coff[i + 1]].copy() for i in range(n)], st[:n], ms.value
+
+
 class PairSet:
     """Center + queries resident in HBM (halign_pairset_*)."""
 
@@ -229,6 +311,7 @@ class PairSet:
                                            C.byref(h))
         ctx._check(rc, "halign_pairset_create")
         self.h = h
+        ctx._children.append(self)
         self.n = len(off) - 1
 
     def run(self, first, count, plain=False):
@@ -252,7 +335,8 @@ class PairSet:
 
     def close(self):
         if self.h:
-            self.ctx.lib.halign_pairset_free(self.h)
+            if self.ctx.h:
+                self.ctx.lib.halign_pairset_free(self.h)
             self.h = None
 
     def __del__(self):

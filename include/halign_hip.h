@@ -134,6 +134,34 @@ int halign_extract_minimizers(halign_ctx *ctx, int32_t n, const char *seqs, cons
                               int32_t w, int32_t non_canonical, uint64_t *x_out, uint64_t *y_out, int64_t cap,
                               int64_t *hit_offsets, int32_t *status_out);
 
+/* A center kept resident for the per-query path (RefAligner with one
+ * center, reference src/align/ref_aligner.cpp:40-41,91-125): its sequence,
+ * its minimizers (extractMinimizer(center, 15, 10, non-canonical)) and their
+ * index (hits sorted by hash, rid, pos, strand; distinct hashes with their
+ * runs; the reference-occurrence threshold of compute_ref_occ_threshold,
+ * src/seed/anchor.cpp:111-117). */
+typedef struct halign_center halign_center;
+int halign_center_create(halign_ctx *ctx, const char *center, int32_t len, halign_center **out);
+void halign_center_free(halign_center *c);
+/* minimizer::collect_anchors(center hits, query hits) (src/seed/minimizer.cpp:
+ * 220-356) for n ASCII queries (queries[offsets[i] .. offsets[i+1])), default
+ * SeedFilterParams. anchors_out: 32-byte anchor::Anchor records
+ * (include/anchor.h:39-50: hash u64, rid_ref, pos_ref, rid_qry, pos_qry, span,
+ * is_rev u32), query i's at anchor_offsets[i] .. anchor_offsets[i+1]. */
+int halign_center_anchors(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                          void *anchors_out, int64_t cap, int64_t *anchor_offsets);
+/* anchor::chainAnchors on those anchors (src/seed/anchor.cpp:332-461, default
+ * ChainParams): the best chain of each query, in position order. */
+int halign_center_chains(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                         void *chains_out, int64_t cap, int64_t *chain_offsets);
+/* RefAligner::globalAlign(center, query) (src/align/ref_aligner.cpp:91-125 =
+ * extractMinimizer -> collect_anchors -> globalAlignMM2, src/align/align.cpp:
+ * 189-305) for n queries: query i's CIGAR at cigar_out[cigar_offsets[i] ..
+ * cigar_offsets[i+1]). stats (optional, 4 per query): segments, chain length,
+ * minimizer status bits, anchors. ms (optional): device time of the batch. */
+int halign_center_align(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                        uint32_t *cigar_out, int64_t cap, int64_t *cigar_offsets, int32_t *stats, double *ms);
+
 /* Column counts and consensus of an alignment (the counting core of
  * consensus::generateConsensusSequence, reference src/consensus/consensus.cpp:
  * 611-703, pickConsensusChar :18-35, SiteCount include/consensus.h:78-99).

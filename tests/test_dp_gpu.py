@@ -157,6 +157,29 @@ def test_dp_long_banded_pairs(ctx, oracle):
     assert _compare(oracle, cases, res, cigs, True, "long") == 0
 
 
+def test_dp_asymmetric_unbanded(ctx, oracle):
+    """Tall-thin and short-wide pairs with w = -1 (auto_band's ratio > 0.5):
+    the band slides along the whole longer side (the empty-chain whole pair of
+    a short query)."""
+    rng = np.random.default_rng(31)
+    cases = []
+    for it in range(60):
+        lt = int(rng.integers(300, 5000))
+        ls = int(rng.integers(1, 200))
+        t = rand_codes(rng, lt)
+        a = int(rng.integers(0, lt - 1))
+        q = mutate_codes(rng, t[a:a + ls], 0.05, 0.02) if it % 3 else rand_codes(rng, ls)
+        if len(q) == 0:
+            q = rand_codes(rng, 1)
+        flag = FLAG_RIGHT if it % 2 else FLAG_LEFT
+        cases.append((q, t, -1, flag) if it % 4 < 2 else (t, q, -1, flag))
+    tasks, qc, tc = _tasks(cases)
+    print("dp asymmetric unbanded: 60 cases", file=sys.stderr, flush=True)
+    for full in (False, True):
+        res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=full, plain=False)
+        assert _compare(oracle, cases, res, cigs, full, "asym") == 0
+
+
 def test_global_align_batch(ctx, oracle):
     rng = np.random.default_rng(11)
     refs, qrys, flags = [], [], []

@@ -1,0 +1,136 @@
+"""GPU parity of the per-query path against a resident center
+(RefAligner::globalAlign, src/align/ref_aligner.cpp:91-125): anchors
+(collect_anchors), chains (chainAnchors) and the final CIGAR (globalAlignMM2),
+bit-exact against the reference-written fixtures and against the oracle on
+seeded batches."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle_lib import ANCHOR_DTYPE, Oracle, cigar_str
+from seqgen import codes_to_ascii, mutate_codes, rand_codes
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with np.load(os.path.join(GOLD, name), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def seg(d, o, i):
+    return d[o[i]:o[i + 1]]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import halign
+    c = halign.Context(0)
+    yield c
+    c.close()
+
+
+def test_anchors_chains_mm2_golden(ctx):
+    """Each fixture case is its own center: anchors, the chain and the
+    globalAlignMM2 CIGAR of the reference (anchors_chain.npz)."""
+    import halign
+    g = load("anchors_chain.npz")
+    n = len(g["ref_off"]) - 1
+    rec = ANCHOR_DTYPE.itemsize
+    print(f"anchors/chains/mm2 golden: {n} cases", file=sys.stderr, flush=True)
+    bad = {"anchors": [], "chain": [], "mm2": []}
+    for i in range(n):
+        ref = seg(g["ref"], g["ref_off"], i).tobytes()
+        qry = seg(g["qry"], g["qry_off"], i).tobytes()
+        c = halign.Center(ctx, ref)
+        try:
+            exp_a = g["anchors"][g["anchors_off"][i] * rec:g["anchors_off"][i + 1] * rec].view(ANCHOR_DTYPE)
+            got_a = c.anchors([qry])[0]
+            if not np.array_equal(got_a.view(np.uint8), exp_a.view(np.uint8)):
+                bad["anchors"].append(i)
+            exp_c = g["chain"][g["chain_off"][i] * rec:g["chain_off"][i + 1] * rec].view(ANCHOR_DTYPE)
+            got_c = c.chains([qry])[0]
+            if not np.array_equal(got_c.view(np.uint8), exp_c.view(np.uint8)):
+                bad["chain"].append(i)
+            cig, st, _ = c.align([qry])
+            exp = seg(g["mm2"], g["mm2_off"], i)
+            if not np.array_equal(cig[0], exp):
+                bad["mm2"].append(i)
+                if len(bad["mm2"]) <= 3:
+                    direct, _ = ctx.global_align_batch([ref], [qry], [0x06])
+                    print(f"mm2 case {i}: ref {len(ref)} query {len(qry)} chain {len(exp_c)} stats {st[0]}\n"
+                          f"  gpu    {cigar_str(cig[0])[:160]}\n  expect {cigar_str(exp)[:160]}\n"
+                          f"  direct globalAlignKSW2 {cigar_str(direct[0])[:160]}", file=sys.stderr, flush=True)
+        finally:
+            c.close()
+    assert not any(bad.values()), {k: (len(v), v[:8]) for k, v in bad.items()}
+
+
+def test_ref_global_golden(ctx):
+    """RefAligner::globalAlign fixtures (ref_global.npz): 150 short pairs, each
+    its own center, and 30 config-1 queries against the 16,569 bp center in
+    one batch."""
+    import halign
+    g = load("ref_global.npz")
+    n = len(g["ref_off"]) - 1
+    print(f"ref_global golden: {n} cases", file=sys.stderr, flush=True)
+    groups = {}
+    for i in range(n):
+        groups.setdefault(seg(g["ref"], g["ref_off"], i).tobytes(), []).append(i)
+    bad = []
+    for ref, idx in groups.items():
+        c = halign.Center(ctx, ref)
+        try:
+            cigs, st, _ = c.align([seg(g["qry"], g["qry_off"], i).tobytes() for i in idx])
+            bad += [i for j, i in enumerate(idx) if not np.array_equal(cigs[j], seg(g["cigar"], g["cigar_off"], i))]
+        finally:
+            c.close()
+    assert not bad, (len(bad), bad[:10])
+
+
+def test_align_batch_vs_oracle(ctx):
+    """One 12 kb center, 300 queries of 0-15 kb: exact windows, mutated
+    substrings, extensions with random ends, long indels (large segments and
+    band overflows), queries too short for anchors, N runs."""
+    import halign
+    O = Oracle()
+    rng = np.random.default_rng(77)
+    center = rand_codes(rng, 12000)
+    cstr = codes_to_ascii(center)
+    qs = []
+    for i in range(300):
+        kind = i % 6
+        if kind == 0:
+            a = int(rng.integers(0, 11000))
+            q = mutate_codes(rng, center[a:a + int(rng.integers(1, 4000))], 0.01, 0.002)
+        elif kind == 1:
+            q = mutate_codes(rng, center, 0.05, 0.01)
+        elif kind == 2:
+            q = np.concatenate([rand_codes(rng, int(rng.integers(0, 800))), mutate_codes(rng, center, 0.01, 0.002),
+                                rand_codes(rng, int(rng.integers(0, 800)))])
+        elif kind == 3:
+            q = center.copy()
+            a = int(rng.integers(100, 11000))
+            q = np.concatenate([q[:a], rand_codes(rng, int(rng.integers(50, 500))), q[a + int(rng.integers(0, 500)):]])
+            q = mutate_codes(rng, q, 0.01, 0.002)
+        elif kind == 4:
+            q = rand_codes(rng, int(rng.integers(0, 500)))
+        else:
+            q = mutate_codes(rng, center, 0.02, 0.005)
+            q[rng.random(len(q)) < 0.001] = 4
+        qs.append(codes_to_ascii(q))
+    print("align batch vs oracle: 300 queries, 12 kb center", file=sys.stderr, flush=True)
+    c = halign.Center(ctx, cstr)
+    cigs, st, ms = c.align(qs)
+    bad = []
+    for i, q in enumerate(qs):
+        exp, _, _, ub = O.ref_global_align(cstr, q)
+        if ub & 3:
+            continue
+        if not np.array_equal(cigs[i], exp):
+            bad.append(i)
+    assert not bad, f"{len(bad)} differ, first {bad[:5]}"
+    c.close()
