@@ -79,8 +79,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times), to scale HBM counters")
-    ap.add_argument("--workload", default="dp", choices=["dp", "consensus"],
-                    help="dp: the headline pairwise DP; consensus: column counting over rows in HBM")
+    ap.add_argument("--workload", default="dp", choices=["dp", "consensus", "e2e"],
+                    help="dp: the headline pairwise DP; consensus: column counting over rows in HBM; "
+                         "e2e: the whole per-query path (--config 1, 3 or 4)")
+    ap.add_argument("--config", type=int, default=1, help="e2e workload configuration (1, 3, 4)")
+    ap.add_argument("--queries", type=int, default=0, help="e2e: queries per step (0 = the config default)")
     ap.add_argument("--rows", type=int, default=100000, help="consensus workload: rows of 29,903 columns")
     ap.add_argument("--kernel", default="auto",
                     help="auto | multiwave | WxS (ring kernel with W waves and S slots per lane)")
@@ -103,6 +106,8 @@ def main():
 
     if args.workload == "consensus":
         return consensus_bench(args)
+    if args.workload == "e2e":
+        return e2e_bench(args)
 
     import halign
     from halign import synth
@@ -190,6 +195,70 @@ def main():
                      "traffic": traffic_per_launch(tot_cells / args.steps)},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+E2E = {1: ("plain method, small: center 16569 bp, 1000 queries at 1% sub / 0.2% indel", 1000, 1000),
+       3: ("uneven batch: center 20000 bp, 1000000 lognormal-length queries (median 1.5 kb)", 16384, 1000000),
+       4: ("divergent stress: center 10000 bp, 200000 queries at 5% sub / 1% indel + long indels", 8192, 200000)}
+
+
+def e2e_bench(args):
+    """RefAligner::globalAlign per query (minimizers, anchors, chain, MM2 with
+    its segment DPs) against a resident center, one C-ABI call per step. The
+    queries are passed as host buffers: the wall time includes their upload
+    and the CIGAR download. cpu_baseline: the compiled reference's
+    RefAligner::globalAlign (center minimizers computed once) on a sample."""
+    import halign
+    from halign import synth
+    cfg = args.config
+    label, per_step, total = E2E[cfg]
+    per = args.queries or per_step
+    need = min(total, per * (args.warmup + args.steps))
+    center = synth.center(cfg)
+    qb, off = synth.queries(cfg, 0, need)
+    log(f"e2e config {cfg}: {need} queries generated, {off[-1] / 1e6:.1f} Mbases")
+    ctx = halign.Context(0)
+    c = halign.Center(ctx, center)
+    steps = []
+    for s in range(args.warmup + args.steps):
+        a = (s * per) % need
+        b = min(need, a + per)
+        t0 = time.perf_counter()
+        cig, coff, st, ms = c.align_flat(qb, off[a:b + 1])
+        dt = time.perf_counter() - t0
+        if s >= args.warmup:
+            steps.append((b - a, int(off[b] - off[a]), dt, ms))
+            log(f"step {s - args.warmup}: {b - a} queries, {dt * 1e3:.1f} ms wall, {ms:.1f} ms device")
+    nq = sum(x[0] for x in steps)
+    nb = sum(x[1] for x in steps)
+    tw = sum(x[2] for x in steps)
+    cpu = None
+    if not args.no_cpu:
+        from oracle_lib import REF_SO, Oracle, RefLib
+        threads = min(16, os.cpu_count() or 1)
+        kind = "reference" if os.path.exists(REF_SO) else "port"
+        lib = RefLib() if kind == "reference" else Oracle()
+        sample = min(need, args.cpu_sample or threads * {1: 4, 3: 64, 4: 8}[cfg])
+        if kind == "reference":
+            hits = lib.minimizers(center)
+            fn = lambda i: lib.ref_global_align_pre(center, hits, qb[off[i]:off[i + 1]])
+        else:
+            fn = lambda i: lib.ref_global_align(center, qb[off[i]:off[i + 1]])
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(fn, range(sample)))
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(sample / dt, 2), "unit": "seqs/s", "cores": threads, "kind": kind,
+               "sample": f"{sample} config-{cfg} queries (RefAligner::globalAlign), {dt:.1f}s on {threads} threads"}
+        log(f"cpu baseline ({kind}): {cpu['value']} seqs/s")
+    out = {"metric": "seqs/s end-to-end per-query path (RefAligner::globalAlign)", "value": round(nq / tw, 2),
+           "unit": "seqs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(tw * 1e3 / len(steps), 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int8", "data": f"synthetic (seeded, csrc/synth.c config {cfg})",
+           "config": {"workload": f"config {cfg}: {label}", "queries_per_step": per},
+           "bases_per_s": round(nb / tw, 1), "device_ms_per_step": round(sum(x[3] for x in steps) / len(steps), 3),
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

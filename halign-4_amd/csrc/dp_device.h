@@ -43,6 +43,11 @@ __device__ __forceinline__ uint32_t pk_subsat(uint32_t a, uint32_t b)
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(s16x2, a),
                                                                       __builtin_bit_cast(s16x2, b)));
 }
+// each 16-bit half: all ones if negative as int16, else 0
+__device__ __forceinline__ uint32_t pk_ashr15(uint32_t a)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(s16x2, a) >> (s16x2){15, 15});
+}
 __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { return (a & m) | (b & ~m); }
 // cells (2j-1, 2j) from pairs hi=(2j,2j+1), lo=(2j-2,2j-1)
 __device__ __forceinline__ uint32_t cat16(uint32_t hi, uint32_t lo) { return __builtin_amdgcn_alignbit(hi, lo, 16); }

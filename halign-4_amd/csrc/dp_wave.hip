@@ -79,22 +79,127 @@ __device__ __forceinline__ void set_cell(uint32_t (&a)[8], int c, uint32_t v16)
     }
 }
 
+// lane i receives lane i-1 of its group of SUB lanes (SUB = 16: DPP row_ror:1;
+// SUB = 4: DPP quad_perm [3,0,1,2])
+template <int SUB>
+__device__ __forceinline__ uint32_t group_ror1(uint32_t v)
+{
+    static_assert(SUB == 4 || SUB == 16, "groups of 4 or 16 lanes");
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, SUB == 16 ? 0x121 : 0x93, 0xF, 0xF, false);
+}
+
+// Backtrack of include/ksw2.h:129-161 for a job owned by a group of SUB lanes
+// (sub-wave mode). Every lane of the group walks the same path; traceback rows
+// are staged 16 anti-diagonals at a time into the group's LDS chunk (16 x 16
+// bytes: along 16 anti-diagonals the path moves at most 15 target columns, so two
+// blocks cover it), lane t of the group loading anti-diagonals rc - t - SUB*m.
+// Cell layout 1 of backtrack_to_cigar; ops go to J.cigar last op first, written
+// by every lane of the group with the same value.
+template <int SUB>
+__device__ inline int group_backtrack(const DpJob &J, const uint8_t *__restrict__ tb, size_t rowb, int w,
+                                      bool right, int i0, int j0, uint32_t (*chunk_)[4])
+{
+    volatile uint32_t(*chunk)[4] = chunk_;
+    const int t = threadIdx.x & (SUB - 1);
+    const int qlen = J.qlen, tlen = J.tlen;
+    int nout = 0, i = i0, j = j0, state = 0, rc = -1, bb0 = 0, cur_op = -1, cur_len = 0;
+    auto push = [&](int op, int len) {
+        if (op == cur_op) {
+            cur_len += len;
+            return;
+        }
+        if (cur_len > 0) {
+            if (nout < J.cig_cap) J.cigar[nout] = (uint32_t)cur_len << 4 | (uint32_t)cur_op;
+            ++nout;
+        }
+        cur_op = op;
+        cur_len = len;
+    };
+    while (i >= 0 && j >= 0) {
+        const int r = i + j;
+        if (rc < 0 || r < rc - 15) {
+            rc = r;
+            bb0 = (i - 15 > 0 ? i - 15 : 0) >> 4;
+#pragma unroll
+            for (int m = 0; m < 16 / SUB; ++m) {
+                const int d = t + SUB * m, rr = rc - d;
+                uint32_t wv[4] = {0, 0, 0, 0};
+                Bounds br;
+                if (rr >= 0 && diag_bounds(rr, qlen, tlen, w, br)) {
+                    const int sb = br.st >> 4, eb = br.en >> 4;
+                    const uint8_t *row = tb + (size_t)rr * rowb;
+#pragma unroll
+                    for (int q2 = 0; q2 < 2; ++q2) {
+                        const int bb = bb0 + q2;
+                        if (bb >= sb && bb <= eb) {
+                            const uint64_t v = __hip_atomic_load((const uint64_t *)(row + (size_t)(bb - sb) * 8),
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            wv[2 * q2] = (uint32_t)v;
+                            wv[2 * q2 + 1] = (uint32_t)(v >> 32);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q2 = 0; q2 < 4; ++q2) chunk[d][q2] = wv[q2];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        Bounds b;
+        diag_bounds(r, qlen, tlen, w, b);
+        int force = -1, d = 0;
+        if (i < b.st) force = 2;
+        if (i > b.en) force = 1;
+        if (force < 0) {
+            const int c = i & 15;
+            const int dw = (c & 7) >> 2, sh = (((c & 1) << 1) | (c >> 3)) * 8 + (((c & 7) >> 1) & 1) * 4;
+            const uint32_t word = chunk[rc - r][2 * ((i >> 4) - bb0) + dw];
+            const uint32_t nib = (word >> sh) & 15u;
+            uint32_t fA = (nib >> 3) & 1, fB = (nib >> 2) & 1, fX = (nib >> 1) & 1, fY = nib & 1;
+            if (right) { fA ^= 1; fB ^= 1; fX ^= 1; fY ^= 1; }
+            d = (int)((fB ? 2u : fA) | (fX << 3) | (fY << 4));
+        }
+        if (state == 0) state = d & 7;
+        else if (!((d >> (state + 2)) & 1)) state = 0;
+        if (state == 0) state = d & 7;
+        if (force >= 0) state = force;
+        if (state == 0) { push(0, 1); --i; --j; }
+        else if (state == 1) { push(2, 1); --i; }
+        else { push(1, 1); --j; }
+    }
+    if (i >= 0) push(2, i + 1);
+    if (j >= 0) push(1, j + 1);
+    push(-1, 0);   // flush the last run
+    return nout;
+}
+
 template <int W, int S>
 struct WaveSmem {
     uint32_t xch[2][W][S][3];   // last lane of each wave: x, v of cell 15 and query window dword 3
-    uint32_t chunk[64][10];     // backtrack staging
+    union {
+        uint32_t chunk[64][10];       // backtrack staging (one job per workgroup)
+        uint32_t group[16][16][4];    // backtrack staging per job (sub-wave mode)
+    };
     uint8_t logu[2], logv[2];   // H-edge bytes of the previous anti-diagonal (W > 1)
     int job;
 };
 
-template <int W, int S, bool RT>
+// Sub-wave mode (SUB = 4 or 16, W = 1): each group of SUB lanes of the wave
+// runs its own job, so one wave advances 64/SUB narrow bands at once. The ring
+// is the group's SUB lanes x S slots (neighbours by DPP within the group), band
+// bounds are per group, the band-edge masks are computed from the bounds, the
+// H-edge bytes come from the owning lane by ds_bpermute one anti-diagonal later,
+// and the backtrack is group_backtrack.
+template <int W, int S, bool RT, int SUB = 0>
 __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restrict__ out,
                             WaveSmem<W, S> &sm)
 {
-    constexpr int NL = 64 * W;   // lanes of the workgroup
-    constexpr int R = NL * S;    // ring size in blocks
-    const int L = threadIdx.x, lane = L & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(L >> 6);
+    constexpr bool ROW = SUB != 0;
+    static_assert(!ROW || (W == 1 && (SUB == 4 || SUB == 16)), "sub-wave groups of 4 or 16 lanes in one wave");
+    constexpr int NL = ROW ? SUB : 64 * W;   // lanes per job
+    constexpr int R = NL * S;                // ring size in blocks
+    const int L = ROW ? (int)(threadIdx.x & (NL - 1)) : (int)threadIdx.x, lane = L & 63;
+    const int wave = ROW ? 0 : __builtin_amdgcn_readfirstlane(L >> 6);
     const int qlen = J.qlen, tlen = J.tlen;
     int w = J.w;
     if (w < 0) w = tlen > qlen ? tlen : qlen;
@@ -164,6 +269,7 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
         if (rr == ndiag - 1 && en0 == tlen - 1) score = He;
     };
     int p_st0 = 0, p_en0 = 0, p_en = 0, rlast = 0;   // bounds of the diagonal whose edge is pending
+    uint32_t pu = 0, pv = 0;                          // its edge bytes (row mode)
 
     // Bounds of r and r+1 are known at the top of diagonal r. Loads for a
     // diagonal (new block windows, the first block's new query base) are
@@ -196,10 +302,22 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
         cells += en0 - st0 + 1;
         // band-edge masks (uniform): cells < st0 of the first block, cells <= en0 of the last
         uint32_t leftm[8], lastm[8];
+        if constexpr (ROW) {
+            // per-row bounds: half (c < k) is all ones iff c - k < 0 as int16, for
+            // the cell pair (j, j+8) of register j
+            const uint32_t kl = (uint32_t)(st0 & 15) * 0x00010001u, ke1 = (uint32_t)((en0 & 15) + 1) * 0x00010001u;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            leftm[j] = kHalfMask.m[st0 & 15][j];
-            lastm[j] = kHalfMask.m[(en0 & 15) + 1][j];
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t cj = (uint32_t)j | (uint32_t)(j + 8) << 16;
+                leftm[j] = pk_ashr15(pk_sub(cj, kl));
+                lastm[j] = pk_ashr15(pk_sub(cj, ke1));
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                leftm[j] = kHalfMask.m[st0 & 15][j];
+                lastm[j] = kHalfMask.m[(en0 & 15) + 1][j];
+            }
         }
         int blk[S];
         bool act[S];
@@ -218,9 +336,15 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
             uint32_t rx[S], rv[S], rq[S];
 #pragma unroll
             for (int k = 0; k < S; ++k) {
-                rx[k] = wave_ror1(X[k][7]);
-                rv[k] = wave_ror1(V[k][7]);
-                rq[k] = wave_ror1(Q4[k][3]);
+                if constexpr (ROW) {
+                    rx[k] = group_ror1<NL>(X[k][7]);
+                    rv[k] = group_ror1<NL>(V[k][7]);
+                    rq[k] = group_ror1<NL>(Q4[k][3]);
+                } else {
+                    rx[k] = wave_ror1(X[k][7]);
+                    rv[k] = wave_ror1(V[k][7]);
+                    rq[k] = wave_ror1(Q4[k][3]);
+                }
             }
             if constexpr (W > 1) {
                 const int p = r & 1;
@@ -254,6 +378,10 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                     nx[k] = lane ? rx[k] : rx[kp];
                     nv[k] = lane ? rv[k] : rv[kp];
                     nq[k] = lane ? rq[k] : rq[kp];
+                }
+                // row mode: the edge bytes of diagonal r-1 (bpermute issued a diagonal ago)
+                if constexpr (ROW) {
+                    if (r > 0) edge(r - 1, p_st0, p_en0, p_en, (int)pu, (int)pv);
                 }
             }
         }
@@ -377,7 +505,12 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
             if (k == kv) vl_v = (uint32_t)cell_byte(V[k], tv & 15);
         }
 
-        if constexpr (W == 1) {
+        if constexpr (ROW) {
+            const int base = (int)(threadIdx.x & ~(uint32_t)(NL - 1));
+            pu = (uint32_t)__builtin_amdgcn_ds_bpermute((base + le) << 2, (int)ue_v);
+            pv = (uint32_t)__builtin_amdgcn_ds_bpermute((base + lv) << 2, (int)vl_v);
+            p_st0 = st0; p_en0 = en0; p_en = en;
+        } else if constexpr (W == 1) {
             edge(r, st0, en0, en, (int)__builtin_amdgcn_readlane(ue_v, le),
                  (int)__builtin_amdgcn_readlane(vl_v, lv));
         } else {
@@ -425,10 +558,18 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
         __syncthreads();
         edge(rlast, p_st0, p_en0, p_en, sm.logu[rlast & 1], sm.logv[rlast & 1]);
     }
+    if constexpr (ROW) {
+        edge(rlast, p_st0, p_en0, p_en, (int)pu, (int)pv);
+        // the row's traceback stores complete before its lanes read each other's rows
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
 
     int nout = 0, status = kStOk;
     if (zd) status = kStNeedStats;
-    else if (wave == 0) {
+    else if constexpr (ROW) {
+        nout = group_backtrack<NL>(J, tb, rowb, w, RT, tlen - 1, qlen - 1, sm.group[threadIdx.x / NL]);
+        if (nout > J.cig_cap) status = kStCigarOverflow;
+    } else if (wave == 0) {
         nout = backtrack_to_cigar(J, tb, rowb, w, RT, tlen - 1, qlen - 1, sm.chunk, 1);
         if (nout > J.cig_cap) status = kStCigarOverflow;
     }
@@ -477,16 +618,46 @@ __global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__rest
     }
 }
 
-// (W waves, S slots per lane); a shape serves n_col_ <= 64*W*S - 2
+// Sub-wave persistent kernel: one wave, G = 64/SUB jobs per ticket (group g
+// of SUB lanes takes job G*t + g); groups past the last job repeat the last
+// job, writing the same values to the same outputs. Group g of workgroup b owns
+// traceback slot G*b + g.
+template <int SUB, int S, bool RT>
+__global__ __launch_bounds__(64) void dp_sub_persistent(const DpJob *__restrict__ jobs, int njobs,
+                                                        int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
+                                                        size_t tb_slot_bytes, DpOut *__restrict__ outs)
+{
+    constexpr int G = 64 / SUB;
+    __shared__ WaveSmem<1, S> sm;
+    const int g = (int)(threadIdx.x / SUB);
+    uint8_t *tb = tb_pool + ((size_t)blockIdx.x * G + (size_t)g) * tb_slot_bytes;
+    for (;;) {
+        // ticket as in dp_wave_persistent: all lanes run the atomic, lane 0 adds 1
+        const int t = __builtin_amdgcn_readfirstlane(atomicAdd(counter, threadIdx.x == 0 ? 1 : 0));
+        if (G * t >= njobs) break;
+        const int jid = G * t + g < njobs ? G * t + g : njobs - 1;
+        const DpJob J = jobs[jid];
+        dp_wave_job<1, S, RT, SUB>(J, tb, &outs[jid], sm);
+    }
+}
+
+// Ring shapes: (W waves, S slots per lane) serve n_col_ <= 64*W*S - 2;
+// sub-wave shapes (SUB lanes x S slots per job, 64/SUB jobs per wave) serve
+// n_col_ <= SUB*S - 2.
 #define HAL_WAVE_SHAPES(X) \
     X(1, 1) X(1, 2) X(2, 1) X(1, 3) X(3, 1) X(2, 2) X(4, 1) X(5, 1) X(3, 2) X(6, 1) X(4, 2) X(8, 1) X(8, 2) X(16, 1)
+#define HAL_SUB_SHAPES(X) X(16, 1) X(16, 2) X(16, 3) X(4, 1) X(4, 2)
 
-static const int kWaveShapes[][2] = {
-#define X(w, s) {w, s},
+struct ShapeDesc { int waves, sub, slots; };   // waves = 0: sub-wave shape with groups of sub lanes
+static const ShapeDesc kShapes[] = {
+#define X(w, s) {w, 0, s},
     HAL_WAVE_SHAPES(X)
 #undef X
+#define X(g, s) {0, g, s},
+    HAL_SUB_SHAPES(X)
+#undef X
 };
-constexpr int kNumWaveShapes = sizeof(kWaveShapes) / sizeof(kWaveShapes[0]);
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 static const void *wave_kernel(int idx, bool rt)
 {
@@ -494,29 +665,55 @@ static const void *wave_kernel(int idx, bool rt)
 #define X(w, s) {(const void *)&dp_wave_persistent<w, s, false>, (const void *)&dp_wave_persistent<w, s, true>},
         HAL_WAVE_SHAPES(X)
 #undef X
+#define X(g, s) {(const void *)&dp_sub_persistent<g, s, false>, (const void *)&dp_sub_persistent<g, s, true>},
+        HAL_SUB_SHAPES(X)
+#undef X
     };
     return tab[idx][rt ? 1 : 0];
 }
 
-int dp_wave_num_shapes() { return kNumWaveShapes; }
-int dp_wave_shape_waves(int idx) { return kWaveShapes[idx][0]; }
-int dp_wave_shape_ring(int idx) { return 64 * kWaveShapes[idx][0] * kWaveShapes[idx][1]; }
+static int block_threads(int idx) { return kShapes[idx].waves ? 64 * kShapes[idx].waves : 64; }
+
+int dp_wave_num_shapes() { return kNumShapes; }
+int dp_wave_shape_waves(int idx) { return kShapes[idx].waves; }
+int dp_wave_shape_lanes(int idx) { return kShapes[idx].waves ? 64 * kShapes[idx].waves : kShapes[idx].sub; }
+int dp_wave_shape_ring(int idx) { return dp_wave_shape_lanes(idx) * kShapes[idx].slots; }
+int dp_wave_jobs_per_group(int idx) { return kShapes[idx].waves ? 1 : 64 / kShapes[idx].sub; }
+
+// Instructions per anti-diagonal of one wave with every slot active, counted
+// in the gfx950 code of each kernel's diagonal loop (hipcc -O3): all issued
+// instructions (one job's critical path: a wave issues one instruction at a
+// time) and VALU instructions (the SIMD throughput cost when waves share it).
+// 0: not counted (shapes for bands wider than a wave).
+void dp_wave_shape_cost(int idx, int *issue, int *valu)
+{
+    const ShapeDesc &d = kShapes[idx];
+    int is = 0, va = 0;
+    if (d.waves == 1 && d.slots == 1) { is = 695; va = 382; }
+    else if (d.waves == 1 && d.slots == 2) { is = 1250; va = 786; }
+    else if (d.waves == 0 && d.slots == 1) { is = 747; va = 580; }
+    else if (d.waves == 0 && d.slots == 2) { is = 1297; va = 1027; }
+    else if (d.waves == 0 && d.slots == 3) { is = 1779; va = 1441; }
+    *issue = is;
+    *valu = va;
+}
 
 int dp_wave_max_groups(int idx, bool rt, int num_cu)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(idx, rt), 64 * kWaveShapes[idx][0], 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(idx, rt), block_threads(idx), 0) !=
         hipSuccess)
         return 0;
     return per_cu * num_cu;
 }
 
+// tb_pool holds groups * dp_wave_jobs_per_group(idx) slots of tb_slot_bytes
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
                           uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream)
 {
     void *args[] = {(void *)&jobs, (void *)&njobs, (void *)&counter, (void *)&tb_pool,
                     (void *)&tb_slot_bytes, (void *)&outs};
-    return hipLaunchKernel(wave_kernel(idx, rt), dim3(groups), dim3(64 * kWaveShapes[idx][0]), args, 0, stream);
+    return hipLaunchKernel(wave_kernel(idx, rt), dim3(groups), dim3(block_threads(idx)), args, 0, stream);
 }
 
 }  // namespace hal

@@ -26,6 +26,9 @@ hipError_t dp_launch(int idx, bool stats, int groups, const DpJob *jobs, int njo
 int dp_wave_num_shapes();
 int dp_wave_shape_waves(int idx);
 int dp_wave_shape_ring(int idx);
+int dp_wave_jobs_per_group(int idx);
+int dp_wave_shape_lanes(int idx);
+void dp_wave_shape_cost(int idx, int *issue, int *valu);
 int dp_wave_max_groups(int idx, bool rt, int num_cu);
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
                           uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
@@ -90,13 +93,18 @@ int pick_shape(int ncol, bool stats, int policy)
         // Measured preference (config-2 band, n_col_ = 293, kernel GCUPS: 3x2 909,
         // 4x2 913, 5x1 861, 8x1 802, 6x1 786, 16x1 508): two slots per lane pay
         // for the per-diagonal work better than a tighter ring, one slot only for
-        // bands of a wave's width. First shape of this list that holds the band.
-        static const int kAuto[][2] = {{1, 1}, {1, 2}, {2, 2}, {3, 2}, {4, 2}, {8, 2}};
+        // bands of a wave's width. Narrow bands first go to the sub-wave shapes
+        // (waves 0: groups of 4 or 16 lanes, 16 or 4 jobs per wave); run_specs
+        // revisits that choice per batch (pick_bucket_shape). First shape of this
+        // list, {waves, lanes per job, slots}, that holds the band.
+        static const int kAuto[][3] = {{0, 4, 1},  {0, 4, 2},  {0, 16, 1},  {0, 16, 2}, {0, 16, 3},
+                                       {1, 64, 1}, {1, 64, 2}, {2, 128, 2}, {3, 192, 2}, {4, 256, 2},
+                                       {8, 512, 2}};
         int best = -1;
         for (const auto &ws : kAuto) {
             for (int i = 0; i < nw && best < 0; ++i)
-                if (dp_wave_shape_waves(i) == ws[0] && dp_wave_shape_ring(i) == 64 * ws[0] * ws[1] &&
-                    ncol <= dp_wave_shape_ring(i) - 2)
+                if (dp_wave_shape_waves(i) == ws[0] && dp_wave_shape_lanes(i) == ws[1] &&
+                    dp_wave_shape_ring(i) == ws[1] * ws[2] && ncol <= dp_wave_shape_ring(i) - 2)
                     best = i;
             if (best >= 0) break;
         }
@@ -106,6 +114,39 @@ int pick_shape(int ncol, bool stats, int policy)
     for (int i = 0; i < ns; ++i)
         if (ncol <= dp_shape_ring(i) - 1) return i;
     return -1;
+}
+
+// Batch-level shape choice for a bucket of narrow-band CIGAR jobs (automatic
+// policy): among the one-wave and sub-wave shapes that hold the bucket's widest
+// band, the one with the least estimated time max(longest job's anti-diagonals
+// x instructions per anti-diagonal, all anti-diagonals x VALU instructions per
+// anti-diagonal / (jobs per wave x SIMDs)). Many short bands favour sub-wave
+// shapes (more jobs per instruction), few long ones the one-wave ring (shorter
+// critical path per anti-diagonal).
+static int pick_bucket_shape(const std::vector<Spec> &specs, const std::vector<int> &b, int shape, int num_cu)
+{
+    const int wi = shape - kWaveBase;
+    if (dp_wave_shape_waves(wi) > 1) return shape;
+    int max_nc = 0;
+    double max_nd = 0, sum_nd = 0;
+    for (int i : b) {
+        const Spec &s = specs[i];
+        const double nd = (double)s.qlen + s.tlen - 1;
+        max_nc = std::max(max_nc, n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen)));
+        max_nd = std::max(max_nd, nd);
+        sum_nd += nd;
+    }
+    const double nsimd = 4.0 * num_cu;
+    int best = shape;
+    double best_t = -1;
+    for (int i = 0; i < dp_wave_num_shapes(); ++i) {
+        int is = 0, va = 0;
+        dp_wave_shape_cost(i, &is, &va);
+        if (!is || max_nc > dp_wave_shape_ring(i) - 2) continue;
+        const double t = std::max(max_nd * is, sum_nd * va / (dp_wave_jobs_per_group(i) * nsimd));
+        if (best_t < 0 || t < best_t) { best_t = t; best = kWaveBase + i; }
+    }
+    return best;
 }
 
 // Run a list of jobs whose sequences already sit in ctx->seq (spread u16) and
@@ -128,6 +169,19 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     for (size_t i = 0; i < n; ++i)
         buckets[specs[i].shape * 4 + (specs[i].stats ? 2 : 0) + ((specs[i].flags & kFlagRight) ? 1 : 0)]
             .push_back((int)i);
+    if (ctx->policy == 0 && !plain) {
+        std::vector<std::vector<int>> moved(nb);
+        for (int bi = 0; bi < nb; ++bi) {
+            auto &b = buckets[bi];
+            if (b.empty()) continue;
+            const int shape = bi / 4;
+            const int to = (shape >= kWaveBase && !(bi & 2)) ? pick_bucket_shape(specs, b, shape, ctx->num_cu) : shape;
+            for (int i : b) specs[i].shape = to;
+            auto &dst = moved[to * 4 + (bi & 3)];
+            dst.insert(dst.end(), b.begin(), b.end());
+        }
+        buckets.swap(moved);
+    }
 
     HCHECK(ctx->jobs.ensure(n * sizeof(DpJob)), "alloc jobs");
     HCHECK(ctx->outs.ensure(n * sizeof(DpOut)), "alloc outs");
@@ -135,8 +189,16 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     std::vector<DpJob> hjobs(n);
     std::vector<int> order;
     order.reserve(n);
-    for (auto &b : buckets)
+    for (int bi = 0; bi < nb; ++bi) {
+        auto &b = buckets[bi];
+        // row shapes run four jobs in one wave: neighbours of similar length
+        // (longest first) keep the rows of a wave busy for the same time
+        if (bi / 4 >= kWaveBase && dp_wave_jobs_per_group(bi / 4 - kWaveBase) > 1)
+            std::stable_sort(b.begin(), b.end(), [&](int x, int y) {
+                return specs[x].qlen + specs[x].tlen > specs[y].qlen + specs[y].tlen;
+            });
         for (int i : b) order.push_back(i);
+    }
     for (size_t k = 0; k < n; ++k) {
         const Spec &s = specs[order[k]];
         DpJob &j = hjobs[k];
@@ -176,8 +238,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         slot = (slot + 255) & ~(size_t)255;
         const bool wave = shape >= kWaveBase;
         const int wi = shape - kWaveBase;
+        const size_t jpg = wave ? (size_t)dp_wave_jobs_per_group(wi) : 1;   // traceback slots per workgroup
         if (plain) {
-            HCHECK(ctx->tb.ensure(slot), "alloc traceback");
+            HCHECK(ctx->tb.ensure(slot * jpg), "alloc traceback");
             for (size_t k = 0; k < b.size(); ++k) {
                 hipError_t e = wave ? dp_wave_launch(wi, rt, 1, djobs + pos + k, 1, ctr + pos + k,
                                                      ctx->tb.as<uint8_t>(), slot, douts + pos + k, ctx->stream)
@@ -189,9 +252,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             int groups = wave ? dp_wave_max_groups(wi, rt, ctx->num_cu)
                               : dp_shape_max_groups(shape, stats, ctx->num_cu);
             if (groups <= 0) return fail(ctx, HALIGN_E_HIP, "occupancy query failed");
-            groups = std::min<size_t>(groups, b.size());
-            groups = (int)std::min<size_t>(groups, std::max<size_t>(1, ctx->tb_budget / slot));
-            HCHECK(ctx->tb.ensure(slot * groups), "alloc traceback pool");
+            groups = (int)std::min<size_t>(groups, (b.size() + jpg - 1) / jpg);
+            groups = (int)std::min<size_t>(groups, std::max<size_t>(1, ctx->tb_budget / (slot * jpg)));
+            HCHECK(ctx->tb.ensure(slot * jpg * groups), "alloc traceback pool");
             hipError_t e = wave ? dp_wave_launch(wi, rt, groups, djobs + pos, (int)b.size(), ctr + bi,
                                                  ctx->tb.as<uint8_t>(), slot, douts + pos, ctx->stream)
                                 : dp_launch(shape, stats, groups, djobs + pos, (int)b.size(), ctr + bi,
@@ -304,11 +367,12 @@ int halign_band_info(int32_t qlen, int32_t tlen, int32_t w, int32_t *ncol, int32
     return HALIGN_OK;
 }
 
-int halign_ring_shapes(int32_t *waves, int32_t *ring, int32_t cap)
+int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t cap)
 {
     const int n = dp_wave_num_shapes();
     for (int i = 0; i < n && i < cap; ++i) {
         if (waves) waves[i] = dp_wave_shape_waves(i);
+        if (lanes) lanes[i] = dp_wave_shape_lanes(i);
         if (ring) ring[i] = dp_wave_shape_ring(i);
     }
     return n;

@@ -67,7 +67,7 @@ struct halign_center {
     // per-batch buffers
     DevBuf hcnt, scratch, hidx, nanch, totals, acount, aoff, anchors, keys, keys2, vals, vals2, sort_tmp;
     DevBuf fw, pw, tw, chain_idx, chain_cnt, qlens, soff, segs, nseg, caps, cig_off, scan_tmp, region, bregion;
-    DevBuf qrev_off, lane_ws, ncig, hlist, hcount, hsegs, hvals, ooff, out, nout;
+    DevBuf qrev_off, lane_ws, ncig, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
 };
 
 namespace {
@@ -77,6 +77,26 @@ __global__ void gather_segs(const SegRec *__restrict__ segs, const int64_t *__re
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = segs[list[i]];
+}
+
+// CIGAR lengths for the output scan; a negative length (a CIGAR that does not
+// consume the pair) counts 0 here and fails the call on the host
+__global__ void clamp_counts(const int64_t *__restrict__ nout, int n, int64_t *__restrict__ cnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = nout[i] > 0 ? nout[i] : 0;
+}
+
+// one workgroup per query: its CIGAR from the per-query capacity layout to the packed output
+__global__ void compact_cigars(const uint32_t *__restrict__ src, const int64_t *__restrict__ ooff,
+                               const int64_t *__restrict__ cnt, const int64_t *__restrict__ coff,
+                               uint32_t *__restrict__ dst)
+{
+    const int q = blockIdx.x;
+    const uint32_t *a = src + ooff[q];
+    uint32_t *b = dst + coff[q];
+    const int64_t m = cnt[q];
+    for (int64_t k = threadIdx.x; k < m; k += blockDim.x) b[k] = a[k];
 }
 
 __global__ void scatter_ncig(const int64_t *__restrict__ list, const int32_t *__restrict__ vals, int64_t n,
@@ -124,6 +144,7 @@ struct Batch {
     std::vector<int64_t> off, qlen, hit_off, hit_cnt, aoff, na;
     std::vector<int32_t> seed_status, chain_cnt, nseg;
     std::vector<int64_t> soff, ooff, nout;
+    int64_t packed = 0;   // total CIGAR words of the batch (packed in c->packed at c->coff)
     int64_t anchors = 0, segs = 0, host_segs = 0, lane_segs = 0, cells_host = 0;
     double ms = 0;
 };
@@ -359,7 +380,31 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                                (uint32_t)c->rlen, c->ooff.as<int64_t>(), n, c->out.as<uint32_t>(),
                                c->nout.as<int64_t>(), st),
            "launch assemble");
-    HCHECK(down(b.nout, c->nout.p, n, st), "download CIGAR lengths");
+    // packed output: exclusive scan of the lengths, then one copy per query
+    HCHECK(c->ocnt.ensure((size_t)(n + 1) * 8), "alloc");
+    HCHECK(c->coff.ensure((size_t)(n + 1) * 8), "alloc");
+    hipLaunchKernelGGL(clamp_counts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c->nout.as<int64_t>(), n,
+                       c->ocnt.as<int64_t>());
+    HCHECK(hipGetLastError(), "launch clamp");
+    HCHECK(hipMemsetAsync(c->ocnt.as<int64_t>() + n, 0, 8, st), "clear");
+    {
+        size_t tmp = 0;
+        HCHECK(rocprim::exclusive_scan(nullptr, tmp, c->ocnt.as<int64_t>(), c->coff.as<int64_t>(), (int64_t)0,
+                                       (size_t)n + 1, rocprim::plus<int64_t>(), st),
+               "scan storage size");
+        HCHECK(c->scan_tmp.ensure(std::max<size_t>(tmp, 1)), "alloc scan storage");
+        HCHECK(rocprim::exclusive_scan(c->scan_tmp.p, tmp, c->ocnt.as<int64_t>(), c->coff.as<int64_t>(), (int64_t)0,
+                                       (size_t)n + 1, rocprim::plus<int64_t>(), st),
+               "scan CIGAR offsets");
+    }
+    HCHECK(c->packed.ensure((size_t)std::max<int64_t>(b.ooff[n], 1) * 4), "alloc packed output");
+    hipLaunchKernelGGL(compact_cigars, dim3((unsigned)n), dim3(256), 0, st, c->out.as<uint32_t>(),
+                       c->ooff.as<int64_t>(), c->ocnt.as<int64_t>(), c->coff.as<int64_t>(), c->packed.as<uint32_t>());
+    HCHECK(hipGetLastError(), "launch compact");
+    b.nout.resize(n);
+    HCHECK(hipMemcpyAsync(b.nout.data(), c->nout.p, (size_t)n * 8, hipMemcpyDeviceToHost, st), "download");
+    HCHECK(hipMemcpyAsync(&b.packed, c->coff.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, st), "download");
+    HCHECK(hipStreamSynchronize(st), "assembly");
     return HALIGN_OK;
 }
 
@@ -461,7 +506,7 @@ void halign_center_free(halign_center *c)
                       &c->vals2, &c->sort_tmp, &c->fw, &c->pw, &c->tw, &c->chain_idx, &c->chain_cnt, &c->qlens,
                       &c->soff, &c->segs, &c->nseg, &c->caps, &c->cig_off, &c->scan_tmp, &c->region, &c->bregion,
                       &c->qrev_off, &c->lane_ws, &c->ncig, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
-                      &c->out, &c->nout})
+                      &c->out, &c->nout, &c->ocnt, &c->coff, &c->packed})
         b->release();
     delete c;
 }
@@ -536,7 +581,6 @@ int halign_center_align(halign_center *c, int32_t n, const char *queries, const 
     }
     HCHECK(hipEventRecord(e1, ctx->stream), "event");
     int64_t o = 0;
-    std::vector<uint32_t> buf;
     for (int i = 0; i < n; ++i) {
         if (b.nout[i] < 0) {
             (void)hipEventDestroy(e0);
@@ -550,11 +594,13 @@ int halign_center_align(halign_center *c, int32_t n, const char *queries, const 
         (void)hipEventDestroy(e1);
         return fail(ctx, HALIGN_E_CAPACITY, "CIGAR output too small");
     }
-    HCHECK(down(buf, c->out.p, (size_t)b.ooff[n], ctx->stream), "download CIGARs");
+    if (cigar_out && b.packed > 0)
+        HCHECK(hipMemcpyAsync(cigar_out, c->packed.p, (size_t)b.packed * 4, hipMemcpyDeviceToHost, ctx->stream),
+               "download CIGARs");
+    HCHECK(hipStreamSynchronize(ctx->stream), "download CIGARs");
     o = 0;
     for (int i = 0; i < n; ++i) {
         cigar_offsets[i] = o;
-        if (cigar_out && b.nout[i]) memcpy(cigar_out + o, buf.data() + b.ooff[i], (size_t)b.nout[i] * 4);
         o += b.nout[i];
         if (stats) {
             stats[4 * i + 0] = b.chain_cnt[i] ? 2 * b.chain_cnt[i] + 1 : 1;

@@ -47,7 +47,7 @@ def _load():
     lib.halign_last_error.restype = C.c_char_p
     lib.halign_version.restype = C.c_char_p
     lib.halign_set_kernel_policy.argtypes = [_P, C.c_int]
-    lib.halign_ring_shapes.argtypes = [_P, _P, C.c_int32]
+    lib.halign_ring_shapes.argtypes = [_P, _P, _P, C.c_int32]
     lib.halign_dp_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, _P,
                                     C.c_int64, C.POINTER(C.c_double)]
     lib.halign_global_align_batch.argtypes = [_P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int64,
@@ -91,20 +91,25 @@ def _ptr(a):
 
 
 def ring_shapes():
-    """[(waves per workgroup, ring size in blocks)] of the ring-form kernel, by shape index."""
-    n = lib().halign_ring_shapes(None, None, 0)
+    """[(waves per workgroup, lanes per job, ring size in blocks)] of the
+    ring-form kernel, by shape index; waves 0 is a sub-wave shape (groups of 4
+    or 16 lanes, one job each)."""
+    n = lib().halign_ring_shapes(None, None, None, 0)
     w = np.zeros(n, np.int32)
+    g = np.zeros(n, np.int32)
     r = np.zeros(n, np.int32)
-    lib().halign_ring_shapes(_ptr(w), _ptr(r), n)
-    return [(int(a), int(b)) for a, b in zip(w, r)]
+    lib().halign_ring_shapes(_ptr(w), _ptr(g), _ptr(r), n)
+    return [(int(a), int(b), int(c)) for a, b, c in zip(w, g, r)]
 
 
-def ring_policy(waves, slots):
-    """Kernel policy that forces the ring shape (waves, slots per lane)."""
-    for i, (w, r) in enumerate(ring_shapes()):
-        if w == waves and r == 64 * waves * slots:
+def ring_policy(waves, slots, lanes=None):
+    """Kernel policy that forces the ring shape (waves, slots per lane); waves 0
+    selects the sub-wave shape with groups of `lanes` lanes (default 16)."""
+    lanes = 64 * waves if waves else (lanes or 16)
+    for i, (w, g, r) in enumerate(ring_shapes()):
+        if w == waves and g == lanes and r == lanes * slots:
             return 2 + i
-    raise ValueError(f"no ring shape with {waves} waves and {slots} slots")
+    raise ValueError(f"no ring shape with {waves} waves, {lanes} lanes and {slots} slots")
 
 
 class Context:
@@ -286,16 +291,27 @@ class Center:
     def align(self, queries):
         """RefAligner::globalAlign per query: (list of CIGAR arrays, stats[n,4], ms)."""
         blob, off = _blob(queries)
+        cig, coff, st, ms = self.align_flat(blob, off)
+        return [cig[coff[i]:coff[i + 1]] for i in range(len(off) - 1)], st, ms
+
+    def align_flat(self, blob, off):
+        """The same on concatenated queries (bytes or uint8 array, int64
+        offsets): (flat CIGAR array, CIGAR offsets, stats[n,4], ms)."""
+        keep = None
+        if not isinstance(blob, (bytes, bytearray)):   # a uint8 array is passed in place
+            keep = np.ascontiguousarray(blob, np.uint8)
+            blob = C.cast(_ptr(keep), C.c_char_p)
+        off = np.ascontiguousarray(off, np.int64)
         n = len(off) - 1
-        cap = int(off[-1]) + n * (self.rlen + 1) + 1
-        cig = np.zeros(cap, np.uint32)
+        cap = int(off[-1] - off[0]) + n * (self.rlen + 1) + 1
+        cig = np.empty(cap, np.uint32)   # only the first coff[n] words are written
         coff = np.zeros(n + 1, np.int64)
         st = np.zeros((max(n, 1), 4), np.int32)
         ms = C.c_double(0)
         self.ctx._check(self.ctx.lib.halign_center_align(self.h, n, blob, _ptr(off), _ptr(cig), cap,
                                                          _ptr(coff), _ptr(st), C.byref(ms)),
                         "halign_center_align")
-        return [cig[coff[i]:coff[i + 1]].copy() for i in range(n)], st[:n], ms.value
+        return cig, coff, st[:n], ms.value
 
 
 class PairSet:

@@ -77,9 +77,10 @@ int halign_set_kernel_policy(halign_ctx *ctx, int policy);
 int halign_band_info(int32_t qlen, int32_t tlen, int32_t w, int32_t *ncol, int32_t *overflow,
                      int64_t *cells);
 /* Ring shapes of the ring-form kernel: returns their number and fills up to
- * cap entries of waves per workgroup and ring size in 16-cell blocks (a shape
- * serves bands of n_col_ <= ring - 2 blocks). No reference counterpart. */
-int halign_ring_shapes(int32_t *waves, int32_t *ring, int32_t cap);
+ * cap entries of waves per workgroup (0 for a sub-wave shape, whose wave runs
+ * 64/lanes jobs at once), lanes per job and ring size in 16-cell blocks (a
+ * shape serves bands of n_col_ <= ring - 2 blocks). No reference counterpart. */
+int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t cap);
 
 /* Drop-in for ksw_extz2_sse (reference include/ksw2.h:64-65, called at
  * src/align/align.cpp:49-55): the plain one-pair-per-launch GPU form with every

@@ -130,6 +130,20 @@ int ref_ref_global_align(const char* ref, int rlen, const char* qry, int qlen, u
     return copy_cigar(align::globalAlignMM2(r, q, anchors), cigar, cap);
 }
 
+// The same with the center's minimizers computed once by the caller, as
+// RefAligner does for its centers (src/align/ref_aligner.cpp:40-41,224-225).
+int ref_ref_global_align_pre(const char* ref, int rlen, const uint64_t* rx, const uint64_t* ry, int nref,
+                             const char* qry, int qlen, uint32_t* cigar, int cap)
+{
+    const std::string r(ref, (size_t)rlen), q(qry, (size_t)qlen);
+    minimizer::MinimizerHits rm;
+    rm.reserve((size_t)nref);
+    for (int i = 0; i < nref; ++i) rm.emplace_back(rx[i], ry[i]);
+    auto qm = minimizer::extractMinimizer(q, 15, 10, true);
+    auto anchors = minimizer::collect_anchors(rm, qm);
+    return copy_cigar(align::globalAlignMM2(r, q, anchors), cigar, cap);
+}
+
 // consensus::generateConsensusSequence on a FASTA file; the per-column counts
 // land in out_json (cereal JSON) for the caller to read.
 int ref_consensus(const char* aligned_fasta, const char* out_fasta, const char* out_json,

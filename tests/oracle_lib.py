@@ -259,6 +259,8 @@ class RefLib(_Base):
         L.ref_global_align_mm2.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _P, C.c_int,
                                            _P, C.c_int]
         L.ref_ref_global_align.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _P, C.c_int]
+        L.ref_ref_global_align_pre.argtypes = [C.c_char_p, C.c_int, _P, _P, C.c_int, C.c_char_p, C.c_int,
+                                               _P, C.c_int]
         L.ref_consensus.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_int,
                                     C.c_int, C.c_char_p, C.c_int]
 
@@ -323,6 +325,19 @@ class RefLib(_Base):
         cap = len(ref) + len(qry) + 4
         cig = np.zeros(cap, np.uint32)
         n = self.lib.ref_ref_global_align(ref, len(ref), qry, len(qry), _ptr(cig), cap)
+        assert n >= 0
+        return cig[:n].copy()
+
+    def ref_global_align_pre(self, ref, ref_hits, qry):
+        """RefAligner::globalAlign with the center's minimizers given (x, y arrays)."""
+        ref = ref.encode() if isinstance(ref, str) else ref
+        qry = qry.encode() if isinstance(qry, str) else qry
+        rx = np.ascontiguousarray(ref_hits[0], np.uint64)
+        ry = np.ascontiguousarray(ref_hits[1], np.uint64)
+        cap = len(ref) + len(qry) + 4
+        cig = np.zeros(cap, np.uint32)
+        n = self.lib.ref_ref_global_align_pre(ref, len(ref), _ptr(rx), _ptr(ry), len(rx), qry, len(qry),
+                                              _ptr(cig), cap)
         assert n >= 0
         return cig[:n].copy()
 

@@ -49,11 +49,14 @@ def test_ring_shapes(lib):
     import halign
     shapes = halign.ring_shapes()
     assert len(shapes) >= 8
-    for waves, ring in shapes:
-        assert 1 <= waves <= 16 and ring % (64 * waves) == 0
+    for waves, lanes, ring in shapes:
+        assert 0 <= waves <= 16 and ring % lanes == 0
+        assert lanes == 64 * waves if waves else lanes in (4, 16)
+    assert {g for w, g, _ in shapes if w == 0} == {4, 16}
     # the config-2 band (n_col_ = 293 blocks) has a shape that holds it
-    assert any(ring - 2 >= 293 for _, ring in shapes)
+    assert any(ring - 2 >= 293 for _, _, ring in shapes)
     assert halign.ring_policy(5, 1) >= 2
+    assert halign.ring_policy(0, 2) >= 2 and halign.ring_policy(0, 1, lanes=4) >= 2
 
 
 def test_null_context_is_an_argument_error(lib):

@@ -18,11 +18,12 @@ FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "
 
 
 # auto, multi-wave only, and forced ring shapes (waves, slots per lane) so that
-# small cases also run through the multi-wave LDS exchange of the ring kernel
-POLICIES = ["auto", "multiwave", (2, 1), (3, 2), (5, 1)]
+# small cases also run through the multi-wave LDS exchange of the ring kernel;
+# waves 0: sub-wave shapes (slots, lanes per job), forced where the band fits them
+POLICIES = ["auto", "multiwave", (2, 1), (3, 2), (5, 1), (0, 1, 16), (0, 3, 16), (0, 1, 4), (0, 2, 4)]
 
 
-@pytest.fixture(scope="module", params=POLICIES, ids=lambda p: p if isinstance(p, str) else f"ring{p[0]}x{p[1]}")
+@pytest.fixture(scope="module", params=POLICIES, ids=lambda p: p if isinstance(p, str) else (f"ring{p[0]}x{p[1]}" if p[0] else f"sub{p[2]}x{p[1]}"))
 def ctx(request):
     import halign
     c = halign.Context(0)
