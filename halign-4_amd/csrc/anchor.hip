@@ -160,19 +160,22 @@ __global__ __launch_bounds__(256) void anchors_write(CenterIndexDev ci, const ui
 }
 
 // chainScoreSimple (src/seed/anchor.cpp:254-301); INT_MIN = no link
-__device__ __forceinline__ int32_t chain_score(const AnchorRec &ai, const AnchorRec &aj, const ChainParamsDev &cp,
-                                               const float *__restrict__ lut)
+// Chaining score of predecessor j for anchor i (src/seed/anchor.cpp:232-262):
+// fields of j, anchor i in scalars. INT_MIN: j cannot precede i.
+__device__ __forceinline__ int32_t chain_score(uint32_t i_rid, uint32_t i_rev, uint32_t i_pr, uint32_t i_pq,
+                                               uint32_t j_rid, uint32_t j_rev, uint32_t j_pr, uint32_t j_pq,
+                                               uint32_t j_span, const ChainParamsDev &cp, const float *__restrict__ lut)
 {
-    if (ai.rid_ref != aj.rid_ref) return INT_MIN;
-    if (ai.is_rev != aj.is_rev) return INT_MIN;
-    const int32_t dq = (int32_t)ai.pos_qry - (int32_t)aj.pos_qry;
+    if (i_rid != j_rid) return INT_MIN;
+    if (i_rev != j_rev) return INT_MIN;
+    const int32_t dq = (int32_t)i_pq - (int32_t)j_pq;
     if (dq <= 0 || dq > cp.max_dist_x) return INT_MIN;
-    const int32_t dr = (int32_t)ai.pos_ref - (int32_t)aj.pos_ref;
+    const int32_t dr = (int32_t)i_pr - (int32_t)j_pr;
     if (dr <= 0 || dr > cp.max_dist_y) return INT_MIN;
     const int32_t dd = dr > dq ? dr - dq : dq - dr;
     if (dd > cp.bw) return INT_MIN;
     const int32_t dg = dr < dq ? dr : dq;
-    const int32_t q_span = (int32_t)aj.span;
+    const int32_t q_span = (int32_t)j_span;
     int32_t sc = q_span < dg ? q_span : dg;
     if (dd > 0 || dg > q_span) {
         // each operation rounded on its own, no contraction: the reference's
@@ -184,68 +187,218 @@ __device__ __forceinline__ int32_t chain_score(const AnchorRec &ai, const Anchor
     return sc;
 }
 
-// One lane per query. order: anchors of query q sorted by (pos_ref, pos_qry)
-// as indices into `anchors` at [aoff[q], aoff[q] + na[q]); f, pr, tk: int32
-// workspaces of the same shape. The best chain goes to chain_idx at aoff[q]
-// (anchor indices, in sorted order) and its length to chain_cnt[q].
-__global__ void chain_queries(const AnchorRec *__restrict__ anchors, const uint32_t *__restrict__ order,
-                              const int64_t *__restrict__ aoff, const int64_t *__restrict__ na_all, int nq_all,
-                              ChainParamsDev cp, const float *__restrict__ lut, int32_t *__restrict__ fw,
-                              int32_t *__restrict__ pw, int32_t *__restrict__ tw, uint32_t *__restrict__ chain_idx,
-                              int32_t *__restrict__ chain_cnt)
+// lane l receives lane l-1 (lane 0 keeps its own value): DPP wave_shr:1
+__device__ __forceinline__ int32_t wave_shr1(int32_t v)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq_all) return;
-    const int64_t base = aoff[q], n = na_all[q];
+    return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);
+}
+
+constexpr int kChainMaxIter = 8192;   // bound on ChainParams::max_iter (LDS mark bitmap)
+
+// chainAnchors (src/seed/anchor.cpp:264-330), one wave per query. order:
+// anchors of query q sorted by (pos_ref, pos_qry), as indices into `anchors`
+// at [aoff[q], aoff[q] + na[q]); f, pr: int32 workspaces of the same shape.
+// The best chain goes to chain_idx at aoff[q] (anchor indices, in sorted
+// order) and its length to chain_cnt[q].
+//
+// The predecessor scan of anchor i (j = i-1 down to iter_start, stopping after
+// max_skip skipped predecessors) runs 64 candidates at a time: lane l takes
+// j = jtop - l. A candidate "improves" iff its total beats the running maximum
+// of the candidates before it (an exclusive prefix max over lanes); the skip
+// counter then runs over the improving / skipped candidates of the chunk in
+// order (ballot bit masks, scalar loop), which gives the break position, and
+// the last improving candidate before it gives f[i] and pr[i]. The reference's
+// mark t[pr[j]] = i, set by every scanned candidate j with a score, decides
+// "skipped" for a later candidate: t[j] == i iff some scored candidate in
+// (j, i-1] has predecessor j, so the marks of a whole chunk may be set before
+// any of its candidates are tested (a candidate below the break marks only
+// indices below the break). They live in an LDS bitmap over the scan window.
+// The 64 most recent anchors (fields, f, pr) ride in a register ring shifted one
+// lane per anchor; older candidates (chunks after the first) come from memory.
+__global__ __launch_bounds__(64) void chain_wave(const AnchorRec *__restrict__ anchors,
+                                                 const uint32_t *__restrict__ order, const int64_t *__restrict__ aoff,
+                                                 const int64_t *__restrict__ na_all, ChainParamsDev cp,
+                                                 const float *__restrict__ lut, int32_t *__restrict__ fw,
+                                                 int32_t *__restrict__ pw, uint32_t *__restrict__ chain_idx,
+                                                 int32_t *__restrict__ chain_cnt)
+{
+    __shared__ uint32_t marks[kChainMaxIter / 32];
+    const int q = blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    const int64_t base = aoff[q];
+    const int n = (int)na_all[q];
     const uint32_t *ord = order + base;
-    int32_t *f = fw + base, *p = pw + base, *t = tw + base;
-    for (int64_t i = 0; i < n; ++i) t[i] = 0;
-    int64_t st = 0, best_end = -1;
+    int32_t *f = fw + base, *pr = pw + base;
+
+    // ring: lane l holds anchor i-1-l at the top of iteration i
+    uint32_t r_pr = 0, r_pq = 0, r_sp = 0, r_rr = 0;   // r_rr = rid_ref * 2 + is_rev
+    int32_t r_f = 0, r_p = -1;
+    // st window: lane l holds anchor st_base + l (pos_ref, rid_ref * 2 + is_rev)
+    int st = 0, st_base = 0;
+    uint32_t s_pr = 0, s_rr = 0;
+    auto load_st = [&]() {
+        const int k = st_base + lane;
+        if (k < n) {
+            const AnchorRec a = anchors[ord[k]];
+            s_pr = a.pos_ref;
+            s_rr = a.rid_ref * 2u + a.is_rev;
+        }
+    };
+    load_st();
+    // ord block: lane l holds ord[ob + l]
+    int ob = 0;
+    uint32_t o_v = lane < n ? ord[lane] : 0u;
     int32_t max_f_global = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        const AnchorRec ai = anchors[ord[i]];
-        int32_t max_f = (int32_t)ai.span;
-        int64_t max_j = -1;
-        int32_t n_skip = 0;
-        while (st < i) {
-            const AnchorRec as = anchors[ord[st]];
-            if (as.rid_ref != ai.rid_ref || as.is_rev != ai.is_rev ||
-                (int32_t)(ai.pos_ref - as.pos_ref) > cp.max_dist_x)
-                ++st;
-            else
+    int best_end = -1;
+
+    for (int i = 0; i < n; ++i) {
+        if (i - ob >= 64) {
+            ob += 64;
+            o_v = ob + lane < n ? ord[ob + lane] : 0u;
+        }
+        const uint32_t oi = __builtin_amdgcn_readlane(o_v, i - ob);
+        const AnchorRec ai = anchors[oi];
+        const uint32_t ai_rr = ai.rid_ref * 2u + ai.is_rev;
+
+        // st: first anchor from st on that is in range of i (the reference's while loop)
+        for (;;) {
+            const int k = st_base + lane;
+            const bool cand = k >= st && k < i;
+            const bool out = s_rr != ai_rr || (int32_t)(ai.pos_ref - s_pr) > cp.max_dist_x;
+            const uint64_t stop = __ballot(cand && !out);
+            if (stop) {
+                st = st_base + (int)__builtin_ctzll(stop);
                 break;
-        }
-        const int64_t iter_start = (i - st > cp.max_iter) ? (i - cp.max_iter) : st;
-        for (int64_t j = i - 1; j >= iter_start; --j) {
-            const int32_t sc = chain_score(ai, anchors[ord[j]], cp, lut);
-            if (sc == INT_MIN) continue;
-            const int32_t total = f[j] + sc;
-            if (total > max_f) {
-                max_f = total;
-                max_j = j;
-                if (n_skip > 0) --n_skip;
-            } else if (t[j] == (int32_t)i) {
-                if (++n_skip > cp.max_skip) break;
             }
-            if (p[j] >= 0) t[p[j]] = (int32_t)i;
+            const int lim = i < st_base + 64 ? i : st_base + 64;
+            if (st < lim) st = lim;
+            if (st >= i) break;
+            st_base = st;
+            load_st();
         }
-        f[i] = max_f;
-        p[i] = (int32_t)max_j;
+        const int iter_start = (i - st > cp.max_iter) ? i - cp.max_iter : st;
+
+        int32_t max_f = (int32_t)ai.span;
+        int max_j = -1, n_skip = 0;
+        const int wdw = (i - iter_start + 31) >> 5;   // bitmap words of the window
+        for (int d = lane; d < wdw; d += 64) marks[d] = 0u;
+        for (int jtop = i - 1; jtop >= iter_start; jtop -= 64) {
+            const int j = jtop - lane;
+            const bool inwin = j >= iter_start;
+            uint32_t a_pr, a_pq, a_sp, a_rr;
+            int32_t a_f, a_p;
+            if (jtop == i - 1) {
+                a_pr = r_pr; a_pq = r_pq; a_sp = r_sp; a_rr = r_rr; a_f = r_f; a_p = r_p;
+            } else {
+                // values written at least 64 anchors ago; complete this wave's stores first
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                a_pr = a_pq = a_sp = a_rr = 0;
+                a_f = 0; a_p = -1;
+                if (inwin) {
+                    const AnchorRec aj = anchors[ord[j]];
+                    a_pr = aj.pos_ref; a_pq = aj.pos_qry; a_sp = aj.span; a_rr = aj.rid_ref * 2u + aj.is_rev;
+                    a_f = __hip_atomic_load(f + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    a_p = __hip_atomic_load(pr + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            int32_t sc = INT_MIN;
+            if (inwin)
+                sc = chain_score(ai_rr >> 1, ai_rr & 1u, ai.pos_ref, ai.pos_qry, a_rr >> 1, a_rr & 1u, a_pr, a_pq,
+                                 a_sp, cp, lut);
+            const bool valid = sc != INT_MIN;
+            const int32_t total = valid ? a_f + sc : INT_MIN;
+            if (valid && a_p >= iter_start) {
+                const int bt = i - 1 - a_p;
+                atomicOr(&marks[bt >> 5], 1u << (bt & 31));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const int bj = inwin ? i - 1 - j : 0;
+            const bool marked = inwin && ((marks[bj >> 5] >> (bj & 31)) & 1u);
+            // exclusive prefix max of the scored totals over the lanes before
+            int32_t incl = total;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int32_t o = __shfl_up(incl, d);
+                if (lane >= d) incl = incl > o ? incl : o;
+            }
+            const int32_t prev = __shfl_up(incl, 1);
+            const int32_t before = lane ? (prev > max_f ? prev : max_f) : max_f;
+            const bool imp = valid && total > before;
+            const uint64_t I = __ballot(imp);
+            const uint64_t M = __ballot(valid && !imp && marked);
+            uint64_t E = I | M;
+            int brk = 64;
+            while (E) {
+                const int bl = (int)__builtin_ctzll(E);
+                E &= E - 1;
+                if ((I >> bl) & 1ull) {
+                    if (n_skip > 0) --n_skip;
+                } else if (++n_skip > cp.max_skip) {
+                    brk = bl;
+                    break;
+                }
+            }
+            const uint64_t Iv = brk == 64 ? I : (I & ((1ull << brk) - 1ull));
+            if (Iv) {
+                const int lb = 63 - (int)__builtin_clzll(Iv);
+                max_f = __builtin_amdgcn_readlane(total, lb);
+                max_j = jtop - lb;
+            }
+            if (brk < 64) break;
+        }
+        f[i] = max_f;   // every lane stores the same value
+        pr[i] = max_j;
         if (max_f > max_f_global) {
             max_f_global = max_f;
             best_end = i;
         }
+        // shift the ring by one lane; anchor i enters at lane 0
+        r_pr = (uint32_t)wave_shr1((int32_t)r_pr);
+        r_pq = (uint32_t)wave_shr1((int32_t)r_pq);
+        r_sp = (uint32_t)wave_shr1((int32_t)r_sp);
+        r_rr = (uint32_t)wave_shr1((int32_t)r_rr);
+        r_f = wave_shr1(r_f);
+        r_p = wave_shr1(r_p);
+        const bool l0 = lane == 0;
+        r_pr = l0 ? ai.pos_ref : r_pr;
+        r_pq = l0 ? ai.pos_qry : r_pq;
+        r_sp = l0 ? ai.span : r_sp;
+        r_rr = l0 ? ai_rr : r_rr;
+        r_f = l0 ? max_f : r_f;
+        r_p = l0 ? max_j : r_p;
     }
-    int32_t cnt = 0;
+
+    // the chain: predecessor links from best_end, read 64 at a time
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    int cnt = 0;
     if (!(max_f_global < cp.min_score || best_end < 0)) {
-        for (int64_t cur = best_end; cur >= 0; cur = p[cur]) ++cnt;
+        int wb = -1;
+        int32_t pv = 0;
+        for (int cur = best_end; cur >= 0;) {
+            if (wb < 0 || cur < wb) {
+                wb = cur - 63 > 0 ? cur - 63 : 0;
+                pv = wb + lane <= cur ? __hip_atomic_load(pr + wb + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                      : -1;
+            }
+            ++cnt;
+            cur = __builtin_amdgcn_readlane(pv, cur - wb);
+        }
         if (cnt < cp.min_cnt) {
             cnt = 0;
         } else {
-            int64_t cur = best_end;
-            for (int k = cnt - 1; k >= 0; --k) {
-                chain_idx[base + k] = ord[cur];
-                cur = p[cur];
+            wb = -1;
+            uint32_t ov = 0;
+            int k = cnt - 1;
+            for (int cur = best_end; cur >= 0; --k) {
+                if (wb < 0 || cur < wb) {
+                    wb = cur - 63 > 0 ? cur - 63 : 0;
+                    const bool in = wb + lane <= cur;
+                    pv = in ? __hip_atomic_load(pr + wb + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : -1;
+                    ov = in ? ord[wb + lane] : 0u;
+                }
+                chain_idx[base + k] = __builtin_amdgcn_readlane(ov, cur - wb);   // every lane, same value
+                cur = __builtin_amdgcn_readlane(pv, cur - wb);
             }
         }
     }
@@ -293,12 +446,13 @@ hipError_t anchor_keys_launch(const AnchorRec *anchors, int64_t n, uint64_t *key
 }
 
 hipError_t chain_launch(const AnchorRec *anchors, const uint32_t *order, const int64_t *aoff, const int64_t *na,
-                        int nq, const ChainParamsDev &cp, const float *lut, int32_t *f, int32_t *p, int32_t *t,
-                        uint32_t *chain_idx, int32_t *chain_cnt, hipStream_t stream)
+                        int nq, const ChainParamsDev &cp, const float *lut, int32_t *f, int32_t *p, uint32_t *chain_idx,
+                        int32_t *chain_cnt, hipStream_t stream)
 {
     if (nq <= 0) return hipSuccess;
-    hipLaunchKernelGGL(chain_queries, dim3((nq + 63) / 64), dim3(64), 0, stream, anchors, order, aoff, na, nq, cp, lut,
-                       f, p, t, chain_idx, chain_cnt);
+    if (cp.max_iter < 0 || cp.max_iter > kChainMaxIter) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(chain_wave, dim3(nq), dim3(64), 0, stream, anchors, order, aoff, na, cp, lut, f, p, chain_idx,
+                       chain_cnt);
     return hipGetLastError();
 }
 

@@ -158,6 +158,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     outs.assign(n, DpOut{});
     if (kernel_ms) *kernel_ms = 0.0;
     if (n == 0) return HALIGN_OK;
+    PhaseClock clk("run_specs");
     const uint16_t *seq = ctx->seq.as<uint16_t>();
     const uint8_t *seqb = ctx->seqb.as<uint8_t>();
     uint32_t *cig = ctx->cig.as<uint32_t>();
@@ -213,11 +214,13 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         j.cig_cap = s.cig_cap;
         j.pad_ = 0;
     }
+    clk.mark("buckets and jobs");
     HCHECK(hipMemcpyAsync(ctx->jobs.p, hjobs.data(), n * sizeof(DpJob), hipMemcpyHostToDevice, ctx->stream),
            "upload jobs");
     HCHECK(hipMemsetAsync(ctx->counters.p, 0, ctx->counters.cap, ctx->stream), "clear counters");
 
-    // traceback pool: one slot per resident workgroup
+    // traceback pool: one slot per resident workgroup (per job group of a
+    // sub-wave workgroup)
     size_t need_slot = 0;
     for (auto &s : specs) need_slot = std::max(need_slot, s.tb_bytes);
     if (need_slot > ctx->tb_budget) return fail(ctx, HALIGN_E_NOMEM, "traceback of one job exceeds the device budget");
@@ -226,20 +229,20 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     DpJob *djobs = ctx->jobs.as<DpJob>();
     DpOut *douts = ctx->outs.as<DpOut>();
     int *ctr = ctx->counters.as<int>();
-    size_t pos = 0;
-    for (int bi = 0; bi < nb; ++bi) {
-        auto &b = buckets[bi];
-        if (b.empty()) continue;
-        const int shape = bi / 4;
-        const bool stats = (bi & 2) != 0;
-        const bool rt = (bi & 1) != 0;
-        size_t slot = 0;
-        for (int i : b) slot = std::max(slot, specs[i].tb_bytes);
-        slot = (slot + 255) & ~(size_t)255;
-        const bool wave = shape >= kWaveBase;
-        const int wi = shape - kWaveBase;
-        const size_t jpg = wave ? (size_t)dp_wave_jobs_per_group(wi) : 1;   // traceback slots per workgroup
-        if (plain) {
+    if (plain) {
+        size_t pos = 0;
+        for (int bi = 0; bi < nb; ++bi) {
+            auto &b = buckets[bi];
+            if (b.empty()) continue;
+            const int shape = bi / 4;
+            const bool stats = (bi & 2) != 0;
+            const bool rt = (bi & 1) != 0;
+            size_t slot = 0;
+            for (int i : b) slot = std::max(slot, specs[i].tb_bytes);
+            slot = (slot + 255) & ~(size_t)255;
+            const bool wave = shape >= kWaveBase;
+            const int wi = shape - kWaveBase;
+            const size_t jpg = wave ? (size_t)dp_wave_jobs_per_group(wi) : 1;
             HCHECK(ctx->tb.ensure(slot * jpg), "alloc traceback");
             for (size_t k = 0; k < b.size(); ++k) {
                 hipError_t e = wave ? dp_wave_launch(wi, rt, 1, djobs + pos + k, 1, ctr + pos + k,
@@ -248,26 +251,128 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                                                 ctx->tb.as<uint8_t>(), slot, douts + pos + k, ctx->stream);
                 HCHECK(e, "launch dp (plain)");
             }
-        } else {
-            int groups = wave ? dp_wave_max_groups(wi, rt, ctx->num_cu)
-                              : dp_shape_max_groups(shape, stats, ctx->num_cu);
+            pos += b.size();
+        }
+    } else {
+        // One launch per bucket. Buckets are independent (own jobs, outputs,
+        // counter and traceback region), so they run side by side on the main
+        // stream and two side streams, longest estimate first on the least
+        // loaded stream: narrow-band buckets are bound by their jobs' anti-
+        // diagonal chains, not by the chip, and overlap well.
+        struct Launch { int bi; size_t pos, slot, jpg, off; int groups; double est; int stream; };
+        std::vector<Launch> ls;
+        const double nsimd = 4.0 * ctx->num_cu;
+        size_t pos = 0;
+        for (int bi = 0; bi < nb; ++bi) {
+            auto &b = buckets[bi];
+            if (b.empty()) continue;
+            const int shape = bi / 4;
+            const bool stats = (bi & 2) != 0;
+            const bool rt = (bi & 1) != 0;
+            Launch L{bi, pos, 0, 1, 0, 0, 0.0, 0};
+            for (int i : b) L.slot = std::max(L.slot, specs[i].tb_bytes);
+            L.slot = (L.slot + 255) & ~(size_t)255;
+            const bool wave = shape >= kWaveBase;
+            const int wi = shape - kWaveBase;
+            L.jpg = wave ? (size_t)dp_wave_jobs_per_group(wi) : 1;
+            int groups = wave ? dp_wave_max_groups(wi, rt, ctx->num_cu) : dp_shape_max_groups(shape, stats, ctx->num_cu);
             if (groups <= 0) return fail(ctx, HALIGN_E_HIP, "occupancy query failed");
-            groups = (int)std::min<size_t>(groups, (b.size() + jpg - 1) / jpg);
-            groups = (int)std::min<size_t>(groups, std::max<size_t>(1, ctx->tb_budget / (slot * jpg)));
-            HCHECK(ctx->tb.ensure(slot * jpg * groups), "alloc traceback pool");
-            hipError_t e = wave ? dp_wave_launch(wi, rt, groups, djobs + pos, (int)b.size(), ctr + bi,
-                                                 ctx->tb.as<uint8_t>(), slot, douts + pos, ctx->stream)
-                                : dp_launch(shape, stats, groups, djobs + pos, (int)b.size(), ctr + bi,
-                                            ctx->tb.as<uint8_t>(), slot, douts + pos, ctx->stream);
+            L.groups = (int)std::min<size_t>(groups, (b.size() + L.jpg - 1) / L.jpg);
+            // estimated time: the cost model where it has counts, else cells
+            int is = 0, va = 0;
+            if (wave) dp_wave_shape_cost(wi, &is, &va);
+            double max_nd = 0, sum_nd = 0, blocks = 0;
+            for (int i : b) {
+                const Spec &s = specs[i];
+                const double nd = (double)s.qlen + s.tlen - 1;
+                max_nd = std::max(max_nd, nd);
+                sum_nd += nd;
+                blocks += nd * n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen));
+            }
+            L.est = is ? std::max(max_nd * is, sum_nd * va / ((double)L.jpg * nsimd)) : blocks * 400.0 / nsimd;
+            ls.push_back(L);
+            pos += b.size();
+        }
+        // traceback regions: each launch its own; shrink the widest launches to the budget
+        auto pool_bytes = [&]() {
+            size_t t = 0;
+            for (auto &L : ls) t += L.slot * L.jpg * (size_t)L.groups;
+            return t;
+        };
+        size_t min_bytes = 0;
+        for (auto &L : ls) min_bytes += L.slot * L.jpg;
+        const bool side = min_bytes <= ctx->tb_budget;   // else one launch at a time in one region
+        if (side) {
+            while (pool_bytes() > ctx->tb_budget) {
+                Launch *big = nullptr;
+                for (auto &L : ls)
+                    if (L.groups > 1 && (!big || L.slot * L.jpg * L.groups > big->slot * big->jpg * big->groups))
+                        big = &L;
+                if (!big) break;
+                big->groups = std::max(1, (int)(big->groups * 0.75));
+            }
+            size_t off = 0;
+            for (auto &L : ls) {
+                L.off = off;
+                off += L.slot * L.jpg * (size_t)L.groups;
+            }
+            clk.mark("launch plan");
+            HCHECK(ctx->tb.ensure(std::max<size_t>(off, 1)), "alloc traceback pool");
+            clk.mark("traceback pool");
+            std::vector<int> ord(ls.size());
+            for (size_t k = 0; k < ls.size(); ++k) ord[k] = (int)k;
+            std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return ls[x].est > ls[y].est; });
+            double load[3] = {0, 0, 0};
+            for (int k : ord) {
+                int sidx = 0;
+                for (int q = 1; q < 3; ++q)
+                    if (load[q] < load[sidx]) sidx = q;
+                ls[k].stream = sidx;
+                load[sidx] += ls[k].est;
+            }
+        } else {
+            for (auto &L : ls) {
+                L.groups = (int)std::min<size_t>(L.groups, std::max<size_t>(1, ctx->tb_budget / (L.slot * L.jpg)));
+                L.off = 0;
+                L.stream = 0;
+            }
+            size_t mx = 0;
+            for (auto &L : ls) mx = std::max(mx, L.slot * L.jpg * (size_t)L.groups);
+            HCHECK(ctx->tb.ensure(std::max<size_t>(mx, 1)), "alloc traceback pool");
+        }
+        bool used[3] = {true, false, false};
+        for (auto &L : ls) used[L.stream] = true;
+        if (used[1] || used[2]) HCHECK(hipEventRecord(ctx->fork, ctx->stream), "event");
+        for (int q = 1; q < 3; ++q)
+            if (used[q]) HCHECK(hipStreamWaitEvent(ctx->aux[q - 1], ctx->fork, 0), "stream wait");
+        for (auto &L : ls) {
+            const int shape = L.bi / 4;
+            const bool stats = (L.bi & 2) != 0;
+            const bool rt = (L.bi & 1) != 0;
+            const bool wave = shape >= kWaveBase;
+            const int wi = shape - kWaveBase;
+            hipStream_t st = L.stream ? ctx->aux[L.stream - 1] : ctx->stream;
+            uint8_t *tb = ctx->tb.as<uint8_t>() + L.off;
+            const int nj = (int)buckets[L.bi].size();
+            hipError_t e = wave ? dp_wave_launch(wi, rt, L.groups, djobs + L.pos, nj, ctr + L.bi, tb, L.slot,
+                                                 douts + L.pos, st)
+                                : dp_launch(shape, stats, L.groups, djobs + L.pos, nj, ctr + L.bi, tb, L.slot,
+                                            douts + L.pos, st);
             HCHECK(e, "launch dp");
         }
-        pos += b.size();
+        for (int q = 1; q < 3; ++q) {
+            if (!used[q]) continue;
+            HCHECK(hipEventRecord(ctx->join[q - 1], ctx->aux[q - 1]), "event");
+            HCHECK(hipStreamWaitEvent(ctx->stream, ctx->join[q - 1], 0), "stream wait");
+        }
     }
+    clk.mark("launches");
     HCHECK(hipEventRecord(ctx->ev1, ctx->stream), "event");
     std::vector<DpOut> tmp(n);
     HCHECK(hipMemcpyAsync(tmp.data(), ctx->outs.p, n * sizeof(DpOut), hipMemcpyDeviceToHost, ctx->stream),
            "download outs");
     HCHECK(hipStreamSynchronize(ctx->stream), "dp kernels");
+    clk.mark("kernels");
     float ms = 0.f;
     HCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
     if (kernel_ms) *kernel_ms = ms;
@@ -311,8 +416,15 @@ int halign_init(int device, halign_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; return HALIGN_E_HIP; }
     ctx->num_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&ctx->ev0) == hipSuccess && hipEventCreate(&ctx->ev1) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->pfork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->pjoin, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < halign_ctx::kAux; ++k)
+        ok = hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         delete ctx;
         return HALIGN_E_HIP;
     }
@@ -340,6 +452,14 @@ void halign_free(halign_ctx *ctx)
         b->release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    for (int k = 0; k < halign_ctx::kAux; ++k) {
+        (void)hipStreamSynchronize(ctx->aux[k]);
+        (void)hipStreamDestroy(ctx->aux[k]);
+        (void)hipEventDestroy(ctx->join[k]);
+    }
+    (void)hipEventDestroy(ctx->fork);
+    (void)hipEventDestroy(ctx->pfork);
+    (void)hipEventDestroy(ctx->pjoin);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

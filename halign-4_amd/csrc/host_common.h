@@ -4,7 +4,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 
@@ -41,6 +43,11 @@ struct halign_ctx {
     int num_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Side streams (default priority): aux[0], aux[1] run DP buckets beside the
+    // main stream; aux[2] runs the per-query path's lane DP beside them.
+    static constexpr int kAux = 3;
+    hipStream_t aux[kAux] = {};
+    hipEvent_t fork = nullptr, join[kAux] = {}, pfork = nullptr, pjoin = nullptr;
     std::mutex mu;
     std::string err;
     hal::DevBuf seq, seqb, jobs, outs, counters, tb, cig;
@@ -53,6 +60,25 @@ struct halign_ctx {
 };
 
 namespace hal {
+
+// Host phase timing on stderr when the environment sets HALIGN_TRACE
+struct PhaseClock {
+    bool on;
+    std::chrono::steady_clock::time_point t;
+    const char *scope;
+    explicit PhaseClock(const char *sc)
+        : on(std::getenv("HALIGN_TRACE") != nullptr), t(std::chrono::steady_clock::now()), scope(sc)
+    {
+    }
+    void mark(const char *what)
+    {
+        if (!on) return;
+        const auto n = std::chrono::steady_clock::now();
+        fprintf(stderr, "[halign %s] %-28s %9.3f ms\n", scope, what,
+                std::chrono::duration<double, std::milli>(n - t).count());
+        t = n;
+    }
+};
 
 inline int fail(halign_ctx *ctx, int code, const char *what, hipError_t e = hipSuccess)
 {

@@ -34,8 +34,8 @@ hipError_t anchors_launch_write(const CenterIndexDev &ci, const uint64_t *qx, co
 hipError_t anchor_keys_launch(const AnchorRec *anchors, int64_t n, uint64_t *keys, uint32_t *vals,
                               hipStream_t stream);
 hipError_t chain_launch(const AnchorRec *anchors, const uint32_t *order, const int64_t *aoff, const int64_t *na,
-                        int nq, const ChainParamsDev &cp, const float *lut, int32_t *f, int32_t *p, int32_t *t,
-                        uint32_t *chain_idx, int32_t *chain_cnt, hipStream_t stream);
+                        int nq, const ChainParamsDev &cp, const float *lut, int32_t *f, int32_t *p, uint32_t *chain_idx,
+                        int32_t *chain_cnt, hipStream_t stream);
 hipError_t mm2_segments_launch(const AnchorRec *anchors, const uint32_t *chain_idx, const int64_t *aoff,
                                const int32_t *chain_cnt, const int64_t *qlens, uint32_t rlen, const int64_t *soff,
                                int nq, SegRec *segs, int32_t *nseg, int64_t *caps, hipStream_t stream);
@@ -66,7 +66,7 @@ struct halign_center {
     size_t scratch_groups = 0;
     // per-batch buffers
     DevBuf hcnt, scratch, hidx, nanch, totals, acount, aoff, anchors, keys, keys2, vals, vals2, sort_tmp;
-    DevBuf fw, pw, tw, chain_idx, chain_cnt, qlens, soff, segs, nseg, caps, cig_off, scan_tmp, region, bregion;
+    DevBuf fw, pw, chain_idx, chain_cnt, qlens, soff, segs, nseg, caps, cig_off, scan_tmp, region, bregion;
     DevBuf qrev_off, lane_ws, ncig, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
 };
 
@@ -154,6 +154,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
 {
     halign_ctx *ctx = c->ctx;
     hipStream_t st = ctx->stream;
+    PhaseClock clk("run_batch");
     b.n = n;
     b.off.assign(offsets, offsets + n + 1);
     for (auto &o : b.off) o -= offsets[0];
@@ -171,6 +172,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     // minimizers: extractMinimizer(query, 15, 10, non-canonical)
     int rc = seed_device(ctx, n, b.off, kK, kW, 1, b.hit_off, b.hit_cnt, b.seed_status);
     if (rc != HALIGN_OK) return rc;
+    clk.mark("upload + minimizers");
     HCHECK(up(c->hcnt, b.hit_cnt, st), "upload hit counts");
 
     // anchors: count per query, then write in the reference's order
@@ -193,6 +195,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                                 c->nanch.as<uint32_t>(), c->acount.as<int64_t>(), st),
            "launch anchors_count");
     HCHECK(down(b.na, c->acount.p, n, st), "download anchor counts");
+    clk.mark("anchor counts");
     b.aoff.assign(n + 1, 0);
     for (int i = 0; i < n; ++i) b.aoff[i + 1] = b.aoff[i] + b.na[i];
     b.anchors = b.aoff[n];
@@ -232,15 +235,15 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     }
     HCHECK(c->fw.ensure((size_t)A * 4), "alloc");
     HCHECK(c->pw.ensure((size_t)A * 4), "alloc");
-    HCHECK(c->tw.ensure((size_t)A * 4), "alloc");
     HCHECK(c->chain_idx.ensure((size_t)A * 4), "alloc");
     HCHECK(c->chain_cnt.ensure((size_t)std::max(n, 1) * 4), "alloc");
     const ChainParamsDev cp{5000, 5000, 500, 25, 5000, 3, 40, 0.01f, 0.01f};   // ChainParams defaults
     HCHECK(chain_launch(c->anchors.as<AnchorRec>(), c->vals2.as<uint32_t>(), c->aoff.as<int64_t>(),
                         c->acount.as<int64_t>(), n, cp, c->lut.as<float>(), c->fw.as<int32_t>(), c->pw.as<int32_t>(),
-                        c->tw.as<int32_t>(), c->chain_idx.as<uint32_t>(), c->chain_cnt.as<int32_t>(), st),
+                        c->chain_idx.as<uint32_t>(), c->chain_cnt.as<int32_t>(), st),
            "launch chain");
     HCHECK(down(b.chain_cnt, c->chain_cnt.p, n, st), "download chain lengths");
+    clk.mark("anchors + sort + chains");
     if (stop == 2) return HALIGN_OK;
 
     // MM2 segments
@@ -273,6 +276,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     HCHECK(hipMemcpyAsync(&last[0], c->cig_off.as<int64_t>() + (b.segs - 1), 8, hipMemcpyDeviceToHost, st), "down");
     HCHECK(hipMemcpyAsync(&last[1], c->caps.as<int64_t>() + (b.segs - 1), 8, hipMemcpyDeviceToHost, st), "down");
     HCHECK(hipStreamSynchronize(st), "segments");
+    clk.mark("segments");
     const int64_t cig_total = std::max<int64_t>(last[0] + last[1], 1);
     HCHECK(seg_offsets_launch(c->segs.as<SegRec>(), c->cig_off.as<int64_t>(), b.segs, st), "launch seg offsets");
 
@@ -302,16 +306,26 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                                c->bregion.as<int64_t>(), 0, n, ctx->seq.as<uint16_t>(), ctx->seqb.as<uint8_t>(), st),
            "launch prep_queries");
 
-    // small segments: one lane each
+    // small segments: one lane each, on a side stream beside the large
+    // segments' DP (disjoint segments, CIGAR ranges and ncig entries)
     HCHECK(ctx->cig.ensure((size_t)cig_total * 4), "alloc segment CIGARs");
     HCHECK(c->ncig.ensure((size_t)S * 4), "alloc");
     const int64_t lanes = (int64_t)ctx->num_cu * 256;
     HCHECK(c->lane_ws.ensure((size_t)lanes * kLaneWs), "alloc lane DP workspace");
+    hipStream_t side = ctx->aux[2];
+    HCHECK(hipEventRecord(ctx->pfork, st), "event");
+    HCHECK(hipStreamWaitEvent(side, ctx->pfork, 0), "stream wait");
     HCHECK(lane_dp_launch(c->segs.as<SegRec>(), b.segs, ctx->seqb.as<uint8_t>() + kBytePad,
                           c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(), ctx->seqb.as<uint8_t>(),
                           c->lane_ws.as<uint8_t>(), kLaneWs, lanes, ctx->cig.as<uint32_t>(), c->ncig.as<int32_t>(),
-                          st),
+                          side),
            "launch lane_dp");
+    HCHECK(hipEventRecord(ctx->pjoin, side), "event");
+    // on every return from here on, later work on the main stream waits for the lane DP
+    struct JoinSide {
+        halign_ctx *ctx;
+        ~JoinSide() { (void)hipStreamWaitEvent(ctx->stream, ctx->pjoin, 0); }
+    } join_side{ctx};
 
     // large segments and band overflows: the ring / full-statistics kernels
     HCHECK(c->hlist.ensure((size_t)S * 8), "alloc");
@@ -323,6 +337,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     unsigned long long nh = 0;
     HCHECK(hipMemcpyAsync(&nh, c->hcount.p, 8, hipMemcpyDeviceToHost, st), "down");
     HCHECK(hipStreamSynchronize(st), "collect");
+    clk.mark("prep + collect");
     b.host_segs = (int64_t)nh;
     if (nh > 0) {
         HCHECK(c->hsegs.ensure((size_t)nh * sizeof(SegRec)), "alloc");
@@ -356,7 +371,9 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
         }
         std::vector<DpOut> outs;
         double ms = 0;
+        clk.mark("large segment specs");
         rc = run_specs(ctx, specs, false, outs, &ms);
+        clk.mark("large segment DP");
         if (rc != HALIGN_OK) return rc;
         std::vector<int32_t> nv(nh);
         for (size_t k = 0; k < nh; ++k) {
@@ -368,6 +385,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                            c->hvals.as<int32_t>(), (int64_t)nh, c->ncig.as<int32_t>());
         HCHECK(hipGetLastError(), "launch scatter");
     }
+    HCHECK(hipStreamWaitEvent(st, ctx->pjoin, 0), "stream wait");   // the lane DP is done before assembly
 
     // assembly: appendCigar over the segments of each query
     b.ooff.assign(n + 1, 0);
@@ -405,6 +423,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     HCHECK(hipMemcpyAsync(b.nout.data(), c->nout.p, (size_t)n * 8, hipMemcpyDeviceToHost, st), "download");
     HCHECK(hipMemcpyAsync(&b.packed, c->coff.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, st), "download");
     HCHECK(hipStreamSynchronize(st), "assembly");
+    clk.mark("assembly + compaction");
     return HALIGN_OK;
 }
 
@@ -501,9 +520,10 @@ void halign_center_free(halign_center *c)
     std::lock_guard<std::mutex> lock(c->ctx->mu);
     (void)hipSetDevice(c->ctx->device);
     (void)hipStreamSynchronize(c->ctx->stream);
+    for (int k = 0; k < halign_ctx::kAux; ++k) (void)hipStreamSynchronize(c->ctx->aux[k]);
     for (DevBuf *b : {&c->sx, &c->sy, &c->dh, &c->dstart, &c->docc, &c->lut, &c->hcnt, &c->scratch, &c->hidx,
                       &c->nanch, &c->totals, &c->acount, &c->aoff, &c->anchors, &c->keys, &c->keys2, &c->vals,
-                      &c->vals2, &c->sort_tmp, &c->fw, &c->pw, &c->tw, &c->chain_idx, &c->chain_cnt, &c->qlens,
+                      &c->vals2, &c->sort_tmp, &c->fw, &c->pw, &c->chain_idx, &c->chain_cnt, &c->qlens,
                       &c->soff, &c->segs, &c->nseg, &c->caps, &c->cig_off, &c->scan_tmp, &c->region, &c->bregion,
                       &c->qrev_off, &c->lane_ws, &c->ncig, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
                       &c->out, &c->nout, &c->ocnt, &c->coff, &c->packed})
