@@ -79,12 +79,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times), to scale HBM counters")
-    ap.add_argument("--workload", default="dp", choices=["dp", "consensus", "e2e"],
+    ap.add_argument("--workload", default="dp", choices=["dp", "consensus", "e2e", "serve"],
                     help="dp: the headline pairwise DP; consensus: column counting over rows in HBM; "
-                         "e2e: the whole per-query path (--config 1, 3 or 4)")
+                         "e2e: the whole per-query path (--config 1, 3 or 4); serve: config 5 serving")
     ap.add_argument("--config", type=int, default=1, help="e2e workload configuration (1, 3, 4)")
     ap.add_argument("--queries", type=int, default=0, help="e2e: queries per step (0 = the config default)")
     ap.add_argument("--rows", type=int, default=100000, help="consensus workload: rows of 29,903 columns")
+    ap.add_argument("--requests", type=int, default=48, help="serve: requests per load point")
+    ap.add_argument("--workers", type=int, default=2, help="serve: concurrent batches")
+    ap.add_argument("--batch-queries", type=int, default=8192, help="serve: queries per batch at most")
+    ap.add_argument("--loads", type=float, nargs="*", default=[0.5, 0.8],
+                    help="serve: offered load as fractions of the measured capacity")
     ap.add_argument("--kernel", default="auto",
                     help="auto | multiwave | WxS (ring kernel with W waves and S slots per lane)")
     args = ap.parse_args()
@@ -108,6 +113,8 @@ def main():
         return consensus_bench(args)
     if args.workload == "e2e":
         return e2e_bench(args)
+    if args.workload == "serve":
+        return serve_bench(args)
 
     import halign
     from halign import synth
@@ -259,6 +266,97 @@ def e2e_bench(args):
            "config": {"workload": f"config {cfg}: {label}", "queries_per_step": per},
            "bases_per_s": round(nb / tw, 1), "device_ms_per_step": round(sum(x[3] for x in steps) / len(steps), 3),
            "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def serve_bench(args):
+    """Config 5 serving: a continuous-batching server (halign_server_*) with the
+    29,903 bp center and its index resident, requests of n ~ U{1..4096} queries
+    (lengths U{1..30000}, 1% divergence). First all requests at once (closed
+    loop: the capacity in seqs/s), then Poisson arrivals at fractions of that
+    capacity; latency = submit -> last CIGAR, measured in the library."""
+    import threading
+    import halign
+    from halign import synth
+    center = synth.center(5)
+    rng = np.random.default_rng(0xC5A)
+    R = args.requests
+    sizes = rng.integers(1, 4097, R)
+    need = int(sizes.sum())
+    qb, off = synth.queries(5, 0, need)
+    bounds = np.concatenate([[0], np.cumsum(sizes)])
+    log(f"serve: {R} requests, {need} queries, {off[-1] / 1e6:.1f} Mbases")
+    srv = halign.Server(center, workers=args.workers, max_batch_queries=args.batch_queries)
+
+    def run(arrivals, label):
+        ids = [None] * R
+        lat = np.zeros(R)
+        done_t = np.zeros(R)
+        t0 = time.perf_counter()
+
+        def collect():
+            for i in range(R):
+                while ids[i] is None:
+                    time.sleep(0.0005)
+                _, _, _, timing = srv.result(ids[i])
+                lat[i] = timing[1]
+                done_t[i] = time.perf_counter() - t0
+        th = threading.Thread(target=collect)
+        th.start()
+        for i in range(R):
+            if arrivals is not None:
+                dt = arrivals[i] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+            ids[i] = srv.submit_flat(qb, off[bounds[i]:bounds[i + 1] + 1])
+        th.join()
+        span = float(done_t.max())
+        log(f"{label}: {need / span:.0f} seqs/s, p50 {np.percentile(lat, 50):.1f} ms, "
+            f"p99 {np.percentile(lat, 99):.1f} ms, span {span:.2f} s")
+        return lat, span
+    try:
+        run(None, "warmup")
+        _, span = run(None, "capacity (all at once)")
+        cap = need / span
+        points = []
+        mean_n = float(sizes.mean())
+        for f in args.loads:
+            lam = f * cap / mean_n   # requests per second
+            arr = np.cumsum(np.random.default_rng(0xC5A + int(f * 100)).exponential(1.0 / lam, R))
+            lat, sp = run(arr, f"load {f:.2f}")
+            points.append({"load": f, "offered_req_per_s": round(lam, 3), "seqs_per_s": round(need / sp, 1),
+                           "p50_ms": round(float(np.percentile(lat, 50)), 1),
+                           "p99_ms": round(float(np.percentile(lat, 99)), 1)})
+    finally:
+        srv.close()
+    cpu = None
+    if not args.no_cpu:
+        from oracle_lib import REF_SO, Oracle, RefLib
+        threads = min(16, os.cpu_count() or 1)
+        kind = "reference" if os.path.exists(REF_SO) else "port"
+        lib = RefLib() if kind == "reference" else Oracle()
+        sample = min(need, args.cpu_sample or threads * 2)
+        if kind == "reference":
+            hits = lib.minimizers(center)
+            fn = lambda i: lib.ref_global_align_pre(center, hits, qb[off[i]:off[i + 1]])
+        else:
+            fn = lambda i: lib.ref_global_align(center, qb[off[i]:off[i + 1]])
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(fn, range(sample)))
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(sample / dt, 2), "unit": "seqs/s", "cores": threads, "kind": kind,
+               "sample": f"{sample} config-5 queries (RefAligner::globalAlign), {dt:.1f}s on {threads} threads"}
+        log(f"cpu baseline ({kind}): {cpu['value']} seqs/s")
+    hi = points[-1] if points else {}
+    out = {"metric": "serving seqs/s and p50/p99 request latency (continuous batching)",
+           "value": round(cap, 1), "unit": "seqs/s", "n_gpus": 1, "steps": len(args.loads) + 1, "warmup": 1,
+           "ms_per_step": round(span * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "int8", "data": "synthetic (seeded, csrc/synth.c config 5)",
+           "config": {"workload": "config 5: serving, center 29903 bp resident, requests of U{1..4096} queries "
+                                  "of U{1..30000} bp at 1% divergence", "requests": R, "queries": need,
+                      "workers": args.workers, "max_batch_queries": args.batch_queries},
+           "p50_ms": hi.get("p50_ms"), "p99_ms": hi.get("p99_ms"), "load_points": points, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

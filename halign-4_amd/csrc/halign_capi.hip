@@ -116,37 +116,98 @@ int pick_shape(int ncol, bool stats, int policy)
     return -1;
 }
 
-// Batch-level shape choice for a bucket of narrow-band CIGAR jobs (automatic
-// policy): among the one-wave and sub-wave shapes that hold the bucket's widest
-// band, the one with the least estimated time max(longest job's anti-diagonals
-// x instructions per anti-diagonal, all anti-diagonals x VALU instructions per
-// anti-diagonal / (jobs per wave x SIMDs)). Many short bands favour sub-wave
-// shapes (more jobs per instruction), few long ones the one-wave ring (shorter
-// critical path per anti-diagonal).
-static int pick_bucket_shape(const std::vector<Spec> &specs, const std::vector<int> &b, int shape, int num_cu)
+// Batch-level shape choice for the narrow-band CIGAR buckets (automatic
+// policy). A bucket on shape s takes about crit = its longest job's
+// anti-diagonals x instructions per anti-diagonal of s (a wave issues one
+// instruction at a time), and adds tp = all its anti-diagonals x VALU
+// instructions per anti-diagonal / (jobs per wave x SIMDs) of chip time. The
+// buckets run side by side, so the batch takes about max(max crit, sum tp):
+// start each bucket on its cheapest shape in tp (sub-wave groups: many jobs
+// per instruction), then move the bucket with the longest crit to a shape with
+// a shorter one (a one-wave ring) while that lowers the estimate.
+struct BucketCost {
+    int bi;
+    double max_nd, sum_nd;
+    int max_nc;
+    std::vector<int> shapes;   // candidate shape indices (ring shapes), by falling crit
+    int cur;                   // position in shapes
+};
+
+static void cost_of(const BucketCost &b, int idx, double nsimd, double &crit, double &tp)
 {
-    const int wi = shape - kWaveBase;
-    if (dp_wave_shape_waves(wi) > 1) return shape;
-    int max_nc = 0;
-    double max_nd = 0, sum_nd = 0;
-    for (int i : b) {
-        const Spec &s = specs[i];
-        const double nd = (double)s.qlen + s.tlen - 1;
-        max_nc = std::max(max_nc, n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen)));
-        max_nd = std::max(max_nd, nd);
-        sum_nd += nd;
-    }
+    int is = 0, va = 0;
+    dp_wave_shape_cost(idx, &is, &va);
+    crit = b.max_nd * is;
+    tp = b.sum_nd * va / (dp_wave_jobs_per_group(idx) * nsimd);
+}
+
+static void plan_bucket_shapes(const std::vector<Spec> &specs, const std::vector<std::vector<int>> &buckets,
+                               int num_cu, std::vector<int> &shape_of_bucket)
+{
     const double nsimd = 4.0 * num_cu;
-    int best = shape;
-    double best_t = -1;
-    for (int i = 0; i < dp_wave_num_shapes(); ++i) {
-        int is = 0, va = 0;
-        dp_wave_shape_cost(i, &is, &va);
-        if (!is || max_nc > dp_wave_shape_ring(i) - 2) continue;
-        const double t = std::max(max_nd * is, sum_nd * va / (dp_wave_jobs_per_group(i) * nsimd));
-        if (best_t < 0 || t < best_t) { best_t = t; best = kWaveBase + i; }
+    std::vector<BucketCost> bc;
+    for (size_t bi = 0; bi < buckets.size(); ++bi) {
+        const auto &b = buckets[bi];
+        const int shape = (int)bi / 4;
+        shape_of_bucket[bi] = shape;
+        if (b.empty() || shape < kWaveBase || (bi & 2) || dp_wave_shape_waves(shape - kWaveBase) > 1) continue;
+        BucketCost c{(int)bi, 0, 0, 0, {}, 0};
+        for (int i : b) {
+            const Spec &s = specs[i];
+            const double nd = (double)s.qlen + s.tlen - 1;
+            c.max_nc = std::max(c.max_nc, n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen)));
+            c.max_nd = std::max(c.max_nd, nd);
+            c.sum_nd += nd;
+        }
+        for (int i = 0; i < dp_wave_num_shapes(); ++i) {
+            int is = 0, va = 0;
+            dp_wave_shape_cost(i, &is, &va);
+            if (is && c.max_nc <= dp_wave_shape_ring(i) - 2) c.shapes.push_back(i);
+        }
+        if (c.shapes.empty()) continue;
+        // by rising tp; the walk below only takes steps that shorten crit
+        std::sort(c.shapes.begin(), c.shapes.end(), [&](int x, int y) {
+            double cx, tx, cy, ty;
+            cost_of(c, x, nsimd, cx, tx);
+            cost_of(c, y, nsimd, cy, ty);
+            return tx < ty;
+        });
+        bc.push_back(c);
     }
-    return best;
+    auto estimate = [&]() {
+        double mc = 0, st = 0;
+        for (auto &c : bc) {
+            double cr, tp;
+            cost_of(c, c.shapes[c.cur], nsimd, cr, tp);
+            mc = std::max(mc, cr);
+            st += tp;
+        }
+        return std::max(mc, st);
+    };
+    for (int guard = 0; guard < 64; ++guard) {
+        // bucket with the longest crit
+        BucketCost *w = nullptr;
+        double wc = -1;
+        for (auto &c : bc) {
+            double cr, tp;
+            cost_of(c, c.shapes[c.cur], nsimd, cr, tp);
+            if (cr > wc) { wc = cr; w = &c; }
+        }
+        if (!w) break;
+        const double before = estimate();
+        // next shape (by tp) with a shorter crit
+        int nxt = -1;
+        for (size_t k = w->cur + 1; k < w->shapes.size(); ++k) {
+            double cr, tp;
+            cost_of(*w, w->shapes[k], nsimd, cr, tp);
+            if (cr < wc) { nxt = (int)k; break; }
+        }
+        if (nxt < 0) break;
+        const int old = w->cur;
+        w->cur = nxt;
+        if (estimate() >= before) { w->cur = old; break; }
+    }
+    for (auto &c : bc) shape_of_bucket[c.bi] = kWaveBase + c.shapes[c.cur];
 }
 
 // Run a list of jobs whose sequences already sit in ctx->seq (spread u16) and
@@ -171,12 +232,13 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         buckets[specs[i].shape * 4 + (specs[i].stats ? 2 : 0) + ((specs[i].flags & kFlagRight) ? 1 : 0)]
             .push_back((int)i);
     if (ctx->policy == 0 && !plain) {
+        std::vector<int> to_shape(nb);
+        plan_bucket_shapes(specs, buckets, ctx->num_cu, to_shape);
         std::vector<std::vector<int>> moved(nb);
         for (int bi = 0; bi < nb; ++bi) {
             auto &b = buckets[bi];
             if (b.empty()) continue;
-            const int shape = bi / 4;
-            const int to = (shape >= kWaveBase && !(bi & 2)) ? pick_bucket_shape(specs, b, shape, ctx->num_cu) : shape;
+            const int to = to_shape[bi];
             for (int i : b) specs[i].shape = to;
             auto &dst = moved[to * 4 + (bi & 3)];
             dst.insert(dst.end(), b.begin(), b.end());
@@ -351,6 +413,11 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             const bool rt = (L.bi & 1) != 0;
             const bool wave = shape >= kWaveBase;
             const int wi = shape - kWaveBase;
+            if (clk.on)
+                fprintf(stderr, "[halign run_specs] bucket: %s %d x%d lanes/job %d ring %d, %zu jobs, %d groups, stream %d, est %.3g\n",
+                        wave ? "ring" : "multiwave", wave ? dp_wave_shape_waves(wi) : shape,
+                        wave ? dp_wave_jobs_per_group(wi) : 1, wave ? dp_wave_shape_lanes(wi) : 0,
+                        wave ? dp_wave_shape_ring(wi) : 0, buckets[L.bi].size(), L.groups, L.stream, L.est);
             hipStream_t st = L.stream ? ctx->aux[L.stream - 1] : ctx->stream;
             uint8_t *tb = ctx->tb.as<uint8_t>() + L.off;
             const int nj = (int)buckets[L.bi].size();
@@ -496,6 +563,14 @@ int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t ca
         if (ring) ring[i] = dp_wave_shape_ring(i);
     }
     return n;
+}
+
+int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes)
+{
+    if (!ctx || bytes <= 0) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tb_budget = (size_t)bytes;
+    return HALIGN_OK;
 }
 
 int halign_set_kernel_policy(halign_ctx *ctx, int policy)

@@ -128,16 +128,6 @@ __global__ void seg_set_offsets(SegRec *__restrict__ segs, const int64_t *__rest
     if (i < n) segs[i].cig_off = cig_off[i];
 }
 
-// indices of the segments the host aligns (kSegHost), appended with an atomic
-__global__ void seg_collect_host(const SegRec *__restrict__ segs, int64_t n, int64_t *__restrict__ list,
-                                 unsigned long long *__restrict__ count)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && segs[i].cls == kSegHost) {
-        const unsigned long long at = atomicAdd(count, 1ull);
-        list[at] = i;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // lane DP: ksw_extz2_sse, CIGAR mode (no overflow: those segments go to the
@@ -283,17 +273,55 @@ __device__ int lane_extz2(const uint8_t *__restrict__ tgt, int tlen, const uint8
 // Lanes walk the segment list with a grid stride and align the kLaneDP ones.
 // center: center codes (index 0 = base 0); qrev_base[query] points at the
 // reversed code copy of the query, so query base j is qrev_base[len-1-j].
-__global__ void lane_dp(const SegRec *__restrict__ segs, int64_t n, const uint8_t *__restrict__ center,
+// Small segments split three ways, one thread per segment: an equal-length
+// segment whose bases are identical ACGT (the k-mer of an anchor, nearly every
+// one) has the all-match path as its unique optimum, so ksw_extz2 returns the
+// CIGAR nM: written here, class kSegExact. Other small segments go to the lane
+// list, large ones to the host list (both appended with an atomic).
+__global__ void seg_split(SegRec *__restrict__ segs, int64_t n, const uint8_t *__restrict__ center,
+                          const int64_t *__restrict__ qrev_off, const int64_t *__restrict__ qlens,
+                          const uint8_t *__restrict__ qrev, uint32_t *__restrict__ cigs, int32_t *__restrict__ ncig,
+                          int64_t *__restrict__ lane_list, unsigned long long *__restrict__ lane_count,
+                          int64_t *__restrict__ host_list, unsigned long long *__restrict__ host_count)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SegRec sg = segs[i];
+    if (sg.cls == kSegHost) {
+        host_list[atomicAdd(host_count, 1ull)] = i;
+        return;
+    }
+    if (sg.cls != kSegLane) return;
+    const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
+    bool same = srl == sql;
+    if (same) {
+        const uint8_t *qr = qrev + qrev_off[sg.query] + (qlens[sg.query] - 1 - sg.qs);
+        const uint8_t *t = center + sg.rs;
+        for (int k = 0; k < srl && same; ++k) same = t[k] < 4 && t[k] == qr[-k];
+    }
+    if (same) {
+        cigs[sg.cig_off] = (uint32_t)srl << 4;   // M
+        ncig[i] = 1;
+        segs[i].cls = kSegExact;
+    } else {
+        lane_list[atomicAdd(lane_count, 1ull)] = i;
+    }
+}
+
+// lane DP over the lane list (count read on the device)
+__global__ void lane_dp(const SegRec *__restrict__ segs, const int64_t *__restrict__ list,
+                        const unsigned long long *__restrict__ count, const uint8_t *__restrict__ center,
                         const int64_t *__restrict__ qrev_off, const int64_t *__restrict__ qlens,
                         const uint8_t *__restrict__ qrev, uint8_t *__restrict__ wsall, int64_t ws_lane,
                         uint32_t *__restrict__ cigs, int32_t *__restrict__ ncig)
 {
     const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n = (int64_t)*count;
     uint8_t *ws = wsall + lane * ws_lane;
-    for (int64_t i = lane; i < n; i += stride) {
+    for (int64_t li = lane; li < n; li += stride) {
+        const int64_t i = list[li];
         const SegRec sg = segs[i];
-        if (sg.cls != kSegLane) continue;
         const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
         const int64_t L = qlens[sg.query];
         // query base qs + j sits at reversed index L-1-qs-j
@@ -393,23 +421,26 @@ hipError_t seg_offsets_launch(SegRec *segs, const int64_t *cig_off, int64_t n, h
     return hipGetLastError();
 }
 
-hipError_t seg_collect_launch(const SegRec *segs, int64_t n, int64_t *list, unsigned long long *count,
-                              hipStream_t stream)
+
+hipError_t seg_split_launch(SegRec *segs, int64_t n, const uint8_t *center, const int64_t *qrev_off,
+                            const int64_t *qlens, const uint8_t *qrev, uint32_t *cigs, int32_t *ncig,
+                            int64_t *lane_list, unsigned long long *lane_count, int64_t *host_list,
+                            unsigned long long *host_count, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(seg_collect_host, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, segs, n, list,
-                       count);
+    hipLaunchKernelGGL(seg_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, segs, n, center, qrev_off,
+                       qlens, qrev, cigs, ncig, lane_list, lane_count, host_list, host_count);
     return hipGetLastError();
 }
 
-hipError_t lane_dp_launch(const SegRec *segs, int64_t n, const uint8_t *center, const int64_t *qrev_off,
-                          const int64_t *qlens, const uint8_t *qrev, uint8_t *ws, int64_t ws_lane, int64_t lanes,
-                          uint32_t *cigs, int32_t *ncig, hipStream_t stream)
+hipError_t lane_dp_launch(const SegRec *segs, const int64_t *list, const unsigned long long *count,
+                          const uint8_t *center, const int64_t *qrev_off, const int64_t *qlens, const uint8_t *qrev,
+                          uint8_t *ws, int64_t ws_lane, int64_t lanes, uint32_t *cigs, int32_t *ncig,
+                          hipStream_t stream)
 {
-    if (n <= 0) return hipSuccess;
     const int64_t blocks = (lanes + 63) / 64;
-    hipLaunchKernelGGL(lane_dp, dim3((unsigned)blocks), dim3(64), 0, stream, segs, n, center, qrev_off, qlens, qrev,
-                       ws, ws_lane, cigs, ncig);
+    hipLaunchKernelGGL(lane_dp, dim3((unsigned)blocks), dim3(64), 0, stream, segs, list, count, center, qrev_off, qlens,
+                       qrev, ws, ws_lane, cigs, ncig);
     return hipGetLastError();
 }
 

@@ -40,11 +40,14 @@ hipError_t mm2_segments_launch(const AnchorRec *anchors, const uint32_t *chain_i
                                const int32_t *chain_cnt, const int64_t *qlens, uint32_t rlen, const int64_t *soff,
                                int nq, SegRec *segs, int32_t *nseg, int64_t *caps, hipStream_t stream);
 hipError_t seg_offsets_launch(SegRec *segs, const int64_t *cig_off, int64_t n, hipStream_t stream);
-hipError_t seg_collect_launch(const SegRec *segs, int64_t n, int64_t *list, unsigned long long *count,
-                              hipStream_t stream);
-hipError_t lane_dp_launch(const SegRec *segs, int64_t n, const uint8_t *center, const int64_t *qrev_off,
-                          const int64_t *qlens, const uint8_t *qrev, uint8_t *ws, int64_t ws_lane, int64_t lanes,
-                          uint32_t *cigs, int32_t *ncig, hipStream_t stream);
+hipError_t seg_split_launch(SegRec *segs, int64_t n, const uint8_t *center, const int64_t *qrev_off,
+                            const int64_t *qlens, const uint8_t *qrev, uint32_t *cigs, int32_t *ncig,
+                            int64_t *lane_list, unsigned long long *lane_count, int64_t *host_list,
+                            unsigned long long *host_count, hipStream_t stream);
+hipError_t lane_dp_launch(const SegRec *segs, const int64_t *list, const unsigned long long *count,
+                          const uint8_t *center, const int64_t *qrev_off, const int64_t *qlens, const uint8_t *qrev,
+                          uint8_t *ws, int64_t ws_lane, int64_t lanes, uint32_t *cigs, int32_t *ncig,
+                          hipStream_t stream);
 hipError_t mm2_assemble_launch(const SegRec *segs, const int64_t *soff, const int32_t *nseg, const uint32_t *cigs,
                                const int32_t *ncig, const int64_t *qlens, uint32_t rlen, const int64_t *ooff, int nq,
                                uint32_t *out, int64_t *nout, hipStream_t stream);
@@ -67,7 +70,7 @@ struct halign_center {
     // per-batch buffers
     DevBuf hcnt, scratch, hidx, nanch, totals, acount, aoff, anchors, keys, keys2, vals, vals2, sort_tmp;
     DevBuf fw, pw, chain_idx, chain_cnt, qlens, soff, segs, nseg, caps, cig_off, scan_tmp, region, bregion;
-    DevBuf qrev_off, lane_ws, ncig, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
+    DevBuf qrev_off, lane_ws, ncig, llist, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
 };
 
 namespace {
@@ -306,16 +309,27 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                                c->bregion.as<int64_t>(), 0, n, ctx->seq.as<uint16_t>(), ctx->seqb.as<uint8_t>(), st),
            "launch prep_queries");
 
-    // small segments: one lane each, on a side stream beside the large
-    // segments' DP (disjoint segments, CIGAR ranges and ncig entries)
+    // small segments: exact ones take nM here, the others one lane each on a
+    // side stream beside the large segments' DP (disjoint segments, CIGAR
+    // ranges and ncig entries); large ones to the host list
     HCHECK(ctx->cig.ensure((size_t)cig_total * 4), "alloc segment CIGARs");
     HCHECK(c->ncig.ensure((size_t)S * 4), "alloc");
+    HCHECK(c->llist.ensure((size_t)S * 8), "alloc");
+    HCHECK(c->hlist.ensure((size_t)S * 8), "alloc");
+    HCHECK(c->hcount.ensure(16), "alloc");
+    HCHECK(hipMemsetAsync(c->hcount.p, 0, 16, st), "clear");
+    unsigned long long *lcnt = c->hcount.as<unsigned long long>(), *hcnt = lcnt + 1;
+    HCHECK(seg_split_launch(c->segs.as<SegRec>(), b.segs, ctx->seqb.as<uint8_t>() + kBytePad,
+                            c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(), ctx->seqb.as<uint8_t>(),
+                            ctx->cig.as<uint32_t>(), c->ncig.as<int32_t>(), c->llist.as<int64_t>(), lcnt,
+                            c->hlist.as<int64_t>(), hcnt, st),
+           "launch seg_split");
     const int64_t lanes = (int64_t)ctx->num_cu * 256;
     HCHECK(c->lane_ws.ensure((size_t)lanes * kLaneWs), "alloc lane DP workspace");
     hipStream_t side = ctx->aux[2];
     HCHECK(hipEventRecord(ctx->pfork, st), "event");
     HCHECK(hipStreamWaitEvent(side, ctx->pfork, 0), "stream wait");
-    HCHECK(lane_dp_launch(c->segs.as<SegRec>(), b.segs, ctx->seqb.as<uint8_t>() + kBytePad,
+    HCHECK(lane_dp_launch(c->segs.as<SegRec>(), c->llist.as<int64_t>(), lcnt, ctx->seqb.as<uint8_t>() + kBytePad,
                           c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(), ctx->seqb.as<uint8_t>(),
                           c->lane_ws.as<uint8_t>(), kLaneWs, lanes, ctx->cig.as<uint32_t>(), c->ncig.as<int32_t>(),
                           side),
@@ -328,14 +342,8 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     } join_side{ctx};
 
     // large segments and band overflows: the ring / full-statistics kernels
-    HCHECK(c->hlist.ensure((size_t)S * 8), "alloc");
-    HCHECK(c->hcount.ensure(8), "alloc");
-    HCHECK(hipMemsetAsync(c->hcount.p, 0, 8, st), "clear");
-    HCHECK(seg_collect_launch(c->segs.as<SegRec>(), b.segs, c->hlist.as<int64_t>(),
-                              c->hcount.as<unsigned long long>(), st),
-           "launch collect");
     unsigned long long nh = 0;
-    HCHECK(hipMemcpyAsync(&nh, c->hcount.p, 8, hipMemcpyDeviceToHost, st), "down");
+    HCHECK(hipMemcpyAsync(&nh, hcnt, 8, hipMemcpyDeviceToHost, st), "down");
     HCHECK(hipStreamSynchronize(st), "collect");
     clk.mark("prep + collect");
     b.host_segs = (int64_t)nh;
@@ -525,7 +533,7 @@ void halign_center_free(halign_center *c)
                       &c->nanch, &c->totals, &c->acount, &c->aoff, &c->anchors, &c->keys, &c->keys2, &c->vals,
                       &c->vals2, &c->sort_tmp, &c->fw, &c->pw, &c->chain_idx, &c->chain_cnt, &c->qlens,
                       &c->soff, &c->segs, &c->nseg, &c->caps, &c->cig_off, &c->scan_tmp, &c->region, &c->bregion,
-                      &c->qrev_off, &c->lane_ws, &c->ncig, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
+                      &c->qrev_off, &c->lane_ws, &c->ncig, &c->llist, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
                       &c->out, &c->nout, &c->ocnt, &c->coff, &c->packed})
         b->release();
     delete c;

@@ -44,7 +44,7 @@ struct SegRec {
     uint32_t rs, re, qs, qe;
     uint32_t flags;   // bit 1: RIGHT (ksw2 flag bit); bit 8: whole pair without a length check
     uint32_t query;   // query index in the batch
-    uint32_t cls;     // kSegEmpty / kSegPure / kSegLane / kSegHost
+    uint32_t cls;     // kSegEmpty / kSegPure / kSegLane / kSegHost / kSegExact
     uint32_t pad;
     int64_t cig_off;  // offset of its CIGAR in the segment CIGAR buffer
 };
@@ -57,6 +57,7 @@ constexpr uint32_t kSegEmpty = 0;     // both sides empty: no CIGAR
 constexpr uint32_t kSegPure = 1;      // one side empty: a single I or D
 constexpr uint32_t kSegLane = 2;      // small DP, one lane per segment
 constexpr uint32_t kSegHost = 3;      // large DP or band overflow: ring / full-statistics kernels
+constexpr uint32_t kSegExact = 4;     // small, equal lengths, identical ACGT bases: CIGAR nM without a DP
 
 // lane DP limits: target length and traceback bytes per lane
 constexpr int kLaneMaxT = 512;

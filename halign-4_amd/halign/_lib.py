@@ -36,6 +36,13 @@ class HalignError(RuntimeError):
     pass
 
 
+HALIGN_PENDING = 1
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("max_batch_queries", C.c_int32), ("hbm_budget", C.c_int64)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
@@ -73,6 +80,17 @@ def _load():
                                         C.POINTER(C.c_double)]
     lib.halign_pairset_free.argtypes = [_P]
     lib.halign_pairset_free.restype = None
+    lib.halign_set_traceback_budget.argtypes = [_P, C.c_int64]
+    lib.halign_server_create.argtypes = [C.c_int, C.c_char_p, C.c_int32, C.POINTER(ServerConfig),
+                                         C.POINTER(_P)]
+    lib.halign_server_submit.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.POINTER(C.c_int64)]
+    lib.halign_server_wait.argtypes = [_P, C.c_int64, C.c_int32]
+    lib.halign_server_result.argtypes = [_P, C.c_int64, _P, C.c_int64, _P, _P, _P,
+                                         C.POINTER(C.c_int64)]
+    lib.halign_server_last_error.argtypes = [_P]
+    lib.halign_server_last_error.restype = C.c_char_p
+    lib.halign_server_free.argtypes = [_P]
+    lib.halign_server_free.restype = None
     return lib
 
 
@@ -360,3 +378,78 @@ class PairSet:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching server over a resident center (halign_server_*):
+    submit() queues a request of queries, wait()/result() collect it."""
+
+    def __init__(self, center, device=0, workers=2, max_batch_queries=4096, hbm_budget=0):
+        self.lib = lib()
+        center = center.encode() if isinstance(center, str) else bytes(center)
+        cfg = ServerConfig(workers, max_batch_queries, hbm_budget)
+        h = _P()
+        rc = self.lib.halign_server_create(device, center, len(center), C.byref(cfg), C.byref(h))
+        if rc != HALIGN_OK:
+            raise HalignError(f"halign_server_create failed: {ERRORS.get(rc, rc)}")
+        self.h = h
+        self._nq = {}
+
+    def close(self):
+        if self.h:
+            self.lib.halign_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != HALIGN_OK:
+            msg = self.lib.halign_server_last_error(self.h).decode()
+            raise HalignError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def submit_flat(self, blob, off):
+        """Queue queries blob[off[i]:off[i+1]] (bytes or uint8 array, int64 offsets)."""
+        keep = None
+        if not isinstance(blob, (bytes, bytearray)):
+            keep = np.ascontiguousarray(blob, np.uint8)
+            blob = C.cast(_ptr(keep), C.c_char_p)
+        off = np.ascontiguousarray(off, np.int64)
+        rid = C.c_int64(0)
+        self._check(self.lib.halign_server_submit(self.h, len(off) - 1, blob, _ptr(off), C.byref(rid)),
+                    "halign_server_submit")
+        self._nq[rid.value] = len(off) - 1
+        return rid.value
+
+    def submit(self, queries):
+        qb = [q.encode() if isinstance(q, str) else bytes(q) for q in queries]
+        off = np.zeros(len(qb) + 1, np.int64)
+        off[1:] = np.cumsum([len(q) for q in qb])
+        return self.submit_flat(b"".join(qb), off)
+
+    def wait(self, rid, timeout_ms=-1):
+        """True once the request is done (False if timeout_ms passed first)."""
+        rc = self.lib.halign_server_wait(self.h, rid, int(timeout_ms))
+        if rc == HALIGN_PENDING:
+            return False
+        self._check(rc, "halign_server_wait")
+        return True
+
+    def result(self, rid):
+        """(flat CIGAR array, CIGAR offsets, stats[n,4], (queue ms, total ms, batch
+        queries)); waits for the request and releases it."""
+        self.wait(rid)
+        n = C.c_int64(0)
+        timing = np.zeros(3, np.float64)
+        self._check(self.lib.halign_server_result(self.h, rid, None, 0, None, None, _ptr(timing),
+                                                  C.byref(n)), "halign_server_result")
+        nq = self._nq.pop(rid)
+        cig = np.zeros(max(1, n.value), np.uint32)
+        coff = np.zeros(nq + 1, np.int64)
+        st = np.zeros((max(1, nq), 4), np.int32)
+        self._check(self.lib.halign_server_result(self.h, rid, _ptr(cig), len(cig), _ptr(coff), _ptr(st),
+                                                  _ptr(timing), C.byref(n)), "halign_server_result")
+        return cig[:n.value], coff, st[:nq], tuple(float(x) for x in timing)

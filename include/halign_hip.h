@@ -23,6 +23,7 @@ extern "C" {
 #define HALIGN_E_NOMEM (-4)        /* device or host allocation failed */
 #define HALIGN_E_CAPACITY (-5)     /* caller-provided output buffer too small */
 #define HALIGN_E_KERNEL (-6)       /* a kernel reported an internal status */
+#define HALIGN_PENDING 1           /* not an error: the request is still in flight */
 
 typedef struct halign_ctx halign_ctx;
 
@@ -194,6 +195,45 @@ int halign_pairset_run(halign_pairset *ps, int32_t first, int32_t count, int32_t
 int halign_pairset_result(halign_pairset *ps, int32_t idx, int32_t *score, uint32_t *cigar,
                           int32_t cigar_cap, int32_t *n_cigar);
 void halign_pairset_free(halign_pairset *ps);
+
+/* Device memory the context may take for traceback slots (default 70% of the
+ * HBM free at halign_init). No reference counterpart. */
+int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes);
+
+/* Continuous-batching server for RefAligner::globalAlign against one center
+ * (src/align/ref_aligner.cpp:91-125; the reference's own request loop is the
+ * OpenMP chunk loop of alignQueryToRef, ref_aligner.cpp:348-386). `workers`
+ * contexts each keep the center and its minimizer index resident; requests
+ * (groups of ASCII queries) queue in arrival order and a free worker takes the
+ * requests at the head of the queue that fit max_batch_queries and the HBM
+ * footprint budget (hbm_budget bytes over all workers, 0 = 85% of free HBM) as
+ * one batch, so requests that arrive while others are on the GPU are admitted
+ * as workers and memory free up and their batches overlap on the device. A
+ * query's CIGAR is the same whichever batch carries it. Thread-safe. */
+typedef struct halign_server halign_server;
+typedef struct {
+    int32_t workers;            /* concurrent batches (1..8), default 2 */
+    int32_t max_batch_queries;  /* queries per batch, default 4096 */
+    int64_t hbm_budget;         /* bytes, 0 = 85% of free HBM */
+} halign_server_config;
+int halign_server_create(int device, const char *center, int32_t len, const halign_server_config *cfg,
+                         halign_server **out);
+/* Copies queries[offsets[0] .. offsets[n]) into the server; *request_id names
+ * the request in later calls. */
+int halign_server_submit(halign_server *s, int32_t n, const char *queries, const int64_t *offsets,
+                         int64_t *request_id);
+/* HALIGN_OK once the request is done, HALIGN_PENDING if timeout_ms (>= 0)
+ * passes first; timeout_ms < 0 waits without a limit. */
+int halign_server_wait(halign_server *s, int64_t request_id, int32_t timeout_ms);
+/* Results of a done request: CIGARs as halign_center_align, stats (4 per
+ * query), timing[3] = {ms from submit to batch start, ms from submit to done,
+ * queries in its batch}, n_cigar = total CIGAR words. cigar_out NULL only
+ * reports sizes and keeps the request; otherwise the request is released. A
+ * failed request returns its error code (and is released). */
+int halign_server_result(halign_server *s, int64_t request_id, uint32_t *cigar_out, int64_t cap,
+                         int64_t *cigar_offsets, int32_t *stats, double *timing, int64_t *n_cigar);
+const char *halign_server_last_error(const halign_server *s);
+void halign_server_free(halign_server *s);
 
 #ifdef __cplusplus
 }
