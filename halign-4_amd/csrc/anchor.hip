@@ -40,25 +40,6 @@ __device__ __forceinline__ int find_hash(const uint64_t *__restrict__ dh, int nd
     return -1;
 }
 
-// block-wide inclusive scan of one value per thread (256 threads); *total
-// receives the sum over the block
-__device__ __forceinline__ int64_t block_scan_incl(int64_t v, int64_t *sh, int64_t *total)
-{
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        const int64_t a = t >= d ? sh[t - d] : 0;
-        __syncthreads();
-        sh[t] += a;
-        __syncthreads();
-    }
-    const int64_t r = sh[t];
-    *total = sh[255];
-    __syncthreads();
-    return r;
-}
-
 // anchors a query hit expands to (0 if filtered out); d = its center hash index
 __device__ __forceinline__ uint32_t hit_anchors(const CenterIndexDev &ci, const SeedFilter &f, int d, uint32_t qocc,
                                                 double limit, uint32_t qpos)

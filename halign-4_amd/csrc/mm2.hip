@@ -422,6 +422,88 @@ hipError_t seg_offsets_launch(SegRec *segs, const int64_t *cig_off, int64_t n, h
 }
 
 
+// cigar::delQueryToRefByCigar (src/align/cigar.cpp:212-299) of each query:
+// its row (M/S/=/X copy query bases, D/N write '-', I skips query bases, H/P
+// nothing; for an MM2 CIGAR the row spans the center), its length in
+// row_len, and cigar::hasInsertion (:70-79). One workgroup per query: a block
+// scan over the ops gives each op's end in the row and start on the query (ws,
+// laid out like the CIGARs), then every row column finds its op by binary
+// search. An empty CIGAR leaves the query as it is (:214-216). Rows longer
+// than stride are cut at stride (row_len still gives the full length).
+__global__ __launch_bounds__(256) void project_rows(const uint32_t *__restrict__ cig, const int64_t *__restrict__ coff,
+                                                    const char *__restrict__ qseq, const int64_t *__restrict__ qoff,
+                                                    char *__restrict__ rows, int64_t stride,
+                                                    int64_t *__restrict__ row_len, int64_t *__restrict__ ws,
+                                                    uint8_t *__restrict__ has_ins)
+{
+    __shared__ int64_t sh[256];
+    __shared__ int ins;
+    const int q = blockIdx.x;
+    const uint32_t *c = cig + coff[q];
+    const int64_t n = coff[q + 1] - coff[q];
+    int64_t *w = ws + coff[q];   // (ref end << 32) | query start, per op
+    const char *qs = qseq + qoff[q];
+    const int64_t qlen = qoff[q + 1] - qoff[q];
+    char *row = rows + (int64_t)q * stride;
+    if (threadIdx.x == 0) ins = 0;
+    __syncthreads();
+    int64_t carry = 0;
+    for (int64_t k0 = 0; k0 < n; k0 += 256) {
+        const int64_t k = k0 + threadIdx.x;
+        int64_t v = 0;
+        uint32_t op = 0, len = 0;
+        if (k < n) {
+            op = c[k] & 0xf;
+            len = c[k] >> 4;
+            const bool col = !(op == 1 || op == 5 || op == 6);               // advances the row
+            const bool qry = !(op == 2 || op == 3 || op == 5 || op == 6);    // advances the query
+            v = ((int64_t)(col ? len : 0) << 32) | (qry ? len : 0);
+            if (op == 1) ins = 1;
+        }
+        int64_t tot = 0;
+        const int64_t incl = block_scan_incl(v, sh, &tot) + carry;
+        if (k < n) w[k] = (incl & ~0xffffffffll) | ((incl - v) & 0xffffffffll);
+        carry += tot;
+    }
+    __syncthreads();
+    const int64_t out_len = n == 0 ? qlen : (carry >> 32);
+    if (threadIdx.x == 0) {
+        has_ins[q] = (uint8_t)ins;
+        row_len[q] = out_len;
+    }
+    if (n == 0) {
+        for (int64_t p = threadIdx.x; p < qlen && p < stride; p += 256) row[p] = qs[p];
+        return;
+    }
+    for (int64_t p = threadIdx.x; p < out_len && p < stride; p += 256) {
+        int64_t lo = 0, hi = n;   // first op whose end on the center is > p
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((w[mid] >> 32) > p) hi = mid;
+            else lo = mid + 1;
+        }
+        char ch = '-';
+        if (lo < n) {
+            const uint32_t op = c[lo] & 0xf, len = c[lo] >> 4;
+            if (op != 2 && op != 3) {
+                const int64_t start = (w[lo] >> 32) - (int64_t)len;
+                ch = qs[(w[lo] & 0xffffffffll) + (p - start)];
+            }
+        }
+        row[p] = ch;
+    }
+}
+
+hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
+                               int nq, char *rows, int64_t stride, int64_t *row_len, int64_t *ws, uint8_t *has_ins,
+                               hipStream_t stream)
+{
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(project_rows, dim3((unsigned)nq), dim3(256), 0, stream, cig, coff, qseq, qoff, rows, stride,
+                       row_len, ws, has_ins);
+    return hipGetLastError();
+}
+
 hipError_t seg_split_launch(SegRec *segs, int64_t n, const uint8_t *center, const int64_t *qrev_off,
                             const int64_t *qlens, const uint8_t *qrev, uint32_t *cigs, int32_t *ncig,
                             int64_t *lane_list, unsigned long long *lane_count, int64_t *host_list,

@@ -48,6 +48,9 @@ hipError_t lane_dp_launch(const SegRec *segs, const int64_t *list, const unsigne
                           const uint8_t *center, const int64_t *qrev_off, const int64_t *qlens, const uint8_t *qrev,
                           uint8_t *ws, int64_t ws_lane, int64_t lanes, uint32_t *cigs, int32_t *ncig,
                           hipStream_t stream);
+hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
+                               int nq, char *rows, int64_t stride, int64_t *row_len, int64_t *ws, uint8_t *has_ins,
+                               hipStream_t stream);
 hipError_t mm2_assemble_launch(const SegRec *segs, const int64_t *soff, const int32_t *nseg, const uint32_t *cigs,
                                const int32_t *ncig, const int64_t *qlens, uint32_t rlen, const int64_t *ooff, int nq,
                                uint32_t *out, int64_t *nout, hipStream_t stream);
@@ -71,6 +74,7 @@ struct halign_center {
     DevBuf hcnt, scratch, hidx, nanch, totals, acount, aoff, anchors, keys, keys2, vals, vals2, sort_tmp;
     DevBuf fw, pw, chain_idx, chain_cnt, qlens, soff, segs, nseg, caps, cig_off, scan_tmp, region, bregion;
     DevBuf qrev_off, lane_ws, ncig, llist, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
+    DevBuf rows, rlen_d, rws, hins;   // projected rows, their lengths, op workspace, insertion flags
 };
 
 namespace {
@@ -534,7 +538,8 @@ void halign_center_free(halign_center *c)
                       &c->vals2, &c->sort_tmp, &c->fw, &c->pw, &c->chain_idx, &c->chain_cnt, &c->qlens,
                       &c->soff, &c->segs, &c->nseg, &c->caps, &c->cig_off, &c->scan_tmp, &c->region, &c->bregion,
                       &c->qrev_off, &c->lane_ws, &c->ncig, &c->llist, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
-                      &c->out, &c->nout, &c->ocnt, &c->coff, &c->packed})
+                      &c->out, &c->nout, &c->ocnt, &c->coff, &c->packed, &c->rows,
+                      &c->rlen_d, &c->rws, &c->hins})
         b->release();
     delete c;
 }
@@ -587,10 +592,17 @@ int halign_center_chains(halign_center *c, int32_t n, const char *queries, const
     return HALIGN_OK;
 }
 
-int halign_center_align(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
-                        uint32_t *cigar_out, int64_t cap, int64_t *cigar_offsets, int32_t *stats, double *ms)
+}  // extern "C"
+
+namespace {
+
+// halign_center_align, and with rows_out the projected rows as well
+int center_align(halign_center *c, int32_t n, const char *queries, const int64_t *offsets, uint32_t *cigar_out,
+                 int64_t cap, int64_t *cigar_offsets, int32_t *stats, double *ms, char *rows_out, int64_t row_stride,
+                 int64_t *row_len, uint8_t *has_insertion)
 {
     if (!c || n < 0 || (n > 0 && (!queries || !offsets || !cigar_offsets))) return HALIGN_E_ARG;
+    if (rows_out && (row_stride <= 0 || !row_len || !has_insertion)) return HALIGN_E_ARG;
     halign_ctx *ctx = c->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
     HCHECK(hipSetDevice(ctx->device), "set device");
@@ -599,33 +611,42 @@ int halign_center_align(halign_center *c, int32_t n, const char *queries, const 
     hipEvent_t e0, e1;
     HCHECK(hipEventCreate(&e0), "event");
     HCHECK(hipEventCreate(&e1), "event");
+    struct Events {
+        hipEvent_t a, b;
+        ~Events() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } evs{e0, e1};
     HCHECK(hipEventRecord(e0, ctx->stream), "event");
     Batch b;
     const int rc = run_batch(c, n, queries, offsets, 3, b);
-    if (rc != HALIGN_OK) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        return rc;
-    }
-    HCHECK(hipEventRecord(e1, ctx->stream), "event");
+    if (rc != HALIGN_OK) return rc;
     int64_t o = 0;
     for (int i = 0; i < n; ++i) {
-        if (b.nout[i] < 0) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return fail(ctx, HALIGN_E_KERNEL, "MM2 CIGAR does not consume the pair");
-        }
+        if (b.nout[i] < 0) return fail(ctx, HALIGN_E_KERNEL, "MM2 CIGAR does not consume the pair");
         o += b.nout[i];
     }
-    if (o > cap) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        return fail(ctx, HALIGN_E_CAPACITY, "CIGAR output too small");
+    if (o > cap) return fail(ctx, HALIGN_E_CAPACITY, "CIGAR output too small");
+    if (rows_out) {
+        // rows from the packed CIGARs and the raw queries, still on the device
+        HCHECK(c->rows.ensure((size_t)n * (size_t)row_stride), "alloc rows");
+        HCHECK(c->rlen_d.ensure((size_t)n * 8), "alloc");
+        HCHECK(c->rws.ensure((size_t)std::max<int64_t>(b.packed, 1) * 8), "alloc");
+        HCHECK(c->hins.ensure((size_t)n), "alloc");
+        HCHECK(project_rows_launch(c->packed.as<uint32_t>(), c->coff.as<int64_t>(), ctx->s_seqs.as<char>(),
+                                   ctx->s_off.as<int64_t>(), n, c->rows.as<char>(), row_stride,
+                                   c->rlen_d.as<int64_t>(), c->rws.as<int64_t>(), c->hins.as<uint8_t>(), ctx->stream),
+               "launch project_rows");
     }
+    HCHECK(hipEventRecord(e1, ctx->stream), "event");
     if (cigar_out && b.packed > 0)
         HCHECK(hipMemcpyAsync(cigar_out, c->packed.p, (size_t)b.packed * 4, hipMemcpyDeviceToHost, ctx->stream),
                "download CIGARs");
-    HCHECK(hipStreamSynchronize(ctx->stream), "download CIGARs");
+    if (rows_out) {
+        HCHECK(hipMemcpyAsync(rows_out, c->rows.p, (size_t)n * (size_t)row_stride, hipMemcpyDeviceToHost, ctx->stream),
+               "download rows");
+        HCHECK(hipMemcpyAsync(row_len, c->rlen_d.p, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+        HCHECK(hipMemcpyAsync(has_insertion, c->hins.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "download");
+    }
+    HCHECK(hipStreamSynchronize(ctx->stream), "download results");
     o = 0;
     for (int i = 0; i < n; ++i) {
         cigar_offsets[i] = o;
@@ -643,9 +664,27 @@ int halign_center_align(halign_center *c, int32_t n, const char *queries, const 
         HCHECK(hipEventElapsedTime(&t, e0, e1), "event time");
         *ms = t;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return HALIGN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int halign_center_align(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                        uint32_t *cigar_out, int64_t cap, int64_t *cigar_offsets, int32_t *stats, double *ms)
+{
+    return center_align(c, n, queries, offsets, cigar_out, cap, cigar_offsets, stats, ms, nullptr, 0, nullptr,
+                        nullptr);
+}
+
+int halign_center_align_rows(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                             uint32_t *cigar_out, int64_t cap, int64_t *cigar_offsets, char *rows_out,
+                             int64_t row_stride, int64_t *row_len, uint8_t *has_insertion, int32_t *stats, double *ms)
+{
+    if (!rows_out) return HALIGN_E_ARG;
+    return center_align(c, n, queries, offsets, cigar_out, cap, cigar_offsets, stats, ms, rows_out, row_stride,
+                        row_len, has_insertion);
 }
 
 }  // extern "C"

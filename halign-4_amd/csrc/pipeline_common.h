@@ -2,6 +2,8 @@
 // their host side: the center index, filter and chain parameters, anchors
 // and MM2 segments.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace hal {
@@ -58,6 +60,25 @@ constexpr uint32_t kSegPure = 1;      // one side empty: a single I or D
 constexpr uint32_t kSegLane = 2;      // small DP, one lane per segment
 constexpr uint32_t kSegHost = 3;      // large DP or band overflow: ring / full-statistics kernels
 constexpr uint32_t kSegExact = 4;     // small, equal lengths, identical ACGT bases: CIGAR nM without a DP
+
+// block-wide inclusive scan of one value per thread (256 threads); *total
+// receives the sum over the block
+__device__ __forceinline__ int64_t block_scan_incl(int64_t v, int64_t *sh, int64_t *total)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const int64_t a = t >= d ? sh[t - d] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    const int64_t r = sh[t];
+    *total = sh[255];
+    __syncthreads();
+    return r;
+}
 
 // lane DP limits: target length and traceback bytes per lane
 constexpr int kLaneMaxT = 512;

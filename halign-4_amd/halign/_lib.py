@@ -78,6 +78,8 @@ def _load():
     lib.halign_center_chains.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P]
     lib.halign_center_align.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P, _P,
                                         C.POINTER(C.c_double)]
+    lib.halign_center_align_rows.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P, _P,
+                                             C.c_int64, _P, _P, _P, C.POINTER(C.c_double)]
     lib.halign_pairset_free.argtypes = [_P]
     lib.halign_pairset_free.restype = None
     lib.halign_set_traceback_budget.argtypes = [_P, C.c_int64]
@@ -311,6 +313,29 @@ class Center:
         blob, off = _blob(queries)
         cig, coff, st, ms = self.align_flat(blob, off)
         return [cig[coff[i]:coff[i + 1]] for i in range(len(off) - 1)], st, ms
+
+    def align_rows(self, queries, row_stride=None):
+        """RefAligner::globalAlign plus the keep-length rows (delQueryToRefByCigar)
+        and hasInsertion per query: (list of CIGAR arrays, list of row bytes,
+        has_insertion bool array)."""
+        blob, off = _blob(queries)
+        off = np.ascontiguousarray(off, np.int64)
+        n = len(off) - 1
+        stride = row_stride or max(self.rlen, 1)
+        cap = int(off[-1] - off[0]) + n * (self.rlen + 1) + 1
+        cig = np.empty(cap, np.uint32)
+        coff = np.zeros(n + 1, np.int64)
+        rows = np.zeros((max(n, 1), stride), np.uint8)
+        rlen = np.zeros(max(n, 1), np.int64)
+        hins = np.zeros(max(n, 1), np.uint8)
+        st = np.zeros((max(n, 1), 4), np.int32)
+        ms = C.c_double(0)
+        self.ctx._check(self.ctx.lib.halign_center_align_rows(
+            self.h, n, blob, _ptr(off), _ptr(cig), cap, _ptr(coff), _ptr(rows), stride, _ptr(rlen),
+            _ptr(hins), _ptr(st), C.byref(ms)), "halign_center_align_rows")
+        cigs = [cig[coff[i]:coff[i + 1]].copy() for i in range(n)]
+        rws = [rows[i, :min(int(rlen[i]), stride)].tobytes() for i in range(n)]
+        return cigs, rws, hins[:n].astype(bool)
 
     def align_flat(self, blob, off):
         """The same on concatenated queries (bytes or uint8 array, int64

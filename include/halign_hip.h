@@ -163,6 +163,17 @@ int halign_center_chains(halign_center *c, int32_t n, const char *queries, const
  * minimizer status bits, anchors. ms (optional): device time of the batch. */
 int halign_center_align(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
                         uint32_t *cigar_out, int64_t cap, int64_t *cigar_offsets, int32_t *stats, double *ms);
+/* halign_center_align plus, per query, cigar::delQueryToRefByCigar (src/align/
+ * cigar.cpp:212-299: the keep-length row, center columns; rows_out holds n
+ * rows row_stride bytes apart, row_len[i] its length, a row longer than
+ * row_stride is cut) and cigar::hasInsertion (:70-79, the SAM routing of
+ * RefAligner::alignOneQueryToRef, src/align/ref_aligner.cpp:212-283) in
+ * has_insertion[i]. The rows are made on the device from the CIGARs and the
+ * raw query bytes. */
+int halign_center_align_rows(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                             uint32_t *cigar_out, int64_t cap, int64_t *cigar_offsets, char *rows_out,
+                             int64_t row_stride, int64_t *row_len, uint8_t *has_insertion, int32_t *stats,
+                             double *ms);
 
 /* Column counts and consensus of an alignment (the counting core of
  * consensus::generateConsensusSequence, reference src/consensus/consensus.cpp:

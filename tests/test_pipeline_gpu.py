@@ -134,3 +134,41 @@ def test_align_batch_vs_oracle(ctx):
             bad.append(i)
     assert not bad, f"{len(bad)} differ, first {bad[:5]}"
     c.close()
+
+
+def test_rows_and_insertion_flags(ctx):
+    """delQueryToRefByCigar rows and hasInsertion from the device, against the
+    oracle's projection of the same CIGARs (and the CIGARs against the oracle's
+    RefAligner::globalAlign), on queries with and without insertions, N bases,
+    lower case and queries without anchors (whole-pair CIGAR)."""
+    import halign
+    O = Oracle()
+    rng = np.random.default_rng(21)
+    center = rand_codes(rng, 5000)
+    cstr = codes_to_ascii(center)
+    qs = []
+    for i in range(120):
+        a = int(rng.integers(0, 4000))
+        q = mutate_codes(rng, center[a:a + int(rng.integers(1, 1000))], 0.02, 0.01 if i % 3 else 0.0)
+        if i % 10 == 0:
+            q = rand_codes(rng, int(rng.integers(0, 200)))
+        s = codes_to_ascii(q)
+        if i % 7 == 0 and s:
+            s = s[:len(s) // 2].lower() + s[len(s) // 2:]
+        qs.append(s)
+    c = halign.Center(ctx, cstr)
+    try:
+        cigs, rows, hins = c.align_rows(qs)
+    finally:
+        c.close()
+    print(f"rows: {len(qs)} queries, {int(hins.sum())} with insertions", file=sys.stderr, flush=True)
+    bad = []
+    for i, q in enumerate(qs):
+        exp_cig, _, _, ub = O.ref_global_align(cstr, q)
+        if ub & 3:
+            continue
+        exp_row = O.del_query_to_ref(q, cigs[i])
+        if not np.array_equal(cigs[i], exp_cig) or rows[i] != exp_row or hins[i] != any((int(x) & 0xf) == 1 for x in cigs[i]):
+            bad.append(i)
+    assert not bad, f"{len(bad)} differ, first {bad[:5]}"
+    assert 0 < int(hins.sum()) < len(qs)
