@@ -83,6 +83,27 @@ bool fill_spec_shape(Spec &s, bool full_stats, int policy);
 hipError_t prep_queries_launch(const char *raw, const int64_t *qoff, const int64_t *region, const int64_t *bregion,
                                int first, int count, uint16_t *seq, uint8_t *seqb, hipStream_t stream);
 void spread_target(const uint8_t *codes, int len, uint16_t *even, uint16_t *odd);
+
+// Ring-form kernels (dp_wave.hip): shape table, cost figures, launch
+int dp_wave_num_shapes();
+int dp_wave_shape_waves(int idx);
+int dp_wave_shape_ring(int idx);
+int dp_wave_jobs_per_group(int idx);
+int dp_wave_shape_lanes(int idx);
+void dp_wave_shape_cost(int idx, int *issue, int *valu);
+int dp_wave_max_groups(int idx, bool rt, int num_cu);
+hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
+                          uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream,
+                          const int *range = nullptr);
+// index of the ring shape with these waves (0: sub-wave), lanes per job and slots; -1 if none
+inline int dp_wave_find_shape(int waves, int lanes, int slots)
+{
+    for (int i = 0; i < dp_wave_num_shapes(); ++i)
+        if (dp_wave_shape_waves(i) == waves && dp_wave_shape_lanes(i) == lanes &&
+            dp_wave_shape_ring(i) == lanes * slots)
+            return i;
+    return -1;
+}
 void bytes_target(const uint8_t *codes, int len, uint8_t *out);
 
 }  // namespace hal

@@ -584,12 +584,19 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
 
 // Persistent kernel: each workgroup pulls jobs from a queue and reuses its own
 // traceback slot; all jobs of one launch share the gap direction.
+// range (optional, on the device): {first job, job count} within jobs/outs,
+// for job lists built on the device.
 template <int W, int S, bool RT>
 __global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__restrict__ jobs, int njobs,
                                                              int *__restrict__ counter,
                                                              uint8_t *__restrict__ tb_pool, size_t tb_slot_bytes,
-                                                             DpOut *__restrict__ outs)
+                                                             DpOut *__restrict__ outs, const int *__restrict__ range)
 {
+    if (range) {
+        jobs += range[0];
+        outs += range[0];
+        njobs = range[1];
+    }
     __shared__ WaveSmem<W, S> sm;
     uint8_t *tb = tb_pool + (size_t)blockIdx.x * tb_slot_bytes;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -625,9 +632,15 @@ __global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__rest
 template <int SUB, int S, bool RT>
 __global__ __launch_bounds__(64) void dp_sub_persistent(const DpJob *__restrict__ jobs, int njobs,
                                                         int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
-                                                        size_t tb_slot_bytes, DpOut *__restrict__ outs)
+                                                        size_t tb_slot_bytes, DpOut *__restrict__ outs,
+                                                        const int *__restrict__ range)
 {
     constexpr int G = 64 / SUB;
+    if (range) {
+        jobs += range[0];
+        outs += range[0];
+        njobs = range[1];
+    }
     __shared__ WaveSmem<1, S> sm;
     const int g = (int)(threadIdx.x / SUB);
     uint8_t *tb = tb_pool + ((size_t)blockIdx.x * G + (size_t)g) * tb_slot_bytes;
@@ -707,12 +720,13 @@ int dp_wave_max_groups(int idx, bool rt, int num_cu)
     return per_cu * num_cu;
 }
 
-// tb_pool holds groups * dp_wave_jobs_per_group(idx) slots of tb_slot_bytes
+// tb_pool holds groups * dp_wave_jobs_per_group(idx) slots of tb_slot_bytes;
+// range (device, optional) overrides jobs/outs offset and njobs
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
-                          uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream)
+                          uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream, const int *range)
 {
     void *args[] = {(void *)&jobs, (void *)&njobs, (void *)&counter, (void *)&tb_pool,
-                    (void *)&tb_slot_bytes, (void *)&outs};
+                    (void *)&tb_slot_bytes, (void *)&outs, (void *)&range};
     return hipLaunchKernel(wave_kernel(idx, rt), dim3(groups), dim3(block_threads(idx)), args, 0, stream);
 }
 

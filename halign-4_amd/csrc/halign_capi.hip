@@ -23,15 +23,6 @@ int dp_shape_threads(int idx);
 int dp_shape_max_groups(int idx, bool stats, int num_cu);
 hipError_t dp_launch(int idx, bool stats, int groups, const DpJob *jobs, int njobs, int *counter,
                      uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
-int dp_wave_num_shapes();
-int dp_wave_shape_waves(int idx);
-int dp_wave_shape_ring(int idx);
-int dp_wave_jobs_per_group(int idx);
-int dp_wave_shape_lanes(int idx);
-void dp_wave_shape_cost(int idx, int *issue, int *valu);
-int dp_wave_max_groups(int idx, bool rt, int num_cu);
-hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
-                          uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
 hipError_t consensus_launch(const uint8_t *rows, int64_t row_stride, int64_t aln_len, int64_t row_end, int64_t batch,
                             int num_cu, uint32_t *counts, char *cons, hipStream_t stream);
 }  // namespace hal
@@ -379,7 +370,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                 off += L.slot * L.jpg * (size_t)L.groups;
             }
             clk.mark("launch plan");
-            HCHECK(ctx->tb.ensure(std::max<size_t>(off, 1)), "alloc traceback pool");
+            HCHECK(ctx->tb.ensure(std::max<size_t>(off, 1), ctx->tb_budget), "alloc traceback pool");
             clk.mark("traceback pool");
             std::vector<int> ord(ls.size());
             for (size_t k = 0; k < ls.size(); ++k) ord[k] = (int)k;
@@ -400,7 +391,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             }
             size_t mx = 0;
             for (auto &L : ls) mx = std::max(mx, L.slot * L.jpg * (size_t)L.groups);
-            HCHECK(ctx->tb.ensure(std::max<size_t>(mx, 1)), "alloc traceback pool");
+            HCHECK(ctx->tb.ensure(std::max<size_t>(mx, 1), ctx->tb_budget), "alloc traceback pool");
         }
         bool used[3] = {true, false, false};
         for (auto &L : ls) used[L.stream] = true;

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -17,14 +18,24 @@ namespace hal {
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
-    hipError_t ensure(size_t n)
+    // At least n bytes. A buffer that has to grow grows by half again (up to
+    // limit, when given) so that batches of varying size do not reallocate
+    // (hipFree + hipMalloc of a large buffer costs far more than a batch).
+    hipError_t ensure(size_t n, size_t limit = 0)
     {
         if (n <= cap) return hipSuccess;
+        size_t want = cap ? std::max(n, cap + cap / 2) : n;
+        if (limit && want > limit) want = std::max(n, limit);
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) cap = n;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess && want > n) {
+            (void)hipGetLastError();
+            want = n;
+            e = hipMalloc(&p, want);
+        }
+        if (e == hipSuccess) cap = want;
         return e;
     }
     void release()

@@ -278,34 +278,173 @@ __device__ int lane_extz2(const uint8_t *__restrict__ tgt, int tlen, const uint8
 // one) has the all-match path as its unique optimum, so ksw_extz2 returns the
 // CIGAR nM: written here, class kSegExact. Other small segments go to the lane
 // list, large ones to the host list (both appended with an atomic).
-__global__ void seg_split(SegRec *__restrict__ segs, int64_t n, const uint8_t *__restrict__ center,
-                          const int64_t *__restrict__ qrev_off, const int64_t *__restrict__ qlens,
-                          const uint8_t *__restrict__ qrev, uint32_t *__restrict__ cigs, int32_t *__restrict__ ncig,
-                          int64_t *__restrict__ lane_list, unsigned long long *__restrict__ lane_count,
-                          int64_t *__restrict__ host_list, unsigned long long *__restrict__ host_count)
+__global__ __launch_bounds__(256) void seg_split(SegRec *__restrict__ segs, int64_t n,
+                                                 const uint8_t *__restrict__ center,
+                                                 const int64_t *__restrict__ qrev_off, const int64_t *__restrict__ qlens,
+                                                 const uint8_t *__restrict__ qrev, uint32_t *__restrict__ cigs,
+                                                 int32_t *__restrict__ ncig, int64_t *__restrict__ lane_list,
+                                                 unsigned long long *__restrict__ lane_count,
+                                                 int64_t *__restrict__ host_list,
+                                                 unsigned long long *__restrict__ host_count)
 {
+    // list slots: ranks in LDS, one global atomic per list and workgroup
+    __shared__ unsigned int lc[2];
+    __shared__ unsigned long long lbase[2];
+    if (threadIdx.x < 2) lc[threadIdx.x] = 0;
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const SegRec sg = segs[i];
-    if (sg.cls == kSegHost) {
-        host_list[atomicAdd(host_count, 1ull)] = i;
-        return;
+    int which = -1;   // 0: lane list, 1: host list
+    unsigned int rank = 0;
+    if (i < n) {
+        const SegRec sg = segs[i];
+        if (sg.cls == kSegHost) {
+            which = 1;
+        } else if (sg.cls == kSegLane) {
+            const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
+            bool same = srl == sql;
+            if (same) {
+                const uint8_t *qr = qrev + qrev_off[sg.query] + (qlens[sg.query] - 1 - sg.qs);
+                const uint8_t *t = center + sg.rs;
+                for (int k = 0; k < srl && same; ++k) same = t[k] < 4 && t[k] == qr[-k];
+            }
+            if (same) {
+                cigs[sg.cig_off] = (uint32_t)srl << 4;   // M
+                ncig[i] = 1;
+                segs[i].cls = kSegExact;
+            } else {
+                which = 0;
+            }
+        }
+        if (which >= 0) rank = atomicAdd(&lc[which], 1u);
     }
-    if (sg.cls != kSegLane) return;
+    __syncthreads();
+    if (threadIdx.x < 2 && lc[threadIdx.x])
+        lbase[threadIdx.x] = atomicAdd(threadIdx.x ? host_count : lane_count, (unsigned long long)lc[threadIdx.x]);
+    __syncthreads();
+    if (which == 0) lane_list[lbase[0] + rank] = i;
+    if (which == 1) host_list[lbase[1] + rank] = i;
+}
+
+// ---- small segments through the ring kernels (jobs built on the device) ----
+
+__device__ __forceinline__ int lane_bucket(const SegRec &sg, const LaneShapeBounds &b)
+{
     const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
-    bool same = srl == sql;
-    if (same) {
-        const uint8_t *qr = qrev + qrev_off[sg.query] + (qlens[sg.query] - 1 - sg.qs);
-        const uint8_t *t = center + sg.rs;
-        for (int k = 0; k < srl && same; ++k) same = t[k] < 4 && t[k] == qr[-k];
+    int w = auto_band_dev(srl, sql);
+    if (w < 0) w = srl > sql ? srl : sql;
+    int nc = srl < sql ? srl : sql;
+    nc = ((nc < w + 1 ? nc : w + 1) + 15) / 16 + 1;
+    int sh = 0;
+    while (sh < kLaneShapes - 1 && nc > b.ncol_max[sh]) ++sh;
+    return sh * 2 + ((sg.flags & kSegRight) ? 1 : 0);
+}
+
+// per-bucket job counts (bcnt[2 * kLaneShapes], cleared before): counted in
+// LDS, one global atomic per bucket and workgroup
+__global__ __launch_bounds__(256) void lane_jobs_count(const SegRec *__restrict__ segs,
+                                                       const int64_t *__restrict__ list,
+                                                       const unsigned long long *__restrict__ count,
+                                                       LaneShapeBounds b, int *__restrict__ bcnt)
+{
+    __shared__ int lc[2 * kLaneShapes];
+    if (threadIdx.x < 2 * kLaneShapes) lc[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li < (int64_t)*count) atomicAdd(&lc[lane_bucket(segs[list[li]], b)], 1);
+    __syncthreads();
+    if (threadIdx.x < 2 * kLaneShapes && lc[threadIdx.x]) atomicAdd(&bcnt[threadIdx.x], lc[threadIdx.x]);
+}
+
+// range[2k] = first job of bucket k, range[2k+1] = its job count; bfill cleared
+__global__ void lane_jobs_offsets(const int *__restrict__ bcnt, int *__restrict__ range, int *__restrict__ bfill)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int at = 0;
+    for (int k = 0; k < 2 * kLaneShapes; ++k) {
+        range[2 * k] = at;
+        range[2 * k + 1] = bcnt[k];
+        bfill[k] = 0;
+        at += bcnt[k];
     }
-    if (same) {
-        cigs[sg.cig_off] = (uint32_t)srl << 4;   // M
-        ncig[i] = 1;
-        segs[i].cls = kSegExact;
+}
+
+// one DpJob per small segment, in its bucket's range (byte-code windows of the
+// center and of the reversed query, as the host builds them for large segments)
+__global__ __launch_bounds__(256) void lane_jobs_place(
+    const SegRec *__restrict__ segs, const int64_t *__restrict__ list, const unsigned long long *__restrict__ count,
+    LaneShapeBounds b, const uint8_t *__restrict__ center, const uint8_t *__restrict__ qrev,
+    const int64_t *__restrict__ qrev_off, const int64_t *__restrict__ qlens, uint32_t *__restrict__ cigs,
+    const int *__restrict__ range, int *__restrict__ bfill, DpJob *__restrict__ jobs, int64_t *__restrict__ job_seg)
+{
+    // ranks within the workgroup in LDS, one global atomic per bucket and workgroup
+    __shared__ int lc[2 * kLaneShapes], lbase[2 * kLaneShapes];
+    if (threadIdx.x < 2 * kLaneShapes) lc[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = li < (int64_t)*count;
+    int64_t i = 0;
+    SegRec sg{};
+    int k = 0, rank = 0;
+    if (valid) {
+        i = list[li];
+        sg = segs[i];
+        k = lane_bucket(sg, b);
+        rank = atomicAdd(&lc[k], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kLaneShapes && lc[threadIdx.x])
+        lbase[threadIdx.x] = atomicAdd(&bfill[threadIdx.x], lc[threadIdx.x]);
+    __syncthreads();
+    if (!valid) return;
+    const int pos = range[2 * k] + lbase[k] + rank;
+    const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
+    DpJob j;
+    j.tgt_even = j.tgt_odd = j.qr_even = j.qr_odd = nullptr;   // spread copies: multi-wave kernel only
+    j.tgt_b = center + sg.rs;
+    j.qr_b = qrev + qrev_off[sg.query] + (qlens[sg.query] - sg.qe);   // reversed index of base qe-1
+    j.tlen = srl;
+    j.qlen = sql;
+    j.w = auto_band_dev(srl, sql);
+    j.flags = ((sg.flags & kSegRight) ? kFlagRight : 0) | kFlagHasN;
+    j.cigar = cigs + sg.cig_off;
+    j.cig_cap = srl + sql + 1;
+    j.pad_ = 0;
+    jobs[pos] = j;
+    job_seg[pos] = i;
+}
+
+// CIGAR lengths back to the segments (class kSegRing: ops stored last first);
+// a job that did not end with status ok sets *err
+__global__ void lane_jobs_finish(const DpOut *__restrict__ outs, const int64_t *__restrict__ job_seg,
+                                 const unsigned long long *__restrict__ count, SegRec *__restrict__ segs,
+                                 int32_t *__restrict__ ncig, int *__restrict__ err)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int64_t)*count) return;
+    const int64_t i = job_seg[j];
+    const DpOut o = outs[j];
+    if (o.status != kStOk) atomicOr(err, 1);
+    ncig[i] = o.n_cigar;
+    segs[i].cls = kSegRing;
+}
+
+hipError_t lane_jobs_launch(int phase, int64_t nmax, const SegRec *segs, const int64_t *list,
+                            const unsigned long long *count, const LaneShapeBounds &b, const uint8_t *center,
+                            const uint8_t *qrev, const int64_t *qrev_off, const int64_t *qlens, uint32_t *cigs,
+                            int *bcnt, int *range, int *bfill, DpJob *jobs, int64_t *job_seg, const DpOut *outs,
+                            SegRec *wsegs, int32_t *ncig, int *err, hipStream_t stream)
+{
+    if (nmax <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((nmax + 255) / 256)), blk(256);
+    if (phase == 0) {
+        hipLaunchKernelGGL(lane_jobs_count, grid, blk, 0, stream, segs, list, count, b, bcnt);
+        hipLaunchKernelGGL(lane_jobs_offsets, dim3(1), dim3(64), 0, stream, bcnt, range, bfill);
+        hipLaunchKernelGGL(lane_jobs_place, grid, blk, 0, stream, segs, list, count, b, center, qrev, qrev_off, qlens,
+                           cigs, range, bfill, jobs, job_seg);
     } else {
-        lane_list[atomicAdd(lane_count, 1ull)] = i;
+        hipLaunchKernelGGL(lane_jobs_finish, grid, blk, 0, stream, outs, job_seg, count, wsegs, ncig, err);
     }
+    return hipGetLastError();
 }
 
 // lane DP over the lane list (count read on the device)
@@ -391,7 +530,7 @@ __global__ void mm2_assemble(const SegRec *__restrict__ segs, const int64_t *__r
                 continue;
             }
         }
-        if (s.cls == kSegHost) {   // the ring / full-statistics kernels store the ops last to first
+        if (s.cls == kSegHost || s.cls == kSegRing) {   // the ring / full-statistics kernels store the ops last to first
             for (int i = nc - 1; i >= 0; --i) ac.add(c[i]);
         } else {
             for (int i = 0; i < nc; ++i) ac.add(c[i]);

@@ -44,10 +44,11 @@ hipError_t seg_split_launch(SegRec *segs, int64_t n, const uint8_t *center, cons
                             const int64_t *qlens, const uint8_t *qrev, uint32_t *cigs, int32_t *ncig,
                             int64_t *lane_list, unsigned long long *lane_count, int64_t *host_list,
                             unsigned long long *host_count, hipStream_t stream);
-hipError_t lane_dp_launch(const SegRec *segs, const int64_t *list, const unsigned long long *count,
-                          const uint8_t *center, const int64_t *qrev_off, const int64_t *qlens, const uint8_t *qrev,
-                          uint8_t *ws, int64_t ws_lane, int64_t lanes, uint32_t *cigs, int32_t *ncig,
-                          hipStream_t stream);
+hipError_t lane_jobs_launch(int phase, int64_t nmax, const SegRec *segs, const int64_t *list,
+                            const unsigned long long *count, const LaneShapeBounds &b, const uint8_t *center,
+                            const uint8_t *qrev, const int64_t *qrev_off, const int64_t *qlens, uint32_t *cigs,
+                            int *bcnt, int *range, int *bfill, DpJob *jobs, int64_t *job_seg, const DpOut *outs,
+                            SegRec *wsegs, int32_t *ncig, int *err, hipStream_t stream);
 hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
                                int nq, char *rows, int64_t stride, int64_t *row_len, int64_t *ws, uint8_t *has_ins,
                                hipStream_t stream);
@@ -73,8 +74,9 @@ struct halign_center {
     // per-batch buffers
     DevBuf hcnt, scratch, hidx, nanch, totals, acount, aoff, anchors, keys, keys2, vals, vals2, sort_tmp;
     DevBuf fw, pw, chain_idx, chain_cnt, qlens, soff, segs, nseg, caps, cig_off, scan_tmp, region, bregion;
-    DevBuf qrev_off, lane_ws, ncig, llist, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
+    DevBuf qrev_off, ncig, llist, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
     DevBuf rows, rlen_d, rws, hins;   // projected rows, their lengths, op workspace, insertion flags
+    DevBuf lsmall, ljobs, ljseg, louts, ltb;   // small segments as ring jobs: counters, jobs, outputs, traceback
 };
 
 namespace {
@@ -114,7 +116,12 @@ __global__ void scatter_ncig(const int64_t *__restrict__ list, const int32_t *__
 }
 
 constexpr int kK = 15, kW = 10;   // RefAligner's k-mer size and window (include/config.hpp defaults)
-constexpr int64_t kLaneWs = 5 * kLaneMaxT + kLaneMaxTb;
+
+// small segments (kSegLane) through the sub-wave ring kernels: groups of 4
+// lanes with 1 and 2 slots, of 16 lanes with 1 and 2 slots
+constexpr int kLaneShapeDesc[kLaneShapes][2] = {{4, 1}, {4, 2}, {16, 1}, {16, 2}};
+// small-segment counters: bucket counts, ranges, fill, tickets, error flag
+constexpr int kLsCnt = 0, kLsRange = 8, kLsFill = 24, kLsTicket = 32, kLsErr = 40, kLsInts = 48;
 
 template <class T>
 hipError_t up(DevBuf &b, const std::vector<T> &v, hipStream_t st)
@@ -328,16 +335,50 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                             ctx->cig.as<uint32_t>(), c->ncig.as<int32_t>(), c->llist.as<int64_t>(), lcnt,
                             c->hlist.as<int64_t>(), hcnt, st),
            "launch seg_split");
-    const int64_t lanes = (int64_t)ctx->num_cu * 256;
-    HCHECK(c->lane_ws.ensure((size_t)lanes * kLaneWs), "alloc lane DP workspace");
+    // small segments: DpJobs built on the device, bucketed by shape and gap
+    // direction, run by the sub-wave ring kernels with their job ranges read on
+    // the device (the buckets run one after another and share one traceback
+    // region: a small segment's traceback is at most kLaneMaxTb / 2 bytes)
     hipStream_t side = ctx->aux[2];
     HCHECK(hipEventRecord(ctx->pfork, st), "event");
     HCHECK(hipStreamWaitEvent(side, ctx->pfork, 0), "stream wait");
-    HCHECK(lane_dp_launch(c->segs.as<SegRec>(), c->llist.as<int64_t>(), lcnt, ctx->seqb.as<uint8_t>() + kBytePad,
-                          c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(), ctx->seqb.as<uint8_t>(),
-                          c->lane_ws.as<uint8_t>(), kLaneWs, lanes, ctx->cig.as<uint32_t>(), c->ncig.as<int32_t>(),
-                          side),
-           "launch lane_dp");
+    HCHECK(c->lsmall.ensure(kLsInts * 4), "alloc");
+    HCHECK(c->ljobs.ensure((size_t)S * sizeof(DpJob)), "alloc small-segment jobs");
+    HCHECK(c->ljseg.ensure((size_t)S * 8), "alloc");
+    HCHECK(c->louts.ensure((size_t)S * sizeof(DpOut)), "alloc");
+    HCHECK(hipMemsetAsync(c->lsmall.p, 0, kLsInts * 4, side), "clear");
+    int *ls = c->lsmall.as<int>();
+    LaneShapeBounds lb;
+    int lshape[kLaneShapes], lgroups[kLaneShapes][2];
+    size_t lslot = ((size_t)kLaneMaxTb / 2 + 255) & ~(size_t)255, ltb = 0;
+    for (int k = 0; k < kLaneShapes; ++k) {
+        lshape[k] = dp_wave_find_shape(0, kLaneShapeDesc[k][0], kLaneShapeDesc[k][1]);
+        if (lshape[k] < 0) return fail(ctx, HALIGN_E_HIP, "small-segment shape missing");
+        lb.ncol_max[k] = dp_wave_shape_ring(lshape[k]) - 2;
+        for (int rt = 0; rt < 2; ++rt) {
+            lgroups[k][rt] = std::max(1, dp_wave_max_groups(lshape[k], rt != 0, ctx->num_cu));
+            ltb = std::max(ltb, lslot * (size_t)dp_wave_jobs_per_group(lshape[k]) * (size_t)lgroups[k][rt]);
+        }
+    }
+    HCHECK(c->ltb.ensure(ltb), "alloc small-segment traceback");
+    const uint8_t *cbytes = ctx->seqb.as<uint8_t>() + kBytePad;
+    HCHECK(lane_jobs_launch(0, S, c->segs.as<SegRec>(), c->llist.as<int64_t>(), lcnt, lb, cbytes,
+                            ctx->seqb.as<uint8_t>(), c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(),
+                            ctx->cig.as<uint32_t>(), ls + kLsCnt, ls + kLsRange, ls + kLsFill, c->ljobs.as<DpJob>(),
+                            c->ljseg.as<int64_t>(), nullptr, nullptr, nullptr, nullptr, side),
+           "launch small-segment jobs");
+    for (int k = 0; k < kLaneShapes; ++k)
+        for (int rt = 0; rt < 2; ++rt) {
+            const int bk = 2 * k + rt;
+            HCHECK(dp_wave_launch(lshape[k], rt != 0, lgroups[k][rt], c->ljobs.as<DpJob>(), 0, ls + kLsTicket + bk,
+                                  c->ltb.as<uint8_t>(), lslot, c->louts.as<DpOut>(), side, ls + kLsRange + 2 * bk),
+                   "launch small-segment DP");
+        }
+    HCHECK(lane_jobs_launch(1, S, c->segs.as<SegRec>(), c->llist.as<int64_t>(), lcnt, lb, cbytes,
+                            ctx->seqb.as<uint8_t>(), c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(),
+                            ctx->cig.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, c->ljseg.as<int64_t>(),
+                            c->louts.as<DpOut>(), c->segs.as<SegRec>(), c->ncig.as<int32_t>(), ls + kLsErr, side),
+           "launch small-segment results");
     HCHECK(hipEventRecord(ctx->pjoin, side), "event");
     // on every return from here on, later work on the main stream waits for the lane DP
     struct JoinSide {
@@ -432,9 +473,12 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                        c->ooff.as<int64_t>(), c->ocnt.as<int64_t>(), c->coff.as<int64_t>(), c->packed.as<uint32_t>());
     HCHECK(hipGetLastError(), "launch compact");
     b.nout.resize(n);
+    int lane_err = 0;
     HCHECK(hipMemcpyAsync(b.nout.data(), c->nout.p, (size_t)n * 8, hipMemcpyDeviceToHost, st), "download");
     HCHECK(hipMemcpyAsync(&b.packed, c->coff.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, st), "download");
+    HCHECK(hipMemcpyAsync(&lane_err, ls + kLsErr, 4, hipMemcpyDeviceToHost, st), "download");
     HCHECK(hipStreamSynchronize(st), "assembly");
+    if (lane_err) return fail(ctx, HALIGN_E_KERNEL, "a small-segment DP job ended with a kernel status");
     clk.mark("assembly + compaction");
     return HALIGN_OK;
 }
@@ -537,9 +581,9 @@ void halign_center_free(halign_center *c)
                       &c->nanch, &c->totals, &c->acount, &c->aoff, &c->anchors, &c->keys, &c->keys2, &c->vals,
                       &c->vals2, &c->sort_tmp, &c->fw, &c->pw, &c->chain_idx, &c->chain_cnt, &c->qlens,
                       &c->soff, &c->segs, &c->nseg, &c->caps, &c->cig_off, &c->scan_tmp, &c->region, &c->bregion,
-                      &c->qrev_off, &c->lane_ws, &c->ncig, &c->llist, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
+                      &c->qrev_off, &c->ncig, &c->llist, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
                       &c->out, &c->nout, &c->ocnt, &c->coff, &c->packed, &c->rows,
-                      &c->rlen_d, &c->rws, &c->hins})
+                      &c->rlen_d, &c->rws, &c->hins, &c->lsmall, &c->ljobs, &c->ljseg, &c->louts, &c->ltb})
         b->release();
     delete c;
 }

@@ -46,7 +46,7 @@ struct SegRec {
     uint32_t rs, re, qs, qe;
     uint32_t flags;   // bit 1: RIGHT (ksw2 flag bit); bit 8: whole pair without a length check
     uint32_t query;   // query index in the batch
-    uint32_t cls;     // kSegEmpty / kSegPure / kSegLane / kSegHost / kSegExact
+    uint32_t cls;     // kSegEmpty / kSegPure / kSegLane / kSegHost / kSegExact / kSegRing
     uint32_t pad;
     int64_t cig_off;  // offset of its CIGAR in the segment CIGAR buffer
 };
@@ -60,6 +60,12 @@ constexpr uint32_t kSegPure = 1;      // one side empty: a single I or D
 constexpr uint32_t kSegLane = 2;      // small DP, one lane per segment
 constexpr uint32_t kSegHost = 3;      // large DP or band overflow: ring / full-statistics kernels
 constexpr uint32_t kSegExact = 4;     // small, equal lengths, identical ACGT bases: CIGAR nM without a DP
+constexpr uint32_t kSegRing = 5;      // small DP done by a ring / sub-wave kernel (CIGAR stored last op first)
+
+// small segments as ring-kernel jobs: bucket = shape * 2 + gap direction, the
+// shape the first whose n_col_ bound holds the band
+constexpr int kLaneShapes = 4;
+struct LaneShapeBounds { int ncol_max[kLaneShapes]; };
 
 // block-wide inclusive scan of one value per thread (256 threads); *total
 // receives the sum over the block
