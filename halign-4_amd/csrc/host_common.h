@@ -55,7 +55,7 @@ struct halign_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Side streams (default priority): aux[0], aux[1] run DP buckets beside the
-    // main stream; aux[2] runs the per-query path's lane DP beside them.
+    // main stream; aux[2] runs the per-query path's small-segment DPs beside them.
     static constexpr int kAux = 3;
     hipStream_t aux[kAux] = {};
     hipEvent_t fork = nullptr, join[kAux] = {}, pfork = nullptr, pjoin = nullptr;

@@ -4,19 +4,19 @@
 //  * mm2_segments: one lane per query turns its chain into the 2n+1 segments
 //    of the reference (first segment LEFT-aligned, the others RIGHT; bounds
 //    clipped to the sequences; each segment starts where the previous one
-//    ended) and classifies them: empty, one side empty, small DP (lane DP),
-//    large DP or band overflow (ring / full-statistics kernels, launched by the
-//    host). A query without a chain gets one whole-pair segment whose CIGAR is
-//    taken as it is (align.cpp:204-206).
-//  * lane_dp: one lane per small segment: globalAlignKSW2 (align.cpp:22-65) =
-//    ksw_extz2_sse (src/align/ksw2_extz2_sse.c:23-304, SSE2 semantics incl.
-//    the 16-rounded margins) and ksw_backtrack (include/ksw2.h:129-161),
-//    scalar, with byte arrays in a per-lane workspace.
+//    ended) and classifies them: empty, one side empty, small DP, large DP
+//    or band overflow. A query without a chain gets one whole-pair segment
+//    whose CIGAR is taken as it is (align.cpp:204-206).
+//  * seg_split: small segments that are exact matches get nM directly; the
+//    others become DpJobs built on the device (lane_jobs_*) for the sub-wave
+//    ring kernels (dp_wave.hip), each one globalAlignKSW2 (align.cpp:22-65);
+//    large segments and band overflows go to the host's batch runner.
 //  * mm2_assemble: one lane per query appends the segment CIGARs with
 //    appendCigar's merge (src/align/cigar.cpp:302-329; the ring and
 //    full-statistics kernels store a CIGAR last op first), replacing a segment
 //    whose CIGAR does not consume it exactly by I(query) D(ref)
 //    (align.cpp:243-261).
+//  * project_rows: delQueryToRefByCigar rows and hasInsertion.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -129,150 +129,7 @@ __global__ void seg_set_offsets(SegRec *__restrict__ segs, const int64_t *__rest
 }
 
 
-// ---------------------------------------------------------------------------
-// lane DP: ksw_extz2_sse, CIGAR mode (no overflow: those segments go to the
-// host kernels), dna5 matrix, q = 6, e = 2.
 
-__device__ __forceinline__ int s8(uint32_t v) { return (int)(int8_t)(uint8_t)v; }
-
-__device__ __forceinline__ int8_t dna5(uint32_t a, uint32_t b)
-{
-    if (a > 3 || b > 3) return 0;
-    return a == b ? 5 : -4;
-}
-
-struct LaneCig {
-    uint32_t *c;
-    int n, cap;
-    __device__ void push(uint32_t op, int len)
-    {
-        if (n > 0 && (c[n - 1] & 0xf) == op) {
-            c[n - 1] += (uint32_t)len << 4;
-        } else {
-            if (n < cap) c[n] = (uint32_t)len << 4 | op;
-            ++n;
-        }
-    }
-};
-
-// tgt[t]: target codes; qrv(j): query code j read from the reversed copy
-// (qrev[-j]); ws: workspace of 5*16*ceil(tlen/16) + (qlen+tlen-1)*n_col_*16 bytes.
-__device__ int lane_extz2(const uint8_t *__restrict__ tgt, int tlen, const uint8_t *__restrict__ qrev, int qlen, int w,
-                          bool right, uint8_t *__restrict__ ws, uint32_t *__restrict__ cig, int cap)
-{
-    constexpr int q = 6, e = 2, qe = q + e;   // KSW2AlignConfig q, e (include/align.h:126-135)
-    const uint32_t qe2 = 2 * qe, max_sc = 5 + 2 * qe;
-    if (w < 0) w = tlen > qlen ? tlen : qlen;
-    const int tl16 = (tlen + 15) / 16 * 16;
-    int n_col_ = qlen < tlen ? qlen : tlen;
-    n_col_ = ((n_col_ < w + 1 ? n_col_ : w + 1) + 15) / 16 + 1;
-    const int ncol = n_col_ * 16;
-    const int ndiag = qlen + tlen - 1;
-    uint8_t *u = ws, *v = u + tl16, *x = v + tl16, *y = x + tl16, *s = y + tl16;
-    uint8_t *p = s + tl16;
-    for (int t = 0; t < 5 * tl16; ++t) ws[t] = 0;
-    int last_st = -1, last_en = -1;
-    for (int r = 0; r < ndiag; ++r) {
-        int st = 0, en = tlen - 1;
-        if (st < r - qlen + 1) st = r - qlen + 1;
-        if (en > r) en = r;
-        if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
-        if (en > (r + w) >> 1) en = (r + w) >> 1;
-        const int st0 = st, en0 = en;   // st <= en: overflowing segments never come here
-        st = st / 16 * 16;
-        en = (en + 16) / 16 * 16 - 1;
-        uint32_t x1, v1;
-        if (st > 0) {
-            if (st - 1 >= last_st && st - 1 <= last_en) { x1 = x[st - 1]; v1 = v[st - 1]; }
-            else x1 = v1 = 0;
-        } else {
-            x1 = 0;
-            v1 = r ? (uint32_t)q : 0;
-        }
-        if (en >= r) { y[r] = 0; u[r] = r ? (uint8_t)q : 0; }
-        for (int t = st0; t <= en0; ++t) s[t] = (uint8_t)dna5(tgt[t], qrev[-(r - t)]);
-        uint8_t *pr = p + (size_t)r * ncol;
-        uint32_t xprev = x1, vprev = v1;
-        for (int t = st; t <= en; ++t) {
-            const uint32_t xt1 = xprev, vt1 = vprev;
-            xprev = x[t];
-            vprev = v[t];
-            uint32_t z = (s[t] + qe2) & 0xff;
-            uint32_t a = (xt1 + vt1) & 0xff;
-            const uint32_t ut = u[t];
-            uint32_t b = (y[t] + ut) & 0xff;
-            uint32_t d;
-            if (right) {
-                d = s8(z) > s8(a) ? 0 : 1;
-                if (!(s8(z) > 0)) z = 0;
-                z = z > a ? z : a;
-                d = s8(z) > s8(b) ? d : 2;
-            } else {
-                d = s8(a) > s8(z) ? 1 : 0;
-                if (!(s8(z) > 0)) z = 0;
-                z = z > a ? z : a;
-                d = s8(b) > s8(z) ? 2 : d;
-            }
-            z = z > b ? z : b;
-            z = z < max_sc ? z : max_sc;
-            u[t] = (uint8_t)(z - vt1);
-            v[t] = (uint8_t)(z - ut);
-            z = (z - q) & 0xff;
-            a = (a - z) & 0xff;
-            b = (b - z) & 0xff;
-            if (right) {
-                x[t] = s8(a) < 0 ? 0 : (uint8_t)a;
-                d |= s8(a) < 0 ? 0 : 0x08;
-                y[t] = s8(b) < 0 ? 0 : (uint8_t)b;
-                d |= s8(b) < 0 ? 0 : 0x10;
-            } else {
-                x[t] = s8(a) > 0 ? (uint8_t)a : 0;
-                d |= s8(a) > 0 ? 0x08 : 0;
-                y[t] = s8(b) > 0 ? (uint8_t)b : 0;
-                d |= s8(b) > 0 ? 0x10 : 0;
-            }
-            pr[t - st] = (uint8_t)d;
-        }
-        last_st = st;
-        last_en = en;
-    }
-    // backtrack from (tlen-1, qlen-1), is_rot = 1 (include/ksw2.h:129-161)
-    LaneCig cb{cig, 0, cap};
-    int i = tlen - 1, j = qlen - 1, state = 0;
-    while (i >= 0 && j >= 0) {
-        const int r = i + j;
-        int st0 = 0, en0 = tlen - 1;
-        if (st0 < r - qlen + 1) st0 = r - qlen + 1;
-        if (en0 > r) en0 = r;
-        if (st0 < (r - w + 1) >> 1) st0 = (r - w + 1) >> 1;
-        if (en0 > (r + w) >> 1) en0 = (r + w) >> 1;
-        const int off = st0 / 16 * 16, off_end = (en0 + 16) / 16 * 16 - 1;
-        int force = -1;
-        if (i < off) force = 2;
-        if (i > off_end) force = 1;
-        const uint32_t tmp = force < 0 ? p[(size_t)r * ncol + (i - off)] : 0;
-        if (state == 0) state = tmp & 7;
-        else if (!((tmp >> (state + 2)) & 1)) state = 0;
-        if (state == 0) state = tmp & 7;
-        if (force >= 0) state = force;
-        if (state == 0) { cb.push(0, 1); --i; --j; }
-        else if (state == 1) { cb.push(2, 1); --i; }
-        else { cb.push(1, 1); --j; }
-    }
-    if (i >= 0) cb.push(2, i + 1);
-    if (j >= 0) cb.push(1, j + 1);
-    const int n = cb.n < cap ? cb.n : cap;
-    for (int k = 0; k < n >> 1; ++k) {
-        const uint32_t tt = cig[k];
-        cig[k] = cig[n - 1 - k];
-        cig[n - 1 - k] = tt;
-    }
-    return cb.n;
-}
-
-// Lanes walk the segment list with a grid stride and align the kLaneDP ones.
-// center: center codes (index 0 = base 0); qrev_base[query] points at the
-// reversed code copy of the query, so query base j is qrev_base[len-1-j].
 // Small segments split three ways, one thread per segment: an equal-length
 // segment whose bases are identical ACGT (the k-mer of an anchor, nearly every
 // one) has the all-match path as its unique optimum, so ksw_extz2 returns the
@@ -447,29 +304,6 @@ hipError_t lane_jobs_launch(int phase, int64_t nmax, const SegRec *segs, const i
     return hipGetLastError();
 }
 
-// lane DP over the lane list (count read on the device)
-__global__ void lane_dp(const SegRec *__restrict__ segs, const int64_t *__restrict__ list,
-                        const unsigned long long *__restrict__ count, const uint8_t *__restrict__ center,
-                        const int64_t *__restrict__ qrev_off, const int64_t *__restrict__ qlens,
-                        const uint8_t *__restrict__ qrev, uint8_t *__restrict__ wsall, int64_t ws_lane,
-                        uint32_t *__restrict__ cigs, int32_t *__restrict__ ncig)
-{
-    const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t n = (int64_t)*count;
-    uint8_t *ws = wsall + lane * ws_lane;
-    for (int64_t li = lane; li < n; li += stride) {
-        const int64_t i = list[li];
-        const SegRec sg = segs[i];
-        const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
-        const int64_t L = qlens[sg.query];
-        // query base qs + j sits at reversed index L-1-qs-j
-        const uint8_t *qr = qrev + qrev_off[sg.query] + (L - 1 - sg.qs);
-        const int w = auto_band_dev(srl, sql);
-        ncig[i] = lane_extz2(center + sg.rs, srl, qr, sql, w, (sg.flags & kSegRight) != 0, ws, cigs + sg.cig_off,
-                             srl + sql + 1);
-    }
-}
 
 // CIGAR consumption (src/align/cigar.cpp:332-365): ref = M D N = X, query = M I S = X
 __device__ __forceinline__ void consumed(const uint32_t *c, int n, int64_t &rl, int64_t &ql)
@@ -654,16 +488,6 @@ hipError_t seg_split_launch(SegRec *segs, int64_t n, const uint8_t *center, cons
     return hipGetLastError();
 }
 
-hipError_t lane_dp_launch(const SegRec *segs, const int64_t *list, const unsigned long long *count,
-                          const uint8_t *center, const int64_t *qrev_off, const int64_t *qlens, const uint8_t *qrev,
-                          uint8_t *ws, int64_t ws_lane, int64_t lanes, uint32_t *cigs, int32_t *ncig,
-                          hipStream_t stream)
-{
-    const int64_t blocks = (lanes + 63) / 64;
-    hipLaunchKernelGGL(lane_dp, dim3((unsigned)blocks), dim3(64), 0, stream, segs, list, count, center, qrev_off, qlens,
-                       qrev, ws, ws_lane, cigs, ncig);
-    return hipGetLastError();
-}
 
 hipError_t mm2_assemble_launch(const SegRec *segs, const int64_t *soff, const int32_t *nseg, const uint32_t *cigs,
                                const int32_t *ncig, const int64_t *qlens, uint32_t rlen, const int64_t *ooff, int nq,

@@ -380,7 +380,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                             c->louts.as<DpOut>(), c->segs.as<SegRec>(), c->ncig.as<int32_t>(), ls + kLsErr, side),
            "launch small-segment results");
     HCHECK(hipEventRecord(ctx->pjoin, side), "event");
-    // on every return from here on, later work on the main stream waits for the lane DP
+    // on every return from here on, later work on the main stream waits for the small-segment DPs
     struct JoinSide {
         halign_ctx *ctx;
         ~JoinSide() { (void)hipStreamWaitEvent(ctx->stream, ctx->pjoin, 0); }
@@ -438,7 +438,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                            c->hvals.as<int32_t>(), (int64_t)nh, c->ncig.as<int32_t>());
         HCHECK(hipGetLastError(), "launch scatter");
     }
-    HCHECK(hipStreamWaitEvent(st, ctx->pjoin, 0), "stream wait");   // the lane DP is done before assembly
+    HCHECK(hipStreamWaitEvent(st, ctx->pjoin, 0), "stream wait");   // the small-segment DPs are done before assembly
 
     // assembly: appendCigar over the segments of each query
     b.ooff.assign(n + 1, 0);

@@ -57,7 +57,7 @@ constexpr uint32_t kSegRaw = 0x100;   // empty-chain whole pair: CIGAR taken as 
 
 constexpr uint32_t kSegEmpty = 0;     // both sides empty: no CIGAR
 constexpr uint32_t kSegPure = 1;      // one side empty: a single I or D
-constexpr uint32_t kSegLane = 2;      // small DP, one lane per segment
+constexpr uint32_t kSegLane = 2;      // small DP (a sub-wave ring job)
 constexpr uint32_t kSegHost = 3;      // large DP or band overflow: ring / full-statistics kernels
 constexpr uint32_t kSegExact = 4;     // small, equal lengths, identical ACGT bases: CIGAR nM without a DP
 constexpr uint32_t kSegRing = 5;      // small DP done by a ring / sub-wave kernel (CIGAR stored last op first)
@@ -86,7 +86,8 @@ __device__ __forceinline__ int64_t block_scan_incl(int64_t v, int64_t *sh, int64
     return r;
 }
 
-// lane DP limits: target length and traceback bytes per lane
+// small-segment limits: target length and traceback bytes (16 per block and
+// anti-diagonal; the ring kernels store half of that)
 constexpr int kLaneMaxT = 512;
 constexpr int kLaneMaxTb = 16384;
 
