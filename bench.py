@@ -201,6 +201,7 @@ def main():
                      "unit": "TOPS", "frac": round(achieved_tops / PEAK_VALU_TOPS, 4),
                      "traffic": traffic_per_launch(tot_cells / args.steps)},
         "cpu_baseline": cpu,
+        "hip_graph": dict(zip(("captures", "replays"), ctx.graph_counts())),
     }
     print(json.dumps(out), flush=True)
 

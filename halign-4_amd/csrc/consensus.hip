@@ -6,7 +6,8 @@
 //
 // Layout: rows are stored row-major with a row stride that is a multiple of 4
 // bytes. A lane owns 4 adjacent columns and walks a chunk of rows with one
-// dword load per row (a wave reads 256 contiguous bytes of a row). The class
+// dword load per row (a wave reads 256 contiguous bytes of a row), eight rows'
+// loads in flight at a time. The class
 // of each byte comes from a 256-entry table in LDS; the 7 class counts of a
 // column are 16-bit fields of two 64-bit accumulators (classes 0-3 and 4-6),
 // so a chunk holds at most 65535 rows, and are added to the global 32-bit
@@ -57,19 +58,29 @@ __global__ __launch_bounds__(kCountThreads) void count_columns(const uint8_t *__
 
     uint64_t lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
     const uint8_t *p = rows + r0 * row_stride + col0;
-    int64_t phase = r0 % (batch + 1);   // position inside the read-batch cycle
-    for (int64_t r = r0; r < r1; ++r, p += row_stride) {
-        if (phase == batch) {           // the record read after a full batch is dropped
-            phase = 0;
-            continue;
-        }
-        ++phase;
-        const uint32_t v = *(const uint32_t *)p;
+    int64_t phase = r0 % (batch + 1);   // position inside the read-batch cycle (the same in every lane)
+    // 8 rows per step: their loads are independent and in flight together
+    constexpr int kU = 8;
+    for (int64_t r = r0; r < r1; r += kU, p += kU * row_stride) {
+        uint32_t v[kU];
+        uint32_t use = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint32_t k = cls[(v >> (8 * b)) & 0xffu];
-            lo[b] += k < 4 ? (uint64_t)1 << (16 * k) : 0;
-            hi[b] += k >= 4 ? (uint64_t)1 << (16 * (k - 4)) : 0;
+        for (int u = 0; u < kU; ++u) {
+            // the record read after a full batch is dropped
+            const bool cnt = r + u < r1 && phase != batch;
+            phase = phase == batch ? 0 : phase + 1;
+            v[u] = cnt ? *(const uint32_t *)(p + u * row_stride) : 0u;
+            use |= (uint32_t)cnt << u;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!((use >> u) & 1u)) continue;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t k = cls[(v[u] >> (8 * b)) & 0xffu];
+                lo[b] += k < 4 ? (uint64_t)1 << (16 * k) : 0;
+                hi[b] += k >= 4 ? (uint64_t)1 << (16 * (k - 4)) : 0;
+            }
         }
     }
 #pragma unroll

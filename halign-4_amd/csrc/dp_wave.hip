@@ -658,7 +658,8 @@ __global__ __launch_bounds__(64) void dp_sub_persistent(const DpJob *__restrict_
 // sub-wave shapes (SUB lanes x S slots per job, 64/SUB jobs per wave) serve
 // n_col_ <= SUB*S - 2.
 #define HAL_WAVE_SHAPES(X) \
-    X(1, 1) X(1, 2) X(2, 1) X(1, 3) X(3, 1) X(2, 2) X(4, 1) X(5, 1) X(3, 2) X(6, 1) X(4, 2) X(8, 1) X(8, 2) X(16, 1)
+    X(1, 1) X(1, 2) X(2, 1) X(1, 3) X(3, 1) X(2, 2) X(4, 1) X(5, 1) X(3, 2) X(6, 1) X(4, 2) X(8, 1) X(8, 2) X(16, 1) \
+    X(2, 3) X(1, 5) X(2, 4)
 #define HAL_SUB_SHAPES(X) X(16, 1) X(16, 2) X(16, 3) X(4, 1) X(4, 2)
 
 struct ShapeDesc { int waves, sub, slots; };   // waves = 0: sub-wave shape with groups of sub lanes

@@ -324,7 +324,13 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             const bool rt = (bi & 1) != 0;
             Launch L{bi, pos, 0, 1, 0, 0, 0.0, 0};
             for (int i : b) L.slot = std::max(L.slot, specs[i].tb_bytes);
-            L.slot = (L.slot + 255) & ~(size_t)255;
+            // rounded up to 1/32 of its power of two (at least 256 B), so that
+            // batches of similar jobs share a plan (and its graph)
+            {
+                size_t g = 256;
+                while (g * 64 <= L.slot) g <<= 1;
+                L.slot = (L.slot + g - 1) / g * g;
+            }
             const bool wave = shape >= kWaveBase;
             const int wi = shape - kWaveBase;
             L.jpg = wave ? (size_t)dp_wave_jobs_per_group(wi) : 1;
@@ -395,9 +401,45 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         }
         bool used[3] = {true, false, false};
         for (auto &L : ls) used[L.stream] = true;
-        if (used[1] || used[2]) HCHECK(hipEventRecord(ctx->fork, ctx->stream), "event");
-        for (int q = 1; q < 3; ++q)
-            if (used[q]) HCHECK(hipStreamWaitEvent(ctx->aux[q - 1], ctx->fork, 0), "stream wait");
+        // the plan's signature: a repeat replays the captured graph
+        std::vector<int64_t> sig;
+        for (auto &L : ls) {
+            const int64_t v[] = {L.bi, L.groups, (int64_t)(djobs + L.pos), (int64_t)buckets[L.bi].size(),
+                                 (int64_t)(ctr + L.bi), (int64_t)(ctx->tb.as<uint8_t>() + L.off), (int64_t)L.slot,
+                                 (int64_t)(douts + L.pos), L.stream};
+            sig.insert(sig.end(), v, v + 9);
+        }
+        int hit = -1;
+        for (size_t k = 0; ctx->graphs && k < ctx->gcache.size(); ++k)
+            if (ctx->gcache[k].sig == sig) hit = (int)k;
+        if (hit >= 0) {
+            auto cg = ctx->gcache[hit];
+            ctx->gcache.erase(ctx->gcache.begin() + hit);
+            ctx->gcache.push_back(cg);
+            HCHECK(hipGraphLaunch(cg.exec, ctx->stream), "graph launch");
+            ++ctx->graph_replays;
+            ls.clear();   // nothing to launch directly
+        } else if (ctx->graphs) {
+            // relaxed mode: other threads may allocate while this stream captures
+            HCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "begin capture");
+        }
+        const bool capturing = ctx->graphs && !ls.empty();
+        auto end_capture = [&]() {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+        };
+#define HCHECK_CAP(expr, what)                                              \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess) {                                             \
+            if (capturing) end_capture();                                   \
+            return fail(ctx, HALIGN_E_HIP, what, e_);                        \
+        }                                                                   \
+    } while (0)
+        if (!ls.empty() && (used[1] || used[2])) HCHECK_CAP(hipEventRecord(ctx->fork, ctx->stream), "event");
+        for (int q = 1; q < 3 && !ls.empty(); ++q)
+            if (used[q]) HCHECK_CAP(hipStreamWaitEvent(ctx->aux[q - 1], ctx->fork, 0), "stream wait");
         for (auto &L : ls) {
             const int shape = L.bi / 4;
             const bool stats = (L.bi & 2) != 0;
@@ -416,12 +458,28 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                                                  douts + L.pos, st)
                                 : dp_launch(shape, stats, L.groups, djobs + L.pos, nj, ctr + L.bi, tb, L.slot,
                                             douts + L.pos, st);
-            HCHECK(e, "launch dp");
+            HCHECK_CAP(e, "launch dp");
         }
-        for (int q = 1; q < 3; ++q) {
+        for (int q = 1; q < 3 && !ls.empty(); ++q) {
             if (!used[q]) continue;
-            HCHECK(hipEventRecord(ctx->join[q - 1], ctx->aux[q - 1]), "event");
-            HCHECK(hipStreamWaitEvent(ctx->stream, ctx->join[q - 1], 0), "stream wait");
+            HCHECK_CAP(hipEventRecord(ctx->join[q - 1], ctx->aux[q - 1]), "event");
+            HCHECK_CAP(hipStreamWaitEvent(ctx->stream, ctx->join[q - 1], 0), "stream wait");
+        }
+#undef HCHECK_CAP
+        if (capturing) {
+            hipGraph_t g = nullptr;
+            HCHECK(hipStreamEndCapture(ctx->stream, &g), "end capture");
+            hipGraphExec_t ge = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HCHECK(ie, "instantiate graph");
+            if (ctx->gcache.size() >= halign_ctx::kGraphCache) {
+                (void)hipGraphExecDestroy(ctx->gcache.front().exec);
+                ctx->gcache.erase(ctx->gcache.begin());
+            }
+            ctx->gcache.push_back({sig, ge});
+            ++ctx->graph_captures;
+            HCHECK(hipGraphLaunch(ge, ctx->stream), "graph launch");
         }
     }
     clk.mark("launches");
@@ -508,6 +566,7 @@ void halign_free(halign_ctx *ctx)
                            &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list, &ctx->s_wsh,
                            &ctx->s_wsp})
         b->release();
+    for (auto &cg : ctx->gcache) (void)hipGraphExecDestroy(cg.exec);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     for (int k = 0; k < halign_ctx::kAux; ++k) {
@@ -554,6 +613,23 @@ int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t ca
         if (ring) ring[i] = dp_wave_shape_ring(i);
     }
     return n;
+}
+
+int halign_set_graph_mode(halign_ctx *ctx, int32_t on)
+{
+    if (!ctx) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->graphs = on ? 1 : 0;
+    return HALIGN_OK;
+}
+
+int halign_graph_counts(halign_ctx *ctx, int64_t *captures, int64_t *replays)
+{
+    if (!ctx) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (captures) *captures = ctx->graph_captures;
+    if (replays) *replays = ctx->graph_replays;
+    return HALIGN_OK;
 }
 
 int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes)

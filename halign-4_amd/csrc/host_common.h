@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/halign_hip.h"
 
@@ -67,6 +68,16 @@ struct halign_ctx {
     hal::DevBuf s_seqs, s_off, s_woff, s_moff, s_codes, s_nmask, s_hoff, s_hx, s_hy, s_cnt, s_st, s_list, s_wsh, s_wsp;
     size_t tb_budget = 0;
     int policy = 0;        // 0: auto kernel choice, 1: multi-wave kernels only, 2+i: ring shape i
+    // Batched DP launches are captured into a HIP graph and replayed while the
+    // launch plan (kernels, grids, buffers, streams) repeats.
+    int graphs = 1;
+    struct CachedGraph {
+        std::vector<int64_t> sig;
+        hipGraphExec_t exec;
+    };
+    std::vector<CachedGraph> gcache;   // most recent last, at most kGraphCache
+    static constexpr size_t kGraphCache = 8;
+    int64_t graph_captures = 0, graph_replays = 0;
     int seed_policy = 0;   // 0: wave form where it applies, 1: sequential form only
 };
 

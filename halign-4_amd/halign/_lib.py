@@ -83,6 +83,8 @@ def _load():
     lib.halign_pairset_free.argtypes = [_P]
     lib.halign_pairset_free.restype = None
     lib.halign_set_traceback_budget.argtypes = [_P, C.c_int64]
+    lib.halign_set_graph_mode.argtypes = [_P, C.c_int32]
+    lib.halign_graph_counts.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.halign_server_create.argtypes = [C.c_int, C.c_char_p, C.c_int32, C.POINTER(ServerConfig),
                                          C.POINTER(_P)]
     lib.halign_server_submit.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.POINTER(C.c_int64)]
@@ -166,6 +168,16 @@ class Context:
     def set_kernel_policy(self, policy):
         """0 = automatic, 1 = multi-wave kernels only, 2 + i = ring shape i (see ring_policy)."""
         self._check(self.lib.halign_set_kernel_policy(self.h, int(policy)), "set_kernel_policy")
+
+    def set_graph_mode(self, on):
+        """Capture the batched DP launches into a replayed HIP graph (default on)."""
+        self._check(self.lib.halign_set_graph_mode(self.h, int(bool(on))), "set_graph_mode")
+
+    def graph_counts(self):
+        """(graphs captured, graph replays) on this context."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        self._check(self.lib.halign_graph_counts(self.h, C.byref(a), C.byref(b)), "graph_counts")
+        return a.value, b.value
 
     def dp_batch(self, tasks, query_codes, target_codes, full_stats=True, plain=False):
         """tasks: structured array DP_TASK_DTYPE. Returns (results, cigars list, kernel_ms)."""

@@ -207,6 +207,15 @@ int halign_pairset_result(halign_pairset *ps, int32_t idx, int32_t *score, uint3
                           int32_t cigar_cap, int32_t *n_cigar);
 void halign_pairset_free(halign_pairset *ps);
 
+/* on != 0 (the default): the batched DP launches of a call (one persistent
+ * kernel per length/shape bucket, on the main and side streams) are captured
+ * into a HIP graph, which later calls with the same launch plan replay; 0:
+ * direct launches. Results are identical. No reference counterpart. */
+int halign_set_graph_mode(halign_ctx *ctx, int32_t on);
+/* Graphs captured and graph replays so far on this context. No reference
+ * counterpart. */
+int halign_graph_counts(halign_ctx *ctx, int64_t *captures, int64_t *replays);
+
 /* Device memory the context may take for traceback slots (default 70% of the
  * HBM free at halign_init). No reference counterpart. */
 int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes);
