@@ -79,6 +79,30 @@ __device__ __forceinline__ void set_cell(uint32_t (&a)[8], int c, uint32_t v16)
     }
 }
 
+// Byte of cell c (0..15) of slot k of lane `from`, all three wave-uniform: one
+// v_readlane of the register that holds the cell, picked by a uniform switch
+// (no per-lane select over the block's registers).
+template <int S>
+__device__ __forceinline__ int uniform_cell_byte(const uint32_t (&A)[S][8], int k, int c, int from)
+{
+    uint32_t v = 0;
+    switch (k * 8 + (c & 7)) {
+#define HAL_RL(kk, jj)                                                                       \
+    case (kk) * 8 + (jj):                                                                    \
+        if constexpr ((kk) < S) v = __builtin_amdgcn_readlane(A[(kk) < S ? (kk) : 0][jj], from); \
+        break;
+#define HAL_RL8(kk) HAL_RL(kk, 0) HAL_RL(kk, 1) HAL_RL(kk, 2) HAL_RL(kk, 3) HAL_RL(kk, 4) HAL_RL(kk, 5) \
+    HAL_RL(kk, 6) HAL_RL(kk, 7)
+        HAL_RL8(0) HAL_RL8(1) HAL_RL8(2) HAL_RL8(3) HAL_RL8(4)
+#undef HAL_RL8
+#undef HAL_RL
+    default:
+        break;
+    }
+    static_assert(S <= 5, "uniform_cell_byte covers up to 5 slots");
+    return (int)((v >> ((c >> 3) * 16 + 8)) & 0xffu);
+}
+
 // lane i receives lane i-1 of its group of SUB lanes (SUB = 16: DPP row_ror:1;
 // SUB = 4: DPP quad_perm [3,0,1,2])
 template <int SUB>
@@ -501,8 +525,10 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
             o.x = bfi(0xF0F0F0F0u, C[1], C[0] >> 4);
             o.y = bfi(0xF0F0F0F0u, C[3], C[2] >> 4);
             *(uint2 *)(tb + (size_t)r * rowb + (size_t)(b - bs) * 8) = o;
-            if (k == ke) ue_v = (uint32_t)cell_byte(U[k], en0 & 15);
-            if (k == kv) vl_v = (uint32_t)cell_byte(V[k], tv & 15);
+            if constexpr (ROW) {   // per-group bounds: the owning lane's own copy
+                if (k == ke) ue_v = (uint32_t)cell_byte(U[k], en0 & 15);
+                if (k == kv) vl_v = (uint32_t)cell_byte(V[k], tv & 15);
+            }
         }
 
         if constexpr (ROW) {
@@ -511,12 +537,17 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
             pv = (uint32_t)__builtin_amdgcn_ds_bpermute((base + lv) << 2, (int)vl_v);
             p_st0 = st0; p_en0 = en0; p_en = en;
         } else if constexpr (W == 1) {
-            edge(r, st0, en0, en, (int)__builtin_amdgcn_readlane(ue_v, le),
-                 (int)__builtin_amdgcn_readlane(vl_v, lv));
+            edge(r, st0, en0, en, uniform_cell_byte<S>(U, ke, en0 & 15, le), uniform_cell_byte<S>(V, kv, tv & 15, lv));
         } else {
-            // read by every wave after the next barrier
-            if (L == le) sm.logu[r & 1] = (uint8_t)ue_v;
-            if (L == lv) sm.logv[r & 1] = (uint8_t)vl_v;
+            // read by every wave after the next barrier; the owning wave writes
+            if (wave == (le >> 6)) {
+                const int b = uniform_cell_byte<S>(U, ke, en0 & 15, le & 63);
+                if (lane == 0) sm.logu[r & 1] = (uint8_t)b;
+            }
+            if (wave == (lv >> 6)) {
+                const int b = uniform_cell_byte<S>(V, kv, tv & 15, lv & 63);
+                if (lane == 0) sm.logv[r & 1] = (uint8_t)b;
+            }
             p_st0 = st0; p_en0 = en0; p_en = en;
         }
         rlast = r;

@@ -563,8 +563,7 @@ void halign_free(halign_ctx *ctx)
     ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
     ctx->crows.release(); ctx->ccounts.release(); ctx->ccons.release();
     for (hal::DevBuf *b : {&ctx->s_seqs, &ctx->s_off, &ctx->s_woff, &ctx->s_moff, &ctx->s_codes, &ctx->s_nmask,
-                           &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list, &ctx->s_wsh,
-                           &ctx->s_wsp})
+                           &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list})
         b->release();
     for (auto &cg : ctx->gcache) (void)hipGraphExecDestroy(cg.exec);
     (void)hipEventDestroy(ctx->ev0);

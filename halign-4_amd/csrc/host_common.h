@@ -65,7 +65,7 @@ struct halign_ctx {
     hal::DevBuf seq, seqb, jobs, outs, counters, tb, cig;
     hal::DevBuf crows, ccounts, ccons;   // consensus rows (host input), counts, characters
     // seeding: sequences, offsets, packed codes and N masks, hits, counts, status, workspace
-    hal::DevBuf s_seqs, s_off, s_woff, s_moff, s_codes, s_nmask, s_hoff, s_hx, s_hy, s_cnt, s_st, s_list, s_wsh, s_wsp;
+    hal::DevBuf s_seqs, s_off, s_woff, s_moff, s_codes, s_nmask, s_hoff, s_hx, s_hy, s_cnt, s_st, s_list;
     size_t tb_budget = 0;
     int policy = 0;        // 0: auto kernel choice, 1: multi-wave kernels only, 2+i: ring shape i
     // Batched DP launches are captured into a HIP graph and replayed while the

@@ -16,7 +16,7 @@
 //
 //  * minimizers_seq: one lane per sequence, the reference's queue step by step
 //    (a ring of `win` slots indexed as the reference indexes it, in a larger
-//    per-lane workspace so that the reference's out-of-range slots are
+//    per-lane LDS workspace so that the reference's out-of-range slots are
 //    modelled instead of faulting).
 //
 // The queue differs from a plain windowed minimum in one case: when it holds
@@ -27,6 +27,7 @@
 // minimizers_seq.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 namespace hal {
@@ -247,13 +248,15 @@ __global__ __launch_bounds__(64 * kSeedWaves) void minimizers_wave(
 }
 
 // One lane per listed sequence: the reference's loop (minimizer.cpp:118-170)
-// with its RingMinQueue. The queue lives in ws (4*win+8 entries per lane).
+// with its RingMinQueue. The queue lives in LDS, ws_cap (4*win+8) entries per
+// lane (the dynamic shared memory holds blockDim.x of them): every queue
+// access is on the sequence's dependent chain, so its latency is the kernel's.
 __global__ void minimizers_seq(const char *__restrict__ seqs, const int64_t *__restrict__ off,
                                const int32_t *__restrict__ list, int nlist, int k_, int w_, int non_canonical,
-                               const int64_t *__restrict__ hit_off, uint64_t *__restrict__ ws_h,
-                               uint32_t *__restrict__ ws_p, int ws_cap, uint64_t *__restrict__ hx,
+                               const int64_t *__restrict__ hit_off, int ws_cap, uint64_t *__restrict__ hx,
                                uint64_t *__restrict__ hy, int64_t *__restrict__ counts, int32_t *__restrict__ status)
 {
+    extern __shared__ uint64_t seq_lds[];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nlist) return;
     const int s = list[t];
@@ -270,8 +273,8 @@ __global__ void minimizers_seq(const char *__restrict__ seqs, const int64_t *__r
     const uint32_t win = w < total ? w : total;
     const uint64_t mask = ((uint64_t)1 << (2 * k)) - 1;
     const uint64_t shift = 2ull * (k - 1);
-    uint64_t *qh = ws_h + (int64_t)t * ws_cap;
-    uint32_t *qp = ws_p + (int64_t)t * ws_cap;
+    uint64_t *qh = seq_lds + (int64_t)threadIdx.x * ws_cap;
+    uint32_t *qp = (uint32_t *)(seq_lds + (int64_t)blockDim.x * ws_cap) + (int64_t)threadIdx.x * ws_cap;
     const uint32_t big = 4 * win + 8 <= (uint32_t)ws_cap ? 4 * win + 8 : (uint32_t)ws_cap;
     uint64_t *ox = hx + hit_off[s], *oy = hy + hit_off[s];
     uint64_t fwd = 0, rev = 0;
@@ -376,12 +379,17 @@ hipError_t seed_wave(const int64_t *off, const int64_t *woff, const int64_t *mof
 }
 
 hipError_t seed_seq(const char *seqs, const int64_t *off, const int32_t *list, int nlist, int k, int w,
-                    int non_canonical, const int64_t *hit_off, uint64_t *ws_h, uint32_t *ws_p, int ws_cap,
-                    uint64_t *hx, uint64_t *hy, int64_t *counts, int32_t *status, hipStream_t stream)
+                    int non_canonical, const int64_t *hit_off, int ws_cap, uint64_t *hx, uint64_t *hy,
+                    int64_t *counts, int32_t *status, hipStream_t stream)
 {
     if (nlist <= 0) return hipSuccess;
-    hipLaunchKernelGGL(minimizers_seq, dim3((nlist + 63) / 64), dim3(64), 0, stream, seqs, off, list, nlist, k, w,
-                       non_canonical, hit_off, ws_h, ws_p, ws_cap, hx, hy, counts, status);
+    // 12 bytes of queue per entry and lane; lanes per workgroup so that a
+    // workgroup takes at most 48 KiB of LDS (64 lanes for the path's w = 10)
+    const size_t per_lane = (size_t)ws_cap * 12;
+    int lanes = (int)std::min<size_t>(64, std::max<size_t>(1, (48u << 10) / per_lane));
+    const size_t lds = (size_t)lanes * per_lane;
+    hipLaunchKernelGGL(minimizers_seq, dim3((nlist + lanes - 1) / lanes), dim3(lanes), lds, stream, seqs, off, list,
+                       nlist, k, w, non_canonical, hit_off, ws_cap, hx, hy, counts, status);
     return hipGetLastError();
 }
 

@@ -17,8 +17,8 @@ hipError_t seed_wave(const int64_t *off, const int64_t *woff, const int64_t *mof
                      const uint64_t *nmask, const int64_t *hit_off, int n, int k, int w, uint64_t *hx, uint64_t *hy,
                      int64_t *counts, int32_t *status, hipStream_t stream);
 hipError_t seed_seq(const char *seqs, const int64_t *off, const int32_t *list, int nlist, int k, int w,
-                    int non_canonical, const int64_t *hit_off, uint64_t *ws_h, uint32_t *ws_p, int ws_cap,
-                    uint64_t *hx, uint64_t *hy, int64_t *counts, int32_t *status, hipStream_t stream);
+                    int non_canonical, const int64_t *hit_off, int ws_cap, uint64_t *hx, uint64_t *hy,
+                    int64_t *counts, int32_t *status, hipStream_t stream);
 int seed_max_wave_window();
 
 // Minimizers of n sequences already in ctx->s_seqs (bytes) with offsets off
@@ -79,14 +79,11 @@ int seed_device(halign_ctx *ctx, int n, const std::vector<int64_t> &off, int k, 
         const int ws_cap = 4 * win + 8;
         const int nl = (int)list.size();
         HCHECK(ctx->s_list.ensure((size_t)nl * 4), "alloc seed list");
-        HCHECK(ctx->s_wsh.ensure((size_t)nl * ws_cap * 8), "alloc seed workspace");
-        HCHECK(ctx->s_wsp.ensure((size_t)nl * ws_cap * 4), "alloc seed workspace");
         HCHECK(hipMemcpyAsync(ctx->s_list.p, list.data(), (size_t)nl * 4, hipMemcpyHostToDevice, ctx->stream),
                "upload seed list");
         HCHECK(seed_seq(ctx->s_seqs.as<char>(), ctx->s_off.as<int64_t>(), ctx->s_list.as<int32_t>(), nl, k, w,
-                        non_canonical, ctx->s_hoff.as<int64_t>(), ctx->s_wsh.as<uint64_t>(), ctx->s_wsp.as<uint32_t>(),
-                        ws_cap, ctx->s_hx.as<uint64_t>(), ctx->s_hy.as<uint64_t>(), ctx->s_cnt.as<int64_t>(),
-                        ctx->s_st.as<int32_t>(), ctx->stream),
+                        non_canonical, ctx->s_hoff.as<int64_t>(), ws_cap, ctx->s_hx.as<uint64_t>(),
+                        ctx->s_hy.as<uint64_t>(), ctx->s_cnt.as<int64_t>(), ctx->s_st.as<int32_t>(), ctx->stream),
                "launch minimizers_seq");
     }
     counts.assign(n, 0);
