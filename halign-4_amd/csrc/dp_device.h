@@ -93,6 +93,44 @@ __device__ __forceinline__ void put16(int32_t (&a)[16], int j, int32_t val)
         a[i] = (int32_t)(((uint32_t)a[i] & ~m) | ((uint32_t)val & m));
     }
 }
+// Register-array access with a wave-uniform index through a uniform switch
+// (one move or select instead of a select over every element).
+__device__ __forceinline__ int32_t pick16u(const int32_t (&a)[16], int j)
+{
+    int32_t v = 0;
+    switch (j) {
+#define HAL_C(i) case i: v = a[i]; break;
+        HAL_C(0) HAL_C(1) HAL_C(2) HAL_C(3) HAL_C(4) HAL_C(5) HAL_C(6) HAL_C(7)
+        HAL_C(8) HAL_C(9) HAL_C(10) HAL_C(11) HAL_C(12) HAL_C(13) HAL_C(14) HAL_C(15)
+#undef HAL_C
+    default: break;
+    }
+    return v;
+}
+// a[j] = val in the lanes where `on` holds (j wave-uniform)
+__device__ __forceinline__ void put16u(int32_t (&a)[16], int j, int32_t val, bool on)
+{
+    switch (j) {
+#define HAL_C(i) case i: a[i] = on ? val : a[i]; break;
+        HAL_C(0) HAL_C(1) HAL_C(2) HAL_C(3) HAL_C(4) HAL_C(5) HAL_C(6) HAL_C(7)
+        HAL_C(8) HAL_C(9) HAL_C(10) HAL_C(11) HAL_C(12) HAL_C(13) HAL_C(14) HAL_C(15)
+#undef HAL_C
+    default: break;
+    }
+}
+// byte of cell c (cells 2j, 2j+1 in register j, x<<8 halves), c wave-uniform
+__device__ __forceinline__ int byte_of_u(const uint32_t (&a)[8], int c)
+{
+    uint32_t w = 0;
+    switch (c >> 1) {
+#define HAL_C(i) case i: w = a[i]; break;
+        HAL_C(0) HAL_C(1) HAL_C(2) HAL_C(3) HAL_C(4) HAL_C(5) HAL_C(6) HAL_C(7)
+#undef HAL_C
+    default: break;
+    }
+    return (int)((w >> ((c & 1) * 16 + 8)) & 0xffu);
+}
+
 // unsigned byte value of cell c (0..15) of a block stored as x<<8 halves
 __device__ __forceinline__ int byte_of(const uint32_t (&a)[8], int c)
 {

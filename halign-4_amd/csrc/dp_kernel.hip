@@ -28,6 +28,8 @@
 // Traceback: 4 bits per cell (two state-compare bits and the two gap
 // continuation bits), 8 bytes per block per anti-diagonal, rows of
 // n_col_*8 bytes (n_col_ exactly as the reference sizes its matrix).
+#include <climits>
+
 #include "dp_device.h"
 
 namespace hal {
@@ -320,36 +322,46 @@ __device__ void dp_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restri
             if constexpr (STATS) {
                 // exact H row (:227-261): fix H[en0] from the pre-update H[en0-1], then add
                 // v-qe over [st0, en0)
-                const int ce = en0 - 16 * b;
+                // in the lane that owns en0, ce = en0 & 15 (wave-uniform): the edge cell's
+                // neighbours are read and written through uniform switches
+                const int ce = en0 - 16 * b, cu = en0 & 15;
                 int32_t hfix = 0;
                 const bool own_e = (ce >= 0 && ce < 16);
-                if (own_e) {
-                    if (r == 0) hfix = byte_of(V[k], 0) - 2 * kQE;
-                    else if (en0 == 0) hfix = HC[k][0] + byte_of(V[k], 0) - kQE;
-                    else hfix = (ce > 0 ? pick16(HC[k], ce - 1) : nh[k]) + byte_of(U[k], ce) - kQE;
-                }
+                if (r == 0) hfix = byte_of(V[k], 0) - 2 * kQE;
+                else if (en0 == 0) hfix = HC[k][0] + byte_of(V[k], 0) - kQE;
+                else hfix = (cu > 0 ? pick16u(HC[k], cu - 1) : nh[k]) + byte_of_u(U[k], cu) - kQE;
+                // branch-free: cell c is row column t = 16b + c, in range iff lo <= c < ce
+                const int lo = st0 - 16 * b;
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
-                    const int t = 16 * b + c;
-                    if (t >= st0 && t < en0) HC[k][c] += (int32_t)((V[k][c >> 1] >> ((c & 1) * 16 + 8)) & 0xffu) - kQE;
+                    const int32_t dv = (int32_t)((V[k][c >> 1] >> ((c & 1) * 16 + 8)) & 0xffu) - kQE;
+                    HC[k][c] += (c >= lo && c < ce) ? dv : 0;
                 }
-                if (own_e) put16(HC[k], ce, hfix);
+                put16u(HC[k], cu, hfix, own_e);
                 // per-diagonal max with the reference's tie order (:228-260), folded into a
-                // running lexicographic (H desc, r asc, key asc) best
-                const int en1 = st0 + ((en0 - st0) >> 2 << 2);
+                // running lexicographic (H desc, r asc, key asc) best: the block's largest H
+                // over [st0, en0], then the smallest key among the cells that hold it (the
+                // key is computed for those cells only, usually one)
+                int32_t hb = INT_MIN;
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    const int t = 16 * b + c;
-                    if (t >= st0 && t <= en0) {
-                        const int32_t h = HC[k][c];
+                for (int c = 0; c < 16; ++c) hb = max(hb, (c >= lo && c <= ce) ? HC[k][c] : INT_MIN);
+                uint32_t eq = 0;
+#pragma unroll
+                for (int c = 0; c < 16; ++c) eq |= (uint32_t)(c >= lo && c <= ce && HC[k][c] == hb) << c;
+                if (eq && (hb > bH || (hb == bH && bR == r))) {
+                    const int en1 = st0 + ((en0 - st0) >> 2 << 2);
+                    int kb = INT_MAX;
+                    for (uint32_t m = eq; m; m &= m - 1) {
+                        const int t = 16 * b + __builtin_ctz(m);
                         const int key = (t == en0) ? 0
                                         : (t < en1) ? 1 + (((t - st0) & 3) << 20) + ((t - st0) >> 2)
                                                     : 1 + (4 << 20) + (t - en1);
-                        if (h > bH || (h == bH && bR == r && key < bK)) { bH = h; bR = r; bK = key; }
+                        kb = key < kb ? key : kb;
                     }
+                    if (hb > bH || kb < bK) { bH = hb; bR = r; bK = kb; }
                 }
                 const int cs = st0 - 16 * b;
-                if (r - st0 == qlen - 1 && cs >= 0 && cs < 16) sm.logH[r & 63][1] = pick16(HC[k], cs);
+                if (r - st0 == qlen - 1 && cs >= 0 && cs < 16) sm.logH[r & 63][1] = pick16u(HC[k], st0 & 15);
             }
         }
 
