@@ -52,6 +52,12 @@ __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { re
 // cells (2j-1, 2j) from pairs hi=(2j,2j+1), lo=(2j-2,2j-1)
 __device__ __forceinline__ uint32_t cat16(uint32_t hi, uint32_t lo) { return __builtin_amdgcn_alignbit(hi, lo, 16); }
 
+// lane i receives lane i-1 (lane 0 receives lane 63): DPP wave_ror:1
+__device__ __forceinline__ uint32_t wave_ror1(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x13C, 0xF, 0xF, false);
+}
+
 struct Bounds { int st0, en0, st, en; };
 
 // Band of anti-diagonal r: src/align/ksw2_extz2_sse.c:107-116

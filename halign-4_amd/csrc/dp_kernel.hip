@@ -180,12 +180,12 @@ __device__ void dp_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restri
         {
             uint32_t rx[S], rv[S];
             int32_t rh[S];
-            const int src = (lane + 63) & 63;
+            // lane i receives lane i-1, lane 0 lane 63 (DPP rotate, no LDS round trip)
 #pragma unroll
             for (int k = 0; k < S; ++k) {
-                rx[k] = __shfl(X[k][7], src);
-                rv[k] = __shfl(V[k][7], src);
-                if constexpr (STATS) rh[k] = __shfl(HC[k][15], src); else rh[k] = 0;
+                rx[k] = wave_ror1(X[k][7]);
+                rv[k] = wave_ror1(V[k][7]);
+                if constexpr (STATS) rh[k] = (int32_t)wave_ror1((uint32_t)HC[k][15]); else rh[k] = 0;
             }
             if constexpr (W > 1) {
                 if (lane == 63) {

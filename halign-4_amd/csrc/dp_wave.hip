@@ -24,11 +24,6 @@
 
 namespace hal {
 
-// lane i receives lane i-1 (lane 0 receives lane 63): DPP wave_ror:1
-__device__ __forceinline__ uint32_t wave_ror1(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x13C, 0xF, 0xF, false);
-}
 
 // 16 bytes starting at an arbitrary byte address (aligned dword loads + v_alignbyte)
 __device__ __forceinline__ void load16(const uint8_t *p, uint32_t (&o)[4])
