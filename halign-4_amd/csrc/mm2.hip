@@ -16,7 +16,8 @@
 //    full-statistics kernels store a CIGAR last op first), replacing a segment
 //    whose CIGAR does not consume it exactly by I(query) D(ref)
 //    (align.cpp:243-261).
-//  * project_rows: delQueryToRefByCigar rows and hasInsertion.
+//  * project_rows: delQueryToRefByCigar rows and hasInsertion; rows_scan,
+//    record_sizes, write_records: the keep-length FASTA records.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -395,43 +396,27 @@ hipError_t seg_offsets_launch(SegRec *segs, const int64_t *cig_off, int64_t n, h
 }
 
 
-// cigar::delQueryToRefByCigar (src/align/cigar.cpp:212-299) of each query:
-// its row (M/S/=/X copy query bases, D/N write '-', I skips query bases, H/P
-// nothing; for an MM2 CIGAR the row spans the center), its length in
-// row_len, and cigar::hasInsertion (:70-79). One workgroup per query: a block
-// scan over the ops gives each op's end in the row and start on the query (ws,
-// laid out like the CIGARs), then every row column finds its op by binary
-// search. An empty CIGAR leaves the query as it is (:214-216). Rows longer
-// than stride are cut at stride (row_len still gives the full length).
-__global__ __launch_bounds__(256) void project_rows(const uint32_t *__restrict__ cig, const int64_t *__restrict__ coff,
-                                                    const char *__restrict__ qseq, const int64_t *__restrict__ qoff,
-                                                    char *__restrict__ rows, int64_t stride,
-                                                    int64_t *__restrict__ row_len, int64_t *__restrict__ ws,
-                                                    uint8_t *__restrict__ has_ins)
+// cigar::delQueryToRefByCigar (src/align/cigar.cpp:212-299) of one query in
+// one workgroup of 256: row_scan runs a block scan over the ops giving each
+// op's end in the row and start on the query (w[k] = row end << 32 | query
+// start, laid out like the CIGAR) and returns the row length (M/S/=/X and D/N
+// advance the row, I/H/P do not); *ins (shared) is set when an I op occurs
+// (cigar::hasInsertion, :70-79). row_fill then writes row columns [0, limit):
+// every column finds its op by binary search (M/S/=/X copy query bases, D/N
+// write '-'). An empty CIGAR leaves the query as it is (:214-216).
+__device__ int64_t row_scan(const uint32_t *__restrict__ c, int64_t n, int64_t qlen, int64_t *__restrict__ w,
+                            int64_t *sh, int *ins)
 {
-    __shared__ int64_t sh[256];
-    __shared__ int ins;
-    const int q = blockIdx.x;
-    const uint32_t *c = cig + coff[q];
-    const int64_t n = coff[q + 1] - coff[q];
-    int64_t *w = ws + coff[q];   // (ref end << 32) | query start, per op
-    const char *qs = qseq + qoff[q];
-    const int64_t qlen = qoff[q + 1] - qoff[q];
-    char *row = rows + (int64_t)q * stride;
-    if (threadIdx.x == 0) ins = 0;
-    __syncthreads();
     int64_t carry = 0;
     for (int64_t k0 = 0; k0 < n; k0 += 256) {
         const int64_t k = k0 + threadIdx.x;
         int64_t v = 0;
-        uint32_t op = 0, len = 0;
         if (k < n) {
-            op = c[k] & 0xf;
-            len = c[k] >> 4;
+            const uint32_t op = c[k] & 0xf, len = c[k] >> 4;
             const bool col = !(op == 1 || op == 5 || op == 6);               // advances the row
             const bool qry = !(op == 2 || op == 3 || op == 5 || op == 6);    // advances the query
             v = ((int64_t)(col ? len : 0) << 32) | (qry ? len : 0);
-            if (op == 1) ins = 1;
+            if (op == 1) *ins = 1;
         }
         int64_t tot = 0;
         const int64_t incl = block_scan_incl(v, sh, &tot) + carry;
@@ -439,17 +424,18 @@ __global__ __launch_bounds__(256) void project_rows(const uint32_t *__restrict__
         carry += tot;
     }
     __syncthreads();
-    const int64_t out_len = n == 0 ? qlen : (carry >> 32);
-    if (threadIdx.x == 0) {
-        has_ins[q] = (uint8_t)ins;
-        row_len[q] = out_len;
-    }
+    return n == 0 ? qlen : (carry >> 32);
+}
+
+__device__ void row_fill(const uint32_t *__restrict__ c, int64_t n, const int64_t *__restrict__ w,
+                         const char *__restrict__ qs, int64_t qlen, int64_t limit, char *__restrict__ row)
+{
     if (n == 0) {
-        for (int64_t p = threadIdx.x; p < qlen && p < stride; p += 256) row[p] = qs[p];
+        for (int64_t p = threadIdx.x; p < qlen && p < limit; p += 256) row[p] = qs[p];
         return;
     }
-    for (int64_t p = threadIdx.x; p < out_len && p < stride; p += 256) {
-        int64_t lo = 0, hi = n;   // first op whose end on the center is > p
+    for (int64_t p = threadIdx.x; p < limit; p += 256) {
+        int64_t lo = 0, hi = n;   // first op whose end in the row is > p
         while (lo < hi) {
             const int64_t mid = (lo + hi) >> 1;
             if ((w[mid] >> 32) > p) hi = mid;
@@ -467,6 +453,122 @@ __global__ __launch_bounds__(256) void project_rows(const uint32_t *__restrict__
     }
 }
 
+// rows of a batch: query q's row at rows + q * stride, its length in
+// row_len[q] (rows longer than stride are cut at stride), hasInsertion in
+// has_ins[q]; ws holds the op scan (one int64 per CIGAR word).
+__global__ __launch_bounds__(256) void project_rows(const uint32_t *__restrict__ cig, const int64_t *__restrict__ coff,
+                                                    const char *__restrict__ qseq, const int64_t *__restrict__ qoff,
+                                                    char *__restrict__ rows, int64_t stride,
+                                                    int64_t *__restrict__ row_len, int64_t *__restrict__ ws,
+                                                    uint8_t *__restrict__ has_ins)
+{
+    __shared__ int64_t sh[256];
+    __shared__ int ins;
+    const int q = blockIdx.x;
+    const int64_t n = coff[q + 1] - coff[q];
+    const int64_t qlen = qoff[q + 1] - qoff[q];
+    if (threadIdx.x == 0) ins = 0;
+    __syncthreads();
+    const int64_t out_len = row_scan(cig + coff[q], n, qlen, ws + coff[q], sh, &ins);
+    if (threadIdx.x == 0) {
+        has_ins[q] = (uint8_t)ins;
+        row_len[q] = out_len;
+    }
+    row_fill(cig + coff[q], n, ws + coff[q], qseq + qoff[q], qlen, out_len < stride ? out_len : stride,
+             rows + (int64_t)q * stride);
+}
+
+// Keep-length FASTA records of RefAligner::mergeAlignedResults for one center
+// (src/align/ref_aligner.cpp:676-740, keep_length): record q is
+// '>' name '\n' row '\n' (seq_io::SeqWriter::writeFasta with an unlimited line
+// width, src/utils/seq_io.cpp:236-268), the rows of queries with an insertion
+// first (their SAM files are merged first, :504-535 and :713-717), then the
+// others (:719-732), each group in input order. Three passes:
+//  * rows_scan: the op scan, row length and insertion flag of every query;
+//  * record_sizes: sz[q] (insertion group) and sz[n + q] (the others) hold the
+//    record's size in its group and 0 in the other (0 in both for a query
+//    whose name starts with '@'), sz[2n] = 0, so that one
+//    exclusive scan over the 2n + 1 entries gives every record's byte offset;
+//    counts[0] += queries with an insertion, counts[1] += rows whose length
+//    is not the center's (the reference throws on those, :590-596, :657-663);
+//  * write_records: header, row (always rlen bytes) and newline at the offset.
+// SamReader::next skips SAM lines that start with '@' (src/utils/seq_io.cpp:
+// 433-436), so a query named "@..." never reaches the merged FASTA
+__device__ __forceinline__ bool is_dropped(const char *names, const int64_t *name_off, int q)
+{
+    return name_off[q + 1] > name_off[q] && names[name_off[q]] == '@';
+}
+
+__global__ __launch_bounds__(256) void rows_scan(const uint32_t *__restrict__ cig, const int64_t *__restrict__ coff,
+                                                 const int64_t *__restrict__ qoff, int64_t *__restrict__ row_len,
+                                                 int64_t *__restrict__ ws, uint8_t *__restrict__ has_ins)
+{
+    __shared__ int64_t sh[256];
+    __shared__ int ins;
+    const int q = blockIdx.x;
+    if (threadIdx.x == 0) ins = 0;
+    __syncthreads();
+    const int64_t out_len = row_scan(cig + coff[q], coff[q + 1] - coff[q], qoff[q + 1] - qoff[q], ws + coff[q], sh,
+                                     &ins);
+    if (threadIdx.x == 0) {
+        has_ins[q] = (uint8_t)ins;
+        row_len[q] = out_len;
+    }
+}
+
+__global__ __launch_bounds__(256) void record_sizes(const char *__restrict__ names,
+                                                    const int64_t *__restrict__ name_off,
+                                                    const uint8_t *__restrict__ has_ins,
+                                                    const int64_t *__restrict__ row_len, int nq, int64_t rlen,
+                                                    int64_t *__restrict__ sz, unsigned long long *__restrict__ counts)
+{
+    __shared__ unsigned int n_ins, n_bad;
+    if (threadIdx.x == 0) n_ins = n_bad = 0;
+    __syncthreads();
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q < nq) {
+        const int64_t nl = name_off[q + 1] - name_off[q];
+        const bool keep = !is_dropped(names, name_off, q);
+        const int64_t size = keep ? nl + rlen + 3 : 0;
+        const bool ins = has_ins[q] != 0;
+        sz[q] = ins ? size : 0;
+        sz[nq + q] = ins ? 0 : size;
+        if (keep && ins) atomicAdd(&n_ins, 1u);
+        if (keep && row_len[q] != rlen) atomicAdd(&n_bad, 1u);
+    }
+    if (q == 0) sz[2 * (int64_t)nq] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (n_ins) atomicAdd(&counts[0], (unsigned long long)n_ins);
+        if (n_bad) atomicAdd(&counts[1], (unsigned long long)n_bad);
+    }
+}
+
+__global__ __launch_bounds__(256) void write_records(const uint32_t *__restrict__ cig, const int64_t *__restrict__ coff,
+                                                     const int64_t *__restrict__ ws, const char *__restrict__ qseq,
+                                                     const int64_t *__restrict__ qoff, const char *__restrict__ names,
+                                                     const int64_t *__restrict__ name_off,
+                                                     const uint8_t *__restrict__ has_ins,
+                                                     const int64_t *__restrict__ rec_off, int nq, int64_t rlen,
+                                                     char *__restrict__ out)
+{
+    const int q = blockIdx.x;
+    if (is_dropped(names, name_off, q)) return;
+    char *rec = out + (has_ins[q] ? rec_off[q] : rec_off[nq + q]);
+    const int64_t nl = name_off[q + 1] - name_off[q];
+    const char *nm = names + name_off[q];
+    for (int64_t p = threadIdx.x; p < nl + 2; p += 256) rec[p] = p == 0 ? '>' : (p == nl + 1 ? '\n' : nm[p - 1]);
+    char *row = rec + nl + 2;
+    if (threadIdx.x == 0) row[rlen] = '\n';
+    const int64_t n = coff[q + 1] - coff[q];
+    const int64_t qlen = qoff[q + 1] - qoff[q];
+    // a row shorter than the center (reported through record_sizes) is padded
+    // so that every byte of the record is written
+    const int64_t have = n == 0 ? qlen : ws[coff[q] + n - 1] >> 32;
+    for (int64_t p = (have < rlen ? have : rlen) + threadIdx.x; p < rlen; p += 256) row[p] = '-';
+    row_fill(cig + coff[q], n, ws + coff[q], qseq + qoff[q], qlen, have < rlen ? have : rlen, row);
+}
+
 hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
                                int nq, char *rows, int64_t stride, int64_t *row_len, int64_t *ws, uint8_t *has_ins,
                                hipStream_t stream)
@@ -474,6 +576,25 @@ hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const c
     if (nq <= 0) return hipSuccess;
     hipLaunchKernelGGL(project_rows, dim3((unsigned)nq), dim3(256), 0, stream, cig, coff, qseq, qoff, rows, stride,
                        row_len, ws, has_ins);
+    return hipGetLastError();
+}
+
+hipError_t records_launch(int phase, const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
+                          const char *names, const int64_t *name_off, int nq, int64_t rlen, int64_t *row_len,
+                          int64_t *ws, uint8_t *has_ins, int64_t *sz, unsigned long long *counts, char *out,
+                          hipStream_t stream)
+{
+    // phase 0: rows_scan; 1: record_sizes into sz (2n + 1 entries); 2:
+    // write_records with sz = the exclusive scan of those sizes
+    if (nq <= 0) return hipSuccess;
+    if (phase == 0)
+        hipLaunchKernelGGL(rows_scan, dim3((unsigned)nq), dim3(256), 0, stream, cig, coff, qoff, row_len, ws, has_ins);
+    else if (phase == 1)
+        hipLaunchKernelGGL(record_sizes, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, names, name_off,
+                           has_ins, row_len, nq, rlen, sz, counts);
+    else
+        hipLaunchKernelGGL(write_records, dim3((unsigned)nq), dim3(256), 0, stream, cig, coff, ws, qseq, qoff, names,
+                           name_off, has_ins, sz, nq, rlen, out);
     return hipGetLastError();
 }
 

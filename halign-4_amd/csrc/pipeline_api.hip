@@ -55,6 +55,10 @@ hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const c
 hipError_t mm2_assemble_launch(const SegRec *segs, const int64_t *soff, const int32_t *nseg, const uint32_t *cigs,
                                const int32_t *ncig, const int64_t *qlens, uint32_t rlen, const int64_t *ooff, int nq,
                                uint32_t *out, int64_t *nout, hipStream_t stream);
+hipError_t records_launch(int phase, const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
+                          const char *names, const int64_t *name_off, int nq, int64_t rlen, int64_t *row_len,
+                          int64_t *ws, uint8_t *has_ins, int64_t *sz, unsigned long long *counts, char *out,
+                          hipStream_t stream);
 
 }  // namespace hal
 
@@ -76,6 +80,7 @@ struct halign_center {
     DevBuf fw, pw, chain_idx, chain_cnt, qlens, soff, segs, nseg, caps, cig_off, scan_tmp, region, bregion;
     DevBuf qrev_off, ncig, llist, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
     DevBuf rows, rlen_d, rws, hins;   // projected rows, their lengths, op workspace, insertion flags
+    DevBuf nm, nmoff, rsz, roff, rcnt, recs;   // keep-length FASTA records: names, sizes, offsets, counts, bytes
     DevBuf lsmall, ljobs, ljseg, louts, ltb;   // small segments as ring jobs: counters, jobs, outputs, traceback
 };
 
@@ -489,7 +494,13 @@ extern "C" {
 
 int halign_center_create(halign_ctx *ctx, const char *center, int32_t len, halign_center **out)
 {
-    if (!ctx || !center || len <= 0 || !out) return HALIGN_E_ARG;
+    return halign_center_create_indexed(ctx, center, len, center, len, out);
+}
+
+int halign_center_create_indexed(halign_ctx *ctx, const char *center, int32_t len, const char *index_seq,
+                                 int32_t index_len, halign_center **out)
+{
+    if (!ctx || !center || len <= 0 || !out || !index_seq || index_len < 0) return HALIGN_E_ARG;
     std::lock_guard<std::mutex> lock(ctx->mu);
     HCHECK(hipSetDevice(ctx->device), "set device");
     halign_center *c = new (std::nothrow) halign_center();
@@ -507,19 +518,26 @@ int halign_center_create(halign_ctx *ctx, const char *center, int32_t len, halig
     c->bytes.resize((size_t)len + 2 * kBytePad);
     bytes_target(codes.data(), len, c->bytes.data());
     // the center's minimizers on the device (same kernels as the queries)
-    std::vector<int64_t> off{0, len}, hoff, cnt;
+    // (from index_seq; RefAligner::globalAlign extracts them from the aligned
+    // sequence itself when the given ones are empty, ref_aligner.cpp:103-106)
+    std::vector<int64_t> hoff, cnt;
     std::vector<int32_t> sst;
-    int rc;
-    if (ctx->s_seqs.ensure((size_t)len) != hipSuccess || ctx->s_off.ensure(16) != hipSuccess) {
-        delete c;
-        return fail(ctx, HALIGN_E_NOMEM, "alloc center");
-    }
-    (void)hipMemcpy(ctx->s_seqs.p, center, (size_t)len, hipMemcpyHostToDevice);
-    (void)hipMemcpy(ctx->s_off.p, off.data(), 16, hipMemcpyHostToDevice);
-    rc = seed_device(ctx, 1, off, kK, kW, 1, hoff, cnt, sst);
-    if (rc != HALIGN_OK) {
-        delete c;
-        return rc;
+    for (int pass = 0; pass < 2; ++pass) {
+        const char *src = pass == 0 ? index_seq : center;
+        const int32_t n = pass == 0 ? index_len : len;
+        std::vector<int64_t> off{0, n};
+        if (ctx->s_seqs.ensure((size_t)std::max(n, 1)) != hipSuccess || ctx->s_off.ensure(16) != hipSuccess) {
+            delete c;
+            return fail(ctx, HALIGN_E_NOMEM, "alloc center");
+        }
+        if (n) (void)hipMemcpy(ctx->s_seqs.p, src, (size_t)n, hipMemcpyHostToDevice);
+        (void)hipMemcpy(ctx->s_off.p, off.data(), 16, hipMemcpyHostToDevice);
+        const int rc = seed_device(ctx, 1, off, kK, kW, 1, hoff, cnt, sst);
+        if (rc != HALIGN_OK) {
+            delete c;
+            return rc;
+        }
+        if (cnt[0] > 0 || index_seq == center) break;
     }
     c->hx.resize(cnt[0]);
     c->hy.resize(cnt[0]);
@@ -583,7 +601,8 @@ void halign_center_free(halign_center *c)
                       &c->soff, &c->segs, &c->nseg, &c->caps, &c->cig_off, &c->scan_tmp, &c->region, &c->bregion,
                       &c->qrev_off, &c->ncig, &c->llist, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
                       &c->out, &c->nout, &c->ocnt, &c->coff, &c->packed, &c->rows,
-                      &c->rlen_d, &c->rws, &c->hins, &c->lsmall, &c->ljobs, &c->ljseg, &c->louts, &c->ltb})
+                      &c->rlen_d, &c->rws, &c->hins, &c->nm, &c->nmoff, &c->rsz, &c->roff, &c->rcnt,
+                      &c->recs, &c->lsmall, &c->ljobs, &c->ljseg, &c->louts, &c->ltb})
         b->release();
     delete c;
 }
@@ -729,6 +748,98 @@ int halign_center_align_rows(halign_center *c, int32_t n, const char *queries, c
     if (!rows_out) return HALIGN_E_ARG;
     return center_align(c, n, queries, offsets, cigar_out, cap, cigar_offsets, stats, ms, rows_out, row_stride,
                         row_len, has_insertion);
+}
+
+int halign_center_align_fasta(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                              const char *names, const int64_t *name_offsets, char *out, int64_t cap,
+                              int64_t *out_len, int64_t *ins_bytes, int32_t *n_insertion, double *ms)
+{
+    if (!c || n < 0 || !out_len || !ins_bytes || !n_insertion) return HALIGN_E_ARG;
+    if (n > 0 && (!queries || !offsets || !names || !name_offsets || !out)) return HALIGN_E_ARG;
+    halign_ctx *ctx = c->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    *out_len = *ins_bytes = 0;
+    *n_insertion = 0;
+    if (n == 0) return HALIGN_OK;
+    // every record is '>' name '\n' row '\n' with a row of the center's length;
+    // a query named "@..." gets none (its SAM line would be skipped)
+    std::vector<int64_t> noff((size_t)n + 1);
+    int64_t total = 0;
+    for (int i = 0; i <= n; ++i) {
+        noff[i] = name_offsets[i] - name_offsets[0];
+        if (i > 0) {
+            if (noff[i] < noff[i - 1]) return HALIGN_E_ARG;
+            if (noff[i] > noff[i - 1] && names[name_offsets[i - 1]] == '@') continue;
+            total += noff[i] - noff[i - 1] + c->rlen + 3;
+        }
+    }
+    if (total > cap) return fail(ctx, HALIGN_E_CAPACITY, "record output too small");
+    hipEvent_t e0, e1;
+    HCHECK(hipEventCreate(&e0), "event");
+    HCHECK(hipEventCreate(&e1), "event");
+    struct Events {
+        hipEvent_t a, b;
+        ~Events() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } evs{e0, e1};
+    hipStream_t st = ctx->stream;
+    HCHECK(hipEventRecord(e0, st), "event");
+    Batch b;
+    const int rc = run_batch(c, n, queries, offsets, 3, b);
+    if (rc != HALIGN_OK) return rc;
+    for (int i = 0; i < n; ++i)
+        if (b.nout[i] < 0) return fail(ctx, HALIGN_E_KERNEL, "MM2 CIGAR does not consume the pair");
+    const size_t nn = (size_t)n;
+    HCHECK(c->nm.ensure((size_t)std::max<int64_t>(noff[n], 1)), "alloc names");
+    HCHECK(c->nmoff.ensure((nn + 1) * 8), "alloc");
+    HCHECK(c->rlen_d.ensure(nn * 8), "alloc");
+    HCHECK(c->rws.ensure((size_t)std::max<int64_t>(b.packed, 1) * 8), "alloc");
+    HCHECK(c->hins.ensure(nn), "alloc");
+    HCHECK(c->rsz.ensure((2 * nn + 1) * 8), "alloc");
+    HCHECK(c->roff.ensure((2 * nn + 1) * 8), "alloc");
+    HCHECK(c->rcnt.ensure(16), "alloc");
+    HCHECK(c->recs.ensure((size_t)std::max<int64_t>(total, 1)), "alloc records");
+    if (noff[n])
+        HCHECK(hipMemcpyAsync(c->nm.p, names + name_offsets[0], (size_t)noff[n], hipMemcpyHostToDevice, st),
+               "upload names");
+    HCHECK(hipMemcpyAsync(c->nmoff.p, noff.data(), (nn + 1) * 8, hipMemcpyHostToDevice, st), "upload name offsets");
+    HCHECK(hipMemsetAsync(c->rcnt.p, 0, 16, st), "clear counts");
+    const uint32_t *cig = c->packed.as<uint32_t>();
+    const int64_t *coff = c->coff.as<int64_t>();
+    for (int phase = 0; phase < 3; ++phase) {
+        if (phase == 2) {
+            size_t tmp = 0;
+            HCHECK(rocprim::exclusive_scan(nullptr, tmp, c->rsz.as<int64_t>(), c->roff.as<int64_t>(), (int64_t)0,
+                                           2 * nn + 1, rocprim::plus<int64_t>(), st),
+                   "scan storage");
+            HCHECK(c->scan_tmp.ensure(std::max<size_t>(tmp, 1)), "alloc scan storage");
+            HCHECK(rocprim::exclusive_scan(c->scan_tmp.p, tmp, c->rsz.as<int64_t>(), c->roff.as<int64_t>(),
+                                           (int64_t)0, 2 * nn + 1, rocprim::plus<int64_t>(), st),
+                   "scan record sizes");
+        }
+        HCHECK(records_launch(phase, cig, coff, ctx->s_seqs.as<char>(), ctx->s_off.as<int64_t>(), c->nm.as<char>(),
+                              c->nmoff.as<int64_t>(), n, c->rlen, c->rlen_d.as<int64_t>(), c->rws.as<int64_t>(),
+                              c->hins.as<uint8_t>(), phase == 2 ? c->roff.as<int64_t>() : c->rsz.as<int64_t>(),
+                              c->rcnt.as<unsigned long long>(), c->recs.as<char>(), st),
+               "launch records");
+    }
+    HCHECK(hipEventRecord(e1, st), "event");
+    unsigned long long cnt[2] = {0, 0};
+    int64_t ins = 0;
+    if (total) HCHECK(hipMemcpyAsync(out, c->recs.p, (size_t)total, hipMemcpyDeviceToHost, st), "download records");
+    HCHECK(hipMemcpyAsync(cnt, c->rcnt.p, 16, hipMemcpyDeviceToHost, st), "download");
+    HCHECK(hipMemcpyAsync(&ins, c->roff.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, st), "download");
+    HCHECK(hipStreamSynchronize(st), "download records");
+    if (cnt[1]) return fail(ctx, HALIGN_E_KERNEL, "a row's length differs from the center's");
+    *out_len = total;
+    *ins_bytes = ins;
+    *n_insertion = (int32_t)cnt[0];
+    if (ms) {
+        float t = 0;
+        HCHECK(hipEventElapsedTime(&t, e0, e1), "event time");
+        *ms = t;
+    }
+    return HALIGN_OK;
 }
 
 }  // extern "C"

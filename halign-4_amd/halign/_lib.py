@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libhalign_hip.so")
 
 HALIGN_OK = 0
 ERRORS = {-1: "invalid argument", -2: "unsupported configuration", -3: "HIP runtime error",
-          -4: "out of memory", -5: "output buffer too small", -6: "kernel status"}
+          -4: "out of memory", -5: "output buffer too small", -6: "kernel status", -7: "I/O or format error"}
 
 _P = C.c_void_p
 
@@ -37,6 +37,11 @@ class HalignError(RuntimeError):
 
 
 HALIGN_PENDING = 1
+
+
+class MsaStats(C.Structure):
+    _fields_ = [("records", C.c_int64), ("insertion_records", C.c_int64), ("bytes", C.c_int64),
+                ("seconds", C.c_double), ("gpu_seconds", C.c_double), ("input_wait_seconds", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -72,6 +77,18 @@ def _load():
     lib.halign_extract_minimizers.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.c_int32, C.c_int32,
                                               C.c_int32, _P, _P, C.c_int64, _P, _P]
     lib.halign_center_create.argtypes = [_P, C.c_char_p, C.c_int32, C.POINTER(_P)]
+    lib.halign_center_create_indexed.argtypes = [_P, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32,
+                                                 C.POINTER(_P)]
+    lib.halign_center_align_fasta.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.c_char_p, _P, _P, C.c_int64,
+                                              C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                              C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    lib.halign_fastx_open.argtypes = [C.c_char_p, C.POINTER(_P)]
+    lib.halign_fastx_next.argtypes = [_P, C.c_int32, C.POINTER(C.c_int32)] + [C.POINTER(_P)] * 6
+    lib.halign_fastx_error.argtypes = [_P]
+    lib.halign_fastx_error.restype = C.c_char_p
+    lib.halign_fastx_close.argtypes = [_P]
+    lib.halign_msa_keep_length.argtypes = [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32,
+                                           C.POINTER(MsaStats)]
     lib.halign_center_free.argtypes = [_P]
     lib.halign_center_free.restype = None
     lib.halign_center_anchors.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P]
@@ -278,13 +295,21 @@ def _blob(queries):
 class Center:
     """A resident center for the per-query path (halign_center_*)."""
 
-    def __init__(self, ctx, center):
+    def __init__(self, ctx, center, index_seq=None):
+        """index_seq: build the minimizer index from this sequence instead
+        (halign_center_create_indexed)."""
         self.ctx = ctx
         center = center.encode() if isinstance(center, str) else bytes(center)
         self.rlen = len(center)
         h = _P()
-        ctx._check(ctx.lib.halign_center_create(ctx.h, center, len(center), C.byref(h)),
-                   "halign_center_create")
+        if index_seq is None:
+            ctx._check(ctx.lib.halign_center_create(ctx.h, center, len(center), C.byref(h)),
+                       "halign_center_create")
+        else:
+            index_seq = index_seq.encode() if isinstance(index_seq, str) else bytes(index_seq)
+            ctx._check(ctx.lib.halign_center_create_indexed(ctx.h, center, len(center), index_seq,
+                                                            len(index_seq), C.byref(h)),
+                       "halign_center_create_indexed")
         self.h = h
         ctx._children.append(self)
 
@@ -348,6 +373,20 @@ class Center:
         cigs = [cig[coff[i]:coff[i + 1]].copy() for i in range(n)]
         rws = [rows[i, :min(int(rlen[i]), stride)].tobytes() for i in range(n)]
         return cigs, rws, hins[:n].astype(bool)
+
+    def align_fasta(self, queries, names):
+        """The keep-length MSA records of the batch (halign_center_align_fasta):
+        (record bytes, bytes of the insertion group, queries in it, ms)."""
+        blob, off = _blob(queries)
+        nb, noff = _blob(names)
+        n = len(off) - 1
+        cap = int(noff[-1]) + n * (self.rlen + 3) + 1
+        out = np.empty(cap, np.uint8)
+        out_len, ins, nins, ms = C.c_int64(0), C.c_int64(0), C.c_int32(0), C.c_double(0)
+        self.ctx._check(self.ctx.lib.halign_center_align_fasta(
+            self.h, n, blob, _ptr(off), nb, _ptr(noff), _ptr(out), cap, C.byref(out_len), C.byref(ins),
+            C.byref(nins), C.byref(ms)), "halign_center_align_fasta")
+        return out[:out_len.value].tobytes(), ins.value, nins.value, ms.value
 
     def align_flat(self, blob, off):
         """The same on concatenated queries (bytes or uint8 array, int64
@@ -490,3 +529,43 @@ class Server:
         self._check(self.lib.halign_server_result(self.h, rid, _ptr(cig), len(cig), _ptr(coff), _ptr(st),
                                                   _ptr(timing), C.byref(n)), "halign_server_result")
         return cig[:n.value], coff, st[:nq], tuple(float(x) for x in timing)
+
+
+def read_fastx(path, chunk=4096):
+    """Yield (id, desc, seq) bytes triples of a FASTA/FASTQ file, gzip or
+    plain (halign_fastx_*, seq_io::KseqReader's rules)."""
+    L = lib()
+    h = _P()
+    rc = L.halign_fastx_open(path.encode(), C.byref(h))
+    if rc != HALIGN_OK:
+        raise HalignError(f"halign_fastx_open: {ERRORS.get(rc, rc)}: {path}")
+    try:
+        while True:
+            n = C.c_int32(0)
+            ptrs = [_P() for _ in range(6)]
+            rc = L.halign_fastx_next(h, chunk, C.byref(n), *[C.byref(p) for p in ptrs])
+            if rc != HALIGN_OK:
+                raise HalignError(f"halign_fastx_next: {ERRORS.get(rc, rc)}: {L.halign_fastx_error(h).decode()}")
+            if n.value == 0:
+                return
+            seqs, soff, ids, ioff, descs, doff = ptrs
+            so = np.ctypeslib.as_array(C.cast(soff, C.POINTER(C.c_int64)), (n.value + 1,)).copy()
+            io = np.ctypeslib.as_array(C.cast(ioff, C.POINTER(C.c_int64)), (n.value + 1,)).copy()
+            do = np.ctypeslib.as_array(C.cast(doff, C.POINTER(C.c_int64)), (n.value + 1,)).copy()
+            sb = C.string_at(seqs, int(so[-1])) if so[-1] else b""
+            ib = C.string_at(ids, int(io[-1])) if io[-1] else b""
+            db = C.string_at(descs, int(do[-1])) if do[-1] else b""
+            for i in range(n.value):
+                yield ib[io[i]:io[i + 1]], db[do[i]:do[i + 1]], sb[so[i]:so[i + 1]]
+    finally:
+        L.halign_fastx_close(h)
+
+
+def msa_keep_length(ctx, center_path, input_path, output_path, batch=0):
+    """halign4 -c center --keep-length, file to file (halign_msa_keep_length):
+    returns the run's statistics as a dict."""
+    st = MsaStats()
+    ctx._check(ctx.lib.halign_msa_keep_length(ctx.h, center_path.encode(), input_path.encode(),
+                                              output_path.encode(), int(batch), C.byref(st)),
+               "halign_msa_keep_length")
+    return {f: getattr(st, f) for f, _ in MsaStats._fields_}

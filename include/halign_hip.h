@@ -144,6 +144,14 @@ int halign_extract_minimizers(halign_ctx *ctx, int32_t n, const char *seqs, cons
  * src/seed/anchor.cpp:111-117). */
 typedef struct halign_center halign_center;
 int halign_center_create(halign_ctx *ctx, const char *center, int32_t len, halign_center **out);
+/* As halign_center_create, aligning to `center` with the minimizer index built
+ * from index_seq: RefAligner keeps the minimizers of the center file's record
+ * (ref_aligner.cpp:35-44) but aligns to the consensus of that file (:47-60,
+ * :243-251), which differs from the record where it has N or lower case.
+ * When index_seq has no minimizers the center's own are used (globalAlign
+ * extracts them on the spot then, :103-106). */
+int halign_center_create_indexed(halign_ctx *ctx, const char *center, int32_t len, const char *index_seq,
+                                 int32_t index_len, halign_center **out);
 void halign_center_free(halign_center *c);
 /* minimizer::collect_anchors(center hits, query hits) (src/seed/minimizer.cpp:
  * 220-356) for n ASCII queries (queries[offsets[i] .. offsets[i+1])), default
@@ -174,6 +182,66 @@ int halign_center_align_rows(halign_center *c, int32_t n, const char *queries, c
                              uint32_t *cigar_out, int64_t cap, int64_t *cigar_offsets, char *rows_out,
                              int64_t row_stride, int64_t *row_len, uint8_t *has_insertion, int32_t *stats,
                              double *ms);
+
+/* The keep-length MSA records of one batch (RefAligner::mergeAlignedResults
+ * with keep_length and one center, src/align/ref_aligner.cpp:676-740: rows of
+ * queries whose CIGAR has an insertion come from the merged insertion SAMs,
+ * :504-535 and :715-717, then the others, :719-732; every row is the query
+ * projected by delQueryToRefByCigar / padQueryToRefByCigar, src/align/
+ * cigar.cpp:150-299, i.e. the center's length; records as seq_io::SeqWriter::
+ * writeFasta with an unlimited line width writes them, src/utils/seq_io.cpp:
+ * 236-268). Query i (queries[offsets[i] .. offsets[i+1])) with name
+ * names[name_offsets[i] .. name_offsets[i+1]) becomes '>' name '\n' row '\n'
+ * (none for a name starting with '@': the reference's SamReader skips such
+ * lines, src/utils/seq_io.cpp:433-436); records of queries with an insertion
+ * come first, each group in input order:
+ * out[0 .. *ins_bytes) holds the *n_insertion records of the first group,
+ * out[*ins_bytes .. *out_len) the others. The rows and records are made on the
+ * device. cap >= sum over the kept queries of (name length + center length +
+ * 3). */
+int halign_center_align_fasta(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,
+                              const char *names, const int64_t *name_offsets, char *out, int64_t cap,
+                              int64_t *out_len, int64_t *ins_bytes, int32_t *n_insertion, double *ms);
+
+#define HALIGN_E_IO (-7)   /* file open/read/write error or malformed FASTA/FASTQ */
+
+/* seq_io::KseqReader (src/utils/seq_io.cpp:67-155 over the vendored
+ * include/kseq.h): FASTA and FASTQ records from a gzip or plain file, with
+ * kseq's rules for names (up to the first whitespace), comments (the rest of
+ * the header line), sequence lines (concatenated, empty lines skipped, one
+ * trailing '\r' per line dropped) and quality lines (read, then dropped). A
+ * FASTQ record whose quality is missing or of another length is an error
+ * (HALIGN_E_IO), as kseq_read's -2 makes the reference throw. */
+typedef struct halign_fastx halign_fastx;
+int halign_fastx_open(const char *path, halign_fastx **out);
+/* Up to max_records next records (fewer at the end of the file; *n = 0 at
+ * its end). The views stay valid until the next call: record i's sequence is
+ * seqs[seq_offsets[i] .. seq_offsets[i+1]), its name ids[id_offsets[i] ..
+ * id_offsets[i+1]), its comment descs[desc_offsets[i] .. desc_offsets[i+1]). */
+int halign_fastx_next(halign_fastx *f, int32_t max_records, int32_t *n, const char **seqs,
+                      const int64_t **seq_offsets, const char **ids, const int64_t **id_offsets,
+                      const char **descs, const int64_t **desc_offsets);
+const char *halign_fastx_error(const halign_fastx *f);
+void halign_fastx_close(halign_fastx *f);
+
+/* halign4 -c center.fa --keep-length with one center and the copy MSA
+ * command (src/halign4.cpp:96-147): the RefAligner constructor's center
+ * (consensus of the center file, ref_aligner.cpp:20-70), alignQueryToRef over
+ * chunks of `batch` queries (:286-394, batch <= 0: 25600) and
+ * mergeAlignedResults (:676-740) into output_path: the center records cleaned
+ * (seq_io::cleanSequence, include/utils.h:161-197), then the keep-length rows
+ * as halign_center_align_fasta orders them over the whole input (the
+ * reference's order with one thread). Reading the next chunk, aligning one on
+ * the GPU and writing the previous one's records overlap. A center file with
+ * more than one record is the multi-reference path: HALIGN_E_UNSUPPORTED.
+ * stats (optional): {records written, insertion records, bytes written,
+ * seconds total, seconds of GPU calls, seconds the GPU waited for input}. */
+typedef struct {
+    int64_t records, insertion_records, bytes;
+    double seconds, gpu_seconds, input_wait_seconds;
+} halign_msa_stats;
+int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, const char *input_path,
+                           const char *output_path, int32_t batch, halign_msa_stats *stats);
 
 /* Column counts and consensus of an alignment (the counting core of
  * consensus::generateConsensusSequence, reference src/consensus/consensus.cpp:

@@ -892,3 +892,177 @@ int64_t or_consensus(const char *rows, int64_t nrows, int64_t aln_len,
     }
     return (int64_t)num_seqs;
 }
+
+/* ------------------------------------------------------------------ */
+/* FASTA/FASTQ records: kseq_read (include/kseq.h) as KseqReader uses it */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const unsigned char *b; int64_t n, i;
+} fx_src_t;
+
+typedef struct {
+    char *p; int64_t l, m;
+} fx_str_t;
+
+static void fx_put(fx_str_t *s, char c)
+{
+    if (s->l + 1 > s->m) {
+        s->m = s->m ? 2 * s->m : 64;
+        s->p = (char *)realloc(s->p, (size_t)s->m);
+    }
+    s->p[s->l++] = c;
+}
+
+static int fx_space(int c)
+{
+    return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r';
+}
+
+/* ks_getuntil2: bytes up to the delimiter (a whitespace byte when space != 0,
+ * else '\n'; consumed, returned in *dret, 0 at the end of the data) are
+ * appended to dst. Returns -1, reading nothing and trimming nothing, when the
+ * data was already exhausted. Otherwise a line read (space == 0) leaves dst
+ * one '\r' shorter when dst, counted whole, is longer than one byte and ends
+ * in '\r'. */
+static int fx_until(fx_src_t *s, int space, fx_str_t *dst, int *dret)
+{
+    if (dret) *dret = 0;
+    if (s->i >= s->n) return -1;
+    while (s->i < s->n) {
+        const int c = s->b[s->i++];
+        if (space ? fx_space(c) : c == '\n') {
+            if (dret) *dret = c;
+            break;
+        }
+        fx_put(dst, (char)c);
+    }
+    if (!space && dst->l > 1 && dst->p[dst->l - 1] == '\r') --dst->l;
+    return 0;
+}
+
+static int fx_getc(fx_src_t *s) { return s->i < s->n ? s->b[s->i++] : -1; }
+
+/* kseq_read: 1 = a record in id/desc/seq, 0 = end of data, -2 = malformed
+ * FASTQ. *hdr carries the header byte already consumed ('>' / '@', 0 = none). */
+static int fx_read(fx_src_t *s, int *hdr, fx_str_t *id, fx_str_t *desc, fx_str_t *seq, fx_str_t *qual)
+{
+    int c;
+    if (*hdr == 0) {
+        while ((c = fx_getc(s)) >= 0 && c != '>' && c != '@') {}
+        if (c < 0) return 0;
+        *hdr = c;
+    }
+    id->l = desc->l = seq->l = qual->l = 0;
+    if (fx_until(s, 1, id, &c) < 0) return 0;
+    if (c != '\n') fx_until(s, 0, desc, NULL);
+    while ((c = fx_getc(s)) >= 0 && c != '>' && c != '+' && c != '@') {
+        if (c == '\n') continue;
+        fx_put(seq, (char)c);
+        fx_until(s, 0, seq, NULL);
+    }
+    if (c == '>' || c == '@') *hdr = c;
+    if (c != '+') return 1;
+    while ((c = fx_getc(s)) >= 0 && c != '\n') {}
+    if (c < 0) return -2;
+    while (fx_until(s, 0, qual, NULL) >= 0 && qual->l < seq->l) {}
+    *hdr = 0;
+    return seq->l == qual->l ? 1 : -2;
+}
+
+static int fx_emit(const fx_str_t *v, char *out, int64_t *off, int k, int64_t cap)
+{
+    if (off[k] + v->l > cap) return -1;
+    if (v->l) memcpy(out + off[k], v->p, (size_t)v->l);
+    off[k + 1] = off[k] + v->l;
+    return 0;
+}
+
+int or_fastx_parse(const char *buf, int64_t len, int max_records,
+                   char *id_out, int64_t *id_off, char *desc_out, int64_t *desc_off,
+                   char *seq_out, int64_t *seq_off, int64_t cap)
+{
+    fx_src_t s = {(const unsigned char *)buf, len, 0};
+    fx_str_t id = {0}, desc = {0}, seq = {0}, qual = {0};
+    int hdr = 0, n = 0, rc = 0;
+    id_off[0] = desc_off[0] = seq_off[0] = 0;
+    while (n < max_records) {
+        rc = fx_read(&s, &hdr, &id, &desc, &seq, &qual);
+        if (rc <= 0) break;
+        if (fx_emit(&id, id_out, id_off, n, cap) || fx_emit(&desc, desc_out, desc_off, n, cap) ||
+            fx_emit(&seq, seq_out, seq_off, n, cap)) {
+            rc = -1;
+            break;
+        }
+        ++n;
+        rc = 0;
+    }
+    free(id.p); free(desc.p); free(seq.p); free(qual.p);
+    return rc < 0 ? rc : n;
+}
+
+/* ------------------------------------------------------------------ */
+/* keep-length merge: RefAligner::mergeAlignedResults, one center       */
+/* ------------------------------------------------------------------ */
+/* seq_io::cleanSequence's table (include/utils.h:161-181) */
+static char clean_char(unsigned char c)
+{
+    switch (c) {
+    case 'A': case 'a': return 'A';
+    case 'C': case 'c': return 'C';
+    case 'G': case 'g': return 'G';
+    case 'T': case 't': return 'T';
+    case 'U': case 'u': return 'U';
+    case '-': return '-';
+    default: return 'N';
+    }
+}
+
+int64_t or_keep_length_fasta(const char *center_id, int id_len, const char *center_desc, int desc_len,
+                             const char *center_seq, int64_t center_len, int n, const char *queries,
+                             const int64_t *qoff, const char *names, const int64_t *noff,
+                             const uint32_t *cigars, const int64_t *coff, char *out, int64_t cap)
+{
+    int64_t o = 0, expect = -1;
+#define OR_PUT(src, len_)                                          \
+    do {                                                           \
+        if (o + (int64_t)(len_) > cap) return -1;                  \
+        if ((len_) > 0) memcpy(out + o, (src), (size_t)(len_));    \
+        o += (int64_t)(len_);                                      \
+    } while (0)
+    OR_PUT(">", 1);
+    OR_PUT(center_id, id_len);
+    if (desc_len > 0) {
+        OR_PUT(" ", 1);
+        OR_PUT(center_desc, desc_len);
+    }
+    OR_PUT("\n", 1);
+    if (o + center_len + 1 > cap) return -1;
+    for (int64_t i = 0; i < center_len; ++i) out[o + i] = clean_char((unsigned char)center_seq[i]);
+    o += center_len;
+    OR_PUT("\n", 1);   /* writeFasta: an empty sequence is an empty line */
+    /* insertion group first, then the others */
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int q = 0; q < n; ++q) {
+            const uint32_t *c = cigars + coff[q];
+            const int nc = (int)(coff[q + 1] - coff[q]);
+            if (or_has_insertion(c, nc) != (pass == 0)) continue;
+            const int64_t nl = noff[q + 1] - noff[q];
+            /* SamReader::next skips lines starting with '@' (src/utils/seq_io.cpp:
+               433-436): a query named "@..." does not come back from its SAM */
+            if (nl > 0 && names[noff[q]] == '@') continue;
+            OR_PUT(">", 1);
+            OR_PUT(names + noff[q], nl);
+            OR_PUT("\n", 1);
+            if (o >= cap) return -1;
+            const int rl = or_del_query_to_ref(queries + qoff[q], (int)(qoff[q + 1] - qoff[q]), c, nc, out + o,
+                                               (int)((cap - o) < 0x7fffffff ? cap - o : 0x7fffffff));
+            if (rl < 0) return -1;
+            if (expect < 0) expect = rl;
+            else if (rl != expect) return -1;
+            o += rl;
+            OR_PUT("\n", 1);
+        }
+    }
+#undef OR_PUT
+    return o;
+}

@@ -138,6 +138,36 @@ int64_t or_consensus(const char *rows, int64_t nrows, int64_t aln_len,
                      int64_t batch_size, uint64_t seq_limit,
                      uint32_t *counts, char *cons_out);
 
+/* seq_io::KseqReader::next over the vendored kseq_read (src/utils/seq_io.cpp:
+ * 131-155, include/kseq.h kseq_read / ks_getuntil2), on an in-memory file of
+ * len bytes (already decompressed). Writes up to max_records records: ids,
+ * comments and sequences back to back into id_out / desc_out / seq_out
+ * (capacity cap bytes each) with n+1 offsets each. Returns the number of
+ * records, or -2 for a malformed FASTQ record (kseq's -2; the reference
+ * throws), or -1 when an output is too small. */
+int or_fastx_parse(const char *buf, int64_t len, int max_records,
+                   char *id_out, int64_t *id_off, char *desc_out, int64_t *desc_off,
+                   char *seq_out, int64_t *seq_off, int64_t cap);
+
+/* The final keep-length FASTA of RefAligner::mergeAlignedResults for one
+ * center and the copy MSA command (src/align/ref_aligner.cpp:676-740):
+ * the center record with seq_io::cleanSequence applied (include/utils.h:
+ * 161-197; writeConsensusAndReferences :541-561), then per query '>' name
+ * '\n' row '\n' (seq_io::SeqWriter::writeFasta, src/utils/seq_io.cpp:236-268,
+ * width unlimited) with the row from delQueryToRefByCigar (the insertion SAMs
+ * through mergeConsensusAndSamToFasta :139-209) or padQueryToRefByCigar (the
+ * other SAMs, convertSamToFastaRecord :467-498; the same row for a CIGAR
+ * without I), the queries whose CIGAR has an insertion first (:713-732),
+ * in input order with one thread; a query whose name starts with '@' is
+ * dropped (SamReader::next skips such lines, src/utils/seq_io.cpp:433-436).
+ * center_desc may be empty (no ' ' then).
+ * Returns the bytes written, or -1 (capacity, or a row whose length differs
+ * from the first row's: the reference throws, :590-596, :657-663). */
+int64_t or_keep_length_fasta(const char *center_id, int id_len, const char *center_desc, int desc_len,
+                             const char *center_seq, int64_t center_len, int n, const char *queries,
+                             const int64_t *qoff, const char *names, const int64_t *noff,
+                             const uint32_t *cigars, const int64_t *coff, char *out, int64_t cap);
+
 /* log2f table used by or_chain_anchors: lut[d] = log2f(d), d in [0, 501]. */
 void or_fill_log2_lut(float *lut);
 

@@ -8,8 +8,11 @@
 #include "consensus.h"
 #include "ksw2.h"
 #include "seed.h"
+#include "utils.h"
+#include "config.hpp"
 
 #include <cstring>
+#include <filesystem>
 #include <string>
 #include <vector>
 
@@ -154,6 +157,185 @@ int ref_consensus(const char* aligned_fasta, const char* out_fasta, const char* 
     if ((int)c.size() > cap) return -1;
     std::memcpy(cons_out, c.data(), c.size());
     return (int)c.size();
+}
+
+// seq_io::KseqReader over a file: records' ids, comments and sequences back
+// to back with n+1 offsets each (capacity cap bytes per output). Returns the
+// record count, -1 when an output is too small, -2 when the reader throws.
+int ref_read_fastx(const char* path, int max_records, char* id_out, int64_t* id_off, char* desc_out,
+                   int64_t* desc_off, char* seq_out, int64_t* seq_off, int64_t cap)
+{
+    try {
+        seq_io::KseqReader rd(path);
+        seq_io::SeqRecord rec;
+        int n = 0;
+        id_off[0] = desc_off[0] = seq_off[0] = 0;
+        while (n < max_records && rd.next(rec)) {
+            const std::string* f[3] = {&rec.id, &rec.desc, &rec.seq};
+            char* o[3] = {id_out, desc_out, seq_out};
+            int64_t* off[3] = {id_off, desc_off, seq_off};
+            for (int k = 0; k < 3; ++k) {
+                if (off[k][n] + (int64_t)f[k]->size() > cap) return -1;
+                std::memcpy(o[k] + off[k][n], f[k]->data(), f[k]->size());
+                off[k][n + 1] = off[k][n] + (int64_t)f[k]->size();
+            }
+            ++n;
+        }
+        return n;
+    } catch (...) {
+        return -2;
+    }
+}
+
+// halign4 -c center_path --keep-length with the copy MSA command and one
+// thread, as the reference runs it (src/halign4.cpp:96-147): the RefAligner
+// constructor (src/align/ref_aligner.cpp:20-70: copy the center file,
+// consensus of the copy = the center aligned to; the minimizers of the file's
+// record), alignQueryToRef (:286-394) with alignOneQueryToRef's routing
+// (:212-283) into thread 0's two SAM files, then mergeAlignedResults
+// (:676-740) with processInsertionSequences / mergeConsensusAndSamToFasta
+// (:504-535, :139-209), parseAlignedReferencesToCigar (:400-459),
+// writeConsensusAndReferences (:541-561), writeInsertionSequences
+// (:567-605) and processSamFileBatch / convertSamToFastaRecord (:467-498,
+// :612-672). RefAligner itself is not compiled here (it needs mash and the
+// external-tool runner); this glue calls the same reference functions in the
+// same order. work_dir must exist. Returns 0, or -2 when the reference throws.
+int ref_keep_length_msa(const char* center_path, const char* input_path, const char* work_dir,
+                        const char* out_path)
+{
+    namespace fs = std::filesystem;
+    try {
+        const fs::path wd(work_dir);
+        // constructor: the center file's records and minimizers
+        std::vector<seq_io::SeqRecord> refs;
+        std::vector<minimizer::MinimizerHits> ref_mz;
+        {
+            seq_io::KseqReader rd(center_path);
+            seq_io::SeqRecord rec;
+            while (rd.next(rec)) {
+                ref_mz.push_back(minimizer::extractMinimizer(rec.seq, 15, 10, true));
+                refs.push_back(rec);
+            }
+        }
+        if (refs.size() != 1) return -3;
+        const fs::path cons_aligned = wd / "cons_aligned.fasta";
+        file_io::copyFile(center_path, cons_aligned);   // "cp {input} {output}"
+        seq_io::SeqRecord consensus_seq;
+        consensus_seq.id = "consensus";
+        consensus_seq.seq = consensus::generateConsensusSequence(cons_aligned, wd / "cons.fasta", wd / "cons.json",
+                                                                 0, 1, 4096);
+        // alignQueryToRef, one thread
+        const fs::path sam = wd / "t0.sam", sam_ins = wd / "t0_insertion.sam";
+        {
+            auto out = seq_io::SeqWriter::Sam(sam);
+            auto out_ins = seq_io::SeqWriter::Sam(sam_ins);
+            out.writeSamHeader("@HD\tVN:1.6\tSO:unknown");
+            out_ins.writeSamHeader("@HD\tVN:1.6\tSO:unknown");
+            seq_io::KseqReader rd(input_path);
+            seq_io::SeqRecord q;
+            while (rd.next(q)) {
+                auto qm = minimizer::extractMinimizer(q.seq, 15, 10, true);
+                const minimizer::MinimizerHits* rm = &ref_mz[0];
+                minimizer::MinimizerHits rtmp;
+                if (rm->empty()) {
+                    rtmp = minimizer::extractMinimizer(consensus_seq.seq, 15, 10, true);
+                    rm = &rtmp;
+                }
+                const auto anchors = minimizer::collect_anchors(*rm, qm);
+                const cigar::Cigar_t cg = align::globalAlignMM2(consensus_seq.seq, q.seq, anchors);
+                const auto rec = seq_io::makeSamRecord(q, consensus_seq.id, cigar::cigarToString(cg), 1, 60, 0);
+                if (cigar::hasInsertion(cg)) out_ins.writeSam(rec);
+                else out.writeSam(rec);
+            }
+            out.flush();
+            out_ins.flush();
+        }
+        // mergeAlignedResults
+        auto parse_aligned = [](const fs::path& p, std::unordered_map<std::string, cigar::Cigar_t>& m) {
+            seq_io::KseqReader rd(p);
+            seq_io::SeqRecord rec;
+            while (rd.next(rec)) {
+                cigar::Cigar_t c;
+                char cur = 0;
+                uint32_t len = 0;
+                for (char b : rec.seq) {
+                    const char op = b == '-' ? 'D' : 'M';
+                    if (op == cur) ++len;
+                    else {
+                        if (cur && len) c.push_back(cigar::cigarToInt(cur, len));
+                        cur = op;
+                        len = 1;
+                    }
+                }
+                if (cur && len) c.push_back(cigar::cigarToInt(cur, len));
+                m[rec.id] = c;
+            }
+        };
+        std::unordered_map<std::string, cigar::Cigar_t> ref_map, ins_map;
+        parse_aligned(cons_aligned, ref_map);
+        seq_io::SeqWriter final_writer(out_path, U_MAX);
+        // processInsertionSequences -> mergeConsensusAndSamToFasta(keep = true, width 80), then copy
+        const fs::path ins_fa = wd / "all_insertion.fasta", ins_aligned = wd / "aligned_insertion.fasta";
+        {
+            seq_io::SeqWriter w(ins_fa, 80);
+            w.writeFasta(consensus_seq);
+            seq_io::SamReader r(sam_ins);
+            seq_io::SamRecord s;
+            while (r.next(s)) {
+                seq_io::SeqRecord f = seq_io::samRecordToSeqRecord(s, false);
+                if (!s.cigar.empty()) {
+                    cigar::Cigar_t ops = cigar::stringToCigar(s.cigar);
+                    cigar::delQueryToRefByCigar(f.seq, ops);
+                    if (s.rname != "consensus") cigar::padQueryToRefByCigar(f.seq, ref_map.at(s.rname));
+                }
+                w.writeFasta(f);
+            }
+            w.flush();
+        }
+        file_io::copyFile(ins_fa, ins_aligned);
+        parse_aligned(ins_aligned, ins_map);
+        ref_map[consensus_seq.id] = ins_map[consensus_seq.id];
+        // writeConsensusAndReferences
+        {
+            seq_io::KseqReader rd(cons_aligned);
+            seq_io::SeqRecord rec;
+            while (rd.next(rec)) {
+                seq_io::cleanSequence(rec);
+                final_writer.writeFasta(rec);
+            }
+        }
+        // writeInsertionSequences
+        size_t expect = 0;
+        bool have = false;
+        {
+            seq_io::KseqReader rd(ins_aligned);
+            seq_io::SeqRecord rec;
+            bool first = true;
+            while (rd.next(rec)) {
+                if (first) { first = false; continue; }
+                if (!have) { expect = rec.seq.size(); have = true; }
+                else if (rec.seq.size() != expect) return -4;
+                final_writer.writeFasta(rec);
+            }
+        }
+        // processSamFileBatch over thread 0's SAM file
+        {
+            seq_io::SamReader r(sam);
+            seq_io::SamRecord s;
+            while (r.next(s)) {
+                seq_io::SeqRecord f = seq_io::samRecordToSeqRecord(s, false);
+                if (!s.cigar.empty() && s.cigar != "*") cigar::padQueryToRefByCigar(f.seq, cigar::stringToCigar(s.cigar));
+                cigar::padQueryToRefByCigar(f.seq, ref_map.at(s.rname));
+                if (!have) { expect = f.seq.size(); have = true; }
+                else if (f.seq.size() != expect) return -4;
+                final_writer.writeFasta(f);
+            }
+        }
+        final_writer.flush();
+        return 0;
+    } catch (...) {
+        return -2;
+    }
 }
 
 }  // extern "C"

@@ -20,6 +20,10 @@ Outputs (np.savez_compressed, no pickles):
   ref_global.npz       RefAligner::globalAlign (seed -> anchors -> chain -> MM2)
   consensus.npz        generateConsensusSequence: counts, consensus, row count
                        (case = (row set, seq_limit))
+  fastx_cases.npz      seq_io::KseqReader records of FASTA/FASTQ texts (plain and
+                       gzip), or its error
+  msa_keep_length.npz  halign4 -c center --keep-length (one thread, copy MSA
+                       command): the final FASTA bytes of center + query files
 """
 import json
 import os
@@ -320,6 +324,167 @@ def consensus(R, rng):
          num_seqs=np.array(nseq, np.int64))
 
 
+FASTX_FIXED = [
+    b">a desc here\nACGT\nacgt\n>b\nNNNN\n",
+    b">a\r\nAC\r\nGT\r\n>b x\r\nTT\r\n",
+    b"junk\n>a\n\n\nAC\n\n>b\n",
+    b">a\n\r\n>b\n\r\nA\n",
+    b"@q1 c\nACGT\n+\nIIII\n@q2\nAC\nGT\n+q2\nII\nII\n",
+    b"@q1\nACGT\n+\nIII\n",
+    b"@q1\nACGT\n+",
+    b">a\tb c\nA C G\n@x\n",
+    b">\nACGT\n> d\nAA\n>",
+    b">a \r\nAAA\r",
+    b">a\nAAA\n\r",
+    b"\r",
+    b"",
+    b">only",
+    b">a\n+\n\n>b\nAC\n",
+    b">a b\r\nA\rC\n",
+    b"@r\n\n+\n\n@s\nA\n+\n!\n",
+    b"@r\n+\n@s\nA\n+\n!\n",
+]
+
+
+def fastx_text(rng, size):
+    """Random FASTA/FASTQ-like text over header, separator and base bytes."""
+    alpha = np.frombuffer(b">@+\n\n\r \tACGTacgtNn-", np.uint8)
+    p = np.array([2, 1, 1, 8, 4, 1, 1, 1] + [6] * 8 + [1, 1, 1], np.float64)
+    return alpha[rng.choice(len(alpha), size, p=p / p.sum())].tobytes()
+
+
+def fastx_cases(R, rng):
+    """seq_io::KseqReader (src/utils/seq_io.cpp:67-155 over include/kseq.h) on
+    FASTA/FASTQ texts: fixed edge cases, random texts, and two texts larger
+    than the readers' buffers; each read plain and gzip-compressed."""
+    import gzip
+    texts = list(FASTX_FIXED)
+    texts += [fastx_text(rng, int(rng.integers(1, 300))) for _ in range(400)]
+    recs = []
+    unit = ACGT[rng.integers(0, 4, 997)].tobytes() * 41   # repetitive, so the fixture compresses
+    for i in range(2):   # records of 0.5-40 kb, 3 MB in all
+        parts = []
+        tot = 0
+        k = 0
+        while tot < 3_000_000:
+            n = int(rng.integers(500, 40_000))
+            a = int(rng.integers(0, 997))
+            seq = unit[a:a + n]
+            if i == 0:
+                lines = [seq[j:j + 60] for j in range(0, n, 60)]
+                parts.append(b">r%d d%d\r\n" % (k, k) + b"\r\n".join(lines) + b"\r\n")
+            else:
+                parts.append(b"@r%d\n%s\n+\n%s\n" % (k, seq, b"I" * n))
+            tot += n
+            k += 1
+        texts.append(b"".join(parts))
+    d = tempfile.mkdtemp()
+    text_data, status, ids, descs, seqs, nrec = [], [], [], [], [], []
+    for i, t in enumerate(texts):
+        res = []
+        for gz in (False, True):
+            path = os.path.join(d, "t%d.fa" % i + (".gz" if gz else ""))
+            with (gzip.open(path, "wb") if gz else open(path, "wb")) as f:
+                f.write(t)
+            res.append(R.read_fastx(path))
+        assert res[0] == res[1], i
+        r = res[0]
+        text_data.append(np.frombuffer(t, np.uint8))
+        if isinstance(r, int):
+            status.append(r)
+            nrec.append(0)
+            continue
+        status.append(0)
+        nrec.append(len(r))
+        for a, b, c in r:
+            ids.append(np.frombuffer(a, np.uint8))
+            descs.append(np.frombuffer(b, np.uint8))
+            seqs.append(np.frombuffer(c, np.uint8))
+    td, to = concat(text_data, np.uint8)
+    idd, ido = concat(ids, np.uint8)
+    dd, do = concat(descs, np.uint8)
+    sd, so = concat(seqs, np.uint8)
+    save("fastx_cases.npz", text=td, text_off=to, status=np.array(status, np.int32),
+         nrec=np.array(nrec, np.int64), id=idd, id_off=ido, desc=dd, desc_off=do, seq=sd, seq_off=so)
+
+
+def msa_keep_length(R, rng):
+    """halign4 -c center --keep-length (src/halign4.cpp:96-147) with one
+    thread and the copy MSA command, through the reference's own readers,
+    aligners, SAM and FASTA writers (ref_keep_length_msa): the final FASTA
+    for a few center/query files. Queries cover both SAM groups (with and
+    without an insertion), a query too short for minimizers, an empty one,
+    descriptions, a name starting with '@' (its SAM line is skipped), FASTQ
+    input, and a center with lower case and N (aligned as its consensus,
+    indexed as written)."""
+    d = tempfile.mkdtemp()
+    centers, inputs, outputs = [], [], []
+    for case in range(5):
+        clen = [3000, 5000, 16569, 2000, 4000][case]
+        cc = rand_codes(rng, clen)
+        center = codes_to_ascii(cc)
+        if case == 4:   # scattered N: the queries follow the consensus (A there), so
+            # anchors differ between the index of the center as written and of its consensus
+            pos = rng.choice(clen, 80, replace=False)
+            cc[pos] = 0
+            b = bytearray(codes_to_ascii(cc))
+            for j in pos:
+                b[j] = ord("N")
+            center = bytes(b)
+        if case == 1:   # lower case and N in the center
+            b = bytearray(center)
+            for j in rng.integers(0, clen, 40):
+                b[j] = ord("n") if j % 3 == 0 else (b[j] | 0x20)
+            b[100:160] = b"N" * 60
+            center = bytes(b)
+        cfa = b">center%d some description\n" % case + b"\n".join(center[j:j + 70] for j in range(0, clen, 70)) + b"\n"
+        recs = []
+        nq = [30, 40, 10, 25, 40][case]
+        for i in range(nq):
+            kind = i % 5
+            if kind == 0:     # substitutions only: no insertion
+                q = mutate_codes(rng, cc, 0.02, 0.0)
+            elif kind == 1:   # deletions only
+                q = cc.copy()
+                for _ in range(3):
+                    a = int(rng.integers(0, len(q) - 10))
+                    q = np.concatenate([q[:a], q[a + int(rng.integers(1, 8)):]])
+            else:
+                q = mutate_codes(rng, cc, 0.01, 0.004)
+            if i % 7 == 3:    # a part of the center
+                a = int(rng.integers(0, clen // 2))
+                q = q[a:a + int(rng.integers(200, clen // 2))]
+            qs = codes_to_ascii(q)
+            if i % 9 == 4:
+                qs = qs.lower()
+            name = b"q%d" % i
+            if i == 5:
+                name = b"@at%d" % i
+            desc = b" len=%d" % len(qs) if i % 4 == 0 else b""
+            recs.append((name, desc, qs))
+        recs.append((b"short", b"", b"ACGTACGTAC"))
+        recs.append((b"empty", b"", b""))
+        if case == 3:   # FASTQ input
+            text = b"".join(b"@" + n + dsc + b"\n" + q + b"\n+\n" + b"I" * len(q) + b"\n" for n, dsc, q in recs)
+        else:
+            text = b"".join(b">" + n + dsc + b"\n" + b"\n".join(q[j:j + 60] for j in range(0, len(q), 60)) +
+                            b"\n" for n, dsc, q in recs)
+        cpath, ipath, opath = (os.path.join(d, x + "%d" % case) for x in ("c.fa", "q.fa", "out.fa"))
+        open(cpath, "wb").write(cfa)
+        open(ipath, "wb").write(text)
+        wd = os.path.join(d, "wd%d" % case)
+        os.makedirs(wd)
+        rc = R.keep_length_msa(cpath, ipath, wd, opath)
+        assert rc == 0, (case, rc)
+        centers.append(np.frombuffer(cfa, np.uint8))
+        inputs.append(np.frombuffer(text, np.uint8))
+        outputs.append(np.frombuffer(open(opath, "rb").read(), np.uint8))
+    cd, co = concat(centers, np.uint8)
+    idd, io = concat(inputs, np.uint8)
+    od, oo = concat(outputs, np.uint8)
+    save("msa_keep_length.npz", center=cd, center_off=co, input=idd, input_off=io, output=od, output_off=oo)
+
+
 def main():
     R = RefLib()
     O = Oracle()
@@ -331,6 +496,8 @@ def main():
     ref_global(R, np.random.default_rng(0x6021))
     consensus(R, np.random.default_rng(0x6022))
     whole_pairs(R)
+    fastx_cases(R, np.random.default_rng(0x6023))
+    msa_keep_length(R, np.random.default_rng(0x6024))
 
 
 if __name__ == "__main__":

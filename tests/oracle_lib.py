@@ -111,6 +111,10 @@ class Oracle(_Base):
                                    C.c_char_p]
         L.or_consensus.restype = C.c_int64
         L.or_auto_band.argtypes = [C.c_int, C.c_int]
+        L.or_fastx_parse.argtypes = [C.c_char_p, C.c_int64, C.c_int, _P, _P, _P, _P, _P, _P, C.c_int64]
+        L.or_keep_length_fasta.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int64,
+                                           C.c_int, C.c_char_p, _P, C.c_char_p, _P, _P, _P, _P, C.c_int64]
+        L.or_keep_length_fasta.restype = C.c_int64
         L.or_fill_log2_lut.argtypes = [_P]
         self.lut = np.zeros(502, np.float32)
         L.or_fill_log2_lut(_ptr(self.lut))
@@ -242,6 +246,44 @@ class Oracle(_Base):
         return counts, cons.raw[:aln], int(n)
 
 
+    def fastx_parse(self, data, max_records=1 << 30):
+        """Records (id, desc, seq) of an in-memory FASTA/FASTQ file, or an int
+        error code (-2: malformed FASTQ)."""
+        cap = len(data) + 1
+        bufs = [np.zeros(cap, np.uint8) for _ in range(3)]
+        offs = [np.zeros(len(data) + 2, np.int64) for _ in range(3)]
+        n = self.lib.or_fastx_parse(data, len(data), min(max_records, len(data) + 1), _ptr(bufs[0]),
+                                    _ptr(offs[0]), _ptr(bufs[1]), _ptr(offs[1]), _ptr(bufs[2]),
+                                    _ptr(offs[2]), cap)
+        if n < 0:
+            return n
+        return [tuple(bufs[k][offs[k][i]:offs[k][i + 1]].tobytes() for k in range(3)) for i in range(n)]
+
+    def keep_length_fasta(self, center_id, center_desc, center_seq, queries, names, cigars):
+        """The merged keep-length FASTA bytes (or_keep_length_fasta)."""
+        qb, qo = _concat_bytes(queries)
+        nb, no = _concat_bytes(names)
+        cb = np.concatenate([np.asarray(c, np.uint32) for c in cigars]) if cigars else np.zeros(0, np.uint32)
+        co = np.zeros(len(cigars) + 1, np.int64)
+        co[1:] = np.cumsum([len(c) for c in cigars])
+        cb = np.ascontiguousarray(cb, np.uint32)
+        cap = (len(center_id) + len(center_desc) + len(center_seq) + 4 +
+               sum(len(n) + 3 for n in names) + sum(len(q) for q in queries) +
+               sum(int(sum(int(x) >> 4 for x in c)) for c in cigars) + 16)
+        out = np.zeros(cap, np.uint8)
+        n = self.lib.or_keep_length_fasta(center_id, len(center_id), center_desc, len(center_desc), center_seq,
+                                          len(center_seq), len(queries), qb, _ptr(qo), nb, _ptr(no), _ptr(cb),
+                                          _ptr(co), _ptr(out), cap)
+        assert n >= 0, n
+        return out[:n].tobytes()
+
+
+def _concat_bytes(items):
+    off = np.zeros(len(items) + 1, np.int64)
+    off[1:] = np.cumsum([len(x) for x in items]) if items else []
+    return b"".join(items), off
+
+
 class RefLib(_Base):
     """The compiled reference (same call shapes as Oracle)."""
 
@@ -340,6 +382,30 @@ class RefLib(_Base):
                                               _ptr(cig), cap)
         assert n >= 0
         return cig[:n].copy()
+
+    def read_fastx(self, path, max_records=1 << 30):
+        """seq_io::KseqReader records (id, desc, seq), or -2 when it throws."""
+        size = max(os.path.getsize(path), 1)
+        if path.endswith(".gz"):
+            import gzip
+            size = max(len(gzip.open(path).read()), 1)
+        cap = size + 1
+        bufs = [np.zeros(cap, np.uint8) for _ in range(3)]
+        offs = [np.zeros(size + 2, np.int64) for _ in range(3)]
+        L = self.lib
+        L.ref_read_fastx.argtypes = [C.c_char_p, C.c_int, _P, _P, _P, _P, _P, _P, C.c_int64]
+        n = L.ref_read_fastx(path.encode(), min(max_records, size + 1), _ptr(bufs[0]), _ptr(offs[0]),
+                             _ptr(bufs[1]), _ptr(offs[1]), _ptr(bufs[2]), _ptr(offs[2]), cap)
+        if n < 0:
+            return n
+        return [tuple(bufs[k][offs[k][i]:offs[k][i + 1]].tobytes() for k in range(3)) for i in range(n)]
+
+    def keep_length_msa(self, center_path, input_path, work_dir, out_path):
+        """halign4 -c --keep-length, one thread, copy MSA command (ref_keep_length_msa)."""
+        L = self.lib
+        L.ref_keep_length_msa.argtypes = [C.c_char_p] * 4
+        return L.ref_keep_length_msa(center_path.encode(), input_path.encode(), work_dir.encode(),
+                                     out_path.encode())
 
     def consensus_file(self, fasta, out_fa, out_json, seq_limit=0, batch=4096, threads=1,
                        aln_len=1 << 20):
