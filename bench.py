@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times), to scale HBM counters")
-    ap.add_argument("--workload", default="dp", choices=["dp", "consensus", "e2e", "serve"],
+    ap.add_argument("--workload", default="dp", choices=["dp", "consensus", "e2e", "serve", "msa"],
                     help="dp: the headline pairwise DP; consensus: column counting over rows in HBM; "
                          "e2e: the whole per-query path (--config 1, 3 or 4); serve: config 5 serving")
     ap.add_argument("--config", type=int, default=1, help="e2e workload configuration (1, 3, 4)")
@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--batch-queries", type=int, default=8192, help="serve: queries per batch at most")
     ap.add_argument("--loads", type=float, nargs="*", default=[0.5, 0.8],
                     help="serve: offered load as fractions of the measured capacity")
+    ap.add_argument("--msa-batch", type=int, default=0,
+                    help="msa: queries per chunk (0 = the library default, alignQueryToRef's 25600)")
     ap.add_argument("--kernel", default="auto",
                     help="auto | multiwave | WxS (ring kernel with W waves and S slots per lane)")
     args = ap.parse_args()
@@ -115,6 +117,8 @@ def main():
         return e2e_bench(args)
     if args.workload == "serve":
         return serve_bench(args)
+    if args.workload == "msa":
+        return msa_bench(args)
 
     import halign
     from halign import synth
@@ -265,6 +269,101 @@ def e2e_bench(args):
            "bases_per_s": round(nb / tw, 1), "device_ms_per_step": round(sum(x[3] for x in steps) / len(steps), 3),
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def msa_bench(args):
+    """halign4 -c center.fa --keep-length (one center, the copy MSA command),
+    file to file: read the query FASTA, align every query on the GPU, build
+    the keep-length rows and FASTA records on the device, write the final
+    FASTA (halign_msa_keep_length). A step is one whole run over the config's
+    query file (config 1: the reference's 1,000-query case; config 3: the
+    first --queries of the uneven batch). cpu_baseline: the same run by the
+    reference's own functions (oracle/_ref ref_keep_length_msa: readers,
+    RefAligner::globalAlign, SAM and FASTA writers, merge), one per thread
+    over a time-bounded share of the queries."""
+    import tempfile
+    import halign
+    from halign import synth
+    cfg = args.config
+    label, per_step, total = E2E[cfg]
+    need = min(total, args.queries or (1000 if cfg == 1 else 65536))
+    center = synth.center(cfg)
+    qb, off = synth.queries(cfg, 0, need)
+    d = tempfile.mkdtemp(prefix="halign_msa_")
+    cpath = os.path.join(d, "center.fa")
+    ipath = os.path.join(d, "queries.fa")
+    opath = os.path.join(d, "out.fa")
+    with open(cpath, "wb") as f:
+        f.write(b">center\n" + bytes(center) + b"\n")
+    qv = memoryview(qb if isinstance(qb, (bytes, bytearray)) else bytes(qb))
+    with open(ipath, "wb") as f:
+        for i in range(need):
+            f.write(b">q%d\n" % i)
+            f.write(qv[off[i]:off[i + 1]])
+            f.write(b"\n")
+    in_bytes = os.path.getsize(ipath)
+    log(f"msa config {cfg}: {need} queries, input {in_bytes / 1e6:.1f} MB in {d}")
+    ctx = halign.Context(0)
+    steps = []
+    for s in range(args.warmup + args.steps):
+        if os.path.exists(opath):   # a fresh output file, as a run normally writes
+            os.remove(opath)
+        t0 = time.perf_counter()
+        st = halign.msa_keep_length(ctx, cpath, ipath, opath, batch=args.msa_batch)
+        dt = time.perf_counter() - t0
+        if s >= args.warmup:
+            steps.append((dt, st))
+            log(f"step {s - args.warmup}: {st['records'] - 1} rows, {st['bytes'] / 1e6:.1f} MB written, "
+                f"{dt * 1e3:.1f} ms wall ({st['seconds'] * 1e3:.1f} in the call), "
+                f"{st['gpu_seconds'] * 1e3:.1f} ms in GPU calls, "
+                f"{st['input_wait_seconds'] * 1e3:.1f} ms waiting for input")
+    tw = sum(x[0] for x in steps)
+    out_bytes = steps[-1][1]["bytes"]
+    cpu = None
+    if not args.no_cpu:
+        from oracle_lib import REF_SO, RefLib
+        if os.path.exists(REF_SO):
+            R = RefLib()
+            threads = min(16, os.cpu_count() or 1)
+            # calibrate: one thread on 8 queries, then a share per thread for ~10 s
+            def run_share(k, a, b):
+                wd = os.path.join(d, f"ref{k}")
+                os.makedirs(wd, exist_ok=True)
+                ip = os.path.join(wd, "q.fa")
+                with open(ip, "wb") as f:
+                    for i in range(a, b):
+                        f.write(b">q%d\n" % i + bytes(qv[off[i]:off[i + 1]]) + b"\n")
+                t0 = time.perf_counter()
+                rc = R.keep_length_msa(cpath, ip, wd, os.path.join(wd, "out.fa"))
+                assert rc == 0, rc
+                return time.perf_counter() - t0
+            t8 = run_share(0, 0, min(8, need))
+            per = max(1, min(need, int(10.0 * min(8, need) / max(t8, 1e-6))))
+            shares = [((k * per) % need, min((k * per) % need + per, need)) for k in range(threads)]
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(threads) as ex:
+                list(ex.map(lambda k: run_share(k, *shares[k]), range(threads)))
+            dt = time.perf_counter() - t0
+            done = sum(b - a for a, b in shares)
+            cpu = {"value": round(done / dt, 2), "unit": "seqs/s", "cores": threads, "kind": "reference",
+                   "sample": f"{done} config-{cfg} queries in {threads} independent keep-length runs of "
+                             f"up to {per} queries (one run per thread), {dt:.1f}s"}
+            log(f"cpu baseline (reference): {cpu['value']} seqs/s")
+    out = {"metric": "seqs/s end-to-end keep-length MSA, FASTA file to FASTA file (halign4 -c --keep-length)",
+           "value": round(need * len(steps) / tw, 2), "unit": "seqs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(tw * 1e3 / len(steps), 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int8",
+           "data": f"synthetic (seeded, csrc/synth.c config {cfg}), written to FASTA files",
+           "config": {"workload": f"config {cfg}: {label}; keep-length FASTA file to file", "queries": need,
+                      "chunk_queries": args.msa_batch or 25600},
+           "output_mb_per_s": round(out_bytes * len(steps) / tw / 1e6, 1), "input_bytes": in_bytes,
+           "output_bytes": out_bytes,
+           "gpu_s_per_step": round(sum(x[1]["gpu_seconds"] for x in steps) / len(steps), 4),
+           "input_wait_s_per_step": round(sum(x[1]["input_wait_seconds"] for x in steps) / len(steps), 4),
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    import shutil
+    shutil.rmtree(d, ignore_errors=True)
 
 
 def cpu_timed(fn, n, threads, fixed=0, target_s=12.0):

@@ -557,6 +557,7 @@ int halign_init(int device, halign_ctx **out)
 void halign_free(halign_ctx *ctx)
 {
     if (!ctx) return;
+    if (ctx->msa_center) halign_center_free(ctx->msa_center);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->seq.release(); ctx->seqb.release(); ctx->jobs.release(); ctx->outs.release();
@@ -566,6 +567,7 @@ void halign_free(halign_ctx *ctx)
                            &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list})
         b->release();
     for (auto &cg : ctx->gcache) (void)hipGraphExecDestroy(cg.exec);
+    for (auto &b : ctx->msa_out) b.release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     for (int k = 0; k < halign_ctx::kAux; ++k) {

@@ -48,6 +48,33 @@ struct DevBuf {
     template <class T> T *as() const { return (T *)p; }
 };
 
+// Pinned host memory, grown by half again when it has to grow. The CPU's first
+// read of freshly allocated pinned pages is an order of magnitude slower than
+// later reads, so buffers that receive results chunk after chunk are kept.
+struct HostPinned {
+    char *p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t n)
+    {
+        if (n <= cap) return true;
+        const size_t want = std::max(n, cap + cap / 2);
+        release();
+        if (hipHostMalloc((void **)&p, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return false;
+        }
+        cap = want;
+        return true;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 }  // namespace hal
 
 struct halign_ctx {
@@ -79,6 +106,12 @@ struct halign_ctx {
     static constexpr size_t kGraphCache = 8;
     int64_t graph_captures = 0, graph_replays = 0;
     int seed_policy = 0;   // 0: wave form where it applies, 1: sequential form only
+    // keep-length file runs: one at a time per context, with two record slots
+    // and the last run's center kept for the next run
+    std::mutex msa_mu;
+    hal::HostPinned msa_out[2];
+    struct halign_center *msa_center = nullptr;
+    std::string msa_key;
 };
 
 namespace hal {

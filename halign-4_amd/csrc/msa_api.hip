@@ -9,11 +9,17 @@
 // on the device, the records landing in pinned host memory), and a writer
 // thread appends the previous chunk's records to the output. The reference
 // writes the rows of queries with an insertion before all others (it merges
-// the insertion SAM files first); the writer appends those straight to the
+// the insertion SAM files first); the writer puts those straight into the
 // output and spools the others to a temporary file next to it, appended at
-// the end.
+// the end. Writes are positioned (pwrite) and split over several threads, so
+// that copying into the page cache is not one core's memcpy.
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cerrno>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -36,34 +42,10 @@ using Clock = std::chrono::steady_clock;
 
 double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
-struct PinnedBuf {
-    char *p = nullptr;
-    size_t cap = 0;
-    ~PinnedBuf()
-    {
-        if (p) (void)hipHostFree(p);
-    }
-    bool ensure(size_t n)
-    {
-        if (n <= cap) return true;
-        const size_t want = std::max(n, cap + cap / 2);
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        if (hipHostMalloc((void **)&p, want, hipHostMallocDefault) != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            return false;
-        }
-        cap = want;
-        return true;
-    }
-};
-
 // One chunk's records ready for the writer: out.p[0 .. ins) first group,
 // out.p[ins .. len) the rest.
 struct Slot {
-    PinnedBuf out;
+    HostPinned *out = nullptr;
     int64_t len = 0, ins = 0;
     bool full = false;
 };
@@ -88,7 +70,60 @@ struct Shared {
     }
 };
 
-bool write_all(FILE *f, const char *p, size_t n) { return n == 0 || fwrite(p, 1, n, f) == n; }
+// n bytes at file offset off, in pieces of at least 4 MiB on up to 8 threads
+bool write_at(int fd, const char *p, size_t n, int64_t off)
+{
+    auto one = [fd](const char *q, size_t m, int64_t o) {
+        while (m > 0) {
+            const ssize_t w = pwrite(fd, q, m, (off_t)o);
+            if (w < 0 && errno == EINTR) continue;
+            if (w <= 0) return false;
+            q += w;
+            m -= (size_t)w;
+            o += w;
+        }
+        return true;
+    };
+    constexpr size_t kPiece = 4 << 20;
+    const int parts = (int)std::min<size_t>(8, (n + kPiece - 1) / kPiece);
+    if (parts <= 1) return one(p, n, off);
+    const size_t step = (n + parts - 1) / parts;
+    std::vector<std::thread> th;
+    std::vector<char> ok((size_t)parts, 0);
+    for (int k = 0; k < parts; ++k) {
+        const size_t a = (size_t)k * step, b = std::min(n, a + step);
+        th.emplace_back([&, k, a, b] { ok[(size_t)k] = one(p + a, b - a, off + (int64_t)a); });
+    }
+    for (auto &t : th) t.join();
+    for (char v : ok)
+        if (!v) return false;
+    return true;
+}
+
+// bytes [0, n) of file `src` to file `dst` at offset off
+bool append_file(int src, int dst, int64_t n, int64_t off)
+{
+    loff_t si = 0, di = (loff_t)off;
+    while (n > 0) {
+        const ssize_t c = copy_file_range(src, &si, dst, &di, (size_t)n, 0);
+        if (c > 0) {
+            n -= c;
+            continue;
+        }
+        if (c < 0 && errno == EINTR) continue;
+        break;   // not supported between these files: copy through memory
+    }
+    std::vector<char> buf(8 << 20);
+    while (n > 0) {
+        const ssize_t r = pread(src, buf.data(), (size_t)std::min<int64_t>(n, (int64_t)buf.size()), (off_t)si);
+        if (r <= 0) return false;
+        if (!write_at(dst, buf.data(), (size_t)r, di)) return false;
+        si += r;
+        di += r;
+        n -= r;
+    }
+    return true;
+}
 
 }  // namespace
 
@@ -96,7 +131,9 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
                                       const char *output_path, int32_t batch, halign_msa_stats *stats)
 {
     if (!ctx || !center_path || !input_path || !output_path) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> run_lock(ctx->msa_mu);
     const auto t0 = Clock::now();
+    PhaseClock sclk("msa setup");
     if (batch <= 0) batch = 25600;   // alignQueryToRef's batch in halign4 (include/align.h:432-436)
 
     // the center file: one record (more is the multi-reference path)
@@ -122,37 +159,44 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
                                         counts.data(), &cons[0], &nseq, nullptr);
         if (rc != HALIGN_OK) return rc;
     }
-    halign_center *center = nullptr;
-    int rc = halign_center_create_indexed(ctx, cons.data(), (int32_t)cons.size(), cseq.data(), (int32_t)cseq.size(),
-                                          &center);
-    if (rc != HALIGN_OK) return rc;
-    struct CenterGuard {
-        halign_center *c;
-        ~CenterGuard() { halign_center_free(c); }
-    } cg{center};
+    // the context keeps the last run's center (and its batch buffers) for the
+    // next run against the same center file
+    int rc = HALIGN_OK;
+    const std::string key = cons + '\n' + cseq;
+    if (!ctx->msa_center || ctx->msa_key != key) {
+        if (ctx->msa_center) halign_center_free(ctx->msa_center);
+        ctx->msa_center = nullptr;
+        ctx->msa_key.clear();
+        rc = halign_center_create_indexed(ctx, cons.data(), (int32_t)cons.size(), cseq.data(), (int32_t)cseq.size(),
+                                          &ctx->msa_center);
+        if (rc != HALIGN_OK) return rc;
+        ctx->msa_key = key;
+    }
+    halign_center *center = ctx->msa_center;
+    sclk.mark("center and index");
 
-    FILE *out = fopen(output_path, "wb");
-    if (!out) return fail(ctx, HALIGN_E_IO, (std::string("cannot create ") + output_path).c_str());
+    const int out = open(output_path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (out < 0) return fail(ctx, HALIGN_E_IO, (std::string("cannot create ") + output_path).c_str());
     const std::string rest_path = std::string(output_path) + ".rest.tmp";
-    FILE *rest = fopen(rest_path.c_str(), "w+b");
-    if (!rest) {
-        fclose(out);
+    const int rest = open(rest_path.c_str(), O_RDWR | O_CREAT | O_TRUNC, 0600);
+    if (rest < 0) {
+        close(out);
         return fail(ctx, HALIGN_E_IO, (std::string("cannot create ") + rest_path).c_str());
     }
     struct Files {
-        FILE *a, *b;
+        int a, b;
         std::string bp;
         ~Files()
         {
-            if (a) fclose(a);
-            if (b) fclose(b);
-            std::remove(bp.c_str());
+            if (a >= 0) close(a);
+            close(b);
+            unlink(bp.c_str());
         }
     } files{out, rest, rest_path};
-    setvbuf(out, nullptr, _IOFBF, 8 << 20);
-    setvbuf(rest, nullptr, _IOFBF, 8 << 20);
+    sclk.mark("output files opened");
 
     // writeConsensusAndReferences (:541-561): the center record, cleaned
+    int64_t out_off = 0, rest_off = 0;
     {
         std::string rec = ">" + cid;
         if (!cdesc.empty()) rec += " " + cdesc;
@@ -161,7 +205,8 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
         rec.resize(h + cseq.size());
         for (size_t i = 0; i < cseq.size(); ++i) rec[h + i] = clean_base((unsigned char)cseq[i]);
         rec += "\n";
-        if (!write_all(out, rec.data(), rec.size())) return fail(ctx, HALIGN_E_IO, "write failed");
+        if (!write_at(out, rec.data(), rec.size(), 0)) return fail(ctx, HALIGN_E_IO, "write failed");
+        out_off = (int64_t)rec.size();
     }
 
     Shared sh;
@@ -206,6 +251,8 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
     // writer: slots in the order the GPU loop fills them; slot k's first
     // group to the output, the rest to the spool
     Slot slots[2];
+    slots[0].out = &ctx->msa_out[0];
+    slots[1].out = &ctx->msa_out[1];
     bool producer_done = false;
     std::thread writer([&] {
         for (int k = 0;; k ^= 1) {
@@ -215,8 +262,12 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
                 if (sh.stop || !slots[k].full) return;
             }
             Slot &s = slots[k];
-            const bool ok = write_all(out, s.out.p, (size_t)s.ins) &&
-                            write_all(rest, s.out.p + s.ins, (size_t)(s.len - s.ins));
+            PhaseClock wclk("msa writer");
+            const bool ok = write_at(out, s.out->p, (size_t)s.ins, out_off) &&
+                            write_at(rest, s.out->p + s.ins, (size_t)(s.len - s.ins), rest_off);
+            wclk.mark("chunk written");
+            out_off += s.ins;
+            rest_off += s.len - s.ins;
             if (!ok) {
                 sh.set_error(HALIGN_E_IO, "write failed");
                 return;
@@ -251,15 +302,17 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
             ++kept;
         }
         Slot &s = slots[k];
-        if (!s.out.ensure((size_t)std::max<int64_t>(need, 1))) {
+        if (!s.out->ensure((size_t)std::max<int64_t>(need, 1))) {
             sh.set_error(HALIGN_E_NOMEM, "pinned host buffer");
             break;
         }
         int32_t n_ins = 0;
         const auto g0 = Clock::now();
+        PhaseClock gclk("msa gpu");
         rc = halign_center_align_fasta(center, ch->n, ch->seqs.data(), ch->seq_off.data(), ch->ids.data(),
-                                       ch->id_off.data(), s.out.p, need, &s.len, &s.ins, &n_ins, nullptr);
+                                       ch->id_off.data(), s.out->p, need, &s.len, &s.ins, &n_ins, nullptr);
         gpu_s += secs(g0, Clock::now());
+        gclk.mark("chunk aligned");
         if (rc != HALIGN_OK) {
             sh.set_error(rc, ctx->err);
             break;
@@ -285,16 +338,16 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
     if (sh.err_code != HALIGN_OK) return fail(ctx, sh.err_code, sh.err.c_str());
 
     // mergeAlignedResults' order: the other group after the insertion group
-    if (fflush(rest) != 0 || fseek(rest, 0, SEEK_SET) != 0) return fail(ctx, HALIGN_E_IO, "spool file");
-    std::vector<char> buf(8 << 20);
-    for (;;) {
-        const size_t r = fread(buf.data(), 1, buf.size(), rest);
-        if (r == 0) break;
-        if (!write_all(out, buf.data(), r)) return fail(ctx, HALIGN_E_IO, "write failed");
+    PhaseClock aclk("msa");
+    aclk.mark("all chunks written");
+    if (!append_file(rest, out, rest_off, out_off)) return fail(ctx, HALIGN_E_IO, "appending the spool file failed");
+    aclk.mark("spool appended");
+    if (close(out) != 0) {
+        files.a = -1;
+        return fail(ctx, HALIGN_E_IO, "closing the output failed");
     }
-    if (ferror(rest)) return fail(ctx, HALIGN_E_IO, "spool file read");
-    if (fflush(out) != 0) return fail(ctx, HALIGN_E_IO, "write failed");
-    st.bytes = (int64_t)ftell(out);
+    files.a = -1;
+    st.bytes = out_off + rest_off;
     st.seconds = secs(t0, Clock::now());
     st.gpu_seconds = gpu_s;
     st.input_wait_seconds = wait_s;

@@ -234,6 +234,11 @@ void halign_fastx_close(halign_fastx *f);
  * reference's order with one thread). Reading the next chunk, aligning one on
  * the GPU and writing the previous one's records overlap. A center file with
  * more than one record is the multi-reference path: HALIGN_E_UNSUPPORTED.
+ * Runs on one context take turns; the context keeps two pinned host buffers
+ * of one chunk's records and the last run's center with its index until
+ * halign_free, so later runs against the same center start warm. The rows of
+ * queries without an insertion are spooled to output_path + ".rest.tmp"
+ * (removed at the end).
  * stats (optional): {records written, insertion records, bytes written,
  * seconds total, seconds of GPU calls, seconds the GPU waited for input}. */
 typedef struct {
