@@ -564,10 +564,13 @@ void halign_free(halign_ctx *ctx)
     ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
     ctx->crows.release(); ctx->ccounts.release(); ctx->ccons.release();
     for (hal::DevBuf *b : {&ctx->s_seqs, &ctx->s_off, &ctx->s_woff, &ctx->s_moff, &ctx->s_codes, &ctx->s_nmask,
-                           &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list})
+                           &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list,
+                           &ctx->m_seqs, &ctx->m_off, &ctx->m_koff, &ctx->m_keys, &ctx->m_keys2, &ctx->m_nkeys,
+                           &ctx->m_tmp, &ctx->m_sel})
         b->release();
     for (auto &cg : ctx->gcache) (void)hipGraphExecDestroy(cg.exec);
     for (auto &b : ctx->msa_out) b.release();
+    ctx->msa_tmp.release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     for (int k = 0; k < halign_ctx::kAux; ++k) {

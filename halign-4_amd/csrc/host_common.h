@@ -106,10 +106,13 @@ struct halign_ctx {
     static constexpr size_t kGraphCache = 8;
     int64_t graph_captures = 0, graph_replays = 0;
     int seed_policy = 0;   // 0: wave form where it applies, 1: sequential form only
+    // mash sketches: sequences, offsets, key ranges, keys, sort output, counts,
+    // sort storage, selection outputs
+    hal::DevBuf m_seqs, m_off, m_koff, m_keys, m_keys2, m_nkeys, m_tmp, m_sel;
     // keep-length file runs: one at a time per context, with two record slots
     // and the last run's center kept for the next run
     std::mutex msa_mu;
-    hal::HostPinned msa_out[2];
+    hal::HostPinned msa_out[2], msa_tmp;
     struct halign_center *msa_center = nullptr;
     std::string msa_key;
 };

@@ -428,10 +428,11 @@ __device__ int64_t row_scan(const uint32_t *__restrict__ c, int64_t n, int64_t q
 }
 
 __device__ void row_fill(const uint32_t *__restrict__ c, int64_t n, const int64_t *__restrict__ w,
-                         const char *__restrict__ qs, int64_t qlen, int64_t limit, char *__restrict__ row)
+                         const char *__restrict__ qs, int64_t qlen, int64_t limit, char *__restrict__ row,
+                         const int32_t *__restrict__ map = nullptr)
 {
     if (n == 0) {
-        for (int64_t p = threadIdx.x; p < qlen && p < limit; p += 256) row[p] = qs[p];
+        for (int64_t p = threadIdx.x; p < qlen && p < limit; p += 256) row[map ? map[p] : p] = qs[p];
         return;
     }
     for (int64_t p = threadIdx.x; p < limit; p += 256) {
@@ -449,7 +450,7 @@ __device__ void row_fill(const uint32_t *__restrict__ c, int64_t n, const int64_
                 ch = qs[(w[lo] & 0xffffffffll) + (p - start)];
             }
         }
-        row[p] = ch;
+        row[map ? map[p] : p] = ch;
     }
 }
 
@@ -491,7 +492,11 @@ __global__ __launch_bounds__(256) void project_rows(const uint32_t *__restrict__
 //    exclusive scan over the 2n + 1 entries gives every record's byte offset;
 //    counts[0] += queries with an insertion, counts[1] += rows whose length
 //    is not the center's (the reference throws on those, :590-596, :657-663);
-//  * write_records: header, row (always rlen bytes) and newline at the offset.
+//  * write_records: header, row and newline at the offset. The row has the
+//    center's length, or with a column map (several references: the row of
+//    the chosen reference spread over the columns of its aligned record,
+//    padQueryToRefByCigar with that record's M/D CIGAR, ref_aligner.cpp:
+//    190-196, 486-496) `width` bytes, '-' where the aligned record has a gap.
 // SamReader::next skips SAM lines that start with '@' (src/utils/seq_io.cpp:
 // 433-436), so a query named "@..." never reaches the merged FASTA
 __device__ __forceinline__ bool is_dropped(const char *names, const int64_t *name_off, int q)
@@ -520,7 +525,8 @@ __global__ __launch_bounds__(256) void record_sizes(const char *__restrict__ nam
                                                     const int64_t *__restrict__ name_off,
                                                     const uint8_t *__restrict__ has_ins,
                                                     const int64_t *__restrict__ row_len, int nq, int64_t rlen,
-                                                    int64_t *__restrict__ sz, unsigned long long *__restrict__ counts)
+                                                    int64_t width, int64_t *__restrict__ sz,
+                                                    unsigned long long *__restrict__ counts)
 {
     __shared__ unsigned int n_ins, n_bad;
     if (threadIdx.x == 0) n_ins = n_bad = 0;
@@ -529,7 +535,7 @@ __global__ __launch_bounds__(256) void record_sizes(const char *__restrict__ nam
     if (q < nq) {
         const int64_t nl = name_off[q + 1] - name_off[q];
         const bool keep = !is_dropped(names, name_off, q);
-        const int64_t size = keep ? nl + rlen + 3 : 0;
+        const int64_t size = keep ? nl + width + 3 : 0;
         const bool ins = has_ins[q] != 0;
         sz[q] = ins ? size : 0;
         sz[nq + q] = ins ? 0 : size;
@@ -550,6 +556,7 @@ __global__ __launch_bounds__(256) void write_records(const uint32_t *__restrict_
                                                      const int64_t *__restrict__ name_off,
                                                      const uint8_t *__restrict__ has_ins,
                                                      const int64_t *__restrict__ rec_off, int nq, int64_t rlen,
+                                                     const int32_t *__restrict__ map, int64_t width,
                                                      char *__restrict__ out)
 {
     const int q = blockIdx.x;
@@ -559,14 +566,22 @@ __global__ __launch_bounds__(256) void write_records(const uint32_t *__restrict_
     const char *nm = names + name_off[q];
     for (int64_t p = threadIdx.x; p < nl + 2; p += 256) rec[p] = p == 0 ? '>' : (p == nl + 1 ? '\n' : nm[p - 1]);
     char *row = rec + nl + 2;
-    if (threadIdx.x == 0) row[rlen] = '\n';
+    if (threadIdx.x == 0) row[width] = '\n';
     const int64_t n = coff[q + 1] - coff[q];
     const int64_t qlen = qoff[q + 1] - qoff[q];
-    // a row shorter than the center (reported through record_sizes) is padded
-    // so that every byte of the record is written
     const int64_t have = n == 0 ? qlen : ws[coff[q] + n - 1] >> 32;
-    for (int64_t p = (have < rlen ? have : rlen) + threadIdx.x; p < rlen; p += 256) row[p] = '-';
-    row_fill(cig + coff[q], n, ws + coff[q], qseq + qoff[q], qlen, have < rlen ? have : rlen, row);
+    const int64_t fill = have < rlen ? have : rlen;
+    if (map) {
+        // padQueryToRefByCigar with the reference's aligned row: its gap
+        // columns are '-', its bases take the row's characters
+        for (int64_t p = threadIdx.x; p < width; p += 256) row[p] = '-';
+        __syncthreads();
+    } else {
+        // a row shorter than the center (reported through record_sizes) is
+        // padded so that every byte of the record is written
+        for (int64_t p = fill + threadIdx.x; p < rlen; p += 256) row[p] = '-';
+    }
+    row_fill(cig + coff[q], n, ws + coff[q], qseq + qoff[q], qlen, fill, row, map);
 }
 
 hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
@@ -580,9 +595,9 @@ hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const c
 }
 
 hipError_t records_launch(int phase, const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
-                          const char *names, const int64_t *name_off, int nq, int64_t rlen, int64_t *row_len,
-                          int64_t *ws, uint8_t *has_ins, int64_t *sz, unsigned long long *counts, char *out,
-                          hipStream_t stream)
+                          const char *names, const int64_t *name_off, int nq, int64_t rlen, const int32_t *map,
+                          int64_t width, int64_t *row_len, int64_t *ws, uint8_t *has_ins, int64_t *sz,
+                          unsigned long long *counts, char *out, hipStream_t stream)
 {
     // phase 0: rows_scan; 1: record_sizes into sz (2n + 1 entries); 2:
     // write_records with sz = the exclusive scan of those sizes
@@ -591,10 +606,10 @@ hipError_t records_launch(int phase, const uint32_t *cig, const int64_t *coff, c
         hipLaunchKernelGGL(rows_scan, dim3((unsigned)nq), dim3(256), 0, stream, cig, coff, qoff, row_len, ws, has_ins);
     else if (phase == 1)
         hipLaunchKernelGGL(record_sizes, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, names, name_off,
-                           has_ins, row_len, nq, rlen, sz, counts);
+                           has_ins, row_len, nq, rlen, width, sz, counts);
     else
         hipLaunchKernelGGL(write_records, dim3((unsigned)nq), dim3(256), 0, stream, cig, coff, ws, qseq, qoff, names,
-                           name_off, has_ins, sz, nq, rlen, out);
+                           name_off, has_ins, sz, nq, rlen, map, width, out);
     return hipGetLastError();
 }
 

@@ -33,6 +33,8 @@
 
 #include "fastx.h"
 #include "host_common.h"
+#include "mash_host.h"
+#include "pipeline_host.h"
 
 using namespace hal;
 
@@ -41,6 +43,35 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+// The references of a run. One center: RefAligner aligns to the consensus of
+// the (aligned) center file and seeds with the record's minimizers. Several:
+// each query goes to the reference with the largest sketch Jaccard, aligned
+// to that reference's own sequence, its row spread over the columns of the
+// reference's aligned record (all records one width).
+struct Refs {
+    bool multi = false;
+    halign_center *single = nullptr;         // kept by the context
+    std::vector<halign_center *> centers;    // several references, owned
+    DevBuf rsk, rcnt, qsk, qcnt;             // reference and query sketches
+    int64_t width = 0;                       // row length of every record
+    ~Refs()
+    {
+        for (auto *c : centers) halign_center_free(c);
+        rsk.release();
+        rcnt.release();
+        qsk.release();
+        qcnt.release();
+    }
+};
+
+constexpr int kSketchK = 15, kSketchSize = 2000;   // RefAligner's kmer_size, sketch_size (include/config.hpp)
+constexpr uint32_t kSketchSeed = 42;               // RefAligner's random_seed (include/align.h)
+
+bool dropped_name(const RecordChunk &ch, int32_t i)
+{
+    return ch.id_off[i + 1] > ch.id_off[i] && ch.ids[(size_t)ch.id_off[i]] == '@';
+}
 
 // One chunk's records ready for the writer: out.p[0 .. ins) first group,
 // out.p[ins .. len) the rest.
@@ -127,8 +158,105 @@ bool append_file(int src, int dst, int64_t n, int64_t off)
 
 }  // namespace
 
+namespace {
+
+// a chunk through the GPU into slot memory: records of kept queries,
+// insertion group first, each group in input order
+int align_chunk(halign_ctx *ctx, Refs &R, const RecordChunk &ch, HostPinned &out, int64_t &len, int64_t &ins,
+                int32_t &n_ins, int32_t &kept)
+{
+    int64_t need = 0;
+    kept = 0;
+    for (int32_t i = 0; i < ch.n; ++i) {
+        if (dropped_name(ch, i)) continue;   // no record (its SAM line is skipped)
+        need += ch.id_off[i + 1] - ch.id_off[i] + R.width + 3;
+        ++kept;
+    }
+    if (!out.ensure((size_t)std::max<int64_t>(need, 1))) return fail(ctx, HALIGN_E_NOMEM, "pinned host buffer");
+    if (!R.multi)
+        return center_align_fasta(R.single, ch.n, ch.seqs.data(), ch.seq_off.data(), ch.ids.data(),
+                                  ch.id_off.data(), out.p, need, &len, &ins, &n_ins, nullptr, nullptr);
+    // several references: the choice per query (alignOneQueryToRef :228-244)
+    std::vector<int32_t> best, num, den;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        int rc = mash_sketch_to_device(ctx, ch.n, ch.seqs.data(), ch.seq_off.data(), kSketchK, kSketchSize, true,
+                                       kSketchSeed, R.qsk, R.qcnt);
+        if (rc == HALIGN_OK)
+            rc = mash_select_device(ctx, ch.n, R.qsk, R.qcnt, kSketchSize, (int)R.centers.size(), R.rsk, R.rcnt,
+                                    kSketchSize, best, num, den);
+        if (rc != HALIGN_OK) return rc;
+    }
+    const int nr = (int)R.centers.size();
+    std::vector<std::vector<int32_t>> members((size_t)nr);
+    for (int32_t i = 0; i < ch.n; ++i) members[(size_t)best[i]].push_back(i);
+    if (!ctx->msa_tmp.ensure((size_t)std::max<int64_t>(need, 1))) return fail(ctx, HALIGN_E_NOMEM, "pinned host buffer");
+    std::vector<int64_t> src((size_t)ch.n, 0);
+    std::vector<uint8_t> hins((size_t)ch.n, 0);
+    int64_t toff = 0;
+    n_ins = 0;
+    for (int r = 0; r < nr; ++r) {
+        const auto &m = members[(size_t)r];
+        if (m.empty()) continue;
+        std::string blob, names;
+        std::vector<int64_t> off{0}, noff{0};
+        for (int32_t i : m) {
+            blob.append(ch.seqs, (size_t)ch.seq_off[i], (size_t)(ch.seq_off[i + 1] - ch.seq_off[i]));
+            names.append(ch.ids, (size_t)ch.id_off[i], (size_t)(ch.id_off[i + 1] - ch.id_off[i]));
+            off.push_back((int64_t)blob.size());
+            noff.push_back((int64_t)names.size());
+        }
+        std::vector<uint8_t> flags(m.size());
+        int64_t l = 0, ib = 0;
+        int32_t ni = 0;
+        const int rc = center_align_fasta(R.centers[(size_t)r], (int32_t)m.size(), blob.data(), off.data(),
+                                          names.data(), noff.data(), ctx->msa_tmp.p + toff, need - toff, &l, &ib, &ni,
+                                          nullptr, flags.data());
+        if (rc != HALIGN_OK) return rc;
+        // where each member's record landed in this reference's output
+        int64_t a = toff, b = toff + ib;
+        for (size_t j = 0; j < m.size(); ++j) {
+            const int32_t i = m[j];
+            hins[(size_t)i] = flags[j];
+            if (dropped_name(ch, i)) continue;
+            const int64_t sz = ch.id_off[i + 1] - ch.id_off[i] + R.width + 3;
+            if (flags[j]) {
+                src[(size_t)i] = a;
+                a += sz;
+            } else {
+                src[(size_t)i] = b;
+                b += sz;
+            }
+        }
+        toff += l;
+        n_ins += ni;
+    }
+    // the chunk's records in input order, insertion group first
+    int64_t d = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int32_t i = 0; i < ch.n; ++i) {
+            if (dropped_name(ch, i) || (hins[(size_t)i] != 0) != (pass == 0)) continue;
+            const int64_t sz = ch.id_off[i + 1] - ch.id_off[i] + R.width + 3;
+            std::memcpy(out.p + d, ctx->msa_tmp.p + src[(size_t)i], (size_t)sz);
+            d += sz;
+        }
+        if (pass == 0) ins = d;
+    }
+    len = d;
+    return HALIGN_OK;
+}
+
+}  // namespace
+
 extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, const char *input_path,
                                       const char *output_path, int32_t batch, halign_msa_stats *stats)
+{
+    return halign_msa_keep_length_refs(ctx, center_path, nullptr, input_path, output_path, batch, stats);
+}
+
+extern "C" int halign_msa_keep_length_refs(halign_ctx *ctx, const char *center_path, const char *aligned_path,
+                                           const char *input_path, const char *output_path, int32_t batch,
+                                           halign_msa_stats *stats)
 {
     if (!ctx || !center_path || !input_path || !output_path) return HALIGN_E_ARG;
     std::lock_guard<std::mutex> run_lock(ctx->msa_mu);
@@ -136,44 +264,92 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
     PhaseClock sclk("msa setup");
     if (batch <= 0) batch = 25600;   // alignQueryToRef's batch in halign4 (include/align.h:432-436)
 
-    // the center file: one record (more is the multi-reference path)
-    std::string cid, cdesc, cseq, e;
-    {
+    // the center file's records, and the aligned records the MSA command made
+    // of them (the copy command: the same file)
+    auto read_all = [&](const char *path, RecordChunk &rc) -> int {
         FastxReader rd;
-        if (!rd.open(center_path, e)) return fail(ctx, HALIGN_E_IO, e.c_str());
-        std::string id2, desc2, seq2;
-        const int r = rd.next(cid, cdesc, cseq, e);
+        std::string e, id, desc, seq;
+        if (!rd.open(path, e)) return fail(ctx, HALIGN_E_IO, e.c_str());
+        int r;
+        while ((r = rd.next(id, desc, seq, e)) > 0) rc.add(id, desc, seq);
         if (r < 0) return fail(ctx, HALIGN_E_IO, e.c_str());
-        if (r == 0) return fail(ctx, HALIGN_E_ARG, "center file has no record");
-        const int r2 = rd.next(id2, desc2, seq2, e);
-        if (r2 < 0) return fail(ctx, HALIGN_E_IO, e.c_str());
-        if (r2 > 0) return fail(ctx, HALIGN_E_UNSUPPORTED, "center file has more than one record");
-    }
-    if (cseq.empty()) return fail(ctx, HALIGN_E_ARG, "center sequence is empty");
-    // RefAligner aligns to the consensus of the (copied) center file
-    std::string cons(cseq.size(), 'A');
-    {
-        std::vector<uint32_t> counts(cseq.size() * 7);
+        return HALIGN_OK;
+    };
+    RecordChunk refs, aligned;
+    int rc = read_all(center_path, refs);
+    if (rc != HALIGN_OK) return rc;
+    if (refs.n == 0) return fail(ctx, HALIGN_E_ARG, "center file has no record");
+    rc = read_all(aligned_path ? aligned_path : center_path, aligned);
+    if (rc != HALIGN_OK) return rc;
+    if (aligned.n != refs.n) return fail(ctx, HALIGN_E_ARG, "aligned file and center file differ in record count");
+    auto seq_of = [](const RecordChunk &c, int i) {
+        return std::string(c.seqs, (size_t)c.seq_off[i], (size_t)(c.seq_off[i + 1] - c.seq_off[i]));
+    };
+    auto id_of = [](const RecordChunk &c, int i) {
+        return std::string(c.ids, (size_t)c.id_off[i], (size_t)(c.id_off[i + 1] - c.id_off[i]));
+    };
+    Refs R;
+    if (refs.n == 1) {
+        // the consensus of the aligned file is the center aligned to (:47-60)
+        const int64_t alen = aligned.seq_off[1];
+        if (alen <= 0) return fail(ctx, HALIGN_E_ARG, "center sequence is empty");
+        std::string cons((size_t)alen, 'A');
+        std::vector<uint32_t> counts((size_t)alen * 7);
         uint64_t nseq = 0;
-        const int rc = halign_consensus(ctx, cseq.data(), 1, (int64_t)cseq.size(), (int64_t)cseq.size(), 0, 4096, 0,
-                                        counts.data(), &cons[0], &nseq, nullptr);
+        rc = halign_consensus(ctx, aligned.seqs.data(), 1, alen, alen, 0, 4096, 0, counts.data(), &cons[0], &nseq,
+                              nullptr);
+        if (rc != HALIGN_OK) return rc;
+        // the context keeps the last run's center (and its batch buffers) for
+        // the next run against the same center file
+        const std::string cseq = seq_of(refs, 0);
+        const std::string key = cons + '\n' + cseq;
+        if (!ctx->msa_center || ctx->msa_key != key) {
+            if (ctx->msa_center) halign_center_free(ctx->msa_center);
+            ctx->msa_center = nullptr;
+            ctx->msa_key.clear();
+            rc = halign_center_create_indexed(ctx, cons.data(), (int32_t)cons.size(), cseq.data(),
+                                              (int32_t)cseq.size(), &ctx->msa_center);
+            if (rc != HALIGN_OK) return rc;
+            ctx->msa_key = key;
+        }
+        R.single = ctx->msa_center;
+        R.width = (int64_t)cons.size();
+    } else {
+        // several references: one center each, a column map from its aligned
+        // record (looked up by name, the last record of a name winning, as the
+        // reference's map does), sketches resident
+        R.multi = true;
+        R.width = aligned.seq_off[1] - aligned.seq_off[0];
+        for (int i = 1; i < aligned.n; ++i)
+            if (aligned.seq_off[i + 1] - aligned.seq_off[i] != R.width)
+                return fail(ctx, HALIGN_E_ARG, "aligned references differ in length");
+        for (int r = 0; r < refs.n; ++r) {
+            const std::string seq = seq_of(refs, r), id = id_of(refs, r);
+            int a = -1;
+            for (int j = 0; j < aligned.n; ++j)
+                if (id_of(aligned, j) == id) a = j;
+            if (a < 0) return fail(ctx, HALIGN_E_ARG, ("reference '" + id + "' not found in the aligned file").c_str());
+            if (seq.empty()) return fail(ctx, HALIGN_E_ARG, "a reference sequence is empty");
+            std::vector<int32_t> map;
+            const std::string arow = seq_of(aligned, a);
+            for (int64_t c = 0; c < (int64_t)arow.size(); ++c)
+                if (arow[(size_t)c] != '-') map.push_back((int32_t)c);
+            if (map.size() != seq.size())
+                return fail(ctx, HALIGN_E_ARG, ("aligned record of '" + id + "' does not hold its bases").c_str());
+            halign_center *c = nullptr;
+            rc = halign_center_create_indexed(ctx, seq.data(), (int32_t)seq.size(), seq.data(), (int32_t)seq.size(),
+                                              &c);
+            if (rc != HALIGN_OK) return rc;
+            R.centers.push_back(c);
+            rc = center_set_columns(c, map.data(), R.width);
+            if (rc != HALIGN_OK) return rc;
+        }
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        rc = mash_sketch_to_device(ctx, refs.n, refs.seqs.data(), refs.seq_off.data(), kSketchK, kSketchSize, true,
+                                   kSketchSeed, R.rsk, R.rcnt);
         if (rc != HALIGN_OK) return rc;
     }
-    // the context keeps the last run's center (and its batch buffers) for the
-    // next run against the same center file
-    int rc = HALIGN_OK;
-    const std::string key = cons + '\n' + cseq;
-    if (!ctx->msa_center || ctx->msa_key != key) {
-        if (ctx->msa_center) halign_center_free(ctx->msa_center);
-        ctx->msa_center = nullptr;
-        ctx->msa_key.clear();
-        rc = halign_center_create_indexed(ctx, cons.data(), (int32_t)cons.size(), cseq.data(), (int32_t)cseq.size(),
-                                          &ctx->msa_center);
-        if (rc != HALIGN_OK) return rc;
-        ctx->msa_key = key;
-    }
-    halign_center *center = ctx->msa_center;
-    sclk.mark("center and index");
+    sclk.mark("centers and indexes");
 
     const int out = open(output_path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
     if (out < 0) return fail(ctx, HALIGN_E_IO, (std::string("cannot create ") + output_path).c_str());
@@ -195,23 +371,30 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
     } files{out, rest, rest_path};
     sclk.mark("output files opened");
 
-    // writeConsensusAndReferences (:541-561): the center record, cleaned
+    // writeConsensusAndReferences (:541-561): the aligned file's records, cleaned
     int64_t out_off = 0, rest_off = 0;
     {
-        std::string rec = ">" + cid;
-        if (!cdesc.empty()) rec += " " + cdesc;
-        rec += "\n";
-        const size_t h = rec.size();
-        rec.resize(h + cseq.size());
-        for (size_t i = 0; i < cseq.size(); ++i) rec[h + i] = clean_base((unsigned char)cseq[i]);
-        rec += "\n";
-        if (!write_at(out, rec.data(), rec.size(), 0)) return fail(ctx, HALIGN_E_IO, "write failed");
-        out_off = (int64_t)rec.size();
+        std::string recs;
+        for (int i = 0; i < aligned.n; ++i) {
+            recs += '>';
+            recs.append(aligned.ids, (size_t)aligned.id_off[i], (size_t)(aligned.id_off[i + 1] - aligned.id_off[i]));
+            if (aligned.desc_off[i + 1] > aligned.desc_off[i]) {
+                recs += ' ';
+                recs.append(aligned.descs, (size_t)aligned.desc_off[i],
+                            (size_t)(aligned.desc_off[i + 1] - aligned.desc_off[i]));
+            }
+            recs += '\n';
+            for (int64_t p = aligned.seq_off[i]; p < aligned.seq_off[i + 1]; ++p)
+                recs += clean_base((unsigned char)aligned.seqs[(size_t)p]);
+            recs += '\n';
+        }
+        if (!write_at(out, recs.data(), recs.size(), 0)) return fail(ctx, HALIGN_E_IO, "write failed");
+        out_off = (int64_t)recs.size();
     }
 
     Shared sh;
     halign_msa_stats st{};
-    st.records = 1;
+    st.records = aligned.n;
     st.bytes = 0;
 
     // reader: chunks of `batch` records, at most two parsed ahead
@@ -293,24 +476,11 @@ extern "C" int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, 
             sh.cv.wait(lk, [&] { return sh.stop || !slots[k].full; });   // the writer is done with slot k
             if (sh.stop) break;
         }
-        int64_t need = 0;
-        int32_t kept = 0;
-        for (int32_t i = 0; i < ch->n; ++i) {
-            const int64_t nl = ch->id_off[i + 1] - ch->id_off[i];
-            if (nl > 0 && ch->ids[(size_t)ch->id_off[i]] == '@') continue;   // no record (its SAM line is skipped)
-            need += nl + (int64_t)cons.size() + 3;
-            ++kept;
-        }
         Slot &s = slots[k];
-        if (!s.out->ensure((size_t)std::max<int64_t>(need, 1))) {
-            sh.set_error(HALIGN_E_NOMEM, "pinned host buffer");
-            break;
-        }
-        int32_t n_ins = 0;
+        int32_t n_ins = 0, kept = 0;
         const auto g0 = Clock::now();
         PhaseClock gclk("msa gpu");
-        rc = halign_center_align_fasta(center, ch->n, ch->seqs.data(), ch->seq_off.data(), ch->ids.data(),
-                                       ch->id_off.data(), s.out->p, need, &s.len, &s.ins, &n_ins, nullptr);
+        rc = align_chunk(ctx, R, *ch, *s.out, s.len, s.ins, n_ins, kept);
         gpu_s += secs(g0, Clock::now());
         gclk.mark("chunk aligned");
         if (rc != HALIGN_OK) {

@@ -18,6 +18,7 @@
 
 #include "dp_host.h"
 #include "pipeline_common.h"
+#include "pipeline_host.h"
 
 namespace hal {
 
@@ -56,9 +57,9 @@ hipError_t mm2_assemble_launch(const SegRec *segs, const int64_t *soff, const in
                                const int32_t *ncig, const int64_t *qlens, uint32_t rlen, const int64_t *ooff, int nq,
                                uint32_t *out, int64_t *nout, hipStream_t stream);
 hipError_t records_launch(int phase, const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
-                          const char *names, const int64_t *name_off, int nq, int64_t rlen, int64_t *row_len,
-                          int64_t *ws, uint8_t *has_ins, int64_t *sz, unsigned long long *counts, char *out,
-                          hipStream_t stream);
+                          const char *names, const int64_t *name_off, int nq, int64_t rlen, const int32_t *map,
+                          int64_t width, int64_t *row_len, int64_t *ws, uint8_t *has_ins, int64_t *sz,
+                          unsigned long long *counts, char *out, hipStream_t stream);
 
 }  // namespace hal
 
@@ -81,6 +82,8 @@ struct halign_center {
     DevBuf qrev_off, ncig, llist, hlist, hcount, hsegs, hvals, ooff, out, nout, ocnt, coff, packed;
     DevBuf rows, rlen_d, rws, hins;   // projected rows, their lengths, op workspace, insertion flags
     DevBuf nm, nmoff, rsz, roff, rcnt, recs;   // keep-length FASTA records: names, sizes, offsets, counts, bytes
+    DevBuf cmap;         // column of each center position in its aligned record (cwidth > 0)
+    int64_t cwidth = 0;
     DevBuf lsmall, ljobs, ljseg, louts, ltb;   // small segments as ring jobs: counters, jobs, outputs, traceback
 };
 
@@ -602,7 +605,7 @@ void halign_center_free(halign_center *c)
                       &c->qrev_off, &c->ncig, &c->llist, &c->hlist, &c->hcount, &c->hsegs, &c->hvals, &c->ooff,
                       &c->out, &c->nout, &c->ocnt, &c->coff, &c->packed, &c->rows,
                       &c->rlen_d, &c->rws, &c->hins, &c->nm, &c->nmoff, &c->rsz, &c->roff, &c->rcnt,
-                      &c->recs, &c->lsmall, &c->ljobs, &c->ljseg, &c->louts, &c->ltb})
+                      &c->recs, &c->cmap, &c->lsmall, &c->ljobs, &c->ljseg, &c->louts, &c->ltb})
         b->release();
     delete c;
 }
@@ -754,6 +757,33 @@ int halign_center_align_fasta(halign_center *c, int32_t n, const char *queries, 
                               const char *names, const int64_t *name_offsets, char *out, int64_t cap,
                               int64_t *out_len, int64_t *ins_bytes, int32_t *n_insertion, double *ms)
 {
+    return hal::center_align_fasta(c, n, queries, offsets, names, name_offsets, out, cap, out_len, ins_bytes,
+                                   n_insertion, ms, nullptr);
+}
+
+}  // extern "C"
+
+namespace hal {
+
+int center_set_columns(halign_center *c, const int32_t *map, int64_t width)
+{
+    halign_ctx *ctx = c->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    if (!map) {
+        c->cwidth = 0;
+        return HALIGN_OK;
+    }
+    HCHECK(c->cmap.ensure((size_t)std::max(c->rlen, 1) * 4), "alloc column map");
+    HCHECK(hipMemcpy(c->cmap.p, map, (size_t)c->rlen * 4, hipMemcpyHostToDevice), "upload column map");
+    c->cwidth = width;
+    return HALIGN_OK;
+}
+
+int center_align_fasta(halign_center *c, int32_t n, const char *queries, const int64_t *offsets, const char *names,
+                       const int64_t *name_offsets, char *out, int64_t cap, int64_t *out_len, int64_t *ins_bytes,
+                       int32_t *n_insertion, double *ms, uint8_t *has_ins_out)
+{
     if (!c || n < 0 || !out_len || !ins_bytes || !n_insertion) return HALIGN_E_ARG;
     if (n > 0 && (!queries || !offsets || !names || !name_offsets || !out)) return HALIGN_E_ARG;
     halign_ctx *ctx = c->ctx;
@@ -762,8 +792,11 @@ int halign_center_align_fasta(halign_center *c, int32_t n, const char *queries, 
     *out_len = *ins_bytes = 0;
     *n_insertion = 0;
     if (n == 0) return HALIGN_OK;
-    // every record is '>' name '\n' row '\n' with a row of the center's length;
-    // a query named "@..." gets none (its SAM line would be skipped)
+    // every record is '>' name '\n' row '\n' with a row of the center's length
+    // (or of its aligned record's); a query named "@..." gets none (its SAM
+    // line would be skipped)
+    const int64_t width = c->cwidth > 0 ? c->cwidth : c->rlen;
+    const int32_t *cmap = c->cwidth > 0 ? c->cmap.as<int32_t>() : nullptr;
     std::vector<int64_t> noff((size_t)n + 1);
     int64_t total = 0;
     for (int i = 0; i <= n; ++i) {
@@ -771,7 +804,7 @@ int halign_center_align_fasta(halign_center *c, int32_t n, const char *queries, 
         if (i > 0) {
             if (noff[i] < noff[i - 1]) return HALIGN_E_ARG;
             if (noff[i] > noff[i - 1] && names[name_offsets[i - 1]] == '@') continue;
-            total += noff[i] - noff[i - 1] + c->rlen + 3;
+            total += noff[i] - noff[i - 1] + width + 3;
         }
     }
     if (total > cap) return fail(ctx, HALIGN_E_CAPACITY, "record output too small");
@@ -818,7 +851,8 @@ int halign_center_align_fasta(halign_center *c, int32_t n, const char *queries, 
                    "scan record sizes");
         }
         HCHECK(records_launch(phase, cig, coff, ctx->s_seqs.as<char>(), ctx->s_off.as<int64_t>(), c->nm.as<char>(),
-                              c->nmoff.as<int64_t>(), n, c->rlen, c->rlen_d.as<int64_t>(), c->rws.as<int64_t>(),
+                              c->nmoff.as<int64_t>(), n, c->rlen, cmap, width, c->rlen_d.as<int64_t>(),
+                              c->rws.as<int64_t>(),
                               c->hins.as<uint8_t>(), phase == 2 ? c->roff.as<int64_t>() : c->rsz.as<int64_t>(),
                               c->rcnt.as<unsigned long long>(), c->recs.as<char>(), st),
                "launch records");
@@ -829,6 +863,8 @@ int halign_center_align_fasta(halign_center *c, int32_t n, const char *queries, 
     if (total) HCHECK(hipMemcpyAsync(out, c->recs.p, (size_t)total, hipMemcpyDeviceToHost, st), "download records");
     HCHECK(hipMemcpyAsync(cnt, c->rcnt.p, 16, hipMemcpyDeviceToHost, st), "download");
     HCHECK(hipMemcpyAsync(&ins, c->roff.as<int64_t>() + n, 8, hipMemcpyDeviceToHost, st), "download");
+    if (has_ins_out)
+        HCHECK(hipMemcpyAsync(has_ins_out, c->hins.p, nn, hipMemcpyDeviceToHost, st), "download insertion flags");
     HCHECK(hipStreamSynchronize(st), "download records");
     if (cnt[1]) return fail(ctx, HALIGN_E_KERNEL, "a row's length differs from the center's");
     *out_len = total;
@@ -842,4 +878,4 @@ int halign_center_align_fasta(halign_center *c, int32_t n, const char *queries, 
     return HALIGN_OK;
 }
 
-}  // extern "C"
+}  // namespace hal

@@ -89,6 +89,11 @@ def _load():
     lib.halign_fastx_close.argtypes = [_P]
     lib.halign_msa_keep_length.argtypes = [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32,
                                            C.POINTER(MsaStats)]
+    lib.halign_msa_keep_length_refs.argtypes = [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32,
+                                                C.POINTER(MsaStats)]
+    lib.halign_mash_sketch.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
+                                       _P, C.c_int64, _P]
+    lib.halign_mash_best.argtypes = [_P, C.c_int32, _P, _P, C.c_int32, _P, _P, _P, _P]
     lib.halign_center_free.argtypes = [_P]
     lib.halign_center_free.restype = None
     lib.halign_center_anchors.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P]
@@ -257,6 +262,33 @@ class Context:
         self._check(rc, "halign_consensus")
         return counts, cons.tobytes(), n.value, ms.value
 
+
+    def mash_sketch(self, seqs, k=15, size=2000, non_canonical=True, seed=42):
+        """mash::sketchFromSequence per sequence: list of sorted uint64 arrays."""
+        blob, off = _blob(seqs)
+        n = len(off) - 1
+        cap = max(1, n * max(size, 0))
+        out = np.zeros(cap, np.uint64)
+        soff = np.zeros(n + 1, np.int64)
+        self._check(self.lib.halign_mash_sketch(self.h, n, blob, _ptr(off), k, size, int(non_canonical), seed,
+                                                _ptr(out), cap, _ptr(soff)), "halign_mash_sketch")
+        return [out[soff[i]:soff[i + 1]].copy() for i in range(n)]
+
+    def mash_best(self, query_sketches, ref_sketches):
+        """(best reference index, Jaccard) per query sketch."""
+        def flat(sk):
+            off = np.zeros(len(sk) + 1, np.int64)
+            off[1:] = np.cumsum([len(x) for x in sk])
+            data = np.concatenate([np.asarray(x, np.uint64) for x in sk]) if off[-1] else np.zeros(1, np.uint64)
+            return np.ascontiguousarray(data, np.uint64), off
+        qd, qo = flat(query_sketches)
+        rd, ro = flat(ref_sketches)
+        nq = len(query_sketches)
+        best = np.zeros(max(nq, 1), np.int32)
+        jac = np.zeros(max(nq, 1), np.float64)
+        self._check(self.lib.halign_mash_best(self.h, nq, _ptr(qd), _ptr(qo), len(ref_sketches), _ptr(rd), _ptr(ro),
+                                              _ptr(best), _ptr(jac)), "halign_mash_best")
+        return best[:nq], jac[:nq]
 
     def set_seed_policy(self, policy):
         """0 = minimizer wave form where it applies, 1 = sequential form only."""
@@ -561,11 +593,16 @@ def read_fastx(path, chunk=4096):
         L.halign_fastx_close(h)
 
 
-def msa_keep_length(ctx, center_path, input_path, output_path, batch=0):
-    """halign4 -c center --keep-length, file to file (halign_msa_keep_length):
-    returns the run's statistics as a dict."""
+def msa_keep_length(ctx, center_path, input_path, output_path, batch=0, aligned_path=None):
+    """halign4 -c center --keep-length, file to file (halign_msa_keep_length,
+    or halign_msa_keep_length_refs with the MSA command's output for a center
+    file of several references): returns the run's statistics as a dict."""
     st = MsaStats()
-    ctx._check(ctx.lib.halign_msa_keep_length(ctx.h, center_path.encode(), input_path.encode(),
-                                              output_path.encode(), int(batch), C.byref(st)),
-               "halign_msa_keep_length")
+    if aligned_path is None:
+        rc = ctx.lib.halign_msa_keep_length(ctx.h, center_path.encode(), input_path.encode(),
+                                            output_path.encode(), int(batch), C.byref(st))
+    else:
+        rc = ctx.lib.halign_msa_keep_length_refs(ctx.h, center_path.encode(), aligned_path.encode(),
+                                                 input_path.encode(), output_path.encode(), int(batch), C.byref(st))
+    ctx._check(rc, "halign_msa_keep_length")
     return {f: getattr(st, f) for f, _ in MsaStats._fields_}

@@ -233,7 +233,8 @@ void halign_fastx_close(halign_fastx *f);
  * as halign_center_align_fasta orders them over the whole input (the
  * reference's order with one thread). Reading the next chunk, aligning one on
  * the GPU and writing the previous one's records overlap. A center file with
- * more than one record is the multi-reference path: HALIGN_E_UNSUPPORTED.
+ * more than one record runs the multi-reference path below with the copy
+ * command.
  * Runs on one context take turns; the context keeps two pinned host buffers
  * of one chunk's records and the last run's center with its index until
  * halign_free, so later runs against the same center start warm. The rows of
@@ -247,6 +248,42 @@ typedef struct {
 } halign_msa_stats;
 int halign_msa_keep_length(halign_ctx *ctx, const char *center_path, const char *input_path,
                            const char *output_path, int32_t batch, halign_msa_stats *stats);
+/* The same with a center file of several reference sequences (RefAligner
+ * with more than one reference and keep_length, ref_aligner.cpp:228-266):
+ * each query is aligned to the reference whose sketch has the largest
+ * Jaccard with its own (halign_mash_best), and its row is spread over the
+ * columns of that reference's record in the aligned file (what the MSA
+ * command made of the center file; aligned_path NULL = the copy command, the
+ * center file itself), '-' in its gap columns (padQueryToRefByCigar with the
+ * record's M/D CIGAR, :400-459, :486-496). The output starts with the aligned
+ * file's records, cleaned. Aligned records must all have one width and hold
+ * their reference's bases (the reference's final length check and CIGAR
+ * lengths); otherwise HALIGN_E_ARG. With one record this is
+ * halign_msa_keep_length (aligned_path then gives the file whose consensus
+ * is the center). */
+int halign_msa_keep_length_refs(halign_ctx *ctx, const char *center_path, const char *aligned_path,
+                                const char *input_path, const char *output_path, int32_t batch,
+                                halign_msa_stats *stats);
+
+/* mash::sketchFromSequence (src/mash/mash.cpp:32-146, getHash2bit src/mash/
+ * hash.cpp) for n sequences (seqs[offsets[i] .. offsets[i+1]), ASCII): the
+ * sketch_size smallest distinct XXH3-64 hashes (seeded) of the k-mers without
+ * a non-ACGTU base, forward codes (non_canonical != 0) or canonical ones,
+ * ascending; sequence i's at hashes_out[sketch_offsets[i] .. sketch_offsets
+ * [i+1]). RefAligner uses k = 15, sketch_size = 2000, seed 42, non-canonical
+ * (ref_aligner.cpp:35-44, 217-222). k = 0, k > 32, sketch_size = 0 or a
+ * sequence shorter than k give an empty sketch, as in the reference. */
+int halign_mash_sketch(halign_ctx *ctx, int32_t n, const char *seqs, const int64_t *offsets, int32_t k,
+                       int32_t sketch_size, int32_t non_canonical, uint32_t seed, uint64_t *hashes_out,
+                       int64_t cap, int64_t *sketch_offsets);
+/* RefAligner::alignOneQueryToRef's choice among several references
+ * (ref_aligner.cpp:228-244 with mash::jaccard, src/mash/mash.cpp:151-164):
+ * for each query sketch the first reference sketch with the largest
+ * |a n b| / min(|a|, |b|) (1 for two empty sketches, 0 for one) in best[i],
+ * that value in jaccard[i] (optional). Sketches are sorted unique hash lists
+ * as halign_mash_sketch returns them. */
+int halign_mash_best(halign_ctx *ctx, int32_t nq, const uint64_t *q_hashes, const int64_t *q_offsets, int32_t nr,
+                     const uint64_t *r_hashes, const int64_t *r_offsets, int32_t *best, double *jaccard);
 
 /* Column counts and consensus of an alignment (the counting core of
  * consensus::generateConsensusSequence, reference src/consensus/consensus.cpp:

@@ -1066,3 +1066,85 @@ int64_t or_keep_length_fasta(const char *center_id, int id_len, const char *cent
 #undef OR_PUT
     return o;
 }
+
+/* ------------------------------------------------------------------ */
+/* mash sketches: src/mash/mash.cpp, src/mash/hash.cpp                 */
+/* ------------------------------------------------------------------ */
+static uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+
+uint64_t or_hash2bit(uint64_t code, uint32_t seed)
+{
+    /* the 8 input bytes are the code's big-endian bytes; the first and the
+       last four read little-endian */
+    const uint32_t in1 = __builtin_bswap32((uint32_t)(code >> 32));
+    const uint32_t in2 = __builtin_bswap32((uint32_t)code);
+    uint64_t s = seed;
+    s ^= (uint64_t)__builtin_bswap32((uint32_t)s) << 32;
+    /* default secret, bytes 8..15 and 16..23 read little-endian */
+    const uint64_t bitflip = (0x1cad21f72c81017cULL ^ 0xdb979083e96dd4deULL) - s;
+    uint64_t h = ((uint64_t)in2 + ((uint64_t)in1 << 32)) ^ bitflip;
+    const uint64_t mx2 = 0x9fb21c651e98df25ULL;
+    h ^= rotl64(h, 49) ^ rotl64(h, 24);
+    h *= mx2;
+    h ^= (h >> 35) + 8;
+    h *= mx2;
+    return h ^ (h >> 28);
+}
+
+int or_mash_sketch(const char *seq, int64_t n, int k, int sketch_size, int noncanonical, uint32_t seed,
+                   uint64_t *out, int cap)
+{
+    if (k == 0 || sketch_size <= 0 || n < k || k > 32) return 0;
+    /* (1 << 2k) - 1 with the shift count taken mod 64 as the reference's x86
+       build computes it: k = 32 shifts by 64 there (undefined in C++) and the
+       mask comes out 0, so every k-mer codes as 0 */
+    const uint64_t mask = (1ULL << ((2 * k) & 63)) - 1ULL;
+    const int shift = 2 * (k - 1);
+    uint64_t *all = (uint64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint64_t));
+    int64_t m = 0;
+    uint64_t fwd = 0, rev = 0;
+    int valid = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int c;
+        switch (seq[i]) {
+        case 'A': case 'a': c = 0; break;
+        case 'C': case 'c': c = 1; break;
+        case 'G': case 'g': c = 2; break;
+        case 'T': case 't': case 'U': case 'u': c = 3; break;
+        default: c = 4;
+        }
+        if (c == 4) {
+            fwd = rev = 0;
+            valid = 0;
+            continue;
+        }
+        fwd = ((fwd << 2) | (uint64_t)c) & mask;
+        rev = (rev >> 2) | ((uint64_t)(3 ^ c) << shift);
+        if (valid < k) ++valid;
+        if (valid < k) continue;
+        const uint64_t code = noncanonical ? fwd : (fwd < rev ? fwd : rev);
+        all[m++] = or_hash2bit(code, seed);
+    }
+    qsort(all, (size_t)m, sizeof(uint64_t), cmp_u64);
+    int cnt = 0;
+    for (int64_t i = 0; i < m && cnt < sketch_size; ++i) {
+        if (i > 0 && all[i] == all[i - 1]) continue;
+        if (cnt >= cap) { free(all); return -1; }
+        out[cnt++] = all[i];
+    }
+    free(all);
+    return cnt;
+}
+
+double or_mash_jaccard(const uint64_t *a, int na, const uint64_t *b, int nb)
+{
+    if (na == 0 && nb == 0) return 1.0;
+    if (na == 0 || nb == 0) return 0.0;
+    int i = 0, j = 0, inter = 0;
+    while (i < na && j < nb) {
+        if (a[i] < b[j]) ++i;
+        else if (b[j] < a[i]) ++j;
+        else { ++inter; ++i; ++j; }
+    }
+    return (double)inter / (double)(na < nb ? na : nb);
+}

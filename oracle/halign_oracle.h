@@ -168,6 +168,21 @@ int64_t or_keep_length_fasta(const char *center_id, int id_len, const char *cent
                              const int64_t *qoff, const char *names, const int64_t *noff,
                              const uint32_t *cigars, const int64_t *coff, char *out, int64_t cap);
 
+/* getHash2bit (src/mash/hash.cpp): XXH3-64 with the given seed of the
+ * 8-byte big-endian code (the vendored xxHash's 4..8-byte path). */
+uint64_t or_hash2bit(uint64_t code, uint32_t seed);
+
+/* mash::sketchFromSequence (src/mash/mash.cpp:32-146): the sketch_size
+ * smallest distinct hashes of the valid k-mers (nt4 codes, N resets, forward
+ * code or min(forward, reverse complement)), ascending. Returns their count or
+ * -1 when cap is too small. */
+int or_mash_sketch(const char *seq, int64_t n, int k, int sketch_size, int noncanonical, uint32_t seed,
+                   uint64_t *out, int cap);
+
+/* mash::jaccard (src/mash/mash.cpp:151-164): |a n b| / min(|a|, |b|), 1 for
+ * two empty sketches, 0 for one. */
+double or_mash_jaccard(const uint64_t *a, int na, const uint64_t *b, int nb);
+
 /* log2f table used by or_chain_anchors: lut[d] = log2f(d), d in [0, 501]. */
 void or_fill_log2_lut(float *lut);
 

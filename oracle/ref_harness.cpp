@@ -10,6 +10,7 @@
 #include "seed.h"
 #include "utils.h"
 #include "config.hpp"
+#include "mash.h"
 
 #include <cstring>
 #include <filesystem>
@@ -200,8 +201,8 @@ int ref_read_fastx(const char* path, int max_records, char* id_out, int64_t* id_
 // :612-672). RefAligner itself is not compiled here (it needs mash and the
 // external-tool runner); this glue calls the same reference functions in the
 // same order. work_dir must exist. Returns 0, or -2 when the reference throws.
-int ref_keep_length_msa(const char* center_path, const char* input_path, const char* work_dir,
-                        const char* out_path)
+int ref_keep_length_msa_refs(const char* center_path, const char* aligned_path, const char* input_path,
+                             const char* work_dir, const char* out_path)
 {
     namespace fs = std::filesystem;
     try {
@@ -217,9 +218,12 @@ int ref_keep_length_msa(const char* center_path, const char* input_path, const c
                 refs.push_back(rec);
             }
         }
-        if (refs.size() != 1) return -3;
+        if (refs.empty()) return -3;
+        std::vector<mash::Sketch> ref_sk;
+        for (const auto& r : refs) ref_sk.push_back(mash::sketchFromSequence(r.seq, 15, 2000, true, 42));
         const fs::path cons_aligned = wd / "cons_aligned.fasta";
-        file_io::copyFile(center_path, cons_aligned);   // "cp {input} {output}"
+        // the MSA command's output: a copy ("cp {input} {output}") or the given aligned file
+        file_io::copyFile(aligned_path ? aligned_path : center_path, cons_aligned);
         seq_io::SeqRecord consensus_seq;
         consensus_seq.id = "consensus";
         consensus_seq.seq = consensus::generateConsensusSequence(cons_aligned, wd / "cons.fasta", wd / "cons.json",
@@ -234,16 +238,33 @@ int ref_keep_length_msa(const char* center_path, const char* input_path, const c
             seq_io::KseqReader rd(input_path);
             seq_io::SeqRecord q;
             while (rd.next(q)) {
+                // alignOneQueryToRef (:212-283) with keep_length: the reference sequence
+                // with the largest Jaccard (first strict maximum), or the consensus
+                // when there is one center
+                const mash::Sketch qsk = mash::sketchFromSequence(q.seq, 15, 2000, true, 42);
                 auto qm = minimizer::extractMinimizer(q.seq, 15, 10, true);
-                const minimizer::MinimizerHits* rm = &ref_mz[0];
+                size_t best = 0;
+                const seq_io::SeqRecord* best_ref = &consensus_seq;
+                if (refs.size() > 1) {
+                    double bj = -1.0;
+                    for (size_t r = 0; r < ref_sk.size(); ++r) {
+                        const double j = mash::jaccard(qsk, ref_sk[r]);
+                        if (j > bj) {
+                            bj = j;
+                            best = r;
+                        }
+                    }
+                    best_ref = &refs[best];
+                }
+                const minimizer::MinimizerHits* rm = &ref_mz[best];
                 minimizer::MinimizerHits rtmp;
                 if (rm->empty()) {
-                    rtmp = minimizer::extractMinimizer(consensus_seq.seq, 15, 10, true);
+                    rtmp = minimizer::extractMinimizer(best_ref->seq, 15, 10, true);
                     rm = &rtmp;
                 }
                 const auto anchors = minimizer::collect_anchors(*rm, qm);
-                const cigar::Cigar_t cg = align::globalAlignMM2(consensus_seq.seq, q.seq, anchors);
-                const auto rec = seq_io::makeSamRecord(q, consensus_seq.id, cigar::cigarToString(cg), 1, 60, 0);
+                const cigar::Cigar_t cg = align::globalAlignMM2(best_ref->seq, q.seq, anchors);
+                const auto rec = seq_io::makeSamRecord(q, best_ref->id, cigar::cigarToString(cg), 1, 60, 0);
                 if (cigar::hasInsertion(cg)) out_ins.writeSam(rec);
                 else out.writeSam(rec);
             }
@@ -336,6 +357,37 @@ int ref_keep_length_msa(const char* center_path, const char* input_path, const c
     } catch (...) {
         return -2;
     }
+}
+
+int ref_keep_length_msa(const char* center_path, const char* input_path, const char* work_dir,
+                        const char* out_path)
+{
+    return ref_keep_length_msa_refs(center_path, nullptr, input_path, work_dir, out_path);
+}
+
+// mash::sketchFromSequence (src/mash/mash.cpp:32-146): the sorted hashes.
+// Returns their count, or -1 when cap is too small.
+int ref_mash_sketch(const char* seq, int n, int k, int sketch_size, int noncanonical, int seed, uint64_t* out,
+                    int cap)
+{
+    const mash::Sketch sk = mash::sketchFromSequence(std::string(seq, (size_t)n), (size_t)k, (size_t)sketch_size,
+                                                     noncanonical != 0, seed);
+    if ((int)sk.hashes.size() > cap) return -1;
+    for (size_t i = 0; i < sk.hashes.size(); ++i) out[i] = sk.hashes[i];
+    return (int)sk.hashes.size();
+}
+
+// getHash2bit (src/mash/hash.cpp): XXH3-64 of the big-endian 8-byte code
+uint64_t ref_hash2bit(uint64_t code, uint32_t seed) { return getHash2bit(code, seed); }
+
+// mash::jaccard (src/mash/mash.cpp:151-164) of two sorted unique hash lists
+double ref_mash_jaccard(const uint64_t* a, int na, const uint64_t* b, int nb, int k)
+{
+    mash::Sketch x, y;
+    x.k = y.k = (size_t)k;
+    x.hashes.assign(a, a + na);
+    y.hashes.assign(b, b + nb);
+    return mash::jaccard(x, y);
 }
 
 }  // extern "C"

@@ -24,6 +24,10 @@ Outputs (np.savez_compressed, no pickles):
                        gzip), or its error
   msa_keep_length.npz  halign4 -c center --keep-length (one thread, copy MSA
                        command): the final FASTA bytes of center + query files
+  mash_cases.npz       mash::sketchFromSequence over (k, sketch size, canonical,
+                       seed), and the Jaccard choice of a reference
+  msa_refs.npz         halign4 -c refs --keep-length with several references
+                       (copy command, or a given aligned file): final FASTA bytes
 """
 import json
 import os
@@ -485,6 +489,131 @@ def msa_keep_length(R, rng):
     save("msa_keep_length.npz", center=cd, center_off=co, input=idd, input_off=io, output=od, output_off=oo)
 
 
+def mash_cases(R, rng):
+    """mash::sketchFromSequence (src/mash/mash.cpp:32-146) over k, sketch size,
+    canonical/forward codes and seeds, on sequences with N, lower case and U;
+    and RefAligner's choice of a reference by mash::jaccard (first strict
+    maximum, src/align/ref_aligner.cpp:228-244)."""
+    alpha = np.frombuffer(b"ACGTACGTACGTacgtNUu", np.uint8)
+    seqs, par, sk = [], [], []
+    for i in range(160):
+        size = [2000, 64, 7, 1, 0][i % 5]
+        L = int(rng.integers(0, 700 if size == 2000 else 3000))
+        seq = alpha[rng.integers(0, len(alpha), L)].tobytes()
+        k = [15, 5, 21, 32, 1, 31, 0, 33][i % 8]
+        nc = int(bool(i % 3))
+        seed = [42, 0, 123456789][i % 3]
+        if i in (5, 60):   # full 2,000-hash sketches, RefAligner's parameters
+            seq = ACGT[rng.integers(0, 4, 2600)].tobytes()
+            k, size, nc, seed = 15, 2000, int(i == 5), 42
+        seqs.append(np.frombuffer(seq, np.uint8))
+        par.append((k, size, nc, seed))
+        sk.append(R.mash_sketch(seq, k, size, nc, seed))
+    # reference choice: queries against 5 references, sketches as RefAligner makes them
+    base = ACGT[rng.integers(0, 4, 3000)]
+    refs = []
+    for r in range(5):
+        b = base.copy()
+        b[rng.integers(0, 3000, 60 * (r + 1))] = ACGT[rng.integers(0, 4, 60 * (r + 1))]
+        refs.append(b.tobytes())
+    refs[4] = refs[1]   # a tie: the first of two equal references wins
+    qs = []
+    for i in range(40):
+        src = np.frombuffer(refs[i % 5], np.uint8).copy()
+        src[rng.integers(0, 3000, 90)] = ACGT[rng.integers(0, 4, 90)]
+        a = int(rng.integers(0, 1500))
+        qs.append(src[a:a + int(rng.integers(0, 1500))].tobytes())
+    qs.append(b"")
+    qs.append(b"ACGTN" * 5)
+    rsk = [R.mash_sketch(r) for r in refs]
+    best, jac = [], []
+    for q in qs:
+        qsk = R.mash_sketch(q)
+        bj, bi = -1.0, 0
+        for r, x in enumerate(rsk):
+            j = R.mash_jaccard(qsk, x)
+            if j > bj:
+                bj, bi = j, r
+        best.append(bi)
+        jac.append(bj)
+    sd, so = concat(seqs, np.uint8)
+    kd, ko = concat(sk, np.uint64)
+    rd, ro = concat([np.frombuffer(r, np.uint8) for r in refs], np.uint8)
+    qd, qo = concat([np.frombuffer(q, np.uint8) for q in qs], np.uint8)
+    save("mash_cases.npz", seq=sd, seq_off=so, params=np.array(par, np.int64), sketch=kd, sketch_off=ko,
+         ref=rd, ref_off=ro, qry=qd, qry_off=qo, best=np.array(best, np.int32), jaccard=np.array(jac, np.float64))
+
+
+def msa_refs(R, rng):
+    """halign4 -c refs --keep-length with several references, one thread
+    (ref_keep_length_msa_refs): each query goes to the reference with the
+    largest sketch Jaccard, its row spread over that reference's aligned
+    record. Case 0: three equal-length references, the copy command; case 1:
+    four references of different lengths and an aligned file with gap columns
+    (what an MSA command would write); case 2: FASTQ queries and two identical
+    references (a Jaccard tie)."""
+    d = tempfile.mkdtemp()
+    out = {k: [] for k in ("center", "aligned", "input", "output")}
+    has_aligned = []
+    for case in range(3):
+        W = [3000, 4000, 2500][case]
+        base = ACGT[rng.integers(0, 4, W)]
+        nref = [3, 4, 3][case]
+        refs, rows = [], []
+        for r in range(nref):
+            b = base.copy()
+            b[rng.integers(0, W, 40 * (r + 1))] = ACGT[rng.integers(0, 4, 40 * (r + 1))]
+            present = np.ones(W, bool)
+            if case == 1:   # each reference lacks some columns
+                for _ in range(3):
+                    a = int(rng.integers(0, W - 60))
+                    present[a:a + int(rng.integers(1, 50))] = False
+            refs.append(b[present].tobytes())
+            rows.append(np.where(present, b, ord("-")).astype(np.uint8).tobytes())
+        if case == 2:
+            refs[2] = refs[0]
+            rows[2] = rows[0]
+        names = [b"ref%d" % r for r in range(nref)]
+        cfa = b"".join(b">" + n + b" r\n" + x + b"\n" for n, x in zip(names, refs))
+        afa = b"".join(b">" + n + b"\n" + x + b"\n" for n, x in zip(names, rows)) if case == 1 else None
+        recs = []
+        for i in range(36):
+            lut = np.zeros(256, np.uint8)
+            lut[ACGT] = np.arange(4, dtype=np.uint8)
+            src = lut[np.frombuffer(refs[i % nref], np.uint8)]
+            q = mutate_codes(rng, src, 0.01, 0.0 if i % 4 == 0 else 0.004)
+            if i % 6 == 5:
+                a = int(rng.integers(0, len(q) // 2))
+                q = q[a:a + int(rng.integers(300, len(q) // 2))]
+            qs = codes_to_ascii(q)
+            name = b"@q%d" % i if i == 7 else b"q%d" % i
+            recs.append((name, b" d" if i % 5 == 0 else b"", qs))
+        recs.append((b"empty", b"", b""))
+        recs.append((b"short", b"", b"ACGTACGTAC"))
+        if case == 2:
+            text = b"".join(b"@" + n + dsc + b"\n" + q + b"\n+\n" + b"I" * len(q) + b"\n" for n, dsc, q in recs)
+        else:
+            text = b"".join(b">" + n + dsc + b"\n" + q + b"\n" for n, dsc, q in recs)
+        cp, ap, ip, op = (os.path.join(d, x + "%d" % case) for x in ("c.fa", "a.fa", "q.fa", "o.fa"))
+        open(cp, "wb").write(cfa)
+        open(ip, "wb").write(text)
+        if afa is not None:
+            open(ap, "wb").write(afa)
+        wd = os.path.join(d, "wd%d" % case)
+        os.makedirs(wd)
+        rc = R.keep_length_msa_refs(cp, ap if afa is not None else None, ip, wd, op)
+        assert rc == 0, (case, rc)
+        out["center"].append(np.frombuffer(cfa, np.uint8))
+        out["aligned"].append(np.frombuffer(afa or b"", np.uint8))
+        out["input"].append(np.frombuffer(text, np.uint8))
+        out["output"].append(np.frombuffer(open(op, "rb").read(), np.uint8))
+        has_aligned.append(afa is not None)
+    arrays = {}
+    for k, v in out.items():
+        arrays[k], arrays[k + "_off"] = concat(v, np.uint8)
+    save("msa_refs.npz", has_aligned=np.array(has_aligned, np.int32), **arrays)
+
+
 def main():
     R = RefLib()
     O = Oracle()
@@ -498,6 +627,8 @@ def main():
     whole_pairs(R)
     fastx_cases(R, np.random.default_rng(0x6023))
     msa_keep_length(R, np.random.default_rng(0x6024))
+    mash_cases(R, np.random.default_rng(0x6025))
+    msa_refs(R, np.random.default_rng(0x6026))
 
 
 if __name__ == "__main__":

@@ -115,6 +115,11 @@ class Oracle(_Base):
         L.or_keep_length_fasta.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int64,
                                            C.c_int, C.c_char_p, _P, C.c_char_p, _P, _P, _P, _P, C.c_int64]
         L.or_keep_length_fasta.restype = C.c_int64
+        L.or_hash2bit.argtypes = [C.c_uint64, C.c_uint32]
+        L.or_hash2bit.restype = C.c_uint64
+        L.or_mash_sketch.argtypes = [C.c_char_p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_uint32, _P, C.c_int]
+        L.or_mash_jaccard.argtypes = [_P, C.c_int, _P, C.c_int]
+        L.or_mash_jaccard.restype = C.c_double
         L.or_fill_log2_lut.argtypes = [_P]
         self.lut = np.zeros(502, np.float32)
         L.or_fill_log2_lut(_ptr(self.lut))
@@ -259,6 +264,15 @@ class Oracle(_Base):
             return n
         return [tuple(bufs[k][offs[k][i]:offs[k][i + 1]].tobytes() for k in range(3)) for i in range(n)]
 
+    def mash_sketch(self, seq, k=15, size=2000, noncanonical=True, seed=42):
+        """mash::sketchFromSequence: sorted uint64 hashes."""
+        return _mash_sketch(self.lib.or_mash_sketch, seq, k, size, noncanonical, seed)
+
+    def mash_jaccard(self, a, b):
+        a = np.ascontiguousarray(a, np.uint64)
+        b = np.ascontiguousarray(b, np.uint64)
+        return self.lib.or_mash_jaccard(_ptr(a), len(a), _ptr(b), len(b))
+
     def keep_length_fasta(self, center_id, center_desc, center_seq, queries, names, cigars):
         """The merged keep-length FASTA bytes (or_keep_length_fasta)."""
         qb, qo = _concat_bytes(queries)
@@ -276,6 +290,15 @@ class Oracle(_Base):
                                           _ptr(co), _ptr(out), cap)
         assert n >= 0, n
         return out[:n].tobytes()
+
+
+def _mash_sketch(fn, seq, k, size, noncanonical, seed):
+    seq = seq.encode() if isinstance(seq, str) else bytes(seq)
+    cap = max(1, min(size, len(seq) + 1))
+    out = np.zeros(cap, np.uint64)
+    n = fn(seq, len(seq), k, size, int(noncanonical), seed, _ptr(out), cap)
+    assert n >= 0, n
+    return out[:n].copy()
 
 
 def _concat_bytes(items):
@@ -399,6 +422,33 @@ class RefLib(_Base):
         if n < 0:
             return n
         return [tuple(bufs[k][offs[k][i]:offs[k][i + 1]].tobytes() for k in range(3)) for i in range(n)]
+
+    def mash_sketch(self, seq, k=15, size=2000, noncanonical=True, seed=42):
+        L = self.lib
+        L.ref_mash_sketch.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int]
+        return _mash_sketch(L.ref_mash_sketch, seq, k, size, noncanonical, seed)
+
+    def mash_jaccard(self, a, b, k=15):
+        L = self.lib
+        L.ref_mash_jaccard.argtypes = [_P, C.c_int, _P, C.c_int, C.c_int]
+        L.ref_mash_jaccard.restype = C.c_double
+        a = np.ascontiguousarray(a, np.uint64)
+        b = np.ascontiguousarray(b, np.uint64)
+        return L.ref_mash_jaccard(_ptr(a), len(a), _ptr(b), len(b), k)
+
+    def hash2bit(self, code, seed):
+        L = self.lib
+        L.ref_hash2bit.argtypes = [C.c_uint64, C.c_uint32]
+        L.ref_hash2bit.restype = C.c_uint64
+        return L.ref_hash2bit(code, seed)
+
+    def keep_length_msa_refs(self, center_path, aligned_path, input_path, work_dir, out_path):
+        """halign4 -c refs --keep-length, one thread; aligned_path = the MSA
+        command's output for the refs (None: the copy command)."""
+        L = self.lib
+        L.ref_keep_length_msa_refs.argtypes = [C.c_char_p] * 5
+        return L.ref_keep_length_msa_refs(center_path.encode(), aligned_path.encode() if aligned_path else None,
+                                          input_path.encode(), work_dir.encode(), out_path.encode())
 
     def keep_length_msa(self, center_path, input_path, work_dir, out_path):
         """halign4 -c --keep-length, one thread, copy MSA command (ref_keep_length_msa)."""

@@ -105,13 +105,10 @@ def test_chunked_run_matches_merge_of_gpu_cigars(ctx, tmp_path):
 
 def test_errors(ctx, tmp_path):
     import halign
-    two = _write(tmp_path, "c2.fa", b">a\nACGTACGTAC\n>b\nACGTACGTAA\n")
     one = _write(tmp_path, "c1.fa", b">a\n" + b"ACGT" * 100 + b"\n")
     q = _write(tmp_path, "q.fa", b">q\nACGTACGT\n")
     bad = _write(tmp_path, "bad.fq", b"@q\nACGT\n+\nII\n")
     out = str(tmp_path / "o.fa")
-    with pytest.raises(halign.HalignError, match="unsupported"):
-        halign.msa_keep_length(ctx, two, q, out)
     with pytest.raises(halign.HalignError):
         halign.msa_keep_length(ctx, one, str(tmp_path / "absent.fa"), out)
     with pytest.raises(halign.HalignError, match="quality"):
