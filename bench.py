@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--batch-queries", type=int, default=8192, help="serve: queries per batch at most")
     ap.add_argument("--loads", type=float, nargs="*", default=[0.5, 0.8],
                     help="serve: offered load as fractions of the measured capacity")
+    ap.add_argument("--refs", type=int, default=1,
+                    help="msa: reference sequences in the center file (>1: several references, chosen by Jaccard)")
     ap.add_argument("--msa-batch", type=int, default=0,
                     help="msa: queries per chunk (0 = the library default, alignQueryToRef's 25600)")
     ap.add_argument("--kernel", default="auto",
@@ -294,7 +296,17 @@ def msa_bench(args):
     ipath = os.path.join(d, "queries.fa")
     opath = os.path.join(d, "out.fa")
     with open(cpath, "wb") as f:
-        f.write(b">center\n" + bytes(center) + b"\n")
+        if args.refs <= 1:
+            f.write(b">center\n" + bytes(center) + b"\n")
+        else:   # variants of the center at 2% substitutions: one width, the copy command aligns them
+            rng = np.random.default_rng(0xC0FFEE)
+            base = np.frombuffer(bytes(center), np.uint8)
+            for r in range(args.refs):
+                v = base.copy()
+                if r:
+                    pos = rng.integers(0, len(v), len(v) // 50)
+                    v[pos] = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, len(pos))]
+                f.write(b">ref%d\n" % r + v.tobytes() + b"\n")
     qv = memoryview(qb if isinstance(qb, (bytes, bytearray)) else bytes(qb))
     with open(ipath, "wb") as f:
         for i in range(need):
@@ -302,7 +314,7 @@ def msa_bench(args):
             f.write(qv[off[i]:off[i + 1]])
             f.write(b"\n")
     in_bytes = os.path.getsize(ipath)
-    log(f"msa config {cfg}: {need} queries, input {in_bytes / 1e6:.1f} MB in {d}")
+    log(f"msa config {cfg}: {need} queries, {args.refs} reference(s), input {in_bytes / 1e6:.1f} MB in {d}")
     ctx = halign.Context(0)
     steps = []
     for s in range(args.warmup + args.steps):
@@ -334,7 +346,7 @@ def msa_bench(args):
                     for i in range(a, b):
                         f.write(b">q%d\n" % i + bytes(qv[off[i]:off[i + 1]]) + b"\n")
                 t0 = time.perf_counter()
-                rc = R.keep_length_msa(cpath, ip, wd, os.path.join(wd, "out.fa"))
+                rc = R.keep_length_msa_refs(cpath, None, ip, wd, os.path.join(wd, "out.fa"))
                 assert rc == 0, rc
                 return time.perf_counter() - t0
             t8 = run_share(0, 0, min(8, need))
@@ -354,8 +366,9 @@ def msa_bench(args):
            "warmup": args.warmup, "ms_per_step": round(tw * 1e3 / len(steps), 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "int8",
            "data": f"synthetic (seeded, csrc/synth.c config {cfg}), written to FASTA files",
-           "config": {"workload": f"config {cfg}: {label}; keep-length FASTA file to file", "queries": need,
-                      "chunk_queries": args.msa_batch or 25600},
+           "config": {"workload": f"config {cfg}: {label}; keep-length FASTA file to file"
+                                  + (f", {args.refs} references (variants at 2% substitutions)" if args.refs > 1 else ""),
+                      "queries": need, "references": args.refs, "chunk_queries": args.msa_batch or 25600},
            "output_mb_per_s": round(out_bytes * len(steps) / tw / 1e6, 1), "input_bytes": in_bytes,
            "output_bytes": out_bytes,
            "gpu_s_per_step": round(sum(x[1]["gpu_seconds"] for x in steps) / len(steps), 4),
