@@ -55,7 +55,8 @@ __device__ __forceinline__ uint32_t cat16(uint32_t hi, uint32_t lo) { return __b
 // lane i receives lane i-1 (lane 0 receives lane 63): DPP wave_ror:1
 __device__ __forceinline__ uint32_t wave_ror1(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x13C, 0xF, 0xF, false);
+    // every lane is written (a full-wave rotate), so no old value is needed
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x13C, 0xF, 0xF, false);
 }
 
 struct Bounds { int st0, en0, st, en; };

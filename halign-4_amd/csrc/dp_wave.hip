@@ -194,7 +194,7 @@ __device__ inline int group_backtrack(const DpJob &J, const uint8_t *__restrict_
 
 template <int W, int S>
 struct WaveSmem {
-    uint32_t xch[2][W][S][3];   // last lane of each wave: x, v of cell 15 and query window dword 3
+    uint4 xch[2][W][S];   // last lane of each wave: x, v of cell 15, query window dword 3 (w unused)
     union {
         uint32_t chunk[64][10];       // backtrack staging (one job per workgroup)
         uint32_t group[16][16][4];    // backtrack staging per job (sub-wave mode)
@@ -307,7 +307,7 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 for (int k = 0; k < S; ++k) {
                     if (k == kn) {
                         load16(tgt + 16 * nbe, T4[k]);
-                        load16(qr + (qlen - 2 + 16 * nbe), Q4[k]);
+                        load16(qr + (qlen - 1 + 16 * nbe), Q4[k]);   // window of diagonal 0, slid at 1
                     }
                 }
             }
@@ -369,24 +369,22 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 const int p = r & 1;
                 if (lane == 63) {
 #pragma unroll
-                    for (int k = 0; k < S; ++k) {
-                        sm.xch[p][wave][k][0] = X[k][7];
-                        sm.xch[p][wave][k][1] = V[k][7];
-                        sm.xch[p][wave][k][2] = Q4[k][3];
-                    }
+                    for (int k = 0; k < S; ++k) sm.xch[p][wave][k] = make_uint4(X[k][7], V[k][7], Q4[k][3], 0u);
                 }
                 __syncthreads();
                 // lane 0 of wave i > 0: last lane of wave i-1, same slot; lane 0 of
-                // wave 0: last lane of wave W-1, previous slot
+                // wave 0: last lane of wave W-1, previous slot. Every lane reads the
+                // (same) words, one 16-byte read per slot, and lane 0 selects them:
+                // no lane-0 branch around the reads.
                 const int ws = wave ? wave - 1 : W - 1;
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     const int kp = (k + S - 1) % S;
-                    const uint32_t *e0 = sm.xch[p][ws][k], *e1 = sm.xch[p][ws][kp];
-                    const uint32_t *e = wave ? e0 : e1;
-                    nx[k] = lane ? rx[k] : e[0];
-                    nv[k] = lane ? rv[k] : e[1];
-                    nq[k] = lane ? rq[k] : e[2];
+                    uint4 e = sm.xch[p][ws][wave ? k : kp];
+                    asm volatile("" : "+v"(e.x), "+v"(e.y), "+v"(e.z));   // keep the read unconditional
+                    nx[k] = lane ? rx[k] : e.x;
+                    nv[k] = lane ? rv[k] : e.y;
+                    nq[k] = lane ? rq[k] : e.z;
                 }
                 // the edge bytes of diagonal r-1 were stored before this barrier
                 if (r > 0) edge(r - 1, p_st0, p_en0, p_en, sm.logu[p ^ 1], sm.logv[p ^ 1]);
@@ -404,18 +402,18 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 }
             }
         }
-        // slide the query windows one base for blocks that were already active
+        // slide every query window one base (branch-free): a block loaded ahead
+        // of its entry holds the window of the diagonal it was loaded at and
+        // slides with the rest, taking the true query bytes from its left
+        // neighbour; windows of blocks outside the band are never read
         if (r > 0) {
 #pragma unroll
             for (int k = 0; k < S; ++k) {
-                const int b = blk[k];
-                if (act[k] && b >= last_bs && b <= (last_en >> 4)) {
-                    const uint32_t in = (b == last_bs) ? (fe << 24) : nq[k];
-                    Q4[k][3] = __builtin_amdgcn_alignbyte(Q4[k][3], Q4[k][2], 3);
-                    Q4[k][2] = __builtin_amdgcn_alignbyte(Q4[k][2], Q4[k][1], 3);
-                    Q4[k][1] = __builtin_amdgcn_alignbyte(Q4[k][1], Q4[k][0], 3);
-                    Q4[k][0] = __builtin_amdgcn_alignbyte(Q4[k][0], in, 3);
-                }
+                const uint32_t in = (blk[k] == last_bs) ? (fe << 24) : nq[k];
+                Q4[k][3] = __builtin_amdgcn_alignbyte(Q4[k][3], Q4[k][2], 3);
+                Q4[k][2] = __builtin_amdgcn_alignbyte(Q4[k][2], Q4[k][1], 3);
+                Q4[k][1] = __builtin_amdgcn_alignbyte(Q4[k][1], Q4[k][0], 3);
+                Q4[k][0] = __builtin_amdgcn_alignbyte(Q4[k][0], in, 3);
             }
         }
         // left boundary of the rounded region (:118-122)
@@ -567,7 +565,8 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
 #pragma unroll
                                 for (int j = 0; j < 8; ++j) U[k][j] = V[k][j] = X[k][j] = Y[k][j] = 0;
                                 load16(tgt + 16 * nbe, T4[k]);
-                                load16(qr + (qlen - 1 - (r + 2) + 16 * nbe), Q4[k]);
+                                // the window of diagonal r: slid at r+1 and r+2
+                                load16(qr + (qlen - 1 - r + 16 * nbe), Q4[k]);
                             }
                         }
                     }
