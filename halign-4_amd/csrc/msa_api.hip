@@ -524,3 +524,183 @@ extern "C" int halign_msa_keep_length_refs(halign_ctx *ctx, const char *center_p
     if (stats) *stats = st;
     return HALIGN_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Auto-center pieces: Top-K longest records (preprocessInputFasta with
+// TopKLongestSelector), the consensus of an aligned file with its FASTA and
+// JSON outputs (generateConsensusSequence), and the keep-length run around
+// them.
+
+namespace {
+
+// seq_io::SeqWriter::writeFasta with line width w (src/utils/seq_io.cpp:236-268)
+void append_fasta(std::string &o, const std::string &id, const std::string &desc, const std::string &seq, size_t w)
+{
+    o += '>';
+    o += id;
+    if (!desc.empty()) {
+        o += ' ';
+        o += desc;
+    }
+    o += '\n';
+    if (seq.empty()) {
+        o += '\n';
+        return;
+    }
+    for (size_t i = 0; i < seq.size(); i += w) {
+        o.append(seq, i, std::min(w, seq.size() - i));
+        o += '\n';
+    }
+}
+
+bool write_file(const char *path, const std::string &data)
+{
+    const int fd = open(path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd < 0) return false;
+    const bool ok = write_at(fd, data.data(), data.size(), 0);
+    return close(fd) == 0 && ok;
+}
+
+}  // namespace
+
+extern "C" int halign_topk_longest(halign_ctx *ctx, const char *input_path, int32_t k, const char *output_path,
+                                   int64_t *n_records)
+{
+    if (!input_path || !output_path || k < 0) return HALIGN_E_ARG;   // ctx optional: host only
+    FastxReader rd;
+    std::string e;
+    if (!rd.open(input_path, e)) return fail(ctx, HALIGN_E_IO, e.c_str());
+    // the K longest cleaned records, the earlier one first among equal lengths
+    struct Item {
+        size_t len;
+        uint64_t order;
+        std::string id, desc, seq;
+    };
+    auto worse = [](const Item &a, const Item &b) {   // a ranks below b
+        return a.len != b.len ? a.len < b.len : a.order > b.order;
+    };
+    // heap with the lowest-ranked kept record on top
+    auto heap_cmp = [&](const Item &a, const Item &b) { return worse(b, a); };
+    std::vector<Item> heap;
+    std::string id, desc, seq;
+    uint64_t order = 0;
+    int r;
+    while ((r = rd.next(id, desc, seq, e)) > 0) {
+        for (auto &ch : seq) ch = clean_base((unsigned char)ch);   // cleanSequence
+        Item it{seq.size(), order++, id, desc, seq};
+        if (k == 0) continue;
+        if ((int32_t)heap.size() < k) {
+            heap.push_back(std::move(it));
+            std::push_heap(heap.begin(), heap.end(), heap_cmp);
+        } else if (worse(heap.front(), it)) {
+            std::pop_heap(heap.begin(), heap.end(), heap_cmp);
+            heap.back() = std::move(it);
+            std::push_heap(heap.begin(), heap.end(), heap_cmp);
+        }
+    }
+    if (r < 0) return fail(ctx, HALIGN_E_IO, e.c_str());
+    std::sort(heap.begin(), heap.end(), [&](const Item &a, const Item &b) { return worse(b, a); });
+    std::string o;
+    for (const auto &it : heap) append_fasta(o, it.id, it.desc, it.seq, 80);
+    if (!write_file(output_path, o)) return fail(ctx, HALIGN_E_IO, "write failed");
+    if (n_records) *n_records = (int64_t)order;
+    return HALIGN_OK;
+}
+
+extern "C" int halign_consensus_file(halign_ctx *ctx, const char *aligned_path, const char *out_fasta,
+                                     const char *out_json, uint64_t seq_limit, int64_t batch, char *cons_out,
+                                     int64_t cap, int64_t *cons_len)
+{
+    if (!ctx || !aligned_path || !out_fasta || !out_json || batch <= 0) return HALIGN_E_ARG;
+    FastxReader rd;
+    std::string e, id, desc, seq;
+    if (!rd.open(aligned_path, e)) return fail(ctx, HALIGN_E_IO, e.c_str());
+    int r = rd.next(id, desc, seq, e);
+    if (r < 0) return fail(ctx, HALIGN_E_IO, e.c_str());
+    if (r == 0) return fail(ctx, HALIGN_E_ARG, "aligned fasta is empty");
+    const size_t aln = seq.size();
+    if (aln == 0) return fail(ctx, HALIGN_E_ARG, "first sequence length is 0");
+    // the reference's reading loop: batches of `batch` records, the record
+    // read after each batch dropped, at most seq_limit records kept
+    std::string rows = seq;
+    uint64_t kept = 1;
+    size_t in_batch = 1;
+    for (;;) {
+        if (seq_limit != 0 && kept >= seq_limit) break;
+        while (in_batch < (size_t)batch && (seq_limit == 0 || kept < seq_limit)) {
+            r = rd.next(id, desc, seq, e);
+            if (r < 0) return fail(ctx, HALIGN_E_IO, e.c_str());
+            if (r == 0) break;
+            if (seq.size() != aln) return fail(ctx, HALIGN_E_ARG, "alignment length mismatch when reading");
+            rows += seq;
+            ++kept;
+            ++in_batch;
+        }
+        in_batch = 0;
+        r = rd.next(id, desc, seq, e);   // read and dropped
+        if (r < 0) return fail(ctx, HALIGN_E_IO, e.c_str());
+        if (r == 0) break;
+    }
+    std::vector<uint32_t> counts(aln * 7);
+    std::string cons(aln, 'A');
+    uint64_t nseq = 0;
+    // every row read here is counted: no further batch drop on the device
+    const int rc = halign_consensus(ctx, rows.data(), (int64_t)kept, (int64_t)aln, (int64_t)aln, 0,
+                                    (int64_t)kept + 1, 0, counts.data(), &cons[0], &nseq, nullptr);
+    if (rc != HALIGN_OK) return rc;
+    // writeConsensusFasta and writeCountsJson (src/consensus/consensus.cpp:38-68)
+    std::string fa = ">consensus\n";
+    for (size_t i = 0; i < aln; i += 80) {
+        fa.append(cons, i, std::min<size_t>(80, aln - i));
+        fa += '\n';
+    }
+    std::string js;
+    js.reserve(aln * 160 + 128);
+    js += "{\n    \"consensus\": {\n        \"num_seqs\": " + std::to_string(kept) +
+          ",\n        \"aln_len\": " + std::to_string(aln) + ",\n        \"counts\": [\n";
+    static const char *const kKeys[7] = {"A", "C", "G", "T", "U", "N", "-"};
+    for (size_t i = 0; i < aln; ++i) {
+        js += "            {\n";
+        for (int c = 0; c < 7; ++c) {
+            js += "                \"";
+            js += kKeys[c];
+            js += "\": " + std::to_string(counts[i * 7 + c]);
+            js += c < 6 ? ",\n" : "\n";
+        }
+        js += i + 1 < aln ? "            },\n" : "            }\n";
+    }
+    js += "        ]\n    }\n}";
+    if (!write_file(out_fasta, fa) || !write_file(out_json, js)) return fail(ctx, HALIGN_E_IO, "write failed");
+    if (cons_len) *cons_len = (int64_t)aln;
+    if (cons_out) {
+        if ((int64_t)aln > cap) return fail(ctx, HALIGN_E_CAPACITY, "consensus output too small");
+        std::memcpy(cons_out, cons.data(), aln);
+    }
+    return HALIGN_OK;
+}
+
+extern "C" int halign_msa_auto_center(halign_ctx *ctx, const char *input_path, const char *aligned_topk_path,
+                                      const char *output_path, int32_t cons_n, int32_t batch,
+                                      halign_msa_stats *stats)
+{
+    if (!ctx || !input_path || !output_path || cons_n <= 0) return HALIGN_E_ARG;
+    const std::string base = output_path;
+    const std::string topk = base + ".topk.fa", cons = base + ".consensus.fa", json = base + ".consensus.json";
+    struct Cleanup {
+        std::vector<std::string> f;
+        ~Cleanup()
+        {
+            for (auto &x : f) unlink(x.c_str());
+        }
+    } cl{{topk, cons, json}};
+    // preprocessInputFasta's Top-K file, the MSA command's output for it (the
+    // copy command: the file itself), its consensus (seq_limit cons_n), then
+    // the keep-length run against the consensus record (src/halign4.cpp:
+    // 87, 119-147)
+    int rc = halign_topk_longest(ctx, input_path, cons_n, topk.c_str(), nullptr);
+    if (rc != HALIGN_OK) return rc;
+    rc = halign_consensus_file(ctx, aligned_topk_path ? aligned_topk_path : topk.c_str(), cons.c_str(), json.c_str(),
+                               (uint64_t)cons_n, 4096, nullptr, 0, nullptr);
+    if (rc != HALIGN_OK) return rc;
+    return halign_msa_keep_length(ctx, cons.c_str(), input_path, output_path, batch, stats);
+}

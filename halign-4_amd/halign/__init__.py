@@ -3,7 +3,8 @@ MI355X. The compute runs in libhalign_hip.so (HIP, gfx950) through its C ABI.
 """
 from ._lib import (ANCHOR_DTYPE, DP_RESULT_DTYPE, DP_TASK_DTYPE, Center, Context, HalignError,  # noqa: F401
                    PairSet, Server,
-                   LIB_PATH, lib, msa_keep_length, read_fastx, ring_policy, ring_shapes)
+                   LIB_PATH, consensus_file, lib, msa_auto_center, msa_keep_length, read_fastx, ring_policy,
+                   ring_shapes, topk_longest)
 
 FLAG_RIGHT = 0x06   # KSW_EZ_GENERIC_SC | KSW_EZ_RIGHT (include/align.h:134)
 FLAG_LEFT = 0x04    # KSW_EZ_GENERIC_SC, first MM2 segment (src/align/align.cpp:194-195)

@@ -94,6 +94,11 @@ def _load():
     lib.halign_mash_sketch.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
                                        _P, C.c_int64, _P]
     lib.halign_mash_best.argtypes = [_P, C.c_int32, _P, _P, C.c_int32, _P, _P, _P, _P]
+    lib.halign_topk_longest.argtypes = [_P, C.c_char_p, C.c_int32, C.c_char_p, C.POINTER(C.c_int64)]
+    lib.halign_consensus_file.argtypes = [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64, C.c_int64, _P,
+                                          C.c_int64, C.POINTER(C.c_int64)]
+    lib.halign_msa_auto_center.argtypes = [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32,
+                                           C.POINTER(MsaStats)]
     lib.halign_center_free.argtypes = [_P]
     lib.halign_center_free.restype = None
     lib.halign_center_anchors.argtypes = [_P, C.c_int32, C.c_char_p, _P, _P, C.c_int64, _P]
@@ -605,4 +610,38 @@ def msa_keep_length(ctx, center_path, input_path, output_path, batch=0, aligned_
         rc = ctx.lib.halign_msa_keep_length_refs(ctx.h, center_path.encode(), aligned_path.encode(),
                                                  input_path.encode(), output_path.encode(), int(batch), C.byref(st))
     ctx._check(rc, "halign_msa_keep_length")
+    return {f: getattr(st, f) for f, _ in MsaStats._fields_}
+
+
+def topk_longest(input_path, k, output_path, ctx=None):
+    """preprocessInputFasta's Top-K consensus input (halign_topk_longest, host
+    only): returns the number of records read."""
+    L = lib()
+    n = C.c_int64(0)
+    rc = L.halign_topk_longest(ctx.h if ctx else None, input_path.encode(), int(k), output_path.encode(),
+                               C.byref(n))
+    if rc != HALIGN_OK:
+        msg = L.halign_last_error(ctx.h).decode() if ctx else ""
+        raise HalignError(f"halign_topk_longest: {ERRORS.get(rc, rc)} {msg}")
+    return n.value
+
+
+def consensus_file(ctx, aligned_path, out_fasta, out_json, seq_limit=0, batch=4096):
+    """generateConsensusSequence on a file (halign_consensus_file): the consensus bytes."""
+    n = C.c_int64(0)
+    cap = max(16, os.path.getsize(aligned_path))
+    buf = C.create_string_buffer(cap)
+    ctx._check(ctx.lib.halign_consensus_file(ctx.h, aligned_path.encode(), out_fasta.encode(), out_json.encode(),
+                                             int(seq_limit), int(batch), buf, cap, C.byref(n)),
+               "halign_consensus_file")
+    return buf.raw[:n.value]
+
+
+def msa_auto_center(ctx, input_path, output_path, cons_n=1000, aligned_topk_path=None, batch=0):
+    """halign4 --keep-length without a center (halign_msa_auto_center)."""
+    st = MsaStats()
+    ctx._check(ctx.lib.halign_msa_auto_center(ctx.h, input_path.encode(),
+                                              aligned_topk_path.encode() if aligned_topk_path else None,
+                                              output_path.encode(), int(cons_n), int(batch), C.byref(st)),
+               "halign_msa_auto_center")
     return {f: getattr(st, f) for f, _ in MsaStats._fields_}

@@ -265,6 +265,34 @@ int halign_msa_keep_length_refs(halign_ctx *ctx, const char *center_path, const 
                                 const char *input_path, const char *output_path, int32_t batch,
                                 halign_msa_stats *stats);
 
+/* preprocessInputFasta's consensus input (src/utils/preprocess.cpp:104-160
+ * with TopKLongestSelector, src/consensus/selector.cpp:48-215): the k longest
+ * records of input_path after seq_io::cleanSequence, longest first and the
+ * earlier one first among equal lengths, written as FASTA with 80-column
+ * lines; n_records (optional) = records read. Host only; ctx (optional)
+ * receives the error message. */
+int halign_topk_longest(halign_ctx *ctx, const char *input_path, int32_t k, const char *output_path,
+                        int64_t *n_records);
+/* consensus::generateConsensusSequence on a file (src/consensus/consensus.cpp:
+ * 611-703): the reference's reading loop (batches of `batch` records with the
+ * record after each batch dropped, at most seq_limit kept, 0 = all, every kept
+ * record of the first one's length), the counts on the GPU (halign_consensus),
+ * out_fasta (">consensus", 80-column lines) and out_json (the reference's
+ * JSON: num_seqs, aln_len, counts A C G T U N -). cons_out (optional, cap
+ * bytes) receives the consensus, cons_len its length. An empty file, an empty
+ * first record or a length mismatch: HALIGN_E_ARG (the reference throws). */
+int halign_consensus_file(halign_ctx *ctx, const char *aligned_path, const char *out_fasta, const char *out_json,
+                          uint64_t seq_limit, int64_t batch, char *cons_out, int64_t cap, int64_t *cons_len);
+/* halign4 --keep-length without a center (src/halign4.cpp:87, 119-147): the
+ * cons_n longest cleaned records (halign_topk_longest), their alignment by the
+ * MSA command (aligned_topk_path: its output; NULL: the copy command, the
+ * records as they are, which must then be of one length), its consensus with
+ * seq_limit cons_n, and the keep-length run against that consensus
+ * (halign_msa_keep_length). Intermediate files sit next to output_path and
+ * are removed. */
+int halign_msa_auto_center(halign_ctx *ctx, const char *input_path, const char *aligned_topk_path,
+                           const char *output_path, int32_t cons_n, int32_t batch, halign_msa_stats *stats);
+
 /* mash::sketchFromSequence (src/mash/mash.cpp:32-146, getHash2bit src/mash/
  * hash.cpp) for n sequences (seqs[offsets[i] .. offsets[i+1]), ASCII): the
  * sketch_size smallest distinct XXH3-64 hashes (seeded) of the k-mers without

@@ -11,6 +11,7 @@
 #include "utils.h"
 #include "config.hpp"
 #include "mash.h"
+#include "preprocess.h"
 
 #include <cstring>
 #include <filesystem>
@@ -388,6 +389,44 @@ double ref_mash_jaccard(const uint64_t* a, int na, const uint64_t* b, int nb, in
     x.hashes.assign(a, a + na);
     y.hashes.assign(b, b + nb);
     return mash::jaccard(x, y);
+}
+
+// preprocessInputFasta (src/utils/preprocess.cpp:19-160) into work_dir: the
+// Top-K consensus input lands in work_dir/data/clean_data/consensus_unaligned.fasta.
+// Returns the records read, or -2 when the reference throws.
+int64_t ref_preprocess(const char* input_path, const char* work_dir, int cons_n)
+{
+    try {
+        return (int64_t)preprocessInputFasta(input_path, work_dir, cons_n);
+    } catch (...) {
+        return -2;
+    }
+}
+
+// halign4 --keep-length without -c (src/halign4.cpp:87, 119-147) with the
+// copy MSA command (or the given aligned Top-K file) and one thread:
+// preprocessInputFasta, the consensus of the aligned Top-K with seq_limit
+// cons_n, then RefAligner on the consensus file as ref_keep_length_msa_refs
+// runs it. Returns 0, or -2 when the reference throws.
+int ref_auto_center_keep_length(const char* input_path, const char* aligned_topk, const char* work_dir,
+                                const char* out_path, int cons_n)
+{
+    namespace fs = std::filesystem;
+    try {
+        const fs::path wd(work_dir);
+        preprocessInputFasta(input_path, work_dir, cons_n);
+        const fs::path clean = wd / WORKDIR_DATA / DATA_CLEAN;
+        const fs::path unaligned = clean / CLEAN_CONS_UNALIGNED, aligned = clean / CLEAN_CONS_ALIGNED;
+        file_io::copyFile(aligned_topk ? fs::path(aligned_topk) : unaligned, aligned);
+        consensus::generateConsensusSequence(aligned, clean / CLEAN_CONS_FASTA, clean / CLEAN_CONS_JSON,
+                                             (uint64_t)cons_n, 1, 4096);
+        const fs::path wd2 = wd / "ref_aligner";
+        fs::create_directories(wd2);
+        const std::string cons_fa = (clean / CLEAN_CONS_FASTA).string();
+        return ref_keep_length_msa_refs(cons_fa.c_str(), nullptr, input_path, wd2.string().c_str(), out_path);
+    } catch (...) {
+        return -2;
+    }
 }
 
 }  // extern "C"

@@ -28,6 +28,8 @@ Outputs (np.savez_compressed, no pickles):
                        seed), and the Jaccard choice of a reference
   msa_refs.npz         halign4 -c refs --keep-length with several references
                        (copy command, or a given aligned file): final FASTA bytes
+  auto_center.npz      preprocessInputFasta's Top-K file, generateConsensusSequence's
+                       FASTA and JSON files, and halign4 --keep-length without -c
 """
 import json
 import os
@@ -614,6 +616,97 @@ def msa_refs(R, rng):
     save("msa_refs.npz", has_aligned=np.array(has_aligned, np.int32), **arrays)
 
 
+def auto_center(R, rng):
+    """The auto-center path with --keep-length: preprocessInputFasta's Top-K
+    consensus input (src/utils/preprocess.cpp:19-160), generateConsensusSequence's
+    FASTA and JSON outputs (src/consensus/consensus.cpp:38-68, 611-703) and the
+    whole run (src/halign4.cpp:87, 119-147) with the copy MSA command or a
+    given aligned Top-K file."""
+    d = tempfile.mkdtemp()
+    out = {k: [] for k in ("topk_in", "topk_out", "cons_in", "cons_fa", "cons_json", "run_in", "run_aligned",
+                           "run_out")}
+    topk_k, cons_par, run_par = [], [], []
+    alpha = np.frombuffer(b"ACGTACGTacgtNnRY", np.uint8)
+    # Top-K: lengths with ties, cleaning of lower case and IUPAC
+    for case, (n, k) in enumerate([(60, 7), (30, 30), (12, 50), (40, 1), (25, 0)]):
+        lens = rng.integers(1, 400, n)
+        lens[rng.integers(0, n, n // 3)] = 333   # ties
+        recs = b"".join(b">s%d%s\n" % (i, b" x" if i % 3 == 0 else b"") + alpha[rng.integers(0, len(alpha), int(L))].tobytes()
+                        + b"\n" for i, L in enumerate(lens))
+        ip = os.path.join(d, "t%d.fa" % case)
+        open(ip, "wb").write(recs)
+        wd = os.path.join(d, "tw%d" % case)
+        os.makedirs(wd)
+        assert R.preprocess(ip, wd, k) == n
+        out["topk_in"].append(np.frombuffer(recs, np.uint8))
+        out["topk_out"].append(np.frombuffer(open(os.path.join(wd, "data", "clean_data", "consensus_unaligned.fasta"),
+                                                   "rb").read(), np.uint8))
+        topk_k.append(k)
+    # consensus files: gaps, dots, lower case, the batch drop and seq_limit
+    calpha = np.frombuffer(b"ACGTUN-.acgtuRYX", np.uint8)
+    for case, (nrows, aln, batch, lim) in enumerate([(1, 7, 4096, 0), (5, 90, 4096, 0), (30, 33, 4, 0),
+                                                     (30, 33, 4, 9), (4100, 5, 4096, 0), (17, 161, 3, 12)]):
+        rows = calpha[rng.integers(0, len(calpha), (nrows, aln))]
+        text = b"".join(b">r%d\n" % i + rows[i].tobytes() + b"\n" for i in range(nrows))
+        fa = os.path.join(d, "c%d.fa" % case)
+        open(fa, "wb").write(text)
+        R.consensus_file(fa, fa + ".out.fa", fa + ".out.json", seq_limit=lim, batch=batch)
+        out["cons_in"].append(np.frombuffer(text, np.uint8))
+        out["cons_fa"].append(np.frombuffer(open(fa + ".out.fa", "rb").read(), np.uint8))
+        out["cons_json"].append(np.frombuffer(open(fa + ".out.json", "rb").read(), np.uint8))
+        cons_par.append((lim, batch))
+    # whole runs: the cons_n longest queries share one length (copy command), or
+    # an aligned Top-K file with gap columns
+    for case in range(2):
+        W = 2500
+        base = ACGT[rng.integers(0, 4, W)]
+        recs, aligned = [], []
+        ktop = 5
+        for i in range(ktop):
+            b = base.copy()
+            b[rng.integers(0, W, 30)] = ACGT[rng.integers(0, 4, 30)]
+            present = np.ones(W, bool)
+            if case == 1:
+                a = int(rng.integers(0, W - 100))
+                present[a:a + int(rng.integers(1, 60))] = False
+            recs.append((b"long%d" % i, b[present].tobytes() if case == 1 else b.tobytes()))
+            aligned.append(np.where(present, b, ord("-")).astype(np.uint8).tobytes())
+        lut = np.zeros(256, np.uint8)
+        lut[ACGT] = np.arange(4, dtype=np.uint8)
+        for i in range(20):
+            q = mutate_codes(rng, lut[base], 0.01, 0.004)
+            L = int(rng.integers(300, W - 200))   # shorter than the Top-K records
+            a = int(rng.integers(0, len(q) - L))
+            recs.append((b"q%d" % i, codes_to_ascii(q[a:a + L])))
+        if case == 1:   # the longest records must come first by length: keep them longest
+            recs[ktop:] = [(n, x[:W - 300]) for n, x in recs[ktop:]]
+        text = b"".join(b">" + n + b"\n" + x + b"\n" for n, x in recs)
+        ip = os.path.join(d, "r%d.fa" % case)
+        open(ip, "wb").write(text)
+        ap = None
+        atext = b""
+        if case == 1:
+            # the aligned Top-K in the order the Top-K file lists them (longest first)
+            order = sorted(range(ktop), key=lambda i: (-len(recs[i][1]), i))
+            atext = b"".join(b">" + recs[i][0] + b"\n" + aligned[i] + b"\n" for i in order)
+            ap = os.path.join(d, "r%d.aligned.fa" % case)
+            open(ap, "wb").write(atext)
+        wd = os.path.join(d, "rw%d" % case)
+        os.makedirs(wd)
+        op = os.path.join(d, "r%d.out.fa" % case)
+        rc = R.auto_center_keep_length(ip, ap, wd, op, ktop)
+        assert rc == 0, (case, rc)
+        out["run_in"].append(np.frombuffer(text, np.uint8))
+        out["run_aligned"].append(np.frombuffer(atext, np.uint8))
+        out["run_out"].append(np.frombuffer(open(op, "rb").read(), np.uint8))
+        run_par.append((ktop, int(ap is not None)))
+    arrays = {}
+    for k, v in out.items():
+        arrays[k], arrays[k + "_off"] = concat(v, np.uint8)
+    save("auto_center.npz", topk_k=np.array(topk_k, np.int32), cons_par=np.array(cons_par, np.int64),
+         run_par=np.array(run_par, np.int32), **arrays)
+
+
 def main():
     R = RefLib()
     O = Oracle()
@@ -629,6 +722,7 @@ def main():
     msa_keep_length(R, np.random.default_rng(0x6024))
     mash_cases(R, np.random.default_rng(0x6025))
     msa_refs(R, np.random.default_rng(0x6026))
+    auto_center(R, np.random.default_rng(0x6027))
 
 
 if __name__ == "__main__":

@@ -442,6 +442,18 @@ class RefLib(_Base):
         L.ref_hash2bit.restype = C.c_uint64
         return L.ref_hash2bit(code, seed)
 
+    def preprocess(self, input_path, work_dir, cons_n):
+        L = self.lib
+        L.ref_preprocess.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.ref_preprocess.restype = C.c_int64
+        return L.ref_preprocess(input_path.encode(), work_dir.encode(), cons_n)
+
+    def auto_center_keep_length(self, input_path, aligned_topk, work_dir, out_path, cons_n):
+        L = self.lib
+        L.ref_auto_center_keep_length.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+        return L.ref_auto_center_keep_length(input_path.encode(), aligned_topk.encode() if aligned_topk else None,
+                                             work_dir.encode(), out_path.encode(), cons_n)
+
     def keep_length_msa_refs(self, center_path, aligned_path, input_path, work_dir, out_path):
         """halign4 -c refs --keep-length, one thread; aligned_path = the MSA
         command's output for the refs (None: the copy command)."""
