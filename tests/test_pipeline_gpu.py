@@ -136,6 +136,54 @@ def test_align_batch_vs_oracle(ctx):
     c.close()
 
 
+def test_one_gap_segments_vs_oracle(ctx):
+    """Queries whose end segments have a unique one-gap optimum (CIGAR written
+    without the DP) and near misses that must run the DP: exact windows of the
+    center (a long reference part against the query bases before the first
+    anchor), the same with the first or last base set equal to the center's
+    first or last base (a tie: the DP decides), windows reaching past the
+    center's ends (insertions against a short reference part), and N at the
+    edges. The oracle runs every segment through its DP."""
+    import halign
+    O = Oracle()
+    rng = np.random.default_rng(91)
+    center = rand_codes(rng, 8000)
+    cstr = codes_to_ascii(center)
+    qs = []
+    for i in range(240):
+        kind = i % 8
+        a = int(rng.integers(0, 7000))
+        L = int(rng.integers(200, 3000))
+        q = center[a:a + L].copy()
+        if kind == 1:
+            q[0] = center[0]              # tie at the front
+        elif kind == 2:
+            q[-1] = center[-1]            # tie at the back
+        elif kind == 3:                   # runs past the center's start
+            q = np.concatenate([rand_codes(rng, int(rng.integers(1, 40))), center[:L]])
+        elif kind == 4:                   # runs past its end
+            q = np.concatenate([center[-L:], rand_codes(rng, int(rng.integers(1, 40)))])
+        elif kind == 5:
+            q[0] = 4                      # N at the front
+        elif kind == 6:
+            q = mutate_codes(rng, q, 0.01, 0.002)
+        elif kind == 7:                   # past the start, with the shared first base
+            q = np.concatenate([center[:1], rand_codes(rng, int(rng.integers(1, 40))), center[:L]])
+        qs.append(codes_to_ascii(q))
+    print("one-gap segments vs oracle: 240 queries, 8 kb center", file=sys.stderr, flush=True)
+    c = halign.Center(ctx, cstr)
+    cigs, st, ms = c.align(qs)
+    bad = []
+    for i, q in enumerate(qs):
+        exp, _, _, ub = O.ref_global_align(cstr, q)
+        if ub & 3:
+            continue
+        if not np.array_equal(cigs[i], exp):
+            bad.append(i)
+    assert not bad, f"{len(bad)} differ, first {bad[:5]}"
+    c.close()
+
+
 def test_rows_and_insertion_flags(ctx):
     """delQueryToRefByCigar rows and hasInsertion from the device, against the
     oracle's projection of the same CIGARs (and the CIGARs against the oracle's
