@@ -131,53 +131,65 @@ __global__ void seg_set_offsets(SegRec *__restrict__ segs, const int64_t *__rest
 
 
 
-// One-gap segments: with a full band (auto_band -1) ksw_extz2 finds the
-// optimal alignment exactly, so where that optimum is unique its CIGAR is
-// known without the DP. For a query part Q of length m shorter than the
-// reference part R of length n, every alignment has at least n - m deleted
-// bases; one run of them and m matching ACGT pairs scores 5m - (6 + 2(n-m)),
-// and anything else (a mismatch, a second gap run, an insertion) scores less.
-// One run with all m bases matching is M^a D^(n-m) M^(m-a) with Q[0,a) =
-// R[0,a) and Q[a,m) = R[a+n-m,n). If Q = R[n-m,n) that holds for a = 0 and,
-// beyond it, exactly for the a >= 1 with Q[0,a) = R[0,a), i.e. when Q[0] =
-// R[0]; so D^(n-m) M^m is the unique optimum iff Q = R[n-m,n) and Q[0] !=
-// R[0]. Mirrored: M^m D^(n-m) iff Q = R[0,m) and Q[m-1] != R[n-1]; with the
-// roles swapped (m > n) the same with I. Returns the two ops, or 0.
+// One-gap segments: with a full band (auto_band -1) ksw_extz2 finds an
+// optimal alignment exactly. For a query part Q of length m shorter than the
+// reference part R of length n, every alignment has at least g = n - m
+// deleted bases; one run of them with m matching ACGT pairs scores
+// 5m - (6 + 2g), and anything else (a mismatch or N pair, a second gap run,
+// an insertion) scores less. One run with all m bases matching is
+// M^a D^g M^(m-a) with Q[0,a) = R[0,a) and Q[a,m) = R[a+g,n). If Q = R[g,n)
+// (Q is R's tail) the second part holds for every a, so the optimal
+// alignments are exactly those with a up to the common ACGT prefix of Q and
+// R; if Q = R[0,m) (R's head) they are M^(m-b) D^g M^b with b up to the
+// common suffix. Among them ksw_extz2 returns the leftmost gap (a = 0,
+// resp. the largest b) and, with KSW_EZ_RIGHT, the rightmost one (the
+// largest a, resp. b = 0): its tie order puts gaps left or right. With the
+// roles swapped (m > n) the same holds with I. Returns the number of ops
+// written to op (2 or 3), or 0 when neither case applies.
 __device__ __forceinline__ int one_gap(const uint8_t *__restrict__ r, const uint8_t *__restrict__ qr, int n, int m,
-                                       uint32_t &op0, uint32_t &op1)
+                                       bool right, uint32_t (&op)[3])
 {
     // r[k]: reference base k; qr[-k]: query base k (the query is stored reversed)
     if (n == m || auto_band_dev(n, m) >= 0) return 0;
     const bool del = m < n;
     const int s = del ? m : n, g = del ? n - m : m - n;   // matched bases, gap length
-    const uint32_t gop = (uint32_t)g << 4 | (del ? 2u : 1u), mop = (uint32_t)s << 4;
-    // the shorter side against the longer one's last s bases
+    const uint32_t gop = (uint32_t)g << 4 | (del ? 2u : 1u);
+    int a = -1;   // matched bases before the gap
     bool tail = true;
     for (int k = 0; k < s && tail; ++k) {
-        const uint8_t a = del ? r[g + k] : r[k], b = del ? qr[-k] : qr[-(g + k)];
-        tail = a < 4 && a == b;
+        const uint8_t x = del ? r[g + k] : r[k], y = del ? qr[-k] : qr[-(g + k)];
+        tail = x < 4 && x == y;
     }
-    if (tail && r[0] != qr[0]) {
-        op0 = gop;
-        op1 = mop;
-        return 2;
+    if (tail) {
+        a = 0;
+        if (right)
+            while (a < s && r[a] < 4 && r[a] == qr[-a]) ++a;   // common prefix of R and Q
+    } else {
+        bool head = true;
+        for (int k = 0; k < s && head; ++k) head = r[k] < 4 && r[k] == qr[-k];
+        if (head) {
+            a = s;
+            if (!right) {
+                int b = 0;   // common suffix of R and Q
+                while (b < s && r[n - 1 - b] < 4 && r[n - 1 - b] == qr[-(m - 1 - b)]) ++b;
+                a = s - b;
+            }
+        }
     }
-    bool head = true;
-    for (int k = 0; k < s && head; ++k) head = r[k] < 4 && r[k] == qr[-k];
-    if (head && r[n - 1] != qr[-(m - 1)]) {
-        op0 = mop;
-        op1 = gop;
-        return 2;
-    }
-    return 0;
+    if (a < 0) return 0;
+    int k = 0;
+    if (a > 0) op[k++] = (uint32_t)a << 4;
+    op[k++] = gop;
+    if (s - a > 0) op[k++] = (uint32_t)(s - a) << 4;
+    return k;
 }
 
 // Segments split three ways, one thread per segment: an equal-length small
 // segment whose bases are identical ACGT (the k-mer of an anchor, nearly every
 // one) has the all-match path as its unique optimum, so ksw_extz2 returns the
-// CIGAR nM; a segment with a unique one-gap optimum (one_gap: typically the
-// ends of a query cut from the center, a long reference part against the few
-// query bases before the first anchor) gets its two ops. Both are written here,
+// CIGAR nM; a segment whose optimum is one gap (one_gap: typically the ends
+// of a query cut from the center, a long reference part against the few query
+// bases before the first anchor) gets its two or three ops. Both are written here,
 // class kSegExact. Other small segments go to the lane list, large ones to the
 // host list (both appended with an atomic).
 __global__ __launch_bounds__(256) void seg_split(SegRec *__restrict__ segs, int64_t n,
@@ -199,14 +211,14 @@ __global__ __launch_bounds__(256) void seg_split(SegRec *__restrict__ segs, int6
     unsigned int rank = 0;
     if (i < n) {
         const SegRec sg = segs[i];
-        uint32_t op0 = 0, op1 = 0;
-        const int srl0 = (int)(sg.re - sg.rs), sql0 = (int)(sg.qe - sg.qs);
-        if ((sg.cls == kSegHost || sg.cls == kSegLane) &&
-            one_gap(center + sg.rs, qrev + qrev_off[sg.query] + (qlens[sg.query] - 1 - sg.qs), srl0, sql0, op0,
-                    op1) == 2) {
-            cigs[sg.cig_off] = op0;
-            cigs[sg.cig_off + 1] = op1;
-            ncig[i] = 2;
+        uint32_t op[3];
+        int nop = 0;
+        if (sg.cls == kSegHost || sg.cls == kSegLane)
+            nop = one_gap(center + sg.rs, qrev + qrev_off[sg.query] + (qlens[sg.query] - 1 - sg.qs),
+                          (int)(sg.re - sg.rs), (int)(sg.qe - sg.qs), (sg.flags & kSegRight) != 0, op);
+        if (nop > 0) {
+            for (int k = 0; k < nop; ++k) cigs[sg.cig_off + k] = op[k];
+            ncig[i] = nop;
             segs[i].cls = kSegExact;
         } else if (sg.cls == kSegHost) {
             which = 1;

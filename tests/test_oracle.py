@@ -171,6 +171,69 @@ def test_ksw_full_band_is_gotoh_optimum(O):
             assert (sc, ti, qi) == (best, len(t), len(q)), (it, flag, cigar_str(cig))
 
 
+def one_gap_closed_form(R, Q, right):
+    """The one-gap optimum of globalAlignKSW2 under a full band (auto_band < 0),
+    from the scores alone: a shorter side equal to the longer side's tail or
+    head aligns as one gap run of |n - m| and s = min(n, m) matching ACGT
+    pairs, and every other path scores less. The optimal paths are
+    M^a gap M^(s-a) for a up to the common prefix (tail case) or from s minus
+    the common suffix (head case); ksw2 puts the gap leftmost without
+    KSW_EZ_RIGHT and rightmost with it. None when neither case holds."""
+    n, m = len(R), len(Q)
+    acgt = b"ACGT"
+    if n == m:
+        return None
+    g, s = abs(n - m), min(n, m)
+    gop = g << 4 | (2 if m < n else 1)
+    long_, short = (R, Q) if m < n else (Q, R)
+    tail = all(short[k] in acgt and short[k] == long_[g + k] for k in range(s))
+    head = all(short[k] in acgt and short[k] == long_[k] for k in range(s))
+    lcp = lcs = 0
+    while lcp < s and R[lcp] in acgt and R[lcp] == Q[lcp]:
+        lcp += 1
+    while lcs < s and R[n - 1 - lcs] in acgt and R[n - 1 - lcs] == Q[m - 1 - lcs]:
+        lcs += 1
+    if tail:
+        a = lcp if right else 0
+    elif head:
+        a = s if right else s - lcs
+    else:
+        return None
+    return [x for x in (a << 4, gop, (s - a) << 4) if x >> 4]
+
+
+def test_one_gap_closed_form(O):
+    """Segments whose optimum is one gap run get their CIGAR from the closed
+    form above, without the DP: the form against the oracle's DP on both
+    flags, including the ties (shared bases at the far end, 1- and 2-letter
+    alphabets) where only the flag picks the path, insertions (roles
+    swapped) and near misses (first base changed) that must not qualify."""
+    rng = np.random.default_rng(5)
+    A = b"ACGT"
+    tot = 0
+    for it in range(6000):
+        kind = it % 6
+        m, g = int(rng.integers(1, 30)), int(rng.integers(1, 300))
+        alph = {1: 2, 4: 2, 5: 1}.get(kind, 4)
+        n = m + g
+        R = bytes(A[int(x)] for x in rng.integers(0, alph, n))
+        Q = R[n - m:] if it % 2 == 0 else R[:m]
+        if kind == 3:
+            Q = bytes([A[int(rng.integers(0, 4))]]) + Q[1:]
+        if it % 3 == 0:
+            R, Q = Q, R
+        if O.auto_band(len(R), len(Q)) >= 0:
+            continue
+        for flag in (FLAG_LEFT, FLAG_RIGHT):
+            exp = one_gap_closed_form(R, Q, flag == FLAG_RIGHT)
+            if exp is None:
+                continue
+            tot += 1
+            cig, _ = O.global_align_ksw2(R, Q, flag)
+            assert list(cig) == exp, (it, flag, R[:12], Q[:12], cigar_str(cig), cigar_str(exp))
+    assert tot > 8000
+
+
 def test_banded_cigars_consume_both_sides(O):
     """test/test_align.cpp:1418-1626: a global CIGAR consumes the whole query
     and reference, also under a narrow band (no overflow)."""

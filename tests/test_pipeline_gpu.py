@@ -137,21 +137,25 @@ def test_align_batch_vs_oracle(ctx):
 
 
 def test_one_gap_segments_vs_oracle(ctx):
-    """Queries whose end segments have a unique one-gap optimum (CIGAR written
+    """Queries whose end segments have a one-gap optimum (CIGAR written
     without the DP) and near misses that must run the DP: exact windows of the
     center (a long reference part against the query bases before the first
-    anchor), the same with the first or last base set equal to the center's
-    first or last base (a tie: the DP decides), windows reaching past the
-    center's ends (insertions against a short reference part), and N at the
-    edges. The oracle runs every segment through its DP."""
+    anchor), windows whose first or last bases equal the center's first or
+    last bases (ties, broken by the segment's gap direction), the same with
+    only the edge base set to the center's (near misses), windows reaching
+    past the center's ends (insertions against a short reference part), and
+    N at the edges. The oracle runs every segment through its DP."""
     import halign
     O = Oracle()
     rng = np.random.default_rng(91)
     center = rand_codes(rng, 8000)
     cstr = codes_to_ascii(center)
+    # window starts sharing the center's first 3 bases, ends sharing its last 3
+    starts = [p for p in range(1, 7000) if np.array_equal(center[p:p + 3], center[:3])]
+    ends = [e for e in range(3000, 7999) if np.array_equal(center[e - 3:e], center[-3:])]
     qs = []
-    for i in range(240):
-        kind = i % 8
+    for i in range(300):
+        kind = i % 10
         a = int(rng.integers(0, 7000))
         L = int(rng.integers(200, 3000))
         q = center[a:a + L].copy()
@@ -169,8 +173,15 @@ def test_one_gap_segments_vs_oracle(ctx):
             q = mutate_codes(rng, q, 0.01, 0.002)
         elif kind == 7:                   # past the start, with the shared first base
             q = np.concatenate([center[:1], rand_codes(rng, int(rng.integers(1, 40))), center[:L]])
+        elif kind == 8:                   # tie: starts like the center
+            a = int(rng.choice(starts))
+            q = center[a:a + L].copy()
+        elif kind == 9:                   # tie: ends like the center
+            e = int(rng.choice(ends))
+            q = center[e - L:e].copy()
         qs.append(codes_to_ascii(q))
-    print("one-gap segments vs oracle: 240 queries, 8 kb center", file=sys.stderr, flush=True)
+    print(f"one-gap segments vs oracle: {len(qs)} queries, 8 kb center, {len(starts)} tie starts, "
+          f"{len(ends)} tie ends", file=sys.stderr, flush=True)
     c = halign.Center(ctx, cstr)
     cigs, st, ms = c.align(qs)
     bad = []
