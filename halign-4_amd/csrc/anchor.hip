@@ -140,32 +140,29 @@ __global__ __launch_bounds__(256) void anchors_write(CenterIndexDev ci, const ui
     }
 }
 
-// chainScoreSimple (src/seed/anchor.cpp:254-301); INT_MIN = no link
-// Chaining score of predecessor j for anchor i (src/seed/anchor.cpp:232-262):
-// fields of j, anchor i in scalars. INT_MIN: j cannot precede i.
-__device__ __forceinline__ int32_t chain_score(uint32_t i_rid, uint32_t i_rev, uint32_t i_pr, uint32_t i_pq,
-                                               uint32_t j_rid, uint32_t j_rev, uint32_t j_pr, uint32_t j_pq,
-                                               uint32_t j_span, const ChainParamsDev &cp, const float *__restrict__ lut)
+// Chaining score of predecessor j for anchor i (chainScoreSimple,
+// src/seed/anchor.cpp:232-262): fields of j, anchor i in scalars, the log2
+// table in LDS. INT_MIN: j cannot precede i. Branch-free: every lane
+// evaluates the penalty (the table index clamped) and the tests select.
+__device__ __forceinline__ int32_t chain_score(uint32_t i_rr, uint32_t i_pr, uint32_t i_pq, uint32_t j_rr,
+                                               uint32_t j_pr, uint32_t j_pq, uint32_t j_span,
+                                               const ChainParamsDev &cp, const float *lut)
 {
-    if (i_rid != j_rid) return INT_MIN;
-    if (i_rev != j_rev) return INT_MIN;
+#pragma clang fp contract(off)
+    // each operation rounded on its own, no fused multiply-add: the
+    // reference's production build has none
     const int32_t dq = (int32_t)i_pq - (int32_t)j_pq;
-    if (dq <= 0 || dq > cp.max_dist_x) return INT_MIN;
     const int32_t dr = (int32_t)i_pr - (int32_t)j_pr;
-    if (dr <= 0 || dr > cp.max_dist_y) return INT_MIN;
     const int32_t dd = dr > dq ? dr - dq : dq - dr;
-    if (dd > cp.bw) return INT_MIN;
     const int32_t dg = dr < dq ? dr : dq;
+    const bool ok = i_rr == j_rr && dq > 0 && dq <= cp.max_dist_x && dr > 0 && dr <= cp.max_dist_y && dd <= cp.bw;
     const int32_t q_span = (int32_t)j_span;
     int32_t sc = q_span < dg ? q_span : dg;
-    if (dd > 0 || dg > q_span) {
-        // each operation rounded on its own, no contraction: the reference's
-        // production build has no FMA
-        const float lin = __fadd_rn(__fmul_rn(cp.gap_penalty, (float)dd), __fmul_rn(cp.skip_penalty, (float)dg));
-        const float lp = dd >= 1 ? lut[dd + 1] : 0.0f;
-        sc -= (int32_t)__fadd_rn(lin, __fmul_rn(0.5f, lp));
-    }
-    return sc;
+    const int32_t di = ok && dd >= 1 ? dd + 1 : 0;   // lut[0] = 0
+    const float lin = cp.gap_penalty * (float)dd + cp.skip_penalty * (float)dg;
+    const float pen = lin + 0.5f * lut[di];
+    if (dd > 0 || dg > q_span) sc -= (int32_t)pen;
+    return ok ? sc : INT_MIN;
 }
 
 // lane l receives lane l-1 (lane 0 keeps its own value): DPP wave_shr:1
@@ -174,7 +171,29 @@ __device__ __forceinline__ int32_t wave_shr1(int32_t v)
     return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);
 }
 
+// lane l receives lane l-1, lane 0 receives id
+__device__ __forceinline__ int32_t wave_shr1_id(int32_t v, int32_t id)
+{
+    return __builtin_amdgcn_update_dpp(id, v, 0x138, 0xF, 0xF, false);
+}
+
+// inclusive scan over the 64 lanes with DPP: row_shr 1, 2, 4, 8 inside each
+// row of 16 lanes, then row_bcast:15 into rows 1 and 3 and row_bcast:31 into
+// rows 2 and 3; a lane with no source lane takes the identity id
+template <class Op>
+__device__ __forceinline__ int32_t wave_scan_incl(int32_t v, int32_t id, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x111, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x112, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x114, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x118, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xC, 0xF, false));
+    return v;
+}
+
 constexpr int kChainMaxIter = 8192;   // bound on ChainParams::max_iter (LDS mark bitmap)
+constexpr int kChainLut = 502;        // log2 table entries: bw <= 500
 
 // chainAnchors (src/seed/anchor.cpp:264-330), one wave per query. order:
 // anchors of query q sorted by (pos_ref, pos_qry), as indices into `anchors`
@@ -204,8 +223,11 @@ __global__ __launch_bounds__(64) void chain_wave(const AnchorRec *__restrict__ a
                                                  int32_t *__restrict__ chain_cnt)
 {
     __shared__ uint32_t marks[kChainMaxIter / 32];
+    __shared__ float slut[kChainLut];
     const int q = blockIdx.x;
     const int lane = (int)threadIdx.x;
+    for (int k = lane; k < kChainLut; k += 64) slut[k] = lut[k];
+    __syncthreads();
     const int64_t base = aoff[q];
     const int n = (int)na_all[q];
     const uint32_t *ord = order + base;
@@ -282,11 +304,8 @@ __global__ __launch_bounds__(64) void chain_wave(const AnchorRec *__restrict__ a
                     a_p = __hip_atomic_load(pr + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
-            int32_t sc = INT_MIN;
-            if (inwin)
-                sc = chain_score(ai_rr >> 1, ai_rr & 1u, ai.pos_ref, ai.pos_qry, a_rr >> 1, a_rr & 1u, a_pr, a_pq,
-                                 a_sp, cp, lut);
-            const bool valid = sc != INT_MIN;
+            const int32_t sc = chain_score(ai_rr, ai.pos_ref, ai.pos_qry, a_rr, a_pr, a_pq, a_sp, cp, slut);
+            const bool valid = inwin && sc != INT_MIN;
             const int32_t total = valid ? a_f + sc : INT_MIN;
             if (valid && a_p >= iter_start) {
                 const int bt = i - 1 - a_p;
@@ -297,29 +316,27 @@ __global__ __launch_bounds__(64) void chain_wave(const AnchorRec *__restrict__ a
             const int bj = inwin ? i - 1 - j : 0;
             const bool marked = inwin && ((marks[bj >> 5] >> (bj & 31)) & 1u);
             // exclusive prefix max of the scored totals over the lanes before
-            int32_t incl = total;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int32_t o = __shfl_up(incl, d);
-                if (lane >= d) incl = incl > o ? incl : o;
-            }
-            const int32_t prev = __shfl_up(incl, 1);
-            const int32_t before = lane ? (prev > max_f ? prev : max_f) : max_f;
+            const auto mx = [](int32_t x, int32_t y) { return x > y ? x : y; };
+            const auto mn = [](int32_t x, int32_t y) { return x < y ? x : y; };
+            const auto add = [](int32_t x, int32_t y) { return x + y; };
+            const int32_t prev = wave_shr1_id(wave_scan_incl(total, INT_MIN, mx), INT_MIN);
+            const int32_t before = prev > max_f ? prev : max_f;
             const bool imp = valid && total > before;
+            const bool skp = valid && !imp && marked;
             const uint64_t I = __ballot(imp);
-            const uint64_t M = __ballot(valid && !imp && marked);
-            uint64_t E = I | M;
-            int brk = 64;
-            while (E) {
-                const int bl = (int)__builtin_ctzll(E);
-                E &= E - 1;
-                if ((I >> bl) & 1ull) {
-                    if (n_skip > 0) --n_skip;
-                } else if (++n_skip > cp.max_skip) {
-                    brk = bl;
-                    break;
-                }
-            }
+            // The skip counter over the candidates in lane order: an improving
+            // one lowers it (not below 0), a skipped one raises it and breaks
+            // the scan when it passes max_skip. With step d_l = -1 / +1 / 0 and
+            // S_l the inclusive sum of the steps, the counter after lane l is
+            // S_l - min(-n_skip, min over lanes m <= l of S_m) (the reflected
+            // walk n_l = max(n_(l-1) + d_l, 0)); the first skipped lane with a
+            // counter above max_skip is the break.
+            const int32_t S = wave_scan_incl(imp ? -1 : (skp ? 1 : 0), 0, add);
+            const int32_t P = wave_scan_incl(S, INT_MAX, mn);
+            const int32_t cnt_l = S - (P < -n_skip ? P : -n_skip);
+            const uint64_t B = __ballot(skp && cnt_l > cp.max_skip);
+            const int brk = B ? (int)__builtin_ctzll(B) : 64;
+            if (!B) n_skip = __builtin_amdgcn_readlane(cnt_l, 63);
             const uint64_t Iv = brk == 64 ? I : (I & ((1ull << brk) - 1ull));
             if (Iv) {
                 const int lb = 63 - (int)__builtin_clzll(Iv);
@@ -431,7 +448,7 @@ hipError_t chain_launch(const AnchorRec *anchors, const uint32_t *order, const i
                         int32_t *chain_cnt, hipStream_t stream)
 {
     if (nq <= 0) return hipSuccess;
-    if (cp.max_iter < 0 || cp.max_iter > kChainMaxIter) return hipErrorInvalidValue;
+    if (cp.max_iter < 0 || cp.max_iter > kChainMaxIter || cp.bw > kChainLut - 2) return hipErrorInvalidValue;
     hipLaunchKernelGGL(chain_wave, dim3(nq), dim3(64), 0, stream, anchors, order, aoff, na, cp, lut, f, p, chain_idx,
                        chain_cnt);
     return hipGetLastError();

@@ -136,6 +136,45 @@ def test_align_batch_vs_oracle(ctx):
     c.close()
 
 
+def test_chains_vs_oracle(ctx):
+    """chainAnchors on the device against the oracle's, anchor for anchor, on a
+    center with tandem repeats (many equal-hash anchors: long predecessor
+    windows over several 64-candidate chunks, improving and skipped
+    candidates interleaved, max_skip breaks) and plain random stretches;
+    queries cut from both, mutated, with swapped blocks."""
+    import halign
+    O = Oracle()
+    rng = np.random.default_rng(123)
+    unit = rand_codes(rng, 180)
+    rep = np.concatenate([mutate_codes(rng, unit, 0.02, 0.0) for _ in range(25)])
+    center = np.concatenate([rand_codes(rng, 3000), rep, rand_codes(rng, 3000), np.tile(unit[:37], 40)])
+    cstr = codes_to_ascii(center)
+    rx, ry, _ = O.minimizers(cstr)
+    qs = []
+    for i in range(160):
+        a = int(rng.integers(0, len(center) - 300))
+        q = mutate_codes(rng, center[a:a + int(rng.integers(300, 6000))], 0.02 if i % 2 else 0.005, 0.004)
+        if i % 5 == 0 and len(q) > 600:   # two blocks swapped
+            h = len(q) // 2
+            q = np.concatenate([q[h:], q[:h]])
+        qs.append(codes_to_ascii(q))
+    print(f"chains vs oracle: {len(qs)} queries, {len(center)} bp center with repeats", file=sys.stderr, flush=True)
+    c = halign.Center(ctx, cstr)
+    try:
+        got = c.chains(qs)
+    finally:
+        c.close()
+    bad, long_chains = [], 0
+    for i, q in enumerate(qs):
+        qx, qy, _ = O.minimizers(q)
+        _, exp = O.chain(O.collect_anchors(rx, ry, qx, qy))
+        long_chains += len(exp) > 64
+        if not np.array_equal(got[i].view(np.uint8), exp.view(np.uint8)):
+            bad.append(i)
+    assert not bad, f"{len(bad)} differ, first {bad[:5]}"
+    assert long_chains > 20
+
+
 def test_one_gap_segments_vs_oracle(ctx):
     """Queries whose end segments have a one-gap optimum (CIGAR written
     without the DP) and near misses that must run the DP: exact windows of the
