@@ -54,6 +54,14 @@ __device__ __forceinline__ uint32_t hit_anchors(const CenterIndexDev &ci, const 
     return rocc;
 }
 
+// Per query: the center hash index of every hit (binary search), the query's
+// own occurrence count of each center hash (block counters, cleared again
+// after the query), the anchors each hit expands to and their total.
+// MODE 2: hashes and counters in LDS; 1: counters in LDS; 0: counters in the
+// block's slice of scratch (global atomics, performed at the L2 of the
+// block's XCD: waiting for them before the barrier orders them for the
+// block, no cache maintenance needed).
+template <int MODE>
 __global__ __launch_bounds__(256) void anchors_count(CenterIndexDev ci, const uint64_t *__restrict__ qx,
                                                      const uint64_t *__restrict__ qy,
                                                      const int64_t *__restrict__ hit_off,
@@ -62,23 +70,40 @@ __global__ __launch_bounds__(256) void anchors_count(CenterIndexDev ci, const ui
                                                      uint32_t *__restrict__ nanch, int64_t *__restrict__ totals)
 {
     __shared__ int64_t sh[256];
-    uint32_t *cnt = scratch + (size_t)blockIdx.x * (size_t)ci.nd;
+    extern __shared__ uint64_t dyn[];
+    const int nd = ci.nd;
+    const uint64_t *dh = ci.dh;
+    uint32_t *cnt;
+    if constexpr (MODE == 2) {
+        for (int k = threadIdx.x; k < nd; k += 256) dyn[k] = ci.dh[k];
+        dh = dyn;
+        cnt = (uint32_t *)(dyn + nd);
+    } else if constexpr (MODE == 1) {
+        cnt = (uint32_t *)dyn;
+    } else {
+        cnt = scratch + (size_t)blockIdx.x * (size_t)nd;
+    }
+    if constexpr (MODE != 0)
+        for (int k = threadIdx.x; k < nd; k += 256) cnt[k] = 0;
+    __syncthreads();
     for (int q = blockIdx.x; q < nq_all; q += gridDim.x) {
         const int64_t base = hit_off[q], n = hit_cnt[q];
         const double limit = f.q_occ_frac * (double)n;
         for (int64_t i = threadIdx.x; i < n; i += 256) {
-            const int d = ci.nd > 0 ? find_hash(ci.dh, ci.nd, qx[base + i] >> 8) : -1;
+            const int d = nd > 0 ? find_hash(dh, nd, qx[base + i] >> 8) : -1;
             hidx[base + i] = d;
             if (d >= 0) atomicAdd(&cnt[d], 1u);
         }
-        __threadfence();
+        if constexpr (MODE == 0) __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         int64_t mine = 0;
         for (int64_t i = threadIdx.x; i < n; i += 256) {
             const int d = hidx[base + i];
             uint32_t na = 0;
             if (d >= 0) {
-                const uint32_t qocc = __hip_atomic_load(&cnt[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t qocc;
+                if constexpr (MODE == 0) qocc = __hip_atomic_load(&cnt[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else qocc = cnt[d];
                 na = hit_anchors(ci, f, d, qocc, limit, (uint32_t)(qy[base + i] & 0xffffffffu));
             }
             nanch[base + i] = na;
@@ -90,10 +115,9 @@ __global__ __launch_bounds__(256) void anchors_count(CenterIndexDev ci, const ui
             if (d >= 0) atomicSub(&cnt[d], 1u);
         }
         int64_t tot = 0;
-        (void)block_scan_incl(mine, sh, &tot);
+        if constexpr (MODE == 0) __builtin_amdgcn_s_waitcnt(0);
+        (void)block_scan_incl(mine, sh, &tot);   // its barriers order the clears before the next query
         if (threadIdx.x == 0) totals[q] = tot;
-        __threadfence();
-        __syncthreads();
     }
 }
 
@@ -413,14 +437,24 @@ __global__ void anchor_keys(const AnchorRec *__restrict__ anchors, int64_t n, ui
     vals[i] = (uint32_t)i;
 }
 
+constexpr size_t kAnchorLds = 62 * 1024;   // dynamic LDS of anchors_count (64 KiB with its scan array)
+
 hipError_t anchors_launch_count(const CenterIndexDev &ci, const uint64_t *qx, const uint64_t *qy,
                                 const int64_t *hit_off, const int64_t *hit_cnt, int nq, const SeedFilter &f,
                                 int groups, uint32_t *scratch, int32_t *hidx, uint32_t *nanch, int64_t *totals,
                                 hipStream_t stream)
 {
     if (nq <= 0) return hipSuccess;
-    hipLaunchKernelGGL(anchors_count, dim3(groups), dim3(256), 0, stream, ci, qx, qy, hit_off, hit_cnt, nq, f, scratch,
-                       hidx, nanch, totals);
+    const size_t nd = (size_t)(ci.nd > 0 ? ci.nd : 0);
+    if (nd * 12 <= kAnchorLds)
+        hipLaunchKernelGGL(anchors_count<2>, dim3(groups), dim3(256), nd * 12, stream, ci, qx, qy, hit_off, hit_cnt,
+                           nq, f, scratch, hidx, nanch, totals);
+    else if (nd * 4 <= kAnchorLds)
+        hipLaunchKernelGGL(anchors_count<1>, dim3(groups), dim3(256), nd * 4, stream, ci, qx, qy, hit_off, hit_cnt, nq,
+                           f, scratch, hidx, nanch, totals);
+    else
+        hipLaunchKernelGGL(anchors_count<0>, dim3(groups), dim3(256), 0, stream, ci, qx, qy, hit_off, hit_cnt, nq, f,
+                           scratch, hidx, nanch, totals);
     return hipGetLastError();
 }
 

@@ -136,6 +136,43 @@ def test_align_batch_vs_oracle(ctx):
     c.close()
 
 
+@pytest.mark.parametrize("clen", [8000, 40000, 120000])
+def test_anchors_center_sizes_vs_oracle(ctx, clen):
+    """collect_anchors on the device against the oracle for centers whose
+    distinct-hash table fits the kernel's LDS with the hashes (8 kb), only
+    its per-query counters (40 kb) and neither (120 kb: counters in global
+    scratch); queries with a repeated block exercise the query-occurrence
+    filter (q_occ_frac) and a low-complexity center stretch the sampled
+    high-occurrence hashes."""
+    import halign
+    O = Oracle()
+    rng = np.random.default_rng(clen)
+    center = np.concatenate([rand_codes(rng, clen - 2000), np.tile(rand_codes(rng, 20), 100)])
+    cstr = codes_to_ascii(center)
+    rx, ry, _ = O.minimizers(cstr)
+    qs = []
+    for i in range(40):
+        a = int(rng.integers(0, len(center) - 2500))
+        q = mutate_codes(rng, center[a:a + int(rng.integers(200, 2500))], 0.01, 0.002)
+        if i % 3 == 0:
+            q = np.concatenate([q, np.tile(q[:60], 6)])
+        qs.append(codes_to_ascii(q))
+    qs.append(codes_to_ascii(center[-2000:]))
+    print(f"anchors vs oracle: center {clen} bp, {len(qs)} queries", file=sys.stderr, flush=True)
+    c = halign.Center(ctx, cstr)
+    try:
+        got = c.anchors(qs)
+    finally:
+        c.close()
+    bad = []
+    for i, q in enumerate(qs):
+        qx, qy, _ = O.minimizers(q)
+        exp = O.collect_anchors(rx, ry, qx, qy)
+        if not np.array_equal(got[i].view(np.uint8), exp.view(np.uint8)):
+            bad.append(i)
+    assert not bad, f"{len(bad)} differ, first {bad[:5]}"
+
+
 def test_chains_vs_oracle(ctx):
     """chainAnchors on the device against the oracle's, anchor for anchor, on a
     center with tandem repeats (many equal-hash anchors: long predecessor
