@@ -11,7 +11,7 @@
 //    others become DpJobs built on the device (lane_jobs_*) for the sub-wave
 //    ring kernels (dp_wave.hip), each one globalAlignKSW2 (align.cpp:22-65);
 //    large segments and band overflows go to the host's batch runner.
-//  * mm2_assemble: one lane per query appends the segment CIGARs with
+//  * mm2_assemble: one wave per query appends the segment CIGARs with
 //    appendCigar's merge (src/align/cigar.cpp:302-329; the ring and
 //    full-statistics kernels store a CIGAR last op first), replacing a segment
 //    whose CIGAR does not consume it exactly by I(query) D(ref)
@@ -20,6 +20,7 @@
 //    record_sizes, write_records: the keep-length FASTA records.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 
 #include "dp_common.h"
@@ -382,65 +383,134 @@ __device__ __forceinline__ void consumed(const uint32_t *c, int n, int64_t &rl, 
     }
 }
 
-struct AppendCig {
-    uint32_t *c;
-    int64_t n, cap;
-    __device__ void add(uint32_t v)   // appendCigar: skip empty ops, merge equal neighbours
-    {
-        if ((v >> 4) == 0) return;
-        if (n > 0 && (c[n - 1] & 0xf) == (v & 0xf)) c[n - 1] += v & ~0xfu;
-        else {
-            if (n < cap) c[n] = v;
-            ++n;
-        }
-    }
-};
 
-// One lane per query: the final CIGAR at out[ooff[q] ..], its length in
-// nout[q] (-1 when the total does not consume both sequences: the caller then
-// aligns the whole pair, align.cpp:296-302).
-__global__ void mm2_assemble(const SegRec *__restrict__ segs, const int64_t *__restrict__ soff,
-                             const int32_t *__restrict__ nseg, const uint32_t *__restrict__ cigs,
-                             const int32_t *__restrict__ ncig, const int64_t *__restrict__ qlens, uint32_t rlen,
-                             const int64_t *__restrict__ ooff, int nq, uint32_t *__restrict__ out,
-                             int64_t *__restrict__ nout)
+// One wave per query (four per workgroup): the final CIGAR at out[ooff[q] ..],
+// its length in nout[q] (-1 when the total does not consume both sequences:
+// the caller then aligns the whole pair, align.cpp:296-302; -2 when a segment
+// CIGAR holds a zero-length op, which no source writes).
+//
+// The segments go in chunks of 64, lane k holding segment k of the chunk and
+// its op list: none (both sides empty), the one op of a one-side-empty
+// segment, the forced I(query) D(ref) pair of a CIGAR that does not consume
+// its segment, or the CIGAR itself, forward or last op first. The chunk's ops
+// then go in tiles of 64, lane t taking op t (its segment found by binary
+// search over the op offsets). appendCigar's merge of equal neighbours is a
+// run-length encoding of that op stream: an op whose type differs from the
+// op before it opens output op (number of such heads so far - 1), and the
+// last op of each run writes the run's summed length; the run open at a
+// tile's end carries into the next tile, which rewrites its output op.
+__global__ __launch_bounds__(256) void mm2_assemble(const SegRec *__restrict__ segs, const int64_t *__restrict__ soff,
+                                                    const int32_t *__restrict__ nseg, const uint32_t *__restrict__ cigs,
+                                                    const int32_t *__restrict__ ncig, const int64_t *__restrict__ qlens,
+                                                    uint32_t rlen, const int64_t *__restrict__ ooff, int nq,
+                                                    uint32_t *__restrict__ out, int64_t *__restrict__ nout)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    const int64_t qlen = qlens[q];
-    AppendCig ac{out + ooff[q], 0, (int64_t)rlen + qlen + 1};
-    const SegRec *sg = segs + soff[q];
+    __shared__ int32_t s_incl[4][64], s_nc[4][64];
+    __shared__ uint32_t s_v0[4][64], s_v1[4][64], s_mode[4][64];
+    __shared__ int64_t s_off[4][64];
+    const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int q = (int)blockIdx.x * 4 + wv;
+    if (q >= nq) return;   // uniform per wave
+    const auto add = [](int32_t x, int32_t y) { return x + y; };
+    const auto mx = [](int32_t x, int32_t y) { return x > y ? x : y; };
+    const int64_t qlen = qlens[q], cap = (int64_t)rlen + qlen + 1, sbase = soff[q];
+    uint32_t *o = out + ooff[q];
+    const SegRec *sg = segs + sbase;
     const int ns = nseg[q];
-    for (int k = 0; k < ns; ++k) {
-        const SegRec s = sg[k];
-        const uint32_t srl = s.re - s.rs, sql = s.qe - s.qs;
-        if (s.cls == kSegEmpty) continue;
-        if (s.cls == kSegPure) {   // globalAlignKSW2 on an empty side (align.cpp:26-34)
-            ac.add(srl == 0 ? (sql << 4 | 1u) : (srl << 4 | 2u));
-            continue;
-        }
-        const int nc = ncig[soff[q] + k];
-        const uint32_t *c = cigs + s.cig_off;
-        if (!(s.flags & kSegRaw)) {
-            int64_t rl, ql;
-            consumed(c, nc, rl, ql);
-            if (rl != srl || ql != sql) {   // forced I(query) D(ref)
-                if (sql > 0) ac.add(sql << 4 | 1u);
-                if (srl > 0) ac.add(srl << 4 | 2u);
-                continue;
+    int32_t c_type = -1, c_idx = -1, c_len = 0;   // the run open so far
+    int64_t rl = 0, ql = 0;                       // this lane's share of the consumption
+    bool zero = false;
+    for (int k0 = 0; k0 < ns; k0 += 64) {
+        const int k = k0 + lane;
+        uint32_t mode = 0, v0 = 0, v1 = 0;   // mode 1: ops v0 (v1); 2: CIGAR forward; 3: last op first
+        int32_t m = 0, nc = 0;
+        int64_t off = 0;
+        if (k < ns) {
+            const SegRec sr = sg[k];
+            const uint32_t srl = sr.re - sr.rs, sql = sr.qe - sr.qs;
+            if (sr.cls == kSegPure) {   // globalAlignKSW2 on an empty side (align.cpp:26-34)
+                mode = 1;
+                m = 1;
+                v0 = srl == 0 ? (sql << 4 | 1u) : (srl << 4 | 2u);
+            } else if (sr.cls != kSegEmpty) {
+                nc = ncig[sbase + k];
+                off = sr.cig_off;
+                bool forced = false;
+                if (!(sr.flags & kSegRaw)) {
+                    int64_t crl, cql;
+                    consumed(cigs + off, nc, crl, cql);
+                    forced = crl != srl || cql != sql;
+                }
+                if (forced) {   // I(query) D(ref), empty ones left out
+                    mode = 1;
+                    const uint32_t i_op = sql << 4 | 1u, d_op = srl << 4 | 2u;
+                    m = (sql > 0) + (srl > 0);
+                    v0 = sql > 0 ? i_op : d_op;
+                    v1 = d_op;
+                } else {
+                    mode = (sr.cls == kSegHost || sr.cls == kSegRing) ? 3u : 2u;
+                    m = nc;
+                }
             }
         }
-        if (s.cls == kSegHost || s.cls == kSegRing) {   // the ring / full-statistics kernels store the ops last to first
-            for (int i = nc - 1; i >= 0; --i) ac.add(c[i]);
-        } else {
-            for (int i = 0; i < nc; ++i) ac.add(c[i]);
+        const int32_t incl = wave_scan_incl(m, 0, add);
+        const int32_t T = __builtin_amdgcn_readlane(incl, 63);
+        s_incl[wv][lane] = incl;
+        s_nc[wv][lane] = nc;
+        s_v0[wv][lane] = v0;
+        s_v1[wv][lane] = v1;
+        s_mode[wv][lane] = mode;
+        s_off[wv][lane] = off;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int t0 = 0; t0 < T; t0 += 64) {
+            const int t = t0 + lane;
+            const bool valid = t < T;
+            uint32_t v = 0;
+            if (valid) {
+                int lo = 0, hi = 63;   // first segment whose inclusive op count passes t
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_incl[wv][mid] > t) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int i = t - (lo ? s_incl[wv][lo - 1] : 0);
+                const uint32_t md = s_mode[wv][lo];
+                if (md == 1) v = i ? s_v1[wv][lo] : s_v0[wv][lo];
+                else v = cigs[s_off[wv][lo] + (md == 3 ? s_nc[wv][lo] - 1 - i : i)];
+            }
+            const int32_t ty = valid ? (int32_t)(v & 0xfu) : -2;
+            const int32_t ln = valid ? (int32_t)(v >> 4) : 0;
+            zero |= valid && ln == 0;
+            if (valid) {   // cigar::getRefLength / getQueryLength (src/align/cigar.cpp:332-365)
+                if (ty == 0 || ty == 2 || ty == 3 || ty == 7 || ty == 8) rl += ln;
+                if (ty == 0 || ty == 1 || ty == 4 || ty == 7 || ty == 8) ql += ln;
+            }
+            const int32_t head = valid && ty != wave_shr1_id(ty, c_type) ? 1 : 0;
+            const int32_t idx = c_idx + wave_scan_incl(head, 0, add);
+            const int32_t P = wave_scan_incl(ln, 0, add);
+            const int32_t hb = wave_scan_incl(head ? P - ln : INT_MIN, INT_MIN, mx);
+            const int32_t run = hb == INT_MIN ? c_len + P : P - hb;
+            // the next lane opens a run, or this is the tile's last op
+            const int32_t next_head = __builtin_amdgcn_update_dpp(1, head, 0x130, 0xF, 0xF, false);
+            if (valid && (next_head || t + 1 == T) && idx < cap) o[idx] = (uint32_t)run << 4 | (uint32_t)ty;
+            const int tl = (T - t0 < 64 ? T - t0 : 64) - 1;
+            c_type = __builtin_amdgcn_readlane(ty, tl);
+            c_idx = __builtin_amdgcn_readlane(idx, tl);
+            c_len = __builtin_amdgcn_readlane(run, tl);
         }
+        // the next chunk rewrites the LDS tables
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
-    int64_t rl = 0, ql = 0;
-    const uint32_t *o = out + ooff[q];
-    consumed(o, (int)(ac.n < ac.cap ? ac.n : ac.cap), rl, ql);
+    for (int d = 32; d > 0; d >>= 1) {
+        rl += __shfl_xor(rl, d);
+        ql += __shfl_xor(ql, d);
+    }
+    const bool any_zero = __ballot(zero) != 0;
     const bool raw = ns == 1 && (sg[0].flags & kSegRaw);
-    nout[q] = (!raw && (rl != (int64_t)rlen || ql != qlen)) ? -1 : ac.n;
+    // every lane stores the same value
+    nout[q] = any_zero ? -2 : ((!raw && (rl != (int64_t)rlen || ql != qlen)) ? -1 : (int64_t)c_idx + 1);
 }
 
 hipError_t mm2_segments_launch(const AnchorRec *anchors, const uint32_t *chain_idx, const int64_t *aoff,
@@ -695,7 +765,7 @@ hipError_t mm2_assemble_launch(const SegRec *segs, const int64_t *soff, const in
                                uint32_t *out, int64_t *nout, hipStream_t stream)
 {
     if (nq <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mm2_assemble, dim3((nq + 63) / 64), dim3(64), 0, stream, segs, soff, nseg, cigs, ncig, qlens,
+    hipLaunchKernelGGL(mm2_assemble, dim3((nq + 3) / 4), dim3(256), 0, stream, segs, soff, nseg, cigs, ncig, qlens,
                        rlen, ooff, nq, out, nout);
     return hipGetLastError();
 }

@@ -687,7 +687,9 @@ int center_align(halign_center *c, int32_t n, const char *queries, const int64_t
     if (rc != HALIGN_OK) return rc;
     int64_t o = 0;
     for (int i = 0; i < n; ++i) {
-        if (b.nout[i] < 0) return fail(ctx, HALIGN_E_KERNEL, "MM2 CIGAR does not consume the pair");
+        if (b.nout[i] < 0)
+            return fail(ctx, HALIGN_E_KERNEL,
+                        b.nout[i] == -2 ? "zero-length op in a segment CIGAR" : "MM2 CIGAR does not consume the pair");
         o += b.nout[i];
     }
     if (o > cap) return fail(ctx, HALIGN_E_CAPACITY, "CIGAR output too small");
@@ -821,7 +823,9 @@ int center_align_fasta(halign_center *c, int32_t n, const char *queries, const i
     const int rc = run_batch(c, n, queries, offsets, 3, b);
     if (rc != HALIGN_OK) return rc;
     for (int i = 0; i < n; ++i)
-        if (b.nout[i] < 0) return fail(ctx, HALIGN_E_KERNEL, "MM2 CIGAR does not consume the pair");
+        if (b.nout[i] < 0)
+            return fail(ctx, HALIGN_E_KERNEL,
+                        b.nout[i] == -2 ? "zero-length op in a segment CIGAR" : "MM2 CIGAR does not consume the pair");
     const size_t nn = (size_t)n;
     HCHECK(c->nm.ensure((size_t)std::max<int64_t>(noff[n], 1)), "alloc names");
     HCHECK(c->nmoff.ensure((nn + 1) * 8), "alloc");

@@ -86,6 +86,27 @@ __device__ __forceinline__ int64_t block_scan_incl(int64_t v, int64_t *sh, int64
     return r;
 }
 
+// lane l receives lane l-1, lane 0 receives id
+__device__ __forceinline__ int32_t wave_shr1_id(int32_t v, int32_t id)
+{
+    return __builtin_amdgcn_update_dpp(id, v, 0x138, 0xF, 0xF, false);
+}
+
+// inclusive scan over the 64 lanes with DPP: row_shr 1, 2, 4, 8 inside each
+// row of 16 lanes, then row_bcast:15 into rows 1 and 3 and row_bcast:31 into
+// rows 2 and 3; a lane with no source lane takes the identity id
+template <class Op>
+__device__ __forceinline__ int32_t wave_scan_incl(int32_t v, int32_t id, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x111, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x112, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x114, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x118, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xC, 0xF, false));
+    return v;
+}
+
 // small-segment limits: target length and traceback bytes (16 per block and
 // anti-diagonal; the ring kernels store half of that)
 constexpr int kLaneMaxT = 512;
