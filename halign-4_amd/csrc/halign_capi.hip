@@ -74,6 +74,7 @@ void bytes_query_rev(const uint8_t *codes, int len, uint8_t *out)
 // kWaveBase + i for ring shape i (CIGAR mode).
 constexpr int kWaveBase = 16;
 constexpr int kNumShapeIds = kWaveBase + 32;
+constexpr int kDpStreams = 1 + halign_ctx::kAux;   // main + side streams for DP buckets
 
 int pick_shape(int ncol, bool stats, int policy)
 {
@@ -309,9 +310,10 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     } else {
         // One launch per bucket. Buckets are independent (own jobs, outputs,
         // counter and traceback region), so they run side by side on the main
-        // stream and two side streams, longest estimate first on the least
-        // loaded stream: narrow-band buckets are bound by their jobs' anti-
-        // diagonal chains, not by the chip, and overlap well.
+        // stream and the three side streams (one hardware queue each: the
+        // runtime's default is four per process), longest estimate first on
+        // the least loaded stream: narrow-band buckets are bound by their
+        // jobs' anti-diagonal chains, not by the chip, and overlap well.
         struct Launch { int bi; size_t pos, slot, jpg, off; int groups; double est; int stream; };
         std::vector<Launch> ls;
         const double nsimd = 4.0 * ctx->num_cu;
@@ -381,10 +383,10 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             std::vector<int> ord(ls.size());
             for (size_t k = 0; k < ls.size(); ++k) ord[k] = (int)k;
             std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return ls[x].est > ls[y].est; });
-            double load[3] = {0, 0, 0};
+            double load[kDpStreams] = {};
             for (int k : ord) {
                 int sidx = 0;
-                for (int q = 1; q < 3; ++q)
+                for (int q = 1; q < kDpStreams; ++q)
                     if (load[q] < load[sidx]) sidx = q;
                 ls[k].stream = sidx;
                 load[sidx] += ls[k].est;
@@ -399,7 +401,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             for (auto &L : ls) mx = std::max(mx, L.slot * L.jpg * (size_t)L.groups);
             HCHECK(ctx->tb.ensure(std::max<size_t>(mx, 1), ctx->tb_budget), "alloc traceback pool");
         }
-        bool used[3] = {true, false, false};
+        bool used[kDpStreams] = {true};
         for (auto &L : ls) used[L.stream] = true;
         // the plan's signature: a repeat replays the captured graph
         std::vector<int64_t> sig;
@@ -437,8 +439,10 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             return fail(ctx, HALIGN_E_HIP, what, e_);                        \
         }                                                                   \
     } while (0)
-        if (!ls.empty() && (used[1] || used[2])) HCHECK_CAP(hipEventRecord(ctx->fork, ctx->stream), "event");
-        for (int q = 1; q < 3 && !ls.empty(); ++q)
+        bool any_side = false;
+        for (int q = 1; q < kDpStreams; ++q) any_side |= used[q];
+        if (!ls.empty() && any_side) HCHECK_CAP(hipEventRecord(ctx->fork, ctx->stream), "event");
+        for (int q = 1; q < kDpStreams && !ls.empty(); ++q)
             if (used[q]) HCHECK_CAP(hipStreamWaitEvent(ctx->aux[q - 1], ctx->fork, 0), "stream wait");
         for (auto &L : ls) {
             const int shape = L.bi / 4;
@@ -460,7 +464,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                                             douts + L.pos, st);
             HCHECK_CAP(e, "launch dp");
         }
-        for (int q = 1; q < 3 && !ls.empty(); ++q) {
+        for (int q = 1; q < kDpStreams && !ls.empty(); ++q) {
             if (!used[q]) continue;
             HCHECK_CAP(hipEventRecord(ctx->join[q - 1], ctx->aux[q - 1]), "event");
             HCHECK_CAP(hipStreamWaitEvent(ctx->stream, ctx->join[q - 1], 0), "stream wait");

@@ -82,8 +82,9 @@ struct halign_ctx {
     int num_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // Side streams (default priority): aux[0], aux[1] run DP buckets beside the
-    // main stream; aux[2] runs the per-query path's small-segment DPs beside them.
+    // Side streams (default priority): all three run DP buckets beside the
+    // main stream; aux[2] also runs the per-query path's small-segment DPs,
+    // which come before that batch's large-segment buckets.
     static constexpr int kAux = 3;
     hipStream_t aux[kAux] = {};
     hipEvent_t fork = nullptr, join[kAux] = {}, pfork = nullptr, pjoin = nullptr;
