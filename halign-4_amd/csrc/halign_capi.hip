@@ -575,6 +575,14 @@ void halign_free(halign_ctx *ctx)
     for (auto &cg : ctx->gcache) (void)hipGraphExecDestroy(cg.exec);
     for (auto &b : ctx->msa_out) b.release();
     ctx->msa_tmp.release();
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->up_ev[k]) {
+            (void)hipEventSynchronize(ctx->up_ev[k]);
+            (void)hipEventDestroy(ctx->up_ev[k]);
+        }
+        ctx->up_pin[k].release();
+    }
+    ctx->dn_pin.release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     for (int k = 0; k < halign_ctx::kAux; ++k) {

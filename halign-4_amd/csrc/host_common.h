@@ -116,6 +116,11 @@ struct halign_ctx {
     hal::HostPinned msa_out[2], msa_tmp;
     struct halign_center *msa_center = nullptr;
     std::string msa_key;
+    // per-query path: two pinned staging buffers for query uploads (each
+    // reused after the event recorded behind its last copy) and one for
+    // downloads of the large-segment records
+    hal::HostPinned up_pin[2], dn_pin;
+    hipEvent_t up_ev[2] = {};
 };
 
 namespace hal {

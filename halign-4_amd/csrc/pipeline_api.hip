@@ -14,6 +14,7 @@
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dp_host.h"
@@ -171,6 +172,59 @@ struct Batch {
     double ms = 0;
 };
 
+// memcpy of n bytes split over four host threads (one copy below 1 MiB, or
+// when a thread cannot be started)
+static void par_copy(char *d, const char *s, size_t n)
+{
+    constexpr int kThreads = 4;
+    if (n < ((size_t)1 << 20)) {
+        std::memcpy(d, s, n);
+        return;
+    }
+    const size_t part = ((n + kThreads - 1) / kThreads + 63) & ~(size_t)63;
+    std::thread th[kThreads - 1];
+    for (int t = 1; t < kThreads; ++t) {
+        const size_t a = std::min(n, (size_t)t * part), e = std::min(n, a + part);
+        if (a >= e) break;
+        try {
+            th[t - 1] = std::thread([=] { std::memcpy(d + a, s + a, e - a); });
+        } catch (...) {   // copy the rest here
+            std::memcpy(d + a, s + a, n - a);
+            break;
+        }
+    }
+    std::memcpy(d, s, std::min(n, part));
+    for (auto &x : th)
+        if (x.joinable()) x.join();
+}
+
+// Host-to-device copy of n pageable bytes on stream st through the context's
+// two pinned staging buffers: chunk k is copied into buffer k & 1 by host
+// threads while the DMA of chunk k-1 runs, and a buffer is refilled only
+// after the event recorded behind its previous copy (of this call or the
+// last one) has completed. Small copies go directly.
+static int upload_staged(halign_ctx *ctx, void *dst, const char *src, size_t n, hipStream_t st)
+{
+    constexpr size_t kChunk = (size_t)8 << 20;
+    if (n <= ((size_t)4 << 20)) {
+        HCHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st), "upload");
+        return HALIGN_OK;
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (!ctx->up_pin[k].ensure(kChunk)) return fail(ctx, HALIGN_E_NOMEM, "pinned staging buffer");
+        if (!ctx->up_ev[k]) HCHECK(hipEventCreateWithFlags(&ctx->up_ev[k], hipEventDisableTiming), "event");
+    }
+    for (size_t off = 0, k = 0; off < n; off += kChunk, ++k) {
+        const int b = (int)(k & 1);
+        const size_t len = std::min(kChunk, n - off);
+        HCHECK(hipEventSynchronize(ctx->up_ev[b]), "staging buffer free");
+        par_copy(ctx->up_pin[b].p, src + off, len);
+        HCHECK(hipMemcpyAsync((char *)dst + off, ctx->up_pin[b].p, len, hipMemcpyHostToDevice, st), "upload");
+        HCHECK(hipEventRecord(ctx->up_ev[b], st), "event");
+    }
+    return HALIGN_OK;
+}
+
 // Stages: 1 = anchors, 2 = chains, 3 = CIGARs (everything).
 int run_batch(halign_center *c, int n, const char *queries, const int64_t *offsets, int stop, Batch &b)
 {
@@ -185,9 +239,10 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     const int64_t total = b.off[n];
     HCHECK(ctx->s_seqs.ensure((size_t)std::max<int64_t>(total, 1)), "alloc queries");
     HCHECK(ctx->s_off.ensure((size_t)(n + 1) * 8), "alloc offsets");
-    if (total)
-        HCHECK(hipMemcpyAsync(ctx->s_seqs.p, queries + offsets[0], (size_t)total, hipMemcpyHostToDevice, st),
-               "upload queries");
+    if (total) {
+        const int urc = upload_staged(ctx, ctx->s_seqs.p, queries + offsets[0], (size_t)total, st);
+        if (urc != HALIGN_OK) return urc;
+    }
     HCHECK(hipMemcpyAsync(ctx->s_off.p, b.off.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice, st),
            "upload offsets");
 
@@ -405,10 +460,14 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
         hipLaunchKernelGGL(gather_segs, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, c->segs.as<SegRec>(),
                            c->hlist.as<int64_t>(), (int64_t)nh, c->hsegs.as<SegRec>());
         HCHECK(hipGetLastError(), "launch gather");
-        std::vector<SegRec> hs;
-        std::vector<int64_t> hl;
-        HCHECK(down(hs, c->hsegs.p, nh, st), "download large segments");
-        HCHECK(down(hl, c->hlist.p, nh, st), "download large segments");
+        // through pinned memory: a pageable download here waited for the
+        // small-segment DPs on the side stream
+        if (!ctx->dn_pin.ensure((size_t)nh * sizeof(SegRec))) return fail(ctx, HALIGN_E_NOMEM, "pinned buffer");
+        HCHECK(hipMemcpyAsync(ctx->dn_pin.p, c->hsegs.p, (size_t)nh * sizeof(SegRec), hipMemcpyDeviceToHost, st),
+               "download large segments");
+        HCHECK(hipStreamSynchronize(st), "download large segments");
+        const SegRec *hs = reinterpret_cast<const SegRec *>(ctx->dn_pin.p);
+        clk.mark("large segment download");
         // keep the device list order (atomic order) paired with the records
         std::vector<Spec> specs(nh);
         for (size_t k = 0; k < nh; ++k) {
