@@ -79,9 +79,11 @@ struct Spec {
 int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector<DpOut> &outs, double *kernel_ms);
 // choose the kernel shape and traceback size of a job; false = band too wide
 bool fill_spec_shape(Spec &s, bool full_stats, int policy);
-// Reversed query copies for a batch of raw ASCII queries (see halign_capi.hip)
+// Reversed query copies for a batch of raw ASCII queries (see halign_capi.hip);
+// mean_len sizes the grid (workgroups per query)
 hipError_t prep_queries_launch(const char *raw, const int64_t *qoff, const int64_t *region, const int64_t *bregion,
-                               int first, int count, uint16_t *seq, uint8_t *seqb, hipStream_t stream);
+                               int first, int count, int64_t mean_len, uint16_t *seq, uint8_t *seqb,
+                               hipStream_t stream);
 void spread_target(const uint8_t *codes, int len, uint16_t *even, uint16_t *odd);
 
 // Ring-form kernels (dp_wave.hip): shape table, cost figures, launch

@@ -887,11 +887,13 @@ __global__ void prep_queries(const char *__restrict__ raw, const int64_t *__rest
 }  // namespace
 
 hipError_t hal::prep_queries_launch(const char *raw, const int64_t *qoff, const int64_t *region,
-                                    const int64_t *bregion, int first, int count, uint16_t *seq, uint8_t *seqb,
-                                    hipStream_t stream)
+                                    const int64_t *bregion, int first, int count, int64_t mean_len, uint16_t *seq,
+                                    uint8_t *seqb, hipStream_t stream)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(prep_queries, dim3(64, count), dim3(256), 0, stream, raw, qoff, region, bregion, first, count,
+    // enough workgroups of 256 for a query of the mean length (longer ones loop)
+    const int gx = (int)std::min<int64_t>(64, std::max<int64_t>(1, (mean_len + 2 * kSeqPad + 255) / 256));
+    hipLaunchKernelGGL(prep_queries, dim3(gx, count), dim3(256), 0, stream, raw, qoff, region, bregion, first, count,
                        seq, seqb);
     return hipGetLastError();
 }

@@ -1,7 +1,7 @@
 // mm2.hip -- align::globalAlignMM2 (src/align/align.cpp:189-305) on the
 // device, for many queries against one center:
 //
-//  * mm2_segments: one lane per query turns its chain into the 2n+1 segments
+//  * mm2_segments: one wave per query turns its chain into the 2n+1 segments
 //    of the reference (first segment LEFT-aligned, the others RIGHT; bounds
 //    clipped to the sequences; each segment starts where the previous one
 //    ended) and classifies them: empty, one side empty, small DP, large DP
@@ -66,62 +66,67 @@ __device__ __forceinline__ uint32_t seg_cig_cap(uint32_t cls, uint32_t srl, uint
     return srl + sql + 1;
 }
 
-// One lane per query. chain_idx/chain_cnt from chain_queries; segments of
-// query q go to segs[soff[q] ..]; nseg[q] receives their count; caps[s] the
-// CIGAR capacity of segment s.
-__global__ void mm2_segments(const AnchorRec *__restrict__ anchors, const uint32_t *__restrict__ chain_idx,
-                             const int64_t *__restrict__ aoff, const int32_t *__restrict__ chain_cnt,
-                             const int64_t *__restrict__ qlens, uint32_t rlen, const int64_t *__restrict__ soff,
-                             int nq, SegRec *__restrict__ segs, int32_t *__restrict__ nseg, int64_t *__restrict__ caps)
+// One wave per query (four per workgroup). chain_idx/chain_cnt from
+// chain_queries; segments of query q go to segs[soff[q] ..]; nseg[q] receives
+// their count; caps[s] the CIGAR capacity of segment s.
+//
+// The reference appends 2n+1 segments for a chain of n anchors: to the first
+// anchor, over each anchor's span, to the next anchor, and to the ends. Each
+// starts where the previous one ended, and its end is its target clipped to
+// the sequence and not below its start, so the ends are running maxima of
+// the clipped targets: lane k takes segment k of a chunk of 64, an inclusive
+// max-scan gives its end, the lane before it (or the chunk's carry) its start.
+__global__ __launch_bounds__(256) void mm2_segments(const AnchorRec *__restrict__ anchors,
+                                                    const uint32_t *__restrict__ chain_idx,
+                                                    const int64_t *__restrict__ aoff,
+                                                    const int32_t *__restrict__ chain_cnt,
+                                                    const int64_t *__restrict__ qlens, uint32_t rlen,
+                                                    const int64_t *__restrict__ soff, int nq,
+                                                    SegRec *__restrict__ segs, int32_t *__restrict__ nseg,
+                                                    int64_t *__restrict__ caps)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
+    const int q = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    if (q >= nq) return;   // uniform per wave
     const uint32_t qlen = (uint32_t)qlens[q];
     const int n = chain_cnt[q];
     SegRec *out = segs + soff[q];
     int64_t *cap = caps + soff[q];
-    int k = 0;
-    auto emit = [&](uint32_t rs, uint32_t re, uint32_t qs, uint32_t qe, uint32_t flags) {
-        SegRec s;
-        s.rs = rs; s.re = re; s.qs = qs; s.qe = qe;
-        s.flags = flags; s.query = (uint32_t)q; s.pad = 0; s.cig_off = 0;
-        s.cls = classify(re - rs, qe - qs);
-        out[k] = s;
-        cap[k] = seg_cig_cap(s.cls, re - rs, qe - qs);
-        ++k;
+    auto emit = [&](int k, uint32_t rs, uint32_t re, uint32_t qs, uint32_t qe, uint32_t flags) {
+        SegRec sr;
+        sr.rs = rs; sr.re = re; sr.qs = qs; sr.qe = qe;
+        sr.flags = flags; sr.query = (uint32_t)q; sr.pad = 0; sr.cig_off = 0;
+        sr.cls = classify(re - rs, qe - qs);
+        out[k] = sr;
+        cap[k] = seg_cig_cap(sr.cls, re - rs, qe - qs);
     };
-    if (n == 0) {
-        emit(0, rlen, 0, qlen, kSegRight | kSegRaw);
-        nseg[q] = k;
+    if (n == 0) {   // the whole pair (align.cpp:204-206)
+        if (lane == 0) emit(0, 0, rlen, 0, qlen, kSegRight | kSegRaw);
+        nseg[q] = 1;
         return;
     }
-    uint32_t ref_pos = 0, qry_pos = 0;
-    auto seg = [&](uint64_t rs, uint64_t re, uint64_t qs, uint64_t qe, uint32_t flags) {
-        if (rs > rlen) rs = rlen;
-        if (re > rlen) re = rlen;
-        if (qs > qlen) qs = qlen;
-        if (qe > qlen) qe = qlen;
-        if (re < rs) re = rs;
-        if (qe < qs) qe = qs;
-        emit((uint32_t)rs, (uint32_t)re, (uint32_t)qs, (uint32_t)qe, flags);
-        ref_pos = (uint32_t)re;
-        qry_pos = (uint32_t)qe;
-    };
+    const auto mx = [](int32_t x, int32_t y) { return x > y ? x : y; };
     const uint32_t *ch = chain_idx + aoff[q];
-    {
-        const AnchorRec a0 = anchors[ch[0]];
-        seg(ref_pos, a0.pos_ref, qry_pos, a0.pos_qry, 0);   // first segment: LEFT (align.cpp:194-195)
-    }
-    for (int i = 0; i < n; ++i) {
-        const AnchorRec a = anchors[ch[i]];
-        seg(ref_pos, (uint64_t)a.pos_ref + a.span, qry_pos, (uint64_t)a.pos_qry + a.span, kSegRight);
-        if (i + 1 < n) {
-            const AnchorRec b = anchors[ch[i + 1]];
-            seg(ref_pos, b.pos_ref, qry_pos, b.pos_qry, kSegRight);
+    const int ns = 2 * n + 1;
+    int32_t cr = 0, cq = 0;   // end of the segments so far
+    for (int k0 = 0; k0 < ns; k0 += 64) {
+        const int k = k0 + lane;
+        uint64_t tr = rlen, tq = qlen;   // target end; the last segment runs to the ends
+        if (k < ns - 1) {
+            // k = 0: first anchor's start; k = 2i+1: anchor i's span end; k = 2i+2: anchor i+1's start
+            const AnchorRec a = anchors[ch[k >> 1]];
+            const uint64_t sp = (k & 1) ? a.span : 0;
+            tr = (uint64_t)a.pos_ref + sp;
+            tq = (uint64_t)a.pos_qry + sp;
         }
+        const int32_t er = (int32_t)(tr < rlen ? tr : rlen), eq = (int32_t)(tq < qlen ? tq : qlen);
+        const int32_t re = mx(wave_scan_incl(k < ns ? er : 0, 0, mx), cr);
+        const int32_t qe = mx(wave_scan_incl(k < ns ? eq : 0, 0, mx), cq);
+        const int32_t rs = wave_shr1_id(re, cr), qs = wave_shr1_id(qe, cq);
+        if (k < ns) emit(k, (uint32_t)rs, (uint32_t)re, (uint32_t)qs, (uint32_t)qe, k == 0 ? 0u : kSegRight);
+        cr = __builtin_amdgcn_readlane(re, 63);
+        cq = __builtin_amdgcn_readlane(qe, 63);
     }
-    seg(ref_pos, rlen, qry_pos, qlen, kSegRight);
-    nseg[q] = k;
+    nseg[q] = ns;   // every lane stores the same value
 }
 
 __global__ void seg_set_offsets(SegRec *__restrict__ segs, const int64_t *__restrict__ cig_off, int64_t n)
@@ -518,7 +523,7 @@ hipError_t mm2_segments_launch(const AnchorRec *anchors, const uint32_t *chain_i
                                int nq, SegRec *segs, int32_t *nseg, int64_t *caps, hipStream_t stream)
 {
     if (nq <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mm2_segments, dim3((nq + 63) / 64), dim3(64), 0, stream, anchors, chain_idx, aoff, chain_cnt,
+    hipLaunchKernelGGL(mm2_segments, dim3((nq + 3) / 4), dim3(256), 0, stream, anchors, chain_idx, aoff, chain_cnt,
                        qlens, rlen, soff, nq, segs, nseg, caps);
     return hipGetLastError();
 }

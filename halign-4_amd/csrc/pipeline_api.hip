@@ -380,7 +380,8 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     HCHECK(up(c->bregion, bregion, st), "upload regions");
     HCHECK(up(c->qrev_off, qrev_off, st), "upload regions");
     HCHECK(prep_queries_launch(ctx->s_seqs.as<char>(), ctx->s_off.as<int64_t>(), c->region.as<int64_t>(),
-                               c->bregion.as<int64_t>(), 0, n, ctx->seq.as<uint16_t>(), ctx->seqb.as<uint8_t>(), st),
+                               c->bregion.as<int64_t>(), 0, n, n ? b.off[n] / n : 0, ctx->seq.as<uint16_t>(),
+                               ctx->seqb.as<uint8_t>(), st),
            "launch prep_queries");
 
     // small segments: exact ones take nM here, the others one lane each on a
