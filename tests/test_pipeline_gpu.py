@@ -212,6 +212,41 @@ def test_chains_vs_oracle(ctx):
     assert long_chains > 20
 
 
+def test_large_batch_staged_upload(ctx):
+    """A batch of ~19 MB of queries goes up through the pinned staging buffers
+    in 8 MiB chunks (a query straddles each chunk boundary); a query's CIGAR
+    does not depend on its batch, so every one must equal the CIGAR of the
+    same query aligned in small batches (direct uploads), and a sample of them
+    the oracle's."""
+    import halign
+    O = Oracle()
+    rng = np.random.default_rng(2024)
+    center = rand_codes(rng, 12000)
+    cstr = codes_to_ascii(center)
+    qs = []
+    for i in range(1700):
+        a = int(rng.integers(0, 1500))
+        qs.append(codes_to_ascii(mutate_codes(rng, center[a:a + int(rng.integers(9000, 12000))], 0.01, 0.002)))
+    total = sum(len(q) for q in qs)
+    print(f"staged upload: {len(qs)} queries, {total / 1e6:.1f} MB in one batch", file=sys.stderr, flush=True)
+    assert total > 2 * (8 << 20)
+    c = halign.Center(ctx, cstr)
+    try:
+        big, _, _ = c.align(qs)
+        small = []
+        for k in range(0, len(qs), 150):
+            cg, _, _ = c.align(qs[k:k + 150])
+            small += cg
+    finally:
+        c.close()
+    bad = [i for i in range(len(qs)) if not np.array_equal(big[i], small[i])]
+    assert not bad, f"{len(bad)} differ between the batch sizes, first {bad[:5]}"
+    for i in range(0, len(qs), 170):
+        exp, _, _, ub = O.ref_global_align(cstr, qs[i])
+        if not ub & 3:
+            assert np.array_equal(big[i], exp), i
+
+
 def test_one_gap_segments_vs_oracle(ctx):
     """Queries whose end segments have a one-gap optimum (CIGAR written
     without the DP) and near misses that must run the DP: exact windows of the
