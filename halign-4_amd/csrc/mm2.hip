@@ -222,9 +222,21 @@ __global__ __launch_bounds__(256) void seg_split(SegRec *__restrict__ segs, int6
         if (sg.cls == kSegHost || sg.cls == kSegLane)
             nop = one_gap(center + sg.rs, qrev + qrev_off[sg.query] + (qlens[sg.query] - 1 - sg.qs),
                           (int)(sg.re - sg.rs), (int)(sg.qe - sg.qs), (sg.flags & kSegRight) != 0, op);
+        const int srl0 = (int)(sg.re - sg.rs), sql0 = (int)(sg.qe - sg.qs);
+        int w0 = auto_band_dev(srl0, sql0);
+        if (w0 < 0) w0 = srl0 > sql0 ? srl0 : sql0;
         if (nop > 0) {
             for (int k = 0; k < nop; ++k) cigs[sg.cig_off + k] = op[k];
             ncig[i] = nop;
+            segs[i].cls = kSegExact;
+        } else if (sg.cls == kSegHost && !(sg.flags & kSegRaw) && overflows(sql0, srl0, w0)) {
+            // The band leaves the matrix before the last anti-diagonal: ksw_extz2
+            // stops there (zdropped) and backtracks from its maximum, a cell
+            // short of the last one, so the CIGAR never consumes the segment and
+            // globalAlignMM2 replaces it by I(query) D(ref) whatever its ops
+            // (align.cpp:243-261). No DP: an empty CIGAR, which the assembly
+            // replaces the same way.
+            ncig[i] = 0;
             segs[i].cls = kSegExact;
         } else if (sg.cls == kSegHost) {
             which = 1;

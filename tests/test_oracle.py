@@ -234,6 +234,41 @@ def test_one_gap_closed_form(O):
     assert tot > 8000
 
 
+def test_band_overflow_never_consumes(O):
+    """A segment whose auto band leaves the matrix before the last
+    anti-diagonal (ksw2 sets zdropped and backtracks from its maximum) gets a
+    CIGAR that does not consume both sides, so globalAlignMM2 always replaces
+    it by I(query) D(ref) and the device skips its DP. Checked on the oracle's
+    globalAlignKSW2 over related and unrelated pairs, both flags, every shape
+    found to overflow."""
+    rng = np.random.default_rng(404)
+    A = b"ACGT"
+    seen = 0
+    for it in range(400):
+        n = int(rng.integers(60, 2500))
+        m = int(rng.integers(max(1, n // 2), n))
+        if it % 2:
+            n, m = m, n          # longer query (insertions)
+        w = O.auto_band(n, m)    # ref length n, query length m
+        if w < 0:
+            continue
+        r = m + n - 2            # last anti-diagonal; ksw2's qlen = m, tlen = n
+        if not (w == 0 or r - m + 1 > (r + w) >> 1 or ((r - w + 1) >> 1) > n - 1):
+            continue
+        R = bytes(A[int(x)] for x in rng.integers(0, 4, n))
+        if it % 3 == 0:          # the query a piece of the reference
+            a = int(rng.integers(0, max(1, n - m + 1)))
+            Q = (R[a:a + m] + bytes(A[int(x)] for x in rng.integers(0, 4, m)))[:m]
+        else:
+            Q = bytes(A[int(x)] for x in rng.integers(0, 4, m))
+        for flag in (FLAG_RIGHT, FLAG_LEFT):
+            cig, _ = O.global_align_ksw2(R, Q, flag)
+            _, ti, qi = cigar_score(cig, np.frombuffer(Q, np.uint8) * 0, np.frombuffer(R, np.uint8) * 0)
+            assert (ti, qi) != (n, m), (it, n, m, w, flag)
+            seen += 1
+    assert seen > 100
+
+
 def test_banded_cigars_consume_both_sides(O):
     """test/test_align.cpp:1418-1626: a global CIGAR consumes the whole query
     and reference, also under a narrow band (no overflow)."""
