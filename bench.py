@@ -36,6 +36,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (a shared GPU host grants each GPU a
+    share of its cores that way)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else None, "cpu_model": model}
+
+
 def cpu_baseline(center, qbytes, off, sample_pairs, threads):
     """Time globalAlignKSW2 on `sample_pairs` pairs with `threads` host threads."""
     from oracle_lib import REF_SO, ORACLE_SO, Oracle, RefLib, FLAG_RIGHT
@@ -53,6 +77,77 @@ def cpu_baseline(center, qbytes, off, sample_pairs, threads):
         list(ex.map(one, range(sample_pairs)))
     dt = time.perf_counter() - t0
     return kind, dt
+
+
+def verify_dp(ps, center, qbytes, off, last_range, k=0):
+    """After timing: the score and CIGAR of k pairs of the last timed launch
+    (spread over it, so pairs that reused a persistent workgroup's traceback
+    slot are among them) against the compiled reference's globalAlignKSW2
+    (the oracle port where the reference build is absent), on host threads.
+    Returns "ok/checked" and the checker's kind."""
+    from oracle_lib import REF_SO, FLAG_RIGHT, Oracle, RefLib
+    kind = "reference" if os.path.exists(REF_SO) else "port"
+    lib = RefLib() if kind == "reference" else Oracle()
+    first, cnt = last_range
+    k = k or 8
+    idx = sorted(set(int(first + j) for j in np.linspace(0, cnt - 1, min(k, cnt))))
+
+    got = {i: ps.result(i)[1] for i in idx}
+
+    def one(i):
+        exp = lib.global_align_ksw2(center, qbytes[off[i]:off[i + 1]], FLAG_RIGHT)
+        exp = exp[0] if isinstance(exp, tuple) else exp
+        return bool(np.array_equal(np.asarray(got[i], np.uint32), np.asarray(exp, np.uint32)))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(min(len(idx), host_threads())) as ex:
+        ok = sum(ex.map(one, idx))
+    log(f"verified {ok}/{len(idx)} pairs of the last launch against the {kind} ({time.perf_counter() - t0:.1f}s)")
+    return {"ok": f"{ok}/{len(idx)}", "checker": kind, "what": "score+CIGAR of pairs spread over the last timed launch"}
+
+
+def verify_msa(ctx, d, cpath, qv, off, n, refs):
+    """The keep-length file run on the first n queries against the
+    reference program's own run (oracle/_ref/halign4 -c --keep-length -t 1,
+    copy command) on the same files: the final FASTA byte for byte."""
+    import halign
+    from oracle_lib import REF_BIN, Halign4
+    if not os.path.exists(REF_BIN):
+        log("verify: no reference program here, msa output not checked")
+        return None
+    vd = os.path.join(d, "verify")
+    os.makedirs(vd, exist_ok=True)
+    ip = os.path.join(vd, "q.fa")
+    with open(ip, "wb") as f:
+        for i in range(n):
+            f.write(b">q%d\n" % i + bytes(qv[off[i]:off[i + 1]]) + b"\n")
+    halign.msa_keep_length(ctx, cpath, ip, os.path.join(vd, "gpu.fa"))
+    rc = Halign4().run(ip, os.path.join(vd, "ref.fa"), os.path.join(vd, "w"), center_path=cpath)
+    same = rc == 0 and open(os.path.join(vd, "gpu.fa"), "rb").read() == open(os.path.join(vd, "ref.fa"), "rb").read()
+    log(f"verified msa run of {n} queries ({refs} reference(s)) against the reference program: "
+        f"{'identical' if same else 'DIFFERENT'}")
+    return {"ok": f"{int(same)}/1", "checker": "reference program",
+            "what": f"final FASTA of a {n}-query run, byte for byte"}
+
+
+def verify_queries(center, qb, off, items, what):
+    """RefAligner::globalAlign CIGARs of the GPU (items: (query index, CIGAR))
+    against the compiled reference (center minimizers computed once, as
+    RefAligner does), or the oracle port where the reference build is absent."""
+    from oracle_lib import REF_SO, Oracle, RefLib
+    kind = "reference" if os.path.exists(REF_SO) else "port"
+    lib = RefLib() if kind == "reference" else Oracle()
+    hits = lib.minimizers(center) if kind == "reference" else None
+
+    def one(it):
+        i, got = it
+        q = qb[off[i]:off[i + 1]]
+        exp = lib.ref_global_align_pre(center, hits, q) if kind == "reference" else lib.ref_global_align(center, q)[0]
+        return bool(np.array_equal(np.asarray(got, np.uint32), np.asarray(exp, np.uint32)))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(host_threads()) as ex:
+        ok = sum(ex.map(one, items))
+    log(f"verified {ok}/{len(items)} {what} against the {kind} ({time.perf_counter() - t0:.1f}s)")
+    return {"ok": f"{ok}/{len(items)}", "checker": kind, "what": what}
 
 
 def band_cells(qlen, tlen, w):
@@ -77,6 +172,9 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=0, help="pairs for the CPU baseline (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true", help="skip the check of sampled outputs after timing")
+    ap.add_argument("--verify", type=int, default=0,
+                    help="outputs checked after timing (0 = the workload default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times), to scale HBM counters")
     ap.add_argument("--workload", default="dp", choices=["dp", "consensus", "e2e", "serve", "msa"],
@@ -158,6 +256,8 @@ def main():
     log("warmup done")
     tot_cells = 0
     tot_kernel_ms = 0.0
+    pairs = 0
+    ranges = []
     t_start = time.perf_counter()
     for s in range(args.steps):
         first = ((args.warmup + s) * args.batch) % n_res
@@ -165,22 +265,25 @@ def main():
         ms, cells = ps.run(first, cnt, plain=False)
         tot_cells += cells
         tot_kernel_ms += ms
+        pairs += cnt
+        ranges.append((first, cnt))
         log(f"step {s}: {cnt} pairs, kernel {ms:.1f} ms, {cells / (ms * 1e-3) / 1e9:.1f} GCUPS")
     t_wall = time.perf_counter() - t_start
-    pairs = args.steps * args.batch
+    verified = None if args.no_verify else verify_dp(ps, center, qbytes, off, ranges[-1], args.verify)
     gcups = tot_cells / t_wall / 1e9
     kernel_gcups = tot_cells / (tot_kernel_ms * 1e-3) / 1e9
     achieved_tops = tot_cells * OPS_PER_CELL / (tot_kernel_ms * 1e-3) / 1e12
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         sample = args.cpu_sample or threads * 32
         kind, dt = cpu_baseline(center, qbytes, off, sample, threads)
         w = [halign_auto_band(len(center), int(off[i + 1] - off[i])) for i in range(sample)]
         cells = sum(band_cells(int(off[i + 1] - off[i]), len(center), w[i]) for i in range(sample))
         cpu = {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": kind,
-               "sample": f"{sample} config-2 whole pairs (globalAlignKSW2), {dt:.1f}s on {threads} threads"}
+               "sample": f"{sample} config-2 whole pairs (globalAlignKSW2), {dt:.1f}s on {threads} threads",
+               "host": host_info()}
         log(f"cpu baseline ({kind}): {cpu['value']} GCUPS")
 
     out = {
@@ -199,6 +302,7 @@ def main():
         "config": {"workload": "config 2: kernel-only pairwise DP, 100000 pairs, center 29903 bp, "
                                "queries 29.0-30.0 kb, auto band, batched persistent form",
                    "pairs_per_step": args.batch, "pairs_timed": pairs, "kernel": args.kernel},
+        "verified": verified,
         "seqs_per_s": round(pairs / t_wall, 2),
         "kernel_gcups": round(kernel_gcups, 3),
         "plain_gcups": round(plain_gcups, 4),
@@ -241,16 +345,23 @@ def e2e_bench(args):
         t0 = time.perf_counter()
         cig, coff, st, ms = c.align_flat(qb, off[a:b + 1])
         dt = time.perf_counter() - t0
+        last = (a, b, cig, coff)
         if s >= args.warmup:
             steps.append((b - a, int(off[b] - off[a]), dt, ms))
             log(f"step {s - args.warmup}: {b - a} queries, {dt * 1e3:.1f} ms wall, {ms:.1f} ms device")
     nq = sum(x[0] for x in steps)
     nb = sum(x[1] for x in steps)
     tw = sum(x[2] for x in steps)
+    verified = None
+    if not args.no_verify:
+        a, b, cig, coff = last
+        picks = sorted(set(int(j) for j in np.linspace(0, b - a - 1, min(args.verify or 64, b - a))))
+        verified = verify_queries(center, qb, off, [(a + j, cig[coff[j]:coff[j + 1]]) for j in picks],
+                                  "queries spread over the last timed step")
     cpu = None
     if not args.no_cpu:
         from oracle_lib import REF_SO, Oracle, RefLib
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         kind = "reference" if os.path.exists(REF_SO) else "port"
         lib = RefLib() if kind == "reference" else Oracle()
         if kind == "reference":
@@ -268,6 +379,7 @@ def e2e_bench(args):
            "ms_per_step": round(tw * 1e3 / len(steps), 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "int8", "data": f"synthetic (seeded, csrc/synth.c config {cfg})",
            "config": {"workload": f"config {cfg}: {label}", "queries_per_step": per},
+           "verified": verified,
            "bases_per_s": round(nb / tw, 1), "device_ms_per_step": round(sum(x[3] for x in steps) / len(steps), 3),
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
@@ -331,13 +443,16 @@ def msa_bench(args):
                 f"{st['input_wait_seconds'] * 1e3:.1f} ms waiting for input")
     tw = sum(x[0] for x in steps)
     out_bytes = steps[-1][1]["bytes"]
+    verified = None
+    if not args.no_verify:
+        verified = verify_msa(ctx, d, cpath, qv, off, min(need, args.verify or 300), args.refs)
     cpu = None
     if not args.no_cpu:
-        from oracle_lib import REF_SO, RefLib
-        if os.path.exists(REF_SO):
-            R = RefLib()
-            threads = min(16, os.cpu_count() or 1)
-            # calibrate: one thread on 8 queries, then a share per thread for ~10 s
+        from oracle_lib import REF_BIN, Halign4
+        if os.path.exists(REF_BIN):
+            H = Halign4()
+            threads = host_threads()
+            # calibrate: one run on 8 queries, then a share per thread for ~10 s
             def run_share(k, a, b):
                 wd = os.path.join(d, f"ref{k}")
                 os.makedirs(wd, exist_ok=True)
@@ -346,7 +461,9 @@ def msa_bench(args):
                     for i in range(a, b):
                         f.write(b">q%d\n" % i + bytes(qv[off[i]:off[i + 1]]) + b"\n")
                 t0 = time.perf_counter()
-                rc = R.keep_length_msa_refs(cpath, None, ip, wd, os.path.join(wd, "out.fa"))
+                import shutil
+                shutil.rmtree(os.path.join(wd, "w"), ignore_errors=True)
+                rc = H.run(ip, os.path.join(wd, "out.fa"), os.path.join(wd, "w"), center_path=cpath)
                 assert rc == 0, rc
                 return time.perf_counter() - t0
             t8 = run_share(0, 0, min(8, need))
@@ -373,7 +490,7 @@ def msa_bench(args):
            "output_bytes": out_bytes,
            "gpu_s_per_step": round(sum(x[1]["gpu_seconds"] for x in steps) / len(steps), 4),
            "input_wait_s_per_step": round(sum(x[1]["input_wait_seconds"] for x in steps) / len(steps), 4),
-           "cpu_baseline": cpu}
+           "verified": verified, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     import shutil
     shutil.rmtree(d, ignore_errors=True)
@@ -416,7 +533,10 @@ def serve_bench(args):
     log(f"serve: {R} requests, {need} queries, {off[-1] / 1e6:.1f} Mbases")
     srv = halign.Server(center, workers=args.workers, max_batch_queries=args.batch_queries)
 
+    kept = []
+
     def run(arrivals, label):
+        kept.clear()
         ids = [None] * R
         lat = np.zeros(R)
         done_t = np.zeros(R)
@@ -426,9 +546,12 @@ def serve_bench(args):
             for i in range(R):
                 while ids[i] is None:
                     time.sleep(0.0005)
-                _, _, _, timing = srv.result(ids[i])
+                cig, coff, _, timing = srv.result(ids[i])
                 lat[i] = timing[1]
                 done_t[i] = time.perf_counter() - t0
+                if i % 8 == 0:   # one query of every eighth request is checked after the run
+                    j = int(bounds[i + 1] - bounds[i]) // 2
+                    kept.append((int(bounds[i]) + j, cig[coff[j]:coff[j + 1]].copy()))
         th = threading.Thread(target=collect)
         th.start()
         for i in range(R):
@@ -457,10 +580,12 @@ def serve_bench(args):
                            "p99_ms": round(float(np.percentile(lat, 99)), 1)})
     finally:
         srv.close()
+    verified = None if args.no_verify else verify_queries(
+        center, qb, off, list(kept), "served queries (the middle one of every eighth request at the last load)")
     cpu = None
     if not args.no_cpu:
         from oracle_lib import REF_SO, Oracle, RefLib
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         kind = "reference" if os.path.exists(REF_SO) else "port"
         lib = RefLib() if kind == "reference" else Oracle()
         if kind == "reference":
@@ -481,7 +606,8 @@ def serve_bench(args):
            "config": {"workload": "config 5: serving, center 29903 bp resident, requests of U{1..4096} queries "
                                   "of U{1..30000} bp at 1% divergence", "requests": R, "queries": need,
                       "workers": args.workers, "max_batch_queries": args.batch_queries},
-           "p50_ms": hi.get("p50_ms"), "p99_ms": hi.get("p99_ms"), "load_points": points, "cpu_baseline": cpu}
+           "p50_ms": hi.get("p50_ms"), "p99_ms": hi.get("p99_ms"), "load_points": points, "verified": verified,
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

@@ -10,11 +10,14 @@
  * probability p_long, l uniform in [long_min, long_max]); an insertion emits l
  * random bases before the current base, a deletion skips l bases.
  * Query i of configuration c is generated from seed (0xC0+c) << 32 | i.
+ * Integer arithmetic and exact double comparisons only: the same bytes on
+ * every machine (tests/test_synth.py holds checksums).
  */
-#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+#include "synth_tables.h"
 
 typedef struct { uint64_t s[4]; } rng_t;
 
@@ -92,17 +95,10 @@ int64_t synth_center(int cfg, char *out, int64_t cap)
     return n;
 }
 
-/* lognormal(median 1500, sigma 0.8) clamped to [200, 20000] via Box-Muller */
-static int64_t lognormal_len(rng_t *r)
-{
-    double u1 = unif(r), u2 = unif(r);
-    if (u1 < 1e-300) u1 = 1e-300;
-    const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-    double L = 1500.0 * exp(0.8 * z);
-    if (L < 200.0) L = 200.0;
-    if (L > 20000.0) L = 20000.0;
-    return (int64_t)(L + 0.5);
-}
+/* lognormal(median 1500, sigma 0.8) clamped to [200, 20000]: the committed
+ * quantile table, entry picked by the top 12 bits of one draw (integers only,
+ * so the lengths do not depend on the machine's libm) */
+static int64_t lognormal_len(rng_t *r) { return kLognormalLen[next(r) >> 52]; }
 
 /* one query of configuration cfg into out; returns its length (or -needed) */
 static int64_t one_query(int cfg, int64_t idx, const char *center, char *tmp, int64_t tmpcap,

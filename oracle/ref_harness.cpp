@@ -189,183 +189,6 @@ int ref_read_fastx(const char* path, int max_records, char* id_out, int64_t* id_
     }
 }
 
-// halign4 -c center_path --keep-length with the copy MSA command and one
-// thread, as the reference runs it (src/halign4.cpp:96-147): the RefAligner
-// constructor (src/align/ref_aligner.cpp:20-70: copy the center file,
-// consensus of the copy = the center aligned to; the minimizers of the file's
-// record), alignQueryToRef (:286-394) with alignOneQueryToRef's routing
-// (:212-283) into thread 0's two SAM files, then mergeAlignedResults
-// (:676-740) with processInsertionSequences / mergeConsensusAndSamToFasta
-// (:504-535, :139-209), parseAlignedReferencesToCigar (:400-459),
-// writeConsensusAndReferences (:541-561), writeInsertionSequences
-// (:567-605) and processSamFileBatch / convertSamToFastaRecord (:467-498,
-// :612-672). RefAligner itself is not compiled here (it needs mash and the
-// external-tool runner); this glue calls the same reference functions in the
-// same order. work_dir must exist. Returns 0, or -2 when the reference throws.
-int ref_keep_length_msa_refs(const char* center_path, const char* aligned_path, const char* input_path,
-                             const char* work_dir, const char* out_path)
-{
-    namespace fs = std::filesystem;
-    try {
-        const fs::path wd(work_dir);
-        // constructor: the center file's records and minimizers
-        std::vector<seq_io::SeqRecord> refs;
-        std::vector<minimizer::MinimizerHits> ref_mz;
-        {
-            seq_io::KseqReader rd(center_path);
-            seq_io::SeqRecord rec;
-            while (rd.next(rec)) {
-                ref_mz.push_back(minimizer::extractMinimizer(rec.seq, 15, 10, true));
-                refs.push_back(rec);
-            }
-        }
-        if (refs.empty()) return -3;
-        std::vector<mash::Sketch> ref_sk;
-        for (const auto& r : refs) ref_sk.push_back(mash::sketchFromSequence(r.seq, 15, 2000, true, 42));
-        const fs::path cons_aligned = wd / "cons_aligned.fasta";
-        // the MSA command's output: a copy ("cp {input} {output}") or the given aligned file
-        file_io::copyFile(aligned_path ? aligned_path : center_path, cons_aligned);
-        seq_io::SeqRecord consensus_seq;
-        consensus_seq.id = "consensus";
-        consensus_seq.seq = consensus::generateConsensusSequence(cons_aligned, wd / "cons.fasta", wd / "cons.json",
-                                                                 0, 1, 4096);
-        // alignQueryToRef, one thread
-        const fs::path sam = wd / "t0.sam", sam_ins = wd / "t0_insertion.sam";
-        {
-            auto out = seq_io::SeqWriter::Sam(sam);
-            auto out_ins = seq_io::SeqWriter::Sam(sam_ins);
-            out.writeSamHeader("@HD\tVN:1.6\tSO:unknown");
-            out_ins.writeSamHeader("@HD\tVN:1.6\tSO:unknown");
-            seq_io::KseqReader rd(input_path);
-            seq_io::SeqRecord q;
-            while (rd.next(q)) {
-                // alignOneQueryToRef (:212-283) with keep_length: the reference sequence
-                // with the largest Jaccard (first strict maximum), or the consensus
-                // when there is one center
-                const mash::Sketch qsk = mash::sketchFromSequence(q.seq, 15, 2000, true, 42);
-                auto qm = minimizer::extractMinimizer(q.seq, 15, 10, true);
-                size_t best = 0;
-                const seq_io::SeqRecord* best_ref = &consensus_seq;
-                if (refs.size() > 1) {
-                    double bj = -1.0;
-                    for (size_t r = 0; r < ref_sk.size(); ++r) {
-                        const double j = mash::jaccard(qsk, ref_sk[r]);
-                        if (j > bj) {
-                            bj = j;
-                            best = r;
-                        }
-                    }
-                    best_ref = &refs[best];
-                }
-                const minimizer::MinimizerHits* rm = &ref_mz[best];
-                minimizer::MinimizerHits rtmp;
-                if (rm->empty()) {
-                    rtmp = minimizer::extractMinimizer(best_ref->seq, 15, 10, true);
-                    rm = &rtmp;
-                }
-                const auto anchors = minimizer::collect_anchors(*rm, qm);
-                const cigar::Cigar_t cg = align::globalAlignMM2(best_ref->seq, q.seq, anchors);
-                const auto rec = seq_io::makeSamRecord(q, best_ref->id, cigar::cigarToString(cg), 1, 60, 0);
-                if (cigar::hasInsertion(cg)) out_ins.writeSam(rec);
-                else out.writeSam(rec);
-            }
-            out.flush();
-            out_ins.flush();
-        }
-        // mergeAlignedResults
-        auto parse_aligned = [](const fs::path& p, std::unordered_map<std::string, cigar::Cigar_t>& m) {
-            seq_io::KseqReader rd(p);
-            seq_io::SeqRecord rec;
-            while (rd.next(rec)) {
-                cigar::Cigar_t c;
-                char cur = 0;
-                uint32_t len = 0;
-                for (char b : rec.seq) {
-                    const char op = b == '-' ? 'D' : 'M';
-                    if (op == cur) ++len;
-                    else {
-                        if (cur && len) c.push_back(cigar::cigarToInt(cur, len));
-                        cur = op;
-                        len = 1;
-                    }
-                }
-                if (cur && len) c.push_back(cigar::cigarToInt(cur, len));
-                m[rec.id] = c;
-            }
-        };
-        std::unordered_map<std::string, cigar::Cigar_t> ref_map, ins_map;
-        parse_aligned(cons_aligned, ref_map);
-        seq_io::SeqWriter final_writer(out_path, U_MAX);
-        // processInsertionSequences -> mergeConsensusAndSamToFasta(keep = true, width 80), then copy
-        const fs::path ins_fa = wd / "all_insertion.fasta", ins_aligned = wd / "aligned_insertion.fasta";
-        {
-            seq_io::SeqWriter w(ins_fa, 80);
-            w.writeFasta(consensus_seq);
-            seq_io::SamReader r(sam_ins);
-            seq_io::SamRecord s;
-            while (r.next(s)) {
-                seq_io::SeqRecord f = seq_io::samRecordToSeqRecord(s, false);
-                if (!s.cigar.empty()) {
-                    cigar::Cigar_t ops = cigar::stringToCigar(s.cigar);
-                    cigar::delQueryToRefByCigar(f.seq, ops);
-                    if (s.rname != "consensus") cigar::padQueryToRefByCigar(f.seq, ref_map.at(s.rname));
-                }
-                w.writeFasta(f);
-            }
-            w.flush();
-        }
-        file_io::copyFile(ins_fa, ins_aligned);
-        parse_aligned(ins_aligned, ins_map);
-        ref_map[consensus_seq.id] = ins_map[consensus_seq.id];
-        // writeConsensusAndReferences
-        {
-            seq_io::KseqReader rd(cons_aligned);
-            seq_io::SeqRecord rec;
-            while (rd.next(rec)) {
-                seq_io::cleanSequence(rec);
-                final_writer.writeFasta(rec);
-            }
-        }
-        // writeInsertionSequences
-        size_t expect = 0;
-        bool have = false;
-        {
-            seq_io::KseqReader rd(ins_aligned);
-            seq_io::SeqRecord rec;
-            bool first = true;
-            while (rd.next(rec)) {
-                if (first) { first = false; continue; }
-                if (!have) { expect = rec.seq.size(); have = true; }
-                else if (rec.seq.size() != expect) return -4;
-                final_writer.writeFasta(rec);
-            }
-        }
-        // processSamFileBatch over thread 0's SAM file
-        {
-            seq_io::SamReader r(sam);
-            seq_io::SamRecord s;
-            while (r.next(s)) {
-                seq_io::SeqRecord f = seq_io::samRecordToSeqRecord(s, false);
-                if (!s.cigar.empty() && s.cigar != "*") cigar::padQueryToRefByCigar(f.seq, cigar::stringToCigar(s.cigar));
-                cigar::padQueryToRefByCigar(f.seq, ref_map.at(s.rname));
-                if (!have) { expect = f.seq.size(); have = true; }
-                else if (f.seq.size() != expect) return -4;
-                final_writer.writeFasta(f);
-            }
-        }
-        final_writer.flush();
-        return 0;
-    } catch (...) {
-        return -2;
-    }
-}
-
-int ref_keep_length_msa(const char* center_path, const char* input_path, const char* work_dir,
-                        const char* out_path)
-{
-    return ref_keep_length_msa_refs(center_path, nullptr, input_path, work_dir, out_path);
-}
-
 // mash::sketchFromSequence (src/mash/mash.cpp:32-146): the sorted hashes.
 // Returns their count, or -1 when cap is too small.
 int ref_mash_sketch(const char* seq, int n, int k, int sketch_size, int noncanonical, int seed, uint64_t* out,
@@ -398,32 +221,6 @@ int64_t ref_preprocess(const char* input_path, const char* work_dir, int cons_n)
 {
     try {
         return (int64_t)preprocessInputFasta(input_path, work_dir, cons_n);
-    } catch (...) {
-        return -2;
-    }
-}
-
-// halign4 --keep-length without -c (src/halign4.cpp:87, 119-147) with the
-// copy MSA command (or the given aligned Top-K file) and one thread:
-// preprocessInputFasta, the consensus of the aligned Top-K with seq_limit
-// cons_n, then RefAligner on the consensus file as ref_keep_length_msa_refs
-// runs it. Returns 0, or -2 when the reference throws.
-int ref_auto_center_keep_length(const char* input_path, const char* aligned_topk, const char* work_dir,
-                                const char* out_path, int cons_n)
-{
-    namespace fs = std::filesystem;
-    try {
-        const fs::path wd(work_dir);
-        preprocessInputFasta(input_path, work_dir, cons_n);
-        const fs::path clean = wd / WORKDIR_DATA / DATA_CLEAN;
-        const fs::path unaligned = clean / CLEAN_CONS_UNALIGNED, aligned = clean / CLEAN_CONS_ALIGNED;
-        file_io::copyFile(aligned_topk ? fs::path(aligned_topk) : unaligned, aligned);
-        consensus::generateConsensusSequence(aligned, clean / CLEAN_CONS_FASTA, clean / CLEAN_CONS_JSON,
-                                             (uint64_t)cons_n, 1, 4096);
-        const fs::path wd2 = wd / "ref_aligner";
-        fs::create_directories(wd2);
-        const std::string cons_fa = (clean / CLEAN_CONS_FASTA).string();
-        return ref_keep_length_msa_refs(cons_fa.c_str(), nullptr, input_path, wd2.string().c_str(), out_path);
     } catch (...) {
         return -2;
     }

@@ -30,6 +30,16 @@ Outputs (np.savez_compressed, no pickles):
                        (copy command, or a given aligned file): final FASTA bytes
   auto_center.npz      preprocessInputFasta's Top-K file, generateConsensusSequence's
                        FASTA and JSON files, and halign4 --keep-length without -c
+  config3.npz, config4.npz, config5.npz
+                       samples of the uneven (3), divergent (4) and serving (5)
+                       workloads through the reference program itself
+                       (oracle/_ref/halign4 -c center --keep-length, one thread,
+                       copy command): each query's SAM CIGAR and group, and the
+                       SHA-256 of the final FASTA; the queries are stored as
+                       indices into the seeded workload (csrc/synth.c, pinned
+                       by tests/test_synth.py)
+
+Run a subset:  python tests/golden/make_golden.py config3 config4
 """
 import json
 import os
@@ -43,8 +53,9 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, os.path.join(ROOT, "halign-4_amd"))
 
-from oracle_lib import ANCHOR_DTYPE, FLAG_LEFT, FLAG_RIGHT, Oracle, RefLib  # noqa: E402
+from oracle_lib import ANCHOR_DTYPE, FLAG_LEFT, FLAG_RIGHT, Halign4, Oracle, RefLib  # noqa: E402
 from seqgen import codes_to_ascii, mutate_codes, rand_codes  # noqa: E402
+from workloads import workload_queries  # noqa: E402
 
 ACGT = np.frombuffer(b"ACGT", np.uint8)
 
@@ -414,11 +425,10 @@ def fastx_cases(R, rng):
          nrec=np.array(nrec, np.int64), id=idd, id_off=ido, desc=dd, desc_off=do, seq=sd, seq_off=so)
 
 
-def msa_keep_length(R, rng):
+def msa_keep_length(H, rng):
     """halign4 -c center --keep-length (src/halign4.cpp:96-147) with one
-    thread and the copy MSA command, through the reference's own readers,
-    aligners, SAM and FASTA writers (ref_keep_length_msa): the final FASTA
-    for a few center/query files. Queries cover both SAM groups (with and
+    thread and the copy MSA command, run as the reference program itself
+    (oracle/_ref/halign4): the final FASTA for a few center/query files. Queries cover both SAM groups (with and
     without an insertion), a query too short for minimizers, an empty one,
     descriptions, a name starting with '@' (its SAM line is skipped), FASTQ
     input, and a center with lower case and N (aligned as its consensus,
@@ -478,10 +488,8 @@ def msa_keep_length(R, rng):
         cpath, ipath, opath = (os.path.join(d, x + "%d" % case) for x in ("c.fa", "q.fa", "out.fa"))
         open(cpath, "wb").write(cfa)
         open(ipath, "wb").write(text)
-        wd = os.path.join(d, "wd%d" % case)
-        os.makedirs(wd)
-        rc = R.keep_length_msa(cpath, ipath, wd, opath)
-        assert rc == 0, (case, rc)
+        rc = H.run(ipath, opath, os.path.join(d, "wd%d" % case), center_path=cpath)
+        assert rc == 0, (case, rc, H.last_log[-2000:])
         centers.append(np.frombuffer(cfa, np.uint8))
         inputs.append(np.frombuffer(text, np.uint8))
         outputs.append(np.frombuffer(open(opath, "rb").read(), np.uint8))
@@ -546,9 +554,9 @@ def mash_cases(R, rng):
          ref=rd, ref_off=ro, qry=qd, qry_off=qo, best=np.array(best, np.int32), jaccard=np.array(jac, np.float64))
 
 
-def msa_refs(R, rng):
-    """halign4 -c refs --keep-length with several references, one thread
-    (ref_keep_length_msa_refs): each query goes to the reference with the
+def msa_refs(H, rng):
+    """halign4 -c refs --keep-length with several references, one thread,
+    run as the reference program itself: each query goes to the reference with the
     largest sketch Jaccard, its row spread over that reference's aligned
     record. Case 0: three equal-length references, the copy command; case 1:
     four references of different lengths and an aligned file with gap columns
@@ -601,10 +609,9 @@ def msa_refs(R, rng):
         open(ip, "wb").write(text)
         if afa is not None:
             open(ap, "wb").write(afa)
-        wd = os.path.join(d, "wd%d" % case)
-        os.makedirs(wd)
-        rc = R.keep_length_msa_refs(cp, ap if afa is not None else None, ip, wd, op)
-        assert rc == 0, (case, rc)
+        rc = H.run(ip, op, os.path.join(d, "wd%d" % case), center_path=cp,
+                   msa_cmd=msa_command(ap) if afa is not None else Halign4.COPY)
+        assert rc == 0, (case, rc, H.last_log[-2000:])
         out["center"].append(np.frombuffer(cfa, np.uint8))
         out["aligned"].append(np.frombuffer(afa or b"", np.uint8))
         out["input"].append(np.frombuffer(text, np.uint8))
@@ -616,7 +623,19 @@ def msa_refs(R, rng):
     save("msa_refs.npz", has_aligned=np.array(has_aligned, np.int32), **arrays)
 
 
-def auto_center(R, rng):
+def msa_command(aligned_path):
+    """An MSA command template for the reference program whose output is the
+    given aligned file for an input of several records, and the input itself
+    for a single record (what any MSA tool writes for one sequence): the
+    reference runs the command on the center file (RefAligner's constructor,
+    src/align/ref_aligner.cpp:47-60) and, in the auto-center path, first on
+    the Top-K file (src/halign4.cpp:119-134), then on the one-record
+    consensus file."""
+    return ("if [ \"$(grep -c '>' {input})\" = 1 ]; then cp {input} {output}; else cp %s {output}; fi"
+            % aligned_path)
+
+
+def auto_center(R, H, rng):
     """The auto-center path with --keep-length: preprocessInputFasta's Top-K
     consensus input (src/utils/preprocess.cpp:19-160), generateConsensusSequence's
     FASTA and JSON outputs (src/consensus/consensus.cpp:38-68, 611-703) and the
@@ -691,11 +710,10 @@ def auto_center(R, rng):
             atext = b"".join(b">" + recs[i][0] + b"\n" + aligned[i] + b"\n" for i in order)
             ap = os.path.join(d, "r%d.aligned.fa" % case)
             open(ap, "wb").write(atext)
-        wd = os.path.join(d, "rw%d" % case)
-        os.makedirs(wd)
         op = os.path.join(d, "r%d.out.fa" % case)
-        rc = R.auto_center_keep_length(ip, ap, wd, op, ktop)
-        assert rc == 0, (case, rc)
+        rc = H.run(ip, op, os.path.join(d, "rw%d" % case), keep_length=True, cons_n=ktop,
+                   msa_cmd=msa_command(ap) if ap else Halign4.COPY)
+        assert rc == 0, (case, rc, H.last_log[-2000:])
         out["run_in"].append(np.frombuffer(text, np.uint8))
         out["run_aligned"].append(np.frombuffer(atext, np.uint8))
         out["run_out"].append(np.frombuffer(open(op, "rb").read(), np.uint8))
@@ -707,23 +725,108 @@ def auto_center(R, rng):
          run_par=np.array(run_par, np.int32), **arrays)
 
 
-def main():
+def workload_lengths(cfg, first, count, chunk=50000):
+    from halign import synth
+    out = []
+    for a in range(first, first + count, chunk):
+        _, off = synth.queries(cfg, a, min(chunk, first + count - a))
+        out.append(np.diff(off))
+    return np.concatenate(out)
+
+
+def sample_workload(cfg, rng):
+    """Query indices of the sample of workload cfg: the first 64 (the start of
+    every bench step), random ones over the whole workload, and the cases each
+    workload exists to stress, forced in from a scan of its first queries."""
+    from halign import synth
+    total = synth.CONFIG_COUNTS[cfg]
+    first = list(range(64))
+    if cfg == 3:
+        L = workload_lengths(3, 0, 200000)
+        short = np.flatnonzero(L < 550)          # anchor-less whole pairs among them
+        longest = np.argsort(L, kind="stable")[-200:]
+        forced = list(rng.choice(short, 300, replace=False)) + list(longest)
+        n_rand = 1436
+    elif cfg == 4:                               # every query carries ~2 long indels
+        forced, n_rand = [], 936
+    else:
+        L = workload_lengths(5, 0, 200000)
+        tiny = np.flatnonzero(L < 20)            # no minimizers: whole-pair DP
+        longer = np.flatnonzero(L > 29903)       # longer than the center
+        edge = np.flatnonzero((L >= 14800) & (L <= 15100))   # auto band switches near 14,951
+        forced = (list(rng.choice(tiny, 40, replace=False)) + list(rng.choice(longer, 60, replace=False)) +
+                  list(rng.choice(edge, 40, replace=False)))
+        n_rand = 1296
+    rand = list(rng.choice(np.arange(64, total), n_rand, replace=False))
+    idx, seen = [], set()
+    for i in first + [int(x) for x in forced] + [int(x) for x in rand]:
+        if i not in seen:
+            seen.add(i)
+            idx.append(i)
+    return np.array(idx, np.int64)
+
+
+def config_workload(H, cfg, rng):
+    """RefAligner on a sample of workload cfg through the reference program
+    (halign4 -c center --keep-length -t 1, copy command: src/halign4.cpp:
+    96-147, src/align/ref_aligner.cpp:212-283, 286-394, 676-740): the CIGAR
+    and SAM group (cigar::hasInsertion) of every query, read back from its
+    thread-0 SAM files, and the SHA-256 of the final aligned FASTA."""
+    import hashlib
+    from halign import synth
+    idx = sample_workload(cfg, rng)
+    q, off = workload_queries(cfg, idx)
+    d = tempfile.mkdtemp()
+    cpath, ipath, opath = (os.path.join(d, x) for x in ("center.fa", "q.fa", "out.fa"))
+    open(cpath, "wb").write(b">center\n" + synth.center(cfg) + b"\n")
+    with open(ipath, "wb") as f:
+        for k, i in enumerate(idx):
+            f.write(b">q%d\n" % i + q[off[k]:off[k + 1]] + b"\n")
+    rc = H.run(ipath, opath, os.path.join(d, "wd"), center_path=cpath)
+    assert rc == 0, (cfg, rc, H.last_log[-2000:])
+    recs = {name: (cig, ins) for name, _, cig, ins in Halign4.sam_records(os.path.join(d, "wd"))}
+    assert len(recs) == len(idx)
+    cigs, ins = [], []
+    for i in idx:
+        c, h = recs[b"q%d" % i]
+        cigs.append(c)
+        ins.append(h)
+    cd, co = concat(cigs, np.uint32)
+    digest = np.frombuffer(hashlib.sha256(open(opath, "rb").read()).digest(), np.uint8)
+    save("config%d.npz" % cfg, idx=idx, cigar=cd, cigar_off=co, has_insertion=np.array(ins, np.uint8),
+         fasta_sha256=digest, fasta_bytes=np.array([os.path.getsize(opath)], np.int64))
+
+
+FIXTURES = ["ksw_cases", "global_align", "minimizers", "anchors_chain", "ref_global", "consensus",
+            "whole_pairs", "fastx_cases", "msa_keep_length", "mash_cases", "msa_refs", "auto_center",
+            "config3", "config4", "config5"]
+
+
+def main(which):
     R = RefLib()
     O = Oracle()
-    print("writing golden fixtures from", R.lib._name, flush=True)
-    ksw_cases(R, O, np.random.default_rng(0x601D))
-    global_align(R, np.random.default_rng(0x601E))
-    minimizers(R, O, np.random.default_rng(0x601F))
-    anchors_chain_mm2(R, np.random.default_rng(0x6020))
-    ref_global(R, np.random.default_rng(0x6021))
-    consensus(R, np.random.default_rng(0x6022))
-    whole_pairs(R)
-    fastx_cases(R, np.random.default_rng(0x6023))
-    msa_keep_length(R, np.random.default_rng(0x6024))
-    mash_cases(R, np.random.default_rng(0x6025))
-    msa_refs(R, np.random.default_rng(0x6026))
-    auto_center(R, np.random.default_rng(0x6027))
+    H = Halign4()
+    print("writing golden fixtures from", R.lib._name, "and", H.path, flush=True)
+    run = {
+        "ksw_cases": lambda: ksw_cases(R, O, np.random.default_rng(0x601D)),
+        "global_align": lambda: global_align(R, np.random.default_rng(0x601E)),
+        "minimizers": lambda: minimizers(R, O, np.random.default_rng(0x601F)),
+        "anchors_chain": lambda: anchors_chain_mm2(R, np.random.default_rng(0x6020)),
+        "ref_global": lambda: ref_global(R, np.random.default_rng(0x6021)),
+        "consensus": lambda: consensus(R, np.random.default_rng(0x6022)),
+        "whole_pairs": lambda: whole_pairs(R),
+        "fastx_cases": lambda: fastx_cases(R, np.random.default_rng(0x6023)),
+        "msa_keep_length": lambda: msa_keep_length(H, np.random.default_rng(0x6024)),
+        "mash_cases": lambda: mash_cases(R, np.random.default_rng(0x6025)),
+        "msa_refs": lambda: msa_refs(H, np.random.default_rng(0x6026)),
+        "auto_center": lambda: auto_center(R, H, np.random.default_rng(0x6027)),
+        "config3": lambda: config_workload(H, 3, np.random.default_rng(0x60C3)),
+        "config4": lambda: config_workload(H, 4, np.random.default_rng(0x60C4)),
+        "config5": lambda: config_workload(H, 5, np.random.default_rng(0x60C5)),
+    }
+    for name in which or FIXTURES:
+        run[name]()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

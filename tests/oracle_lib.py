@@ -4,6 +4,8 @@
 - ``RefLib``: oracle/_ref/libhalign_ref.so, the real reference path compiled
   from its own sources (only present where the reference tree was available
   at build time).
+- ``Halign4``: oracle/_ref/halign4, the reference's own command-line program
+  built from the same sources, run as a child process.
 
 Both expose the same call shapes so a test can run one input through both.
 """
@@ -15,6 +17,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libhalign_ref.so")
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "halign4")
 
 RIGHT = 0x02
 GENERIC = 0x04
@@ -448,27 +451,6 @@ class RefLib(_Base):
         L.ref_preprocess.restype = C.c_int64
         return L.ref_preprocess(input_path.encode(), work_dir.encode(), cons_n)
 
-    def auto_center_keep_length(self, input_path, aligned_topk, work_dir, out_path, cons_n):
-        L = self.lib
-        L.ref_auto_center_keep_length.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
-        return L.ref_auto_center_keep_length(input_path.encode(), aligned_topk.encode() if aligned_topk else None,
-                                             work_dir.encode(), out_path.encode(), cons_n)
-
-    def keep_length_msa_refs(self, center_path, aligned_path, input_path, work_dir, out_path):
-        """halign4 -c refs --keep-length, one thread; aligned_path = the MSA
-        command's output for the refs (None: the copy command)."""
-        L = self.lib
-        L.ref_keep_length_msa_refs.argtypes = [C.c_char_p] * 5
-        return L.ref_keep_length_msa_refs(center_path.encode(), aligned_path.encode() if aligned_path else None,
-                                          input_path.encode(), work_dir.encode(), out_path.encode())
-
-    def keep_length_msa(self, center_path, input_path, work_dir, out_path):
-        """halign4 -c --keep-length, one thread, copy MSA command (ref_keep_length_msa)."""
-        L = self.lib
-        L.ref_keep_length_msa.argtypes = [C.c_char_p] * 4
-        return L.ref_keep_length_msa(center_path.encode(), input_path.encode(), work_dir.encode(),
-                                     out_path.encode())
-
     def consensus_file(self, fasta, out_fa, out_json, seq_limit=0, batch=4096, threads=1,
                        aln_len=1 << 20):
         cons = C.create_string_buffer(aln_len)
@@ -480,3 +462,67 @@ class RefLib(_Base):
 
 def ref_available():
     return os.path.exists(REF_SO)
+
+
+def parse_cigar_string(s):
+    """cigar::stringToCigar (src/align/cigar.cpp) of a SAM CIGAR field: uint32
+    len << 4 | op, ops M I D N S H P = X as BAM numbers them."""
+    ops = {b"M": 0, b"I": 1, b"D": 2, b"N": 3, b"S": 4, b"H": 5, b"P": 6, b"=": 7, b"X": 8}
+    out, num = [], 0
+    for ch in s:
+        c = bytes([ch])
+        if c.isdigit():
+            num = num * 10 + ch - 48
+        else:
+            out.append(num << 4 | ops[c])
+            num = 0
+    return np.array(out, np.uint32)
+
+
+class Halign4:
+    """The reference's own program (oracle/_ref/halign4: src/halign4.cpp main
+    with RefAligner, src/align/ref_aligner.cpp), run as a child process with
+    --save-workdir so its per-thread SAM files can be read back."""
+
+    COPY = "cp {input} {output}"
+
+    def __init__(self, path=REF_BIN):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.path = path
+
+    def run(self, input_path, output_path, work_dir, center_path=None, keep_length=True, msa_cmd=COPY,
+            threads=1, cons_n=None, timeout=3600):
+        """Runs halign4 and returns its exit code; work_dir must not exist
+        (the reference wants an empty working directory)."""
+        import subprocess
+        args = [self.path, "-i", input_path, "-o", output_path, "-w", work_dir, "-p", msa_cmd, "-t",
+                str(threads), "--save-workdir"]
+        if center_path:
+            args += ["-c", center_path]
+        if keep_length:
+            args.append("--keep-length")
+        if cons_n is not None:
+            args += ["--cons-n", str(cons_n)]
+        r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=timeout,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+        self.last_log = r.stdout
+        return r.returncode
+
+    @staticmethod
+    def sam_records(work_dir, thread=0):
+        """The records of one thread's two SAM files (result/thread<t>.sam and
+        result/thread<t>_insertion.sam): a list of (qname, rname, CIGAR words,
+        has_insertion) in file order, the plain file first."""
+        out = []
+        res = os.path.join(work_dir, "result")
+        for name, ins in ((f"thread{thread}.sam", False), (f"thread{thread}_insertion.sam", True)):
+            p = os.path.join(res, name)
+            if not os.path.exists(p):
+                continue
+            for line in open(p, "rb").read().split(b"\n"):
+                if not line or line.startswith(b"@"):
+                    continue
+                f = line.split(b"\t")
+                out.append((f[0], f[2], parse_cigar_string(f[5]) if f[5] != b"*" else np.zeros(0, np.uint32), ins))
+        return out

@@ -483,6 +483,32 @@ def test_oracle_matches_compiled_reference_live(O):
         e2, c2 = R.ksw(q, t, w, flag)
         assert e1 == e2 and np.array_equal(c1, c2), it
     for it in range(40):
-        ref = codes_to_ascii(rand_codes(rng, int(rng.integers(50, 3000))))
-        qry = codes_to_ascii(mutate_codes(rng, np.frombuffer(ref, np.uint8) % 5, 0.03, 0.01))
+        rc = rand_codes(rng, int(rng.integers(50, 3000)))
+        ref = codes_to_ascii(rc)
+        qry = codes_to_ascii(mutate_codes(rng, rc, 0.03, 0.01))   # codes, not ASCII: no T read as N
+        assert b"T" in qry and b"N" not in qry
         assert np.array_equal(O.ref_global_align(ref, qry)[0], R.ref_global_align(ref, qry)), it
+
+
+# ------------------------------------------- workload samples (configs 3-5) --
+
+@pytest.mark.parametrize("cfg", [3, 4, 5])
+def test_oracle_matches_reference_program_on_workloads(O, cfg):
+    """The oracle's RefAligner::globalAlign against the CIGARs the reference
+    program wrote for the workload samples (config<c>.npz): the first 40
+    queries and 40 spread over the sample (forced cases included), and the
+    reference's SAM group equals hasInsertion of its CIGAR."""
+    from halign import synth
+    from workloads import workload_queries
+    with np.load(os.path.join(GOLD, "config%d.npz" % cfg), allow_pickle=False) as z:
+        f = {k: z[k] for k in z.files}
+    n = len(f["idx"])
+    pick = np.unique(np.concatenate([np.arange(40), np.linspace(40, n - 1, 40).astype(int)]))
+    q, off = workload_queries(cfg, f["idx"][pick])
+    center = synth.center(cfg)
+    co = f["cigar_off"]
+    for j, k in enumerate(pick):
+        exp = f["cigar"][co[k]:co[k + 1]]
+        assert bool(f["has_insertion"][k]) == bool(np.any((exp & 0xF) == 1)), k
+        got = O.ref_global_align(center, q[off[j]:off[j + 1]])[0]
+        assert np.array_equal(got, exp), (cfg, int(f["idx"][k]))
