@@ -31,6 +31,7 @@ using namespace hal;
 
 
 
+
 namespace hal {
 
 void spread_target(const uint8_t *codes, int len, uint16_t *even, uint16_t *odd)
@@ -520,6 +521,54 @@ bool fill_spec_shape(Spec &s, bool full_stats, int policy)
 
 }  // namespace hal
 
+void hal::ctx_destroy(halign_ctx *ctx)
+{
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->seq.release(); ctx->seqb.release(); ctx->jobs.release(); ctx->outs.release();
+    ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
+    ctx->crows.release(); ctx->ccounts.release(); ctx->ccons.release();
+    for (hal::DevBuf *b : {&ctx->s_seqs, &ctx->s_off, &ctx->s_woff, &ctx->s_moff, &ctx->s_codes, &ctx->s_nmask,
+                           &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list,
+                           &ctx->m_seqs, &ctx->m_off, &ctx->m_koff, &ctx->m_keys, &ctx->m_keys2, &ctx->m_nkeys,
+                           &ctx->m_tmp, &ctx->m_sel})
+        b->release();
+    for (auto &cg : ctx->gcache) (void)hipGraphExecDestroy(cg.exec);
+    for (auto &b : ctx->msa_out) b.release();
+    ctx->msa_tmp.release();
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->up_ev[k]) {
+            (void)hipEventSynchronize(ctx->up_ev[k]);
+            (void)hipEventDestroy(ctx->up_ev[k]);
+        }
+        ctx->up_pin[k].release();
+    }
+    ctx->dn_pin.release();
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    for (int k = 0; k < halign_ctx::kAux; ++k) {
+        (void)hipStreamSynchronize(ctx->aux[k]);
+        (void)hipStreamDestroy(ctx->aux[k]);
+        (void)hipEventDestroy(ctx->join[k]);
+    }
+    (void)hipEventDestroy(ctx->fork);
+    (void)hipEventDestroy(ctx->pfork);
+    (void)hipEventDestroy(ctx->pjoin);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void hal::ctx_detach(halign_ctx *ctx)
+{
+    bool last = false;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        --ctx->children;
+        last = ctx->closing && ctx->children == 0;
+    }
+    if (last) ctx_destroy(ctx);
+}
+
 extern "C" {
 
 const char *halign_version(void) { return "halign-mi355x 0.1 (gfx950)"; }
@@ -561,40 +610,19 @@ int halign_init(int device, halign_ctx **out)
 void halign_free(halign_ctx *ctx)
 {
     if (!ctx) return;
-    if (ctx->msa_center) halign_center_free(ctx->msa_center);
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    ctx->seq.release(); ctx->seqb.release(); ctx->jobs.release(); ctx->outs.release();
-    ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
-    ctx->crows.release(); ctx->ccounts.release(); ctx->ccons.release();
-    for (hal::DevBuf *b : {&ctx->s_seqs, &ctx->s_off, &ctx->s_woff, &ctx->s_moff, &ctx->s_codes, &ctx->s_nmask,
-                           &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list,
-                           &ctx->m_seqs, &ctx->m_off, &ctx->m_koff, &ctx->m_keys, &ctx->m_keys2, &ctx->m_nkeys,
-                           &ctx->m_tmp, &ctx->m_sel})
-        b->release();
-    for (auto &cg : ctx->gcache) (void)hipGraphExecDestroy(cg.exec);
-    for (auto &b : ctx->msa_out) b.release();
-    ctx->msa_tmp.release();
-    for (int k = 0; k < 2; ++k) {
-        if (ctx->up_ev[k]) {
-            (void)hipEventSynchronize(ctx->up_ev[k]);
-            (void)hipEventDestroy(ctx->up_ev[k]);
-        }
-        ctx->up_pin[k].release();
+    if (ctx->msa_center) {   // the keep-length run's cached center is the context's own
+        halign_center *mc = ctx->msa_center;
+        ctx->msa_center = nullptr;
+        halign_center_free(mc);
     }
-    ctx->dn_pin.release();
-    (void)hipEventDestroy(ctx->ev0);
-    (void)hipEventDestroy(ctx->ev1);
-    for (int k = 0; k < halign_ctx::kAux; ++k) {
-        (void)hipStreamSynchronize(ctx->aux[k]);
-        (void)hipStreamDestroy(ctx->aux[k]);
-        (void)hipEventDestroy(ctx->join[k]);
+    bool now = false;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        if (ctx->closing) return;   // freed twice while children were live
+        ctx->closing = true;
+        now = ctx->children == 0;
     }
-    (void)hipEventDestroy(ctx->fork);
-    (void)hipEventDestroy(ctx->pfork);
-    (void)hipEventDestroy(ctx->pjoin);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    if (now) ctx_destroy(ctx);
 }
 
 const char *halign_last_error(const halign_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
@@ -778,10 +806,13 @@ int halign_ksw_extz2(halign_ctx *ctx, void *km, int qlen, const uint8_t *query, 
     ez->max = r.max; ez->zdropped = r.zdropped; ez->max_q = r.max_q; ez->max_t = r.max_t;
     ez->mqe = r.mqe; ez->mqe_t = r.mqe_t; ez->mte = r.mte; ez->mte_q = r.mte_q; ez->score = r.score;
     if (r.n_cigar > ez->m_cigar) {
-        uint32_t *nc = (uint32_t *)realloc(ez->cigar, (size_t)r.n_cigar * 4);
+        // ksw_push_cigar's growth (include/ksw2.h:113-123): 4 when empty, then doubling
+        int m = ez->m_cigar;
+        while (m < r.n_cigar) m = m ? m << 1 : 4;
+        uint32_t *nc = (uint32_t *)realloc(ez->cigar, (size_t)m * 4);
         if (!nc) return fail(ctx, HALIGN_E_NOMEM, "cigar realloc");
         ez->cigar = nc;
-        ez->m_cigar = r.n_cigar;
+        ez->m_cigar = m;
     }
     memcpy(ez->cigar, cig.data(), (size_t)r.n_cigar * 4);
     ez->n_cigar = r.n_cigar;
@@ -1009,6 +1040,7 @@ int halign_pairset_create(halign_ctx *ctx, const char *center, int32_t clen, int
     HCHECK(hipMemcpy(ps->centerb.p, hcb.data(), hcb.size(), hipMemcpyHostToDevice), "upload center bytes");
     HCHECK(hipMemcpy(ps->raw.p, queries + query_offsets[0], (size_t)total, hipMemcpyHostToDevice), "upload queries");
     HCHECK(hipMemcpy(ps->qoff.p, ps->hoff.data(), (size_t)(n + 1) * 8, hipMemcpyHostToDevice), "upload offsets");
+    ctx_attach(ctx);
     *out = ps;
     return HALIGN_OK;
 }
@@ -1113,9 +1145,16 @@ int halign_pairset_result(halign_pairset *ps, int32_t idx, int32_t *score, uint3
 void halign_pairset_free(halign_pairset *ps)
 {
     if (!ps) return;
-    ps->center.release(); ps->centerb.release(); ps->raw.release(); ps->bregion.release(); ps->qoff.release(); ps->work.release();
-    ps->cig.release(); ps->region.release();
+    halign_ctx *ctx = ps->ctx;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ps->center.release(); ps->centerb.release(); ps->raw.release(); ps->bregion.release(); ps->qoff.release();
+        ps->work.release(); ps->cig.release(); ps->region.release();
+    }
     delete ps;
+    ctx_detach(ctx);   // releases the context if it was freed and this was its last child
 }
 
 }  // extern "C"

@@ -121,9 +121,22 @@ struct halign_ctx {
     // downloads of the large-segment records
     hal::HostPinned up_pin[2], dn_pin;
     hipEvent_t up_ev[2] = {};
+    // Centers and pair sets live on the context's streams and buffers. They
+    // are counted here (under mu); halign_free with some still live only
+    // marks the context closing, and the last child's free releases it.
+    int children = 0;
+    bool closing = false;
 };
 
 namespace hal {
+
+// Registers a new center or pair set on ctx (ctx->mu held by the caller).
+inline void ctx_attach(halign_ctx *ctx) { ++ctx->children; }
+// Releases the context itself: streams, events, buffers, graphs.
+void ctx_destroy(halign_ctx *ctx);
+// Unregisters a child whose resources are already released (ctx->mu not
+// held); releases the context when it was freed and this was its last child.
+void ctx_detach(halign_ctx *ctx);
 
 // Host phase timing on stderr when the environment sets HALIGN_TRACE
 struct PhaseClock {

@@ -647,6 +647,7 @@ int halign_center_create_indexed(halign_ctx *ctx, const char *center, int32_t le
         delete c;
         return fail(ctx, HALIGN_E_HIP, "upload center index");
     }
+    ctx_attach(ctx);
     *out = c;
     return HALIGN_OK;
 }
@@ -654,10 +655,12 @@ int halign_center_create_indexed(halign_ctx *ctx, const char *center, int32_t le
 void halign_center_free(halign_center *c)
 {
     if (!c) return;
-    std::lock_guard<std::mutex> lock(c->ctx->mu);
-    (void)hipSetDevice(c->ctx->device);
-    (void)hipStreamSynchronize(c->ctx->stream);
-    for (int k = 0; k < halign_ctx::kAux; ++k) (void)hipStreamSynchronize(c->ctx->aux[k]);
+    halign_ctx *ctx = c->ctx;
+    {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int k = 0; k < halign_ctx::kAux; ++k) (void)hipStreamSynchronize(ctx->aux[k]);
     for (DevBuf *b : {&c->sx, &c->sy, &c->dh, &c->dstart, &c->docc, &c->lut, &c->hcnt, &c->scratch, &c->hidx,
                       &c->nanch, &c->totals, &c->acount, &c->aoff, &c->anchors, &c->keys, &c->keys2, &c->vals,
                       &c->vals2, &c->sort_tmp, &c->fw, &c->pw, &c->chain_idx, &c->chain_cnt, &c->qlens,
@@ -667,7 +670,9 @@ void halign_center_free(halign_center *c)
                       &c->rlen_d, &c->rws, &c->hins, &c->nm, &c->nmoff, &c->rsz, &c->roff, &c->rcnt,
                       &c->recs, &c->cmap, &c->lsmall, &c->ljobs, &c->ljseg, &c->louts, &c->ltb})
         b->release();
+    }
     delete c;
+    ctx_detach(ctx);   // releases the context if it was freed and this was its last child
 }
 
 int halign_center_anchors(halign_center *c, int32_t n, const char *queries, const int64_t *offsets,

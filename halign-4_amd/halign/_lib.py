@@ -44,6 +44,31 @@ class MsaStats(C.Structure):
                 ("seconds", C.c_double), ("gpu_seconds", C.c_double), ("input_wait_seconds", C.c_double)]
 
 
+class ExtzT(C.Structure):
+    """halign_extz_t: the field layout of ksw2's ksw_extz_t (include/ksw2.h:33-42)."""
+    _fields_ = [("max", C.c_uint32, 31), ("zdropped", C.c_uint32, 1), ("max_q", C.c_int), ("max_t", C.c_int),
+                ("mqe", C.c_int), ("mqe_t", C.c_int), ("mte", C.c_int), ("mte_q", C.c_int), ("score", C.c_int),
+                ("m_cigar", C.c_int), ("n_cigar", C.c_int), ("reach_end", C.c_int),
+                ("cigar", C.POINTER(C.c_uint32))]
+
+    def cigar_array(self):
+        return np.array([self.cigar[i] for i in range(self.n_cigar)], np.uint32)
+
+
+# dna5 scoring matrix of the path (reference include/align.h:109-116)
+DNA5_MAT = (C.c_int8 * 25)(5, -4, -4, -4, 0, -4, 5, -4, -4, 0, -4, -4, 5, -4, 0, -4, -4, -4, 5, 0, 0, 0, 0, 0, 0)
+_libc = C.CDLL(None)
+_libc.free.argtypes = [C.c_void_p]
+
+
+def free_extz(ez):
+    """Releases ez.cigar with free(), as the reference caller does (src/align/align.cpp:63)."""
+    if ez.cigar:
+        _libc.free(C.cast(ez.cigar, C.c_void_p))
+    ez.cigar = C.POINTER(C.c_uint32)()
+    ez.m_cigar = ez.n_cigar = 0
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("workers", C.c_int32), ("max_batch_queries", C.c_int32), ("hbm_budget", C.c_int64)]
 
@@ -60,6 +85,8 @@ def _load():
     lib.halign_version.restype = C.c_char_p
     lib.halign_set_kernel_policy.argtypes = [_P, C.c_int]
     lib.halign_ring_shapes.argtypes = [_P, _P, _P, C.c_int32]
+    lib.halign_ksw_extz2.argtypes = [_P, _P, C.c_int, _P, C.c_int, _P, C.c_int8, _P, C.c_int8, C.c_int8,
+                                     C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(ExtzT)]
     lib.halign_dp_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, _P,
                                     C.c_int64, C.POINTER(C.c_double)]
     lib.halign_global_align_batch.argtypes = [_P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int64,
@@ -205,6 +232,16 @@ class Context:
         a, b = C.c_int64(0), C.c_int64(0)
         self._check(self.lib.halign_graph_counts(self.h, C.byref(a), C.byref(b)), "graph_counts")
         return a.value, b.value
+
+    def ksw_extz2(self, query_codes, target_codes, w, flag, ez, m=5, mat=DNA5_MAT, q=6, e=2, zdrop=-1,
+                  end_bonus=0):
+        """halign_ksw_extz2 with ksw_extz2_sse's argument list (include/ksw2.h:64-65):
+        fills ez (an ExtzT reused across calls; its cigar grows by realloc and
+        is released by free_extz). Returns the status code."""
+        qc = np.ascontiguousarray(query_codes, np.uint8)
+        tc = np.ascontiguousarray(target_codes, np.uint8)
+        return self.lib.halign_ksw_extz2(self.h, None, len(qc), _ptr(qc), len(tc), _ptr(tc), m, mat, q, e, w, zdrop,
+                                         end_bonus, flag, C.byref(ez))
 
     def dp_batch(self, tasks, query_codes, target_codes, full_stats=True, plain=False):
         """tasks: structured array DP_TASK_DTYPE. Returns (results, cigars list, kernel_ms)."""

@@ -60,6 +60,10 @@ typedef struct {
 
 /* Context ------------------------------------------------------------------ */
 int halign_init(int device, halign_ctx **out);
+/* Frees the context. Centers and pair sets made on it may be freed before or
+ * after it: while any is live the context's streams and buffers stay
+ * allocated for them (they remain usable) and are released when the last one
+ * is freed. The ctx handle itself must not be passed to any call after this. */
 void halign_free(halign_ctx *ctx);
 const char *halign_last_error(const halign_ctx *ctx);
 /* library version string */
@@ -88,7 +92,12 @@ int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t ca
  * ksw_extz_t field exact. Supports the path's configuration only: m=5, the
  * dna5 matrix (include/align.h:109-116), q=6, e=2, zdrop<0, end_bonus=0, flag
  * GENERIC_SC with or without RIGHT; anything else -> HALIGN_E_UNSUPPORTED.
- * km is ignored (kalloc pool in the reference). */
+ * km is ignored (kalloc pool in the reference). As ksw2: ez is reset first
+ * (ksw_reset_extz, include/ksw2.h:184-189), m <= 0, qlen <= 0 or tlen <= 0
+ * return with the reset ez (src/align/ksw2_extz2_sse.c:56-57), and ez->cigar
+ * grows by realloc as ksw_push_cigar grows it (m_cigar 4 when empty, then
+ * doubling; include/ksw2.h:113-123), so one ez can be reused across calls and
+ * its cigar is released with free(). */
 int halign_ksw_extz2(halign_ctx *ctx, void *km, int qlen, const uint8_t *query, int tlen,
                      const uint8_t *target, int8_t m, const int8_t *mat, int8_t q, int8_t e,
                      int w, int zdrop, int end_bonus, int flag, halign_extz_t *ez);
@@ -152,6 +161,7 @@ int halign_center_create(halign_ctx *ctx, const char *center, int32_t len, halig
  * extracts them on the spot then, :103-106). */
 int halign_center_create_indexed(halign_ctx *ctx, const char *center, int32_t len, const char *index_seq,
                                  int32_t index_len, halign_center **out);
+/* Frees the center (before or after its context, see halign_free). */
 void halign_center_free(halign_center *c);
 /* minimizer::collect_anchors(center hits, query hits) (src/seed/minimizer.cpp:
  * 220-356) for n ASCII queries (queries[offsets[i] .. offsets[i+1])), default
@@ -343,6 +353,7 @@ int halign_pairset_run(halign_pairset *ps, int32_t first, int32_t count, int32_t
 /* Score and CIGAR of query idx from the last run that covered it. */
 int halign_pairset_result(halign_pairset *ps, int32_t idx, int32_t *score, uint32_t *cigar,
                           int32_t cigar_cap, int32_t *n_cigar);
+/* Frees the pair set (before or after its context, see halign_free). */
 void halign_pairset_free(halign_pairset *ps);
 
 /* on != 0 (the default): the batched DP launches of a call (one persistent
