@@ -115,6 +115,22 @@ __global__ void pick_consensus(const uint32_t *__restrict__ counts, int64_t aln_
     cons[i] = ch;
 }
 
+__global__ void zero_words(uint32_t *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// Clears n 32-bit words with a fill kernel (no memset node in a captured graph).
+hipError_t zero_words_launch(uint32_t *p, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(zero_words, dim3((unsigned)g), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
 // Launch both kernels on `stream`. counts must be zeroed by the caller.
 hipError_t consensus_launch(const uint8_t *rows, int64_t row_stride, int64_t aln_len, int64_t row_end, int64_t batch,
                             int num_cu, uint32_t *counts, char *cons, hipStream_t stream)

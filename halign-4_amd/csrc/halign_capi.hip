@@ -25,10 +25,19 @@ hipError_t dp_launch(int idx, bool stats, int groups, const DpJob *jobs, int njo
                      uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
 hipError_t consensus_launch(const uint8_t *rows, int64_t row_stride, int64_t aln_len, int64_t row_end, int64_t batch,
                             int num_cu, uint32_t *counts, char *cons, hipStream_t stream);
+hipError_t zero_words_launch(uint32_t *p, int64_t n, hipStream_t stream);
 }  // namespace hal
 
 using namespace hal;
 
+// Device scratch of halign_consensus_noalloc: the 7 counts per column
+// (uint32) and the consensus characters of one call, for alignments of up to
+// kNoAllocMaxColumns columns. One copy per GPU, alive until the process ends;
+// host code uses the address hipGetSymbolAddress returns for the context's
+// device (halign_ctx::noalloc_scratch), never this name.
+static constexpr int64_t kNoAllocMaxColumns = (int64_t)1 << 20;
+static constexpr size_t kNoAllocScratchBytes = (size_t)kNoAllocMaxColumns * (7 * 4 + 1);
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
 
 
 
@@ -597,6 +606,11 @@ int halign_init(int device, halign_ctx **out)
         delete ctx;
         return HALIGN_E_HIP;
     }
+    // this device's copy of the no-allocation scratch array (device current)
+    if (hipGetSymbolAddress(&ctx->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess || !ctx->noalloc_scratch) {
+        ctx_destroy(ctx);
+        return HALIGN_E_HIP;
+    }
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     // Traceback pool: one slot per resident workgroup. A config-2 pair needs a
@@ -999,6 +1013,45 @@ int halign_consensus(halign_ctx *ctx, const char *rows, int64_t nrows, int64_t a
         HCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
         *kernel_ms = ms;
     }
+    if (num_seqs) *num_seqs = (uint64_t)counted;
+    return HALIGN_OK;
+}
+
+int halign_consensus_noalloc(halign_ctx *ctx, const char *rows_dev, int64_t nrows, int64_t aln_len,
+                             int64_t row_stride, int64_t batch, uint64_t seq_limit, uint32_t *counts_out,
+                             char *cons_out, uint64_t *num_seqs)
+{
+    if (!ctx || !rows_dev || nrows <= 0 || aln_len <= 0 || row_stride < aln_len || batch <= 0 || !counts_out ||
+        !cons_out)
+        return fail(ctx, HALIGN_E_ARG, "consensus_noalloc: empty or inconsistent input");
+    if (aln_len > kNoAllocMaxColumns)
+        return fail(ctx, HALIGN_E_CAPACITY, "consensus_noalloc: more columns than the static scratch holds (1,048,576)");
+    const int64_t stride4 = (aln_len + 3) & ~(int64_t)3;
+    if (row_stride % 4 != 0 || row_stride < stride4 || ((uintptr_t)rows_dev & 3))
+        return fail(ctx, HALIGN_E_ARG, "consensus_noalloc: device rows need a 4-byte aligned stride >= aln_len rounded to 4");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HCHECK(hipSetDevice(ctx->device), "set device");
+    const int64_t B = batch;
+    const int64_t avail = (nrows / (B + 1)) * B + std::min<int64_t>(nrows % (B + 1), B);
+    int64_t counted = avail, row_end = nrows;
+    if (seq_limit > 0 && (uint64_t)avail > seq_limit) {
+        counted = (int64_t)seq_limit;
+        const int64_t m = counted - 1;
+        row_end = (m / B) * (B + 1) + (m % B) + 1;
+    }
+    if (counted < 1) counted = 1;
+    // counts then characters, both in this device's scratch; the previous
+    // call's counts are cleared by a fill kernel
+    uint32_t *counts = (uint32_t *)ctx->noalloc_scratch;
+    char *cons = (char *)ctx->noalloc_scratch + (size_t)kNoAllocMaxColumns * 7 * 4;
+    HCHECK(zero_words_launch(counts, aln_len * 7, ctx->stream), "clear counts");
+    HCHECK(consensus_launch((const uint8_t *)rows_dev, row_stride, aln_len, row_end, B, ctx->num_cu, counts, cons,
+                            ctx->stream),
+           "launch consensus");
+    HCHECK(hipMemcpyAsync(counts_out, counts, (size_t)aln_len * 7 * 4, hipMemcpyDeviceToHost, ctx->stream),
+           "download counts");
+    HCHECK(hipMemcpyAsync(cons_out, cons, (size_t)aln_len, hipMemcpyDeviceToHost, ctx->stream), "download consensus");
+    HCHECK(hipStreamSynchronize(ctx->stream), "consensus kernels");
     if (num_seqs) *num_seqs = (uint64_t)counted;
     return HALIGN_OK;
 }

@@ -126,6 +126,8 @@ struct halign_ctx {
     // marks the context closing, and the last child's free releases it.
     int children = 0;
     bool closing = false;
+    // this device's address of the static scratch of halign_consensus_noalloc
+    void *noalloc_scratch = nullptr;
 };
 
 namespace hal {

@@ -100,6 +100,8 @@ def _load():
     lib.halign_consensus.argtypes = [_P, _P, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int64,
                                      C.c_uint64, _P, _P, C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_double)]
+    lib.halign_consensus_noalloc.argtypes = [_P, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                             C.c_uint64, _P, _P, C.POINTER(C.c_uint64)]
     lib.halign_set_seed_policy.argtypes = [_P, C.c_int]
     lib.halign_extract_minimizers.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.c_int32, C.c_int32,
                                               C.c_int32, _P, _P, C.c_int64, _P, _P]

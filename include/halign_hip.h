@@ -338,6 +338,19 @@ int halign_consensus(halign_ctx *ctx, const char *rows, int64_t nrows, int64_t a
                      int32_t rows_on_device, int64_t batch, uint64_t seq_limit, uint32_t *counts_out,
                      char *cons_out, uint64_t *num_seqs, double *kernel_ms);
 
+/* halign_consensus for rows already on the device (rows_on_device = 1), with
+ * no device or host allocation: no counterpart in the reference; it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The counts and consensus
+ * characters are built in a static device array of this library (one per
+ * GPU, kept until the process ends) and copied to counts_out / cons_out on
+ * the host, so aln_len is at most 1,048,576 columns (HALIGN_E_CAPACITY
+ * beyond). Calls on one context are serialised; calls on two contexts of the
+ * same GPU that overlap race on that array. Results equal halign_consensus's. */
+int halign_consensus_noalloc(halign_ctx *ctx, const char *rows_dev, int64_t nrows, int64_t aln_len,
+                             int64_t row_stride, int64_t batch, uint64_t seq_limit, uint32_t *counts_out,
+                             char *cons_out, uint64_t *num_seqs);
+
 /* Device-resident workload: one center (target) and n queries kept in HBM,
  * aligned pairwise as whole pairs (globalAlignKSW2 semantics). Used for the
  * kernel-only DP throughput measurement and by the serving loop. */
