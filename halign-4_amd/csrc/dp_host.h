@@ -92,6 +92,8 @@ int dp_wave_shape_waves(int idx);
 int dp_wave_shape_ring(int idx);
 int dp_wave_jobs_per_group(int idx);
 int dp_wave_shape_lanes(int idx);
+int dp_wave_shape_uniform(int idx);   // 1: a sub-wave shape's uniform-ticket form
+int dp_wave_uniform_of(int idx);      // the uniform-ticket form of a sub-wave shape (else idx)
 void dp_wave_shape_cost(int idx, int *issue, int *valu);
 int dp_wave_max_groups(int idx, bool rt, int num_cu);
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,

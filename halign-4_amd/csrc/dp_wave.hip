@@ -209,20 +209,30 @@ struct WaveSmem {
 // bounds are per group, the band-edge masks are computed from the bounds, the
 // H-edge bytes come from the owning lane by ds_bpermute one anti-diagonal later,
 // and the backtrack is group_backtrack.
-template <int W, int S, bool RT, int SUB = 0>
+// UNI (sub-wave mode only): every group of the wave holds a job with the same
+// qlen, tlen and band, so the band bounds, masks and ring positions are
+// wave-uniform (scalar) as in the one-wave ring; only the sequences, the
+// H-row edge and the backtrack stay per group.
+template <int W, int S, bool RT, int SUB = 0, bool UNI = false>
 __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restrict__ out,
                             WaveSmem<W, S> &sm)
 {
     constexpr bool ROW = SUB != 0;
+    constexpr bool ROWB = ROW && !UNI;   // per-group band bounds
     static_assert(!ROW || (W == 1 && (SUB == 4 || SUB == 16)), "sub-wave groups of 4 or 16 lanes in one wave");
+    static_assert(!UNI || ROW, "uniform bounds are a sub-wave mode");
     constexpr int NL = ROW ? SUB : 64 * W;   // lanes per job
     constexpr int R = NL * S;                // ring size in blocks
     const int L = ROW ? (int)(threadIdx.x & (NL - 1)) : (int)threadIdx.x, lane = L & 63;
     const int wave = ROW ? 0 : __builtin_amdgcn_readfirstlane(L >> 6);
-    const int qlen = J.qlen, tlen = J.tlen;
-    int w = J.w;
+    const int qlen = UNI ? __builtin_amdgcn_readfirstlane(J.qlen) : J.qlen;
+    const int tlen = UNI ? __builtin_amdgcn_readfirstlane(J.tlen) : J.tlen;
+    int w = UNI ? __builtin_amdgcn_readfirstlane(J.w) : J.w;
     if (w < 0) w = tlen > qlen ? tlen : qlen;
-    const bool hasn = (J.flags & kFlagHasN) != 0;
+    // UNI: the N correction of the score runs for every group when any job has
+    // an N (it subtracts 0 where no N is involved)
+    const bool hasn = UNI ? __builtin_amdgcn_ballot_w64((J.flags & kFlagHasN) != 0) != 0
+                          : (J.flags & kFlagHasN) != 0;
     const int ndiag = qlen + tlen - 1;
     int ncol = qlen < tlen ? qlen : tlen;
     ncol = ((ncol < w + 1 ? ncol : w + 1) + 15) / 16 + 1;
@@ -321,7 +331,7 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
         cells += en0 - st0 + 1;
         // band-edge masks (uniform): cells < st0 of the first block, cells <= en0 of the last
         uint32_t leftm[8], lastm[8];
-        if constexpr (ROW) {
+        if constexpr (ROWB) {
             // per-row bounds: half (c < k) is all ones iff c - k < 0 as int16, for
             // the cell pair (j, j+8) of register j
             const uint32_t kl = (uint32_t)(st0 & 15) * 0x00010001u, ke1 = (uint32_t)((en0 & 15) + 1) * 0x00010001u;
@@ -679,6 +689,33 @@ __global__ __launch_bounds__(64) void dp_sub_persistent(const DpJob *__restrict_
     }
 }
 
+// Sub-wave persistent kernel over uniform tickets: the G jobs of every ticket
+// share qlen, tlen and band (the host pads each key's jobs to a multiple of G
+// with repeats of its last job), so the band bookkeeping runs in scalars.
+template <int SUB, int S, bool RT>
+__global__ __launch_bounds__(64) void dp_usub_persistent(const DpJob *__restrict__ jobs, int njobs,
+                                                         int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
+                                                         size_t tb_slot_bytes, DpOut *__restrict__ outs,
+                                                         const int *__restrict__ range)
+{
+    constexpr int G = 64 / SUB;
+    if (range) {
+        jobs += range[0];
+        outs += range[0];
+        njobs = range[1];
+    }
+    __shared__ WaveSmem<1, S> sm;
+    const int g = (int)(threadIdx.x / SUB);
+    uint8_t *tb = tb_pool + ((size_t)blockIdx.x * G + (size_t)g) * tb_slot_bytes;
+    for (;;) {
+        const int t = __builtin_amdgcn_readfirstlane(atomicAdd(counter, threadIdx.x == 0 ? 1 : 0));
+        if (G * t >= njobs) break;
+        const int jid = G * t + g < njobs ? G * t + g : njobs - 1;
+        const DpJob J = jobs[jid];
+        dp_wave_job<1, S, RT, SUB, true>(J, tb, &outs[jid], sm);
+    }
+}
+
 // Ring shapes: (W waves, S slots per lane) serve n_col_ <= 64*W*S - 2;
 // sub-wave shapes (SUB lanes x S slots per job, 64/SUB jobs per wave) serve
 // n_col_ <= SUB*S - 2.
@@ -687,12 +724,16 @@ __global__ __launch_bounds__(64) void dp_sub_persistent(const DpJob *__restrict_
     X(2, 3) X(1, 5) X(2, 4)
 #define HAL_SUB_SHAPES(X) X(16, 1) X(16, 2) X(16, 3) X(4, 1) X(4, 2)
 
-struct ShapeDesc { int waves, sub, slots; };   // waves = 0: sub-wave shape with groups of sub lanes
+// waves = 0: sub-wave shape with groups of sub lanes; uni: its uniform-ticket form
+struct ShapeDesc { int waves, sub, slots, uni; };
 static const ShapeDesc kShapes[] = {
-#define X(w, s) {w, 0, s},
+#define X(w, s) {w, 0, s, 0},
     HAL_WAVE_SHAPES(X)
 #undef X
-#define X(g, s) {0, g, s},
+#define X(g, s) {0, g, s, 0},
+    HAL_SUB_SHAPES(X)
+#undef X
+#define X(g, s) {0, g, s, 1},
     HAL_SUB_SHAPES(X)
 #undef X
 };
@@ -707,6 +748,9 @@ static const void *wave_kernel(int idx, bool rt)
 #define X(g, s) {(const void *)&dp_sub_persistent<g, s, false>, (const void *)&dp_sub_persistent<g, s, true>},
         HAL_SUB_SHAPES(X)
 #undef X
+#define X(g, s) {(const void *)&dp_usub_persistent<g, s, false>, (const void *)&dp_usub_persistent<g, s, true>},
+        HAL_SUB_SHAPES(X)
+#undef X
     };
     return tab[idx][rt ? 1 : 0];
 }
@@ -718,6 +762,16 @@ int dp_wave_shape_waves(int idx) { return kShapes[idx].waves; }
 int dp_wave_shape_lanes(int idx) { return kShapes[idx].waves ? 64 * kShapes[idx].waves : kShapes[idx].sub; }
 int dp_wave_shape_ring(int idx) { return dp_wave_shape_lanes(idx) * kShapes[idx].slots; }
 int dp_wave_jobs_per_group(int idx) { return kShapes[idx].waves ? 1 : 64 / kShapes[idx].sub; }
+int dp_wave_shape_uniform(int idx) { return kShapes[idx].uni; }
+// the uniform-ticket form of sub-wave shape idx (itself if it has none)
+int dp_wave_uniform_of(int idx)
+{
+    const ShapeDesc &d = kShapes[idx];
+    if (d.waves || d.uni) return idx;
+    for (int i = 0; i < kNumShapes; ++i)
+        if (!kShapes[i].waves && kShapes[i].uni && kShapes[i].sub == d.sub && kShapes[i].slots == d.slots) return i;
+    return idx;
+}
 
 // Instructions per anti-diagonal of one wave with every slot active, counted
 // in the gfx950 code of each kernel's diagonal loop (hipcc -O3): all issued

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/halign_hip.h"
@@ -212,6 +213,50 @@ static void plan_bucket_shapes(const std::vector<Spec> &specs, const std::vector
     for (auto &c : bc) shape_of_bucket[c.bi] = kWaveBase + c.shapes[c.cur];
 }
 
+// Uniform tickets: a sub-wave bucket whose jobs share (qlen, tlen, band) in
+// groups runs on the uniform form of its shape when padding every such key to
+// a whole number of tickets (jobs per wave) adds at most a quarter more jobs:
+// the bucket becomes its keys in falling anti-diagonal count, each padded with
+// repeats of its last job. The uniform form keeps the band bookkeeping in
+// scalars, about half the VALU work per anti-diagonal of the per-group form.
+static void form_uniform_tickets(std::vector<Spec> &specs, std::vector<std::vector<int>> &buckets)
+{
+    const int nb = (int)buckets.size();
+    std::vector<std::vector<int>> moved(nb);
+    for (int bi = 0; bi < nb; ++bi) {
+        auto &b = buckets[bi];
+        const int shape = bi / 4;
+        if (b.empty() || shape < kWaveBase) continue;
+        const int wi = shape - kWaveBase;
+        const int G = dp_wave_jobs_per_group(wi);
+        const int ui = dp_wave_uniform_of(wi);
+        const bool forced = dp_wave_shape_uniform(wi) != 0;   // a uniform shape must get its tickets
+        if (G <= 1 || (ui == wi && !forced)) continue;
+        auto key = [&](int i) {
+            const Spec &x = specs[i];
+            return std::make_tuple(x.qlen + x.tlen, x.qlen, x.tlen, eff_w(x.w, x.qlen, x.tlen));
+        };
+        std::vector<int> srt(b);
+        std::stable_sort(srt.begin(), srt.end(), [&](int x, int y) { return key(x) > key(y); });
+        std::vector<int> padded;
+        padded.reserve(srt.size() + srt.size() / 2);
+        for (size_t a = 0; a < srt.size();) {
+            size_t e = a;
+            while (e < srt.size() && key(srt[e]) == key(srt[a])) ++e;
+            padded.insert(padded.end(), srt.begin() + a, srt.begin() + e);
+            for (size_t c = e - a; c % G; ++c) padded.push_back(srt[e - 1]);
+            a = e;
+        }
+        if (!forced && padded.size() * 4 > b.size() * 5) continue;
+        for (int i : b) specs[i].shape = kWaveBase + ui;
+        auto &dst = moved[(kWaveBase + ui) * 4 + (bi & 3)];
+        dst.insert(dst.end(), padded.begin(), padded.end());
+        b.clear();
+    }
+    for (int bi = 0; bi < nb; ++bi)
+        if (!moved[bi].empty()) buckets[bi].insert(buckets[bi].end(), moved[bi].begin(), moved[bi].end());
+}
+
 // Run a list of jobs whose sequences already sit in ctx->seq (spread u16) and
 // whose CIGARs go to ctx->cig. outs receives one DpOut per job, host side.
 int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector<DpOut> &outs,
@@ -248,23 +293,29 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         buckets.swap(moved);
     }
 
-    HCHECK(ctx->jobs.ensure(n * sizeof(DpJob)), "alloc jobs");
-    HCHECK(ctx->outs.ensure(n * sizeof(DpOut)), "alloc outs");
-    HCHECK(ctx->counters.ensure((size_t)(plain ? n : nb) * sizeof(int)), "alloc counters");
-    std::vector<DpJob> hjobs(n);
+    if (!plain) form_uniform_tickets(specs, buckets);
     std::vector<int> order;
     order.reserve(n);
     for (int bi = 0; bi < nb; ++bi) {
         auto &b = buckets[bi];
-        // row shapes run four jobs in one wave: neighbours of similar length
+        // row shapes run several jobs in one wave: neighbours of similar length
         // (longest first) keep the rows of a wave busy for the same time
-        if (bi / 4 >= kWaveBase && dp_wave_jobs_per_group(bi / 4 - kWaveBase) > 1)
+        // (uniform tickets are already ordered and padded)
+        if (bi / 4 >= kWaveBase && dp_wave_jobs_per_group(bi / 4 - kWaveBase) > 1 &&
+            !dp_wave_shape_uniform(bi / 4 - kWaveBase))
             std::stable_sort(b.begin(), b.end(), [&](int x, int y) {
                 return specs[x].qlen + specs[x].tlen > specs[y].qlen + specs[y].tlen;
             });
         for (int i : b) order.push_back(i);
     }
-    for (size_t k = 0; k < n; ++k) {
+    // with uniform tickets a job may appear more than once (its repeats
+    // compute and write the same results)
+    const size_t nj = order.size();
+    HCHECK(ctx->jobs.ensure(nj * sizeof(DpJob)), "alloc jobs");
+    HCHECK(ctx->outs.ensure(nj * sizeof(DpOut)), "alloc outs");
+    HCHECK(ctx->counters.ensure((size_t)(plain ? nj : nb) * sizeof(int)), "alloc counters");
+    std::vector<DpJob> hjobs(nj);
+    for (size_t k = 0; k < nj; ++k) {
         const Spec &s = specs[order[k]];
         DpJob &j = hjobs[k];
         j.tgt_even = seq + s.tgt_elem;
@@ -279,7 +330,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         j.pad_ = 0;
     }
     clk.mark("buckets and jobs");
-    HCHECK(hipMemcpyAsync(ctx->jobs.p, hjobs.data(), n * sizeof(DpJob), hipMemcpyHostToDevice, ctx->stream),
+    HCHECK(hipMemcpyAsync(ctx->jobs.p, hjobs.data(), nj * sizeof(DpJob), hipMemcpyHostToDevice, ctx->stream),
            "upload jobs");
     HCHECK(hipMemsetAsync(ctx->counters.p, 0, ctx->counters.cap, ctx->stream), "clear counters");
 
@@ -462,7 +513,8 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             const int wi = shape - kWaveBase;
             if (clk.on)
                 fprintf(stderr, "[halign run_specs] bucket: %s %d x%d lanes/job %d ring %d, %zu jobs, %d groups, stream %d, est %.3g\n",
-                        wave ? "ring" : "multiwave", wave ? dp_wave_shape_waves(wi) : shape,
+                        wave ? (dp_wave_shape_uniform(wi) ? "ring(uniform)" : "ring") : "multiwave",
+                        wave ? dp_wave_shape_waves(wi) : shape,
                         wave ? dp_wave_jobs_per_group(wi) : 1, wave ? dp_wave_shape_lanes(wi) : 0,
                         wave ? dp_wave_shape_ring(wi) : 0, buckets[L.bi].size(), L.groups, L.stream, L.est);
             hipStream_t st = L.stream ? ctx->aux[L.stream - 1] : ctx->stream;
@@ -498,15 +550,15 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     }
     clk.mark("launches");
     HCHECK(hipEventRecord(ctx->ev1, ctx->stream), "event");
-    std::vector<DpOut> tmp(n);
-    HCHECK(hipMemcpyAsync(tmp.data(), ctx->outs.p, n * sizeof(DpOut), hipMemcpyDeviceToHost, ctx->stream),
+    std::vector<DpOut> tmp(nj);
+    HCHECK(hipMemcpyAsync(tmp.data(), ctx->outs.p, nj * sizeof(DpOut), hipMemcpyDeviceToHost, ctx->stream),
            "download outs");
     HCHECK(hipStreamSynchronize(ctx->stream), "dp kernels");
     clk.mark("kernels");
     float ms = 0.f;
     HCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
     if (kernel_ms) *kernel_ms = ms;
-    for (size_t k = 0; k < n; ++k) outs[order[k]] = tmp[k];
+    for (size_t k = 0; k < nj; ++k) outs[order[k]] = tmp[k];
     for (size_t i = 0; i < n; ++i) {
         const int st = outs[i].status;
         if (st != kStOk) {

@@ -180,14 +180,16 @@ def ring_shapes():
     return [(int(a), int(b), int(c)) for a, b, c in zip(w, g, r)]
 
 
-def ring_policy(waves, slots, lanes=None):
+def ring_policy(waves, slots, lanes=None, uniform=False):
     """Kernel policy that forces the ring shape (waves, slots per lane); waves 0
-    selects the sub-wave shape with groups of `lanes` lanes (default 16)."""
+    selects the sub-wave shape with groups of `lanes` lanes (default 16), in its
+    per-group form or (uniform=True) its uniform-ticket form, listed second."""
     lanes = 64 * waves if waves else (lanes or 16)
-    for i, (w, g, r) in enumerate(ring_shapes()):
-        if w == waves and g == lanes and r == lanes * slots:
-            return 2 + i
-    raise ValueError(f"no ring shape with {waves} waves, {lanes} lanes and {slots} slots")
+    hits = [2 + i for i, (w, g, r) in enumerate(ring_shapes()) if w == waves and g == lanes and r == lanes * slots]
+    want = 1 if (uniform and not waves) else 0
+    if len(hits) > want:
+        return hits[want]
+    raise ValueError(f"no ring shape with {waves} waves, {lanes} lanes and {slots} slots (uniform={uniform})")
 
 
 class Context:

@@ -84,7 +84,10 @@ int halign_band_info(int32_t qlen, int32_t tlen, int32_t w, int32_t *ncol, int32
 /* Ring shapes of the ring-form kernel: returns their number and fills up to
  * cap entries of waves per workgroup (0 for a sub-wave shape, whose wave runs
  * 64/lanes jobs at once), lanes per job and ring size in 16-cell blocks (a
- * shape serves bands of n_col_ <= ring - 2 blocks). No reference counterpart. */
+ * shape serves bands of n_col_ <= ring - 2 blocks). Each sub-wave shape is
+ * listed twice: first its per-group form, then its uniform-ticket form (all
+ * jobs of a wave share qlen, tlen and band; the library groups the jobs and
+ * pads each group with repeats). No reference counterpart. */
 int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t cap);
 
 /* Drop-in for ksw_extz2_sse (reference include/ksw2.h:64-65, called at

@@ -268,3 +268,63 @@ def test_pairset_golden_whole_pairs(ctx):
         sc, cig = ps.result(i)
         assert np.array_equal(cig, g["cigar"][g["cigar_off"][i]:g["cigar_off"][i + 1]]), i
     ps.close()
+
+
+UNIFORM_SHAPES = [(16, 1, 16), (16, 2, 16), (16, 3, 16), (4, 1, 4), (4, 2, 4)]   # (lanes, slots, max side)
+
+
+@pytest.mark.parametrize("lanes,slots,_", UNIFORM_SHAPES, ids=lambda v: str(v))
+def test_dp_uniform_tickets_forced(oracle, lanes, slots, _):
+    """The uniform-ticket form of each sub-wave shape, forced: jobs in groups
+    of equal (qlen, tlen, band) whose sizes are not multiples of the jobs per
+    wave (the library pads each group with repeats), different sequences in
+    every job, unbanded and auto-band, both gap directions, N bases; every
+    result equals the oracle's."""
+    import halign
+    ctx = halign.Context(0)
+    try:
+        ctx.set_kernel_policy(halign.ring_policy(0, slots, lanes, uniform=True))
+        rng = np.random.default_rng(100 * lanes + slots)
+        side = 16 * (lanes * slots - 3)   # the shorter side's limit for n_col <= ring - 2
+        cases = []
+        for key in range(6):
+            ls = int(rng.integers(1, max(2, side)))
+            lt = int(rng.integers(ls, 6 * ls + 50))
+            w = -1 if key % 2 else oracle.auto_band(lt, ls)
+            flag = FLAG_RIGHT if key % 3 else FLAG_LEFT
+            for j in range(int(rng.integers(1, 8))):
+                t = rand_codes(rng, lt, p_n=0.01 if key == 4 else 0.0)
+                a = int(rng.integers(0, lt - ls + 1))
+                q = mutate_codes(rng, t[a:a + ls], 0.05, 0.0)[:ls]
+                if len(q) < ls:
+                    q = np.concatenate([q, rand_codes(rng, ls - len(q))])
+                cases.append((q, t, w, flag) if key < 3 else (t, q, w, flag))
+        tasks, qc, tc = _tasks(cases)
+        print(f"uniform tickets {lanes}x{slots}: {len(cases)} jobs", file=sys.stderr, flush=True)
+        res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=False, plain=False)
+        assert _compare(oracle, cases, res, cigs, False, f"uniform{lanes}x{slots}") == 0
+    finally:
+        ctx.close()
+
+
+def test_dp_uniform_tickets_automatic(oracle):
+    """Automatic policy: whole pairs of short queries against one long center
+    (unbanded, many queries per length, as the anchor-less queries of the
+    uneven workload) go to the uniform-ticket form; results equal the oracle's."""
+    import halign
+    ctx = halign.Context(0)
+    try:
+        rng = np.random.default_rng(77)
+        t = rand_codes(rng, 3000)
+        cases = []
+        for ls in (180, 300, 420):
+            for j in range(16):   # 8 per gap direction: whole tickets
+                a = int(rng.integers(0, len(t) - ls))
+                q = mutate_codes(rng, t[a:a + ls], 0.02, 0.004)
+                q = np.concatenate([q, rand_codes(rng, max(0, ls - len(q)))])[:ls]
+                cases.append((q, t, -1, FLAG_RIGHT if j % 2 else FLAG_LEFT))
+        tasks, qc, tc = _tasks(cases)
+        res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=False, plain=False)
+        assert _compare(oracle, cases, res, cigs, False, "uniform-auto") == 0
+    finally:
+        ctx.close()
