@@ -22,6 +22,7 @@ int dp_num_shapes();
 int dp_shape_ring(int idx);
 int dp_shape_threads(int idx);
 int dp_shape_max_groups(int idx, bool stats, int num_cu);
+size_t dp_shape_slot_bytes(int idx, int qlen, int tlen, int ncol);
 hipError_t dp_launch(int idx, bool stats, int groups, const DpJob *jobs, int njobs, int *counter,
                      uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream);
 hipError_t consensus_launch(const uint8_t *rows, int64_t row_stride, int64_t aln_len, int64_t row_end, int64_t batch,
@@ -89,6 +90,7 @@ constexpr int kDpStreams = 1 + halign_ctx::kAux;   // main + side streams for DP
 
 int pick_shape(int ncol, bool stats, int policy)
 {
+    if (policy == 2 + dp_wave_num_shapes()) return dp_num_shapes() - 1;   // the wide-band kernel only
     if (!stats && policy != 1) {
         const int nw = dp_wave_num_shapes();
         if (policy >= 2 && policy - 2 < nw && ncol <= dp_wave_shape_ring(policy - 2) - 2)
@@ -576,7 +578,9 @@ bool fill_spec_shape(Spec &s, bool full_stats, int policy)
     const int nc = n_col(s.qlen, s.tlen, w);
     s.stats = full_stats || predict_zdrop(s.qlen, s.tlen, w);
     s.shape = pick_shape(nc, s.stats, policy);
-    s.tb_bytes = (size_t)(s.qlen + s.tlen - 1) * (size_t)nc * 8;
+    if (s.shape == dp_num_shapes() - 1) s.stats = true;   // the wide-band kernel computes every field
+    s.tb_bytes = s.shape >= 0 && s.shape < kWaveBase ? dp_shape_slot_bytes(s.shape, s.qlen, s.tlen, nc)
+                                                     : (size_t)(s.qlen + s.tlen - 1) * (size_t)nc * 8;
     return s.shape >= 0;
 }
 
@@ -752,7 +756,7 @@ int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes)
 
 int halign_set_kernel_policy(halign_ctx *ctx, int policy)
 {
-    if (!ctx || policy < 0 || policy >= 2 + dp_wave_num_shapes()) return HALIGN_E_ARG;
+    if (!ctx || policy < 0 || policy > 2 + dp_wave_num_shapes()) return HALIGN_E_ARG;
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->policy = policy;
     return HALIGN_OK;

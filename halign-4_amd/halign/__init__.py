@@ -3,7 +3,7 @@ MI355X. The compute runs in libhalign_hip.so (HIP, gfx950) through its C ABI.
 """
 from ._lib import (ANCHOR_DTYPE, DNA5_MAT, DP_RESULT_DTYPE, DP_TASK_DTYPE, Center, Context, ExtzT,  # noqa: F401
                    HalignError, PairSet, Server, free_extz,
-                   LIB_PATH, consensus_file, lib, msa_auto_center, msa_keep_length, read_fastx, ring_policy,
+                   LIB_PATH, consensus_file, lib, msa_auto_center, msa_keep_length, read_fastx, ring_policy, wide_policy,
                    ring_shapes, topk_longest)
 
 FLAG_RIGHT = 0x06   # KSW_EZ_GENERIC_SC | KSW_EZ_RIGHT (include/align.h:134)

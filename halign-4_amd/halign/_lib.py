@@ -180,6 +180,11 @@ def ring_shapes():
     return [(int(a), int(b), int(c)) for a, b, c in zip(w, g, r)]
 
 
+def wide_policy():
+    """Kernel policy that runs every job on the wide-band kernel."""
+    return 2 + len(ring_shapes())
+
+
 def ring_policy(waves, slots, lanes=None, uniform=False):
     """Kernel policy that forces the ring shape (waves, slots per lane); waves 0
     selects the sub-wave shape with groups of `lanes` lanes (default 16), in its

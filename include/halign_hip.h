@@ -69,8 +69,10 @@ const char *halign_last_error(const halign_ctx *ctx);
 /* library version string */
 const char *halign_version(void);
 /* Kernel family choice: 0 = automatic (ring-form kernel where the band fits,
- * multi-wave otherwise), 1 = multi-wave kernels only, 2 + i = ring shape i
- * wherever the band fits it (automatic otherwise). Results are identical;
+ * multi-wave otherwise, the wide-band kernel beyond the widest multi-wave
+ * ring), 1 = multi-wave kernels only, 2 + i = ring shape i wherever the band
+ * fits it (automatic otherwise), 2 + (number of ring shapes) = the wide-band
+ * kernel for every job. Results are identical;
  * used to test every family and to tune the shape choice. No reference
  * counterpart (the reference has one CPU kernel). */
 int halign_set_kernel_policy(halign_ctx *ctx, int policy);
@@ -100,7 +102,8 @@ int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t ca
  * return with the reset ez (src/align/ksw2_extz2_sse.c:56-57), and ez->cigar
  * grows by realloc as ksw_push_cigar grows it (m_cigar 4 when empty, then
  * doubling; include/ksw2.h:113-123), so one ez can be reused across calls and
- * its cigar is released with free(). */
+ * its cigar is released with free(). Any band width and length: bands wider
+ * than the widest ring kernel run on the wide-band kernel. */
 int halign_ksw_extz2(halign_ctx *ctx, void *km, int qlen, const uint8_t *query, int tlen,
                      const uint8_t *target, int8_t m, const int8_t *mat, int8_t q, int8_t e,
                      int w, int zdrop, int end_bonus, int flag, halign_extz_t *ez);

@@ -20,7 +20,7 @@ FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "
 # auto, multi-wave only, and forced ring shapes (waves, slots per lane) so that
 # small cases also run through the multi-wave LDS exchange of the ring kernel;
 # waves 0: sub-wave shapes (slots, lanes per job), forced where the band fits them
-POLICIES = ["auto", "multiwave", (2, 1), (3, 2), (5, 1), (0, 1, 16), (0, 3, 16), (0, 1, 4), (0, 2, 4)]
+POLICIES = ["auto", "multiwave", "wide", (2, 1), (3, 2), (5, 1), (0, 1, 16), (0, 3, 16), (0, 1, 4), (0, 2, 4)]
 
 
 @pytest.fixture(scope="module", params=POLICIES, ids=lambda p: p if isinstance(p, str) else (f"ring{p[0]}x{p[1]}" if p[0] else f"sub{p[2]}x{p[1]}"))
@@ -28,7 +28,7 @@ def ctx(request):
     import halign
     c = halign.Context(0)
     p = request.param
-    pol = {"auto": 0, "multiwave": 1}[p] if isinstance(p, str) else halign.ring_policy(*p)
+    pol = {"auto": 0, "multiwave": 1, "wide": halign.wide_policy()}[p] if isinstance(p, str) else halign.ring_policy(*p)
     c.set_kernel_policy(pol)
     print(f"kernel policy {p} -> {pol}", file=sys.stderr, flush=True)
     yield c
@@ -328,3 +328,47 @@ def test_dp_uniform_tickets_automatic(oracle):
         assert _compare(oracle, cases, res, cigs, False, "uniform-auto") == 0
     finally:
         ctx.close()
+
+
+def test_dp_wide_bands(oracle):
+    """Bands wider than the widest ring (n_col_ > 1,535 blocks) run on the
+    wide-band kernel, which ksw_extz2_sse allows for any w and length
+    (src/align/ksw2_extz2_sse.c:72-76): an unbanded ~24,650 x 49,400 pair
+    (n_col_ 1,542) both ways round and with N bases; and, forced onto that
+    kernel, banded pairs of several passes per anti-diagonal (n_col_ 564 and
+    1,100) including a band overflow. Every field and CIGAR equals the oracle's."""
+    import halign
+    rng = np.random.default_rng(2600)
+    big_t = rand_codes(rng, 49400)
+    big_q = mutate_codes(rng, big_t[20000:44650], 0.02, 0.004)   # ~24,650: n_col_ 1,542
+    big_qn = big_q.copy()
+    big_qn[rng.integers(0, len(big_qn), 40)] = 4
+    auto_cases = [(big_q, big_t, -1, FLAG_RIGHT), (big_t, big_qn, -1, FLAG_LEFT)]
+    t1 = rand_codes(rng, 9500)
+    q1 = mutate_codes(rng, t1, 0.03, 0.01)
+    t2 = rand_codes(rng, 18000)
+    q2 = mutate_codes(rng, t2, 0.02, 0.004)
+    t3 = rand_codes(rng, 12000)
+    q3 = mutate_codes(rng, t3[:6100], 0.02, 0.004)
+    forced_cases = [(q1, t1, 8999, FLAG_RIGHT), (q2, t2, 17590, FLAG_LEFT), (q3, t3, 2000, FLAG_RIGHT)]
+    class Once:   # the oracle's answer per case, computed once for both modes
+        def __init__(self):
+            self.memo = {}
+
+        def ksw(self, q, t, w, flag):
+            k = (id(q), id(t), w, flag)
+            if k not in self.memo:
+                self.memo[k] = oracle.ksw(q, t, w, flag)
+            return self.memo[k]
+    once = Once()
+    for label, cases, pol in (("wide auto", auto_cases, 0), ("wide forced", forced_cases, halign.wide_policy())):
+        ctx = halign.Context(0)
+        try:
+            ctx.set_kernel_policy(pol)
+            tasks, qc, tc = _tasks(cases)
+            print(label, [(len(c[0]), len(c[1]), c[2]) for c in cases], file=sys.stderr, flush=True)
+            for full in (True, False):
+                res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=full, plain=False)
+                assert _compare(once, cases, res, cigs, full, label) == 0
+        finally:
+            ctx.close()

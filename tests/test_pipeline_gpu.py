@@ -342,3 +342,30 @@ def test_rows_and_insertion_flags(ctx):
             bad.append(i)
     assert not bad, f"{len(bad)} differ, first {bad[:5]}"
     assert 0 < int(hins.sum()) < len(qs)
+
+
+def test_whole_pair_wider_than_every_ring_vs_oracle(ctx):
+    """An anchor-less query of ~24,650 bases against a 49,400-base center:
+    globalAlignMM2 falls back to the whole pair (src/align/align.cpp:204-206)
+    with auto_band -1 (length ratio above 0.5), so its band is n_col_ 1,542
+    blocks, wider than every ring kernel; it runs on the wide-band kernel
+    inside the batch, beside ordinary queries, and equals the oracle's CIGAR."""
+    import halign
+    rng = np.random.default_rng(49400)
+    center = codes_to_ascii(rand_codes(rng, 49400))
+    lone = codes_to_ascii(rand_codes(rng, 24650))
+    cc = np.frombuffer(center, np.uint8)
+    lut = np.zeros(256, np.uint8)
+    lut[np.frombuffer(b"ACGT", np.uint8)] = np.arange(4, dtype=np.uint8)
+    near = [codes_to_ascii(mutate_codes(rng, lut[cc[a:a + 3000]], 0.01, 0.002)) for a in (100, 20000, 46000)]
+    qs = [near[0], lone, near[1], near[2]]
+    O = Oracle()
+    print("wide whole pair: 49,400 center, 24,650 anchor-less query", file=sys.stderr, flush=True)
+    exp = [O.ref_global_align(center, q)[0] for q in qs]
+    c = halign.Center(ctx, center)
+    try:
+        got, st, _ = c.align(qs)
+    finally:
+        c.close()
+    for i in range(len(qs)):
+        assert np.array_equal(got[i], exp[i]), (i, cigar_str(got[i])[:200], cigar_str(exp[i])[:200])
