@@ -2,7 +2,7 @@
 MI355X. The compute runs in libhalign_hip.so (HIP, gfx950) through its C ABI.
 """
 from ._lib import (ANCHOR_DTYPE, DNA5_MAT, DP_RESULT_DTYPE, DP_TASK_DTYPE, Center, Context, ExtzT,  # noqa: F401
-                   HalignError, PairSet, Server, free_extz,
+                   HalignError, PairSet, RefSet, Server, align_sam, free_extz,
                    LIB_PATH, consensus_file, lib, msa_auto_center, msa_keep_length, read_fastx, ring_policy, wide_policy,
                    ring_shapes, topk_longest)
 

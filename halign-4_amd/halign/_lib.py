@@ -100,6 +100,13 @@ def _load():
     lib.halign_consensus.argtypes = [_P, _P, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int64,
                                      C.c_uint64, _P, _P, C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_double)]
+    lib.halign_refset_create.argtypes = [_P, C.c_int32, _P, _P, C.c_char_p, C.c_int32, C.POINTER(_P)]
+    lib.halign_refset_align.argtypes = [_P, C.c_int32, C.c_char_p, _P, C.c_int32, _P, _P, C.c_int64, _P, _P,
+                                        C.c_int64, _P, _P, _P]
+    lib.halign_refset_free.argtypes = [_P]
+    lib.halign_refset_free.restype = None
+    lib.halign_align_sam.argtypes = [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32,
+                                     C.c_int32, C.POINTER(MsaStats)]
     lib.halign_consensus_noalloc.argtypes = [_P, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
                                              C.c_uint64, _P, _P, C.POINTER(C.c_uint64)]
     lib.halign_set_seed_policy.argtypes = [_P, C.c_int]
@@ -489,6 +496,60 @@ class Center:
                                                          _ptr(coff), _ptr(st), C.byref(ms)),
                         "halign_center_align")
         return cig, coff, st[:n], ms.value
+
+
+class RefSet:
+    """RefAligner's reference set (halign_refset_*): reference records and the
+    consensus of their MSA."""
+
+    def __init__(self, ctx, refs, consensus):
+        self.ctx = ctx
+        rb = [r.encode() if isinstance(r, str) else bytes(r) for r in refs]
+        self._keep = [C.create_string_buffer(r, len(r) + 1) for r in rb]
+        ptrs = (C.c_char_p * len(rb))(*[C.cast(b, C.c_char_p) for b in self._keep])
+        lens = np.array([len(r) for r in rb], np.int32)
+        cons = consensus.encode() if isinstance(consensus, str) else bytes(consensus)
+        self.maxlen = max([len(cons)] + [len(r) for r in rb])
+        h = _P()
+        ctx._check(ctx.lib.halign_refset_create(ctx.h, len(rb), C.cast(ptrs, _P), _ptr(lens), cons, len(cons),
+                                                C.byref(h)), "halign_refset_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.halign_refset_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def align(self, queries, keep_length):
+        """(best_ref, first CIGARs, result CIGARs, has_insertion, result reference (-1: consensus))."""
+        blob, off = _blob(queries)
+        n = len(off) - 1
+        cap = int(off[-1]) + n * (self.maxlen + 1) + 1
+        ic, fc = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+        io, fo = np.zeros(n + 1, np.int64), np.zeros(n + 1, np.int64)
+        best, fr = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        hi = np.zeros(max(n, 1), np.uint8)
+        self.ctx._check(self.ctx.lib.halign_refset_align(self.h, n, blob, _ptr(off), int(keep_length), _ptr(best),
+                                                         _ptr(ic), cap, _ptr(io), _ptr(fc), cap, _ptr(fo), _ptr(hi),
+                                                         _ptr(fr)), "halign_refset_align")
+        return (best[:n].copy(), [ic[io[i]:io[i + 1]].copy() for i in range(n)],
+                [fc[fo[i]:fo[i + 1]].copy() for i in range(n)], hi[:n].astype(bool), fr[:n].copy())
+
+
+def align_sam(ctx, center_path, input_path, sam_path, insertion_sam_path, keep_length=True, batch=0,
+              aligned_path=None):
+    """alignQueryToRef's two SAM files with one thread (halign_align_sam)."""
+    st = MsaStats()
+    ctx._check(ctx.lib.halign_align_sam(ctx.h, center_path.encode(), aligned_path.encode() if aligned_path else None,
+                                        input_path.encode(), sam_path.encode(), insertion_sam_path.encode(),
+                                        int(keep_length), int(batch), C.byref(st)), "halign_align_sam")
+    return {f: getattr(st, f) for f, _ in MsaStats._fields_}
 
 
 class PairSet:

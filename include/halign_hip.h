@@ -281,6 +281,47 @@ int halign_msa_keep_length_refs(halign_ctx *ctx, const char *center_path, const 
                                 const char *input_path, const char *output_path, int32_t batch,
                                 halign_msa_stats *stats);
 
+/* RefAligner over its whole reference set (src/align/ref_aligner.cpp:20-73,
+ * alignOneQueryToRef :212-283): nref reference records (refs[r], ref_lens[r]
+ * bytes, as read from the center file) and the consensus of the MSA command's
+ * output for them (the sequence RefAligner names "consensus"). One reference:
+ * queries are aligned to the consensus, seeded with the record's minimizers.
+ * Several: each query goes to the first reference with the largest sketch
+ * Jaccard (k 15, 2,000 hashes, seed 42), aligned to and seeded from it; that
+ * is the result with keep_length, otherwise the query is aligned again to the
+ * consensus seeded with the consensus's own minimizers (consensus_minimizer,
+ * :71-72) and that second alignment is the result (:268-282). */
+typedef struct halign_refset halign_refset;
+int halign_refset_create(halign_ctx *ctx, int32_t nref, const char *const *refs, const int32_t *ref_lens,
+                         const char *consensus, int32_t cons_len, halign_refset **out);
+/* For n queries (queries[offsets[i] .. offsets[i+1])): best_ref[i] (optional)
+ * the reference chosen (0 with one reference), the first alignment's CIGAR at
+ * init_cigar[init_off[i] .. init_off[i+1]) (to that reference, or to the
+ * consensus with one reference), the result's CIGAR at final_cigar[final_off
+ * [i] .. final_off[i+1]), has_ins[i] (optional) = cigar::hasInsertion of the
+ * result (the SAM file it goes to), final_ref[i] (optional) = the reference
+ * the result is against, -1 for the consensus. Offsets have n+1 entries. */
+int halign_refset_align(halign_refset *rs, int32_t n, const char *queries, const int64_t *offsets,
+                        int32_t keep_length, int32_t *best_ref, uint32_t *init_cigar, int64_t init_cap,
+                        int64_t *init_off, uint32_t *final_cigar, int64_t final_cap, int64_t *final_off,
+                        uint8_t *has_ins, int32_t *final_ref);
+void halign_refset_free(halign_refset *rs);
+/* RefAligner::alignQueryToRef with one thread (src/align/ref_aligner.cpp:
+ * 286-394): the SAM files of thread 0, sam_path (queries whose result has no
+ * insertion) and insertion_sam_path (the others), each with the header line
+ * "@HD VN:1.6 SO:unknown" and one line per query in input order as
+ * writeSamRecord writes it (:128-136, cigar::cigarToString src/align/
+ * cigar.cpp:82-97, seq_io::makeSamRecord / SeqWriter::writeSam src/utils/
+ * seq_io.cpp:289-360): qname, 0, the reference name ("consensus" or the
+ * chosen record's name), 1, 60, the CIGAR, *, 0, 0, the query as read, *.
+ * The consensus is that of aligned_path, the MSA command's output for the
+ * center file (NULL: the copy command, the center file itself). stats
+ * (optional): records, insertion records, bytes written, seconds, seconds in
+ * the GPU calls. */
+int halign_align_sam(halign_ctx *ctx, const char *center_path, const char *aligned_path, const char *input_path,
+                     const char *sam_path, const char *insertion_sam_path, int32_t keep_length, int32_t batch,
+                     halign_msa_stats *stats);
+
 /* preprocessInputFasta's consensus input (src/utils/preprocess.cpp:104-160
  * with TopKLongestSelector, src/consensus/selector.cpp:48-215): the k longest
  * records of input_path after seq_io::cleanSequence, longest first and the

@@ -30,6 +30,9 @@ Outputs (np.savez_compressed, no pickles):
                        (copy command, or a given aligned file): final FASTA bytes
   auto_center.npz      preprocessInputFasta's Top-K file, generateConsensusSequence's
                        FASTA and JSON files, and halign4 --keep-length without -c
+  sam_cases.npz        halign4 -t 1 --save-workdir: the thread-0 SAM files (plain and
+                       insertion) of one and several references, with and without
+                       --keep-length (the second alignment to the consensus)
   config3.npz, config4.npz, config5.npz
                        samples of the uneven (3), divergent (4) and serving (5)
                        workloads through the reference program itself
@@ -797,9 +800,101 @@ def config_workload(H, cfg, rng):
          fasta_sha256=digest, fasta_bytes=np.array([os.path.getsize(opath)], np.int64))
 
 
+def sam_cases(H, rng):
+    """The per-thread SAM files of RefAligner::alignQueryToRef with one thread
+    (src/align/ref_aligner.cpp:286-394, routing :212-283), written by the
+    reference program itself (halign4 -t 1 --save-workdir; result/thread0.sam
+    and result/thread0_insertion.sam). Cases: one center with and without
+    --keep-length (--cons-n 1 so RefAligner runs); three equal-length
+    references (copy command) with and without --keep-length; four references
+    with missing columns and their aligned file, with and without
+    --keep-length. Without --keep-length and several references every query is
+    aligned again to the consensus (:268-282); the queries include copies of
+    full-width references, whose first alignment (to a reference that lacks
+    some columns) has an insertion while the second (to the full-width
+    consensus) may not. Queries also cover an '@' name, an empty and a short
+    query, lower case and N. A run whose merge fails after the SAM files are
+    written (the copy command leaves insertion records of several lengths)
+    still counts: its rc is stored. Each reference set's inputs run with
+    --keep-length (case 2k) and without (case 2k+1)."""
+    d = tempfile.mkdtemp()
+    out = {k: [] for k in ("center", "aligned", "input", "sam", "sam_ins")}
+    par = []
+    lut = np.zeros(256, np.uint8)
+    lut[ACGT] = np.arange(4, dtype=np.uint8)
+    for kind in range(3):          # 0: one center, 1: three references, 2: four with missing columns
+        W = [2500, 3000, 2800][kind]
+        base = ACGT[rng.integers(0, 4, W)]
+        nref = [1, 3, 4][kind]
+        refs, rows = [], []
+        for r in range(nref):
+            b = base.copy()
+            if r:
+                b[rng.integers(0, W, 30 * r)] = ACGT[rng.integers(0, 4, 30 * r)]
+            present = np.ones(W, bool)
+            if kind == 2 and r % 2 == 1:   # odd references lack some columns
+                for _ in range(3):
+                    a = int(rng.integers(0, W - 80))
+                    present[a:a + int(rng.integers(5, 60))] = False
+            refs.append(b[present].tobytes())
+            rows.append(np.where(present, b, ord("-")).astype(np.uint8).tobytes())
+        names = [b"ref%d" % r for r in range(nref)]
+        cfa = b"".join(b">" + n + b" r\n" + x + b"\n" for n, x in zip(names, refs))
+        afa = b"".join(b">" + n + b"\n" + x + b"\n" for n, x in zip(names, rows)) if kind == 2 else None
+        recs = []
+        for i in range(40):
+            src = refs[i % nref]
+            if kind == 2 and i % 5 == 0:
+                # full width with a short reference's substitutions: its Jaccard picks that
+                # reference (an insertion where it lacks columns); the consensus has them
+                row = np.frombuffer(rows[1 + 2 * ((i // 5) % 2)], np.uint8)
+                src = np.where(row == ord("-"), base, row).astype(np.uint8).tobytes()
+            q = mutate_codes(rng, lut[np.frombuffer(src, np.uint8)], 0.01, 0.0 if i % 3 == 0 else 0.004)
+            if i % 7 == 6:
+                a = int(rng.integers(0, len(q) // 2))
+                q = q[a:a + int(rng.integers(300, len(q) // 2))]
+            qs = codes_to_ascii(q)
+            if i % 11 == 4:
+                qs = qs.lower()
+            if i % 13 == 5:
+                b = bytearray(qs)
+                b[10:14] = b"NNNN"
+                qs = bytes(b)
+            name = b"@q%d" % i if i == 9 else b"q%d" % i
+            recs.append((name, qs))
+        recs.append((b"empty", b""))
+        recs.append((b"short", b"ACGTACGTAC"))
+        text = b"".join(b">" + n + b"\n" + q + b"\n" for n, q in recs)
+        for keep in (True, False):
+            case = 2 * kind + (0 if keep else 1)
+            cp, ap, ip, op = (os.path.join(d, x + "%d" % case) for x in ("c.fa", "a.fa", "q.fa", "o.fa"))
+            open(cp, "wb").write(cfa)
+            open(ip, "wb").write(text)
+            if afa is not None:
+                open(ap, "wb").write(afa)
+            wd = os.path.join(d, "wd%d" % case)
+            rc = H.run(ip, op, wd, center_path=cp, keep_length=keep, cons_n=None if keep else 1,
+                       msa_cmd=msa_command(ap) if afa is not None else Halign4.COPY)
+            res = os.path.join(wd, "result")
+            sam, sam_ins = (open(os.path.join(res, f), "rb").read() for f in ("thread0.sam", "thread0_insertion.sam"))
+            assert sam.startswith(b"@HD") and sam_ins.startswith(b"@HD"), (case, rc, H.last_log[-1500:])
+            out["center"].append(np.frombuffer(cfa, np.uint8))
+            out["aligned"].append(np.frombuffer(afa or b"", np.uint8))
+            out["input"].append(np.frombuffer(text, np.uint8))
+            out["sam"].append(np.frombuffer(sam, np.uint8))
+            out["sam_ins"].append(np.frombuffer(sam_ins, np.uint8))
+            par.append((int(keep), int(afa is not None), nref, rc))
+            nl = (sam.count(b"\n") - 1, sam_ins.count(b"\n") - 1)
+            print(f"    sam case {case}: nref {nref} keep {keep} rc {rc} lines {nl[0]}+{nl[1]}")
+    arrays = {}
+    for k, v in out.items():
+        arrays[k], arrays[k + "_off"] = concat(v, np.uint8)
+    save("sam_cases.npz", par=np.array(par, np.int32), **arrays)
+
+
 FIXTURES = ["ksw_cases", "global_align", "minimizers", "anchors_chain", "ref_global", "consensus",
             "whole_pairs", "fastx_cases", "msa_keep_length", "mash_cases", "msa_refs", "auto_center",
-            "config3", "config4", "config5"]
+            "config3", "config4", "config5", "sam_cases"]
 
 
 def main(which):
@@ -823,6 +918,7 @@ def main(which):
         "config3": lambda: config_workload(H, 3, np.random.default_rng(0x60C3)),
         "config4": lambda: config_workload(H, 4, np.random.default_rng(0x60C4)),
         "config5": lambda: config_workload(H, 5, np.random.default_rng(0x60C5)),
+        "sam_cases": lambda: sam_cases(H, np.random.default_rng(0x6028)),
     }
     for name in which or FIXTURES:
         run[name]()
