@@ -201,13 +201,15 @@ def main():
         sys.exit(2)
 
     if args.copy_calibration:
-        import torch
+        import halign
+        halign.lib()
         n = args.copy_calibration << 20
-        a = torch.ones(n, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
+        a, b = DeviceBuffer(n), DeviceBuffer(n)
+        a.upload(np.ones(n, np.uint8))
         for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
+            b.copy_from(a)
+        a.free()
+        b.free()
         print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
         return
 
@@ -611,26 +613,53 @@ def serve_bench(args):
     print(json.dumps(out), flush=True)
 
 
+class DeviceBuffer:
+    """Device memory from the HIP runtime libhalign_hip.so links (loaded with
+    it): plain hipMalloc / hipMemcpy through ctypes."""
+
+    def __init__(self, nbytes):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.ptr = C.c_void_p()
+        self.nbytes = nbytes
+        if self.hip.hipMalloc(C.byref(self.ptr), nbytes) != 0:
+            raise MemoryError(f"hipMalloc({nbytes})")
+
+    def upload(self, host, offset=0):
+        host = np.ascontiguousarray(host)
+        assert self.hip.hipMemcpy(self.ptr.value + offset, host.ctypes.data_as(self.C.c_void_p), host.nbytes, 1) == 0
+
+    def copy_from(self, other):
+        assert self.hip.hipMemcpy(self.ptr, other.ptr, min(self.nbytes, other.nbytes), 3) == 0   # device to device
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = self.C.c_void_p()
+
+
 def consensus_bench(args):
     """Column counting + consensus over rows x 29,903 aligned columns already
     resident in HBM (stride rounded to 4 bytes). Algorithmic bytes: rows x
     columns read + 28 x columns written (7 uint32 counts)."""
-    import torch
     import halign
     aln = 29903
     stride = (aln + 3) & ~3
-    alpha = torch.tensor(list(b"ACGT-N"), dtype=torch.uint8, device="cuda")
-    g = torch.Generator(device="cuda")
-    g.manual_seed(0xC5)
-    rows = torch.empty((args.rows, stride), dtype=torch.uint8, device="cuda")
+    alpha = np.frombuffer(b"ACGT-N", np.uint8)
+    halign.lib()
+    rows = DeviceBuffer(args.rows * stride)
+    rng = np.random.default_rng(0xC5)
     for r0 in range(0, args.rows, 8192):
         r1 = min(args.rows, r0 + 8192)
-        rows[r0:r1] = alpha[torch.randint(0, len(alpha), (r1 - r0, stride), device="cuda", generator=g)]
-    torch.cuda.synchronize()
+        rows.upload(alpha[rng.integers(0, len(alpha), (r1 - r0, stride), dtype=np.uint8)], r0 * stride)
     ctx = halign.Context(0)
     ms_all = []
     for s in range(args.warmup + args.steps):
-        counts, cons, n, ms = ctx.consensus((args.rows, aln), device_ptr=rows.data_ptr(), row_stride=stride)
+        counts, cons, n, ms = ctx.consensus((args.rows, aln), device_ptr=rows.ptr.value, row_stride=stride)
         if s >= args.warmup:
             ms_all.append(ms)
             log(f"step {s - args.warmup}: {args.rows} rows, kernel {ms:.3f} ms")
@@ -640,7 +669,7 @@ def consensus_bench(args):
     out = {"metric": "consensus column counting GB/s (rows in HBM)", "value": round(algo / t / 1e9, 1),
            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(t * 1e3, 3), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "u8", "data": "synthetic (torch seeded, ACGT-N)",
+           "vs_baseline": None, "dtype": "u8", "data": "synthetic (numpy seeded, ACGT-N)",
            "config": {"workload": f"consensus counting, {args.rows} rows x {aln} columns"},
            "roofline": {"bound": "hbm", "achieved": round(algo / t / 1e9, 1), "peak": 8000.0,
                         "unit": "GB/s", "frac": round(algo / t / 8e12, 4), "traffic": None}}

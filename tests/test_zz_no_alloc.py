@@ -12,11 +12,34 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _device_rows(torch, rng, nrows, aln):
+class DeviceRows:
+    """Rows in device memory from the HIP runtime the library itself links
+    (libamdhip64.so.7, already loaded with it), so no second runtime enters
+    the process."""
+
+    def __init__(self, host):
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.ptr = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.ptr), host.nbytes) == 0
+        assert self.hip.hipMemcpy(self.ptr, host.ctypes.data_as(C.c_void_p), host.nbytes, 1) == 0   # host to device
+
+    def data_ptr(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def _device_rows(rng, nrows, aln):
     stride = (aln + 3) & ~3
     alpha = np.frombuffer(b"ACGTUN-.acgtRY", np.uint8)
-    host = alpha[rng.integers(0, len(alpha), (nrows, stride))]
-    return torch.from_numpy(host).to("cuda"), stride
+    host = np.ascontiguousarray(alpha[rng.integers(0, len(alpha), (nrows, stride))])
+    return DeviceRows(host), stride
 
 
 def _noalloc(lib, ctx, rows, nrows, aln, stride, batch, lim):
@@ -30,17 +53,17 @@ def _noalloc(lib, ctx, rows, nrows, aln, stride, batch, lim):
 
 
 def test_noalloc_matches_consensus_twice():
-    import torch
     import halign
     lib = halign.lib()
     ctx = halign.Context(0)
     rng = np.random.default_rng(77)
+    held = []
     try:
         for nrows, aln, batch, lim in ((9000, 5001, 4096, 0), (700, 3003, 64, 500)):
             print(f"no-alloc consensus: {nrows} rows x {aln} columns, batch {batch}, limit {lim}",
                   file=sys.stderr, flush=True)
-            rows, stride = _device_rows(torch, rng, nrows, aln)
-            torch.cuda.synchronize()
+            rows, stride = _device_rows(rng, nrows, aln)
+            held.append(rows)
             exp_counts, exp_cons, exp_n, _ = ctx.consensus((nrows, aln), batch=batch, seq_limit=lim,
                                                            device_ptr=rows.data_ptr(), row_stride=stride)
             rc, counts, cons, n = _noalloc(lib, ctx, rows, nrows, aln, stride, batch, lim)
@@ -54,3 +77,5 @@ def test_noalloc_matches_consensus_twice():
         assert rc == -5
     finally:
         ctx.close()
+        for r in held:
+            r.free()
