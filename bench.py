@@ -183,10 +183,12 @@ def main():
     ap.add_argument("--config", type=int, default=1, help="e2e workload configuration (1, 3, 4)")
     ap.add_argument("--queries", type=int, default=0, help="e2e: queries per step (0 = the config default)")
     ap.add_argument("--rows", type=int, default=100000, help="consensus workload: rows of 29,903 columns")
-    ap.add_argument("--requests", type=int, default=48, help="serve: requests per load point")
+    ap.add_argument("--requests", type=int, default=0, help="serve: requests per load point (0 = --serve-queries)")
+    ap.add_argument("--serve-queries", type=int, default=2000000, help="serve: queries per load point")
+    ap.add_argument("--pool", type=int, default=262144, help="serve: distinct queries the requests draw on")
     ap.add_argument("--workers", type=int, default=2, help="serve: concurrent batches")
     ap.add_argument("--batch-queries", type=int, default=8192, help="serve: queries per batch at most")
-    ap.add_argument("--loads", type=float, nargs="*", default=[0.5, 0.8],
+    ap.add_argument("--loads", type=float, nargs="*", default=[0.5, 0.65, 0.8, 0.95],
                     help="serve: offered load as fractions of the measured capacity")
     ap.add_argument("--refs", type=int, default=1,
                     help="msa: reference sequences in the center file (>1: several references, chosen by Jaccard)")
@@ -519,25 +521,37 @@ def cpu_timed(fn, n, threads, fixed=0, target_s=12.0):
 def serve_bench(args):
     """Config 5 serving: a continuous-batching server (halign_server_*) with the
     29,903 bp center and its index resident, requests of n ~ U{1..4096} queries
-    (lengths U{1..30000}, 1% divergence). First all requests at once (closed
-    loop: the capacity in seqs/s), then Poisson arrivals at fractions of that
-    capacity; latency = submit -> last CIGAR, measured in the library."""
+    (lengths U{1..30000}, 1% divergence), --serve-queries queries in all
+    (2,000,000 by default). The queries of request i are a contiguous range of
+    a seeded pool of --pool distinct queries. First all requests at once
+    (closed loop: the capacity in seqs/s), then Poisson arrivals at fractions
+    of that capacity; latency = submit -> last CIGAR, measured in the
+    library. The server's footprint estimate of its largest batch and the
+    device memory it was seen to use are reported beside the budget."""
     import threading
     import halign
     from halign import synth
     center = synth.center(5)
     rng = np.random.default_rng(0xC5A)
-    R = args.requests
-    sizes = rng.integers(1, 4097, R)
+    sizes = []
+    total = 0
+    while total < args.serve_queries and (args.requests == 0 or len(sizes) < args.requests):
+        n = int(rng.integers(1, 4097))
+        sizes.append(n)
+        total += n
+    sizes = np.array(sizes, np.int64)
+    R = len(sizes)
     need = int(sizes.sum())
-    qb, off = synth.queries(5, 0, need)
-    bounds = np.concatenate([[0], np.cumsum(sizes)])
-    log(f"serve: {R} requests, {need} queries, {off[-1] / 1e6:.1f} Mbases")
+    P = max(8192, min(args.pool, need))
+    qb, off = synth.queries(5, 0, P)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]) % (P - 4096)   # request i: pool[starts[i] : +sizes[i]]
+    bases = int(sum(off[s + n] - off[s] for s, n in zip(starts, sizes)))
+    log(f"serve: {R} requests, {need} queries ({P} distinct), {bases / 1e9:.2f} Gbases per pass")
     srv = halign.Server(center, workers=args.workers, max_batch_queries=args.batch_queries)
 
     kept = []
 
-    def run(arrivals, label):
+    def run(arrivals, label, R=R):
         kept.clear()
         ids = [None] * R
         lat = np.zeros(R)
@@ -551,9 +565,9 @@ def serve_bench(args):
                 cig, coff, _, timing = srv.result(ids[i])
                 lat[i] = timing[1]
                 done_t[i] = time.perf_counter() - t0
-                if i % 8 == 0:   # one query of every eighth request is checked after the run
-                    j = int(bounds[i + 1] - bounds[i]) // 2
-                    kept.append((int(bounds[i]) + j, cig[coff[j]:coff[j + 1]].copy()))
+                if i % 16 == 0:   # one query of every sixteenth request is checked after the run
+                    j = int(sizes[i]) // 2
+                    kept.append((int(starts[i]) + j, cig[coff[j]:coff[j + 1]].copy()))
         th = threading.Thread(target=collect)
         th.start()
         for i in range(R):
@@ -561,14 +575,15 @@ def serve_bench(args):
                 dt = arrivals[i] - (time.perf_counter() - t0)
                 if dt > 0:
                     time.sleep(dt)
-            ids[i] = srv.submit_flat(qb, off[bounds[i]:bounds[i + 1] + 1])
+            ids[i] = srv.submit_flat(qb, off[starts[i]:starts[i] + sizes[i] + 1])
         th.join()
         span = float(done_t.max())
-        log(f"{label}: {need / span:.0f} seqs/s, p50 {np.percentile(lat, 50):.1f} ms, "
+        nq = int(sizes[:R].sum())
+        log(f"{label}: {nq / span:.0f} seqs/s, p50 {np.percentile(lat, 50):.1f} ms, "
             f"p99 {np.percentile(lat, 99):.1f} ms, span {span:.2f} s")
         return lat, span
     try:
-        run(None, "warmup")
+        run(None, "warmup", R=min(R, 64))
         _, span = run(None, "capacity (all at once)")
         cap = need / span
         points = []
@@ -580,10 +595,16 @@ def serve_bench(args):
             points.append({"load": f, "offered_req_per_s": round(lam, 3), "seqs_per_s": round(need / sp, 1),
                            "p50_ms": round(float(np.percentile(lat, 50)), 1),
                            "p99_ms": round(float(np.percentile(lat, 99)), 1)})
+        batches, est, used = srv.stats()
+        free_b, _ = hip_mem_info()
+        memory = {"batches": batches, "largest_batch_estimate_bytes": est, "device_used_bytes": used,
+                  "device_free_bytes_after": free_b}
+        log(f"serve memory: {batches} batches, largest batch estimate {est / 2**30:.2f} GiB, "
+            f"device in use {used / 2**30:.2f} GiB")
     finally:
         srv.close()
     verified = None if args.no_verify else verify_queries(
-        center, qb, off, list(kept), "served queries (the middle one of every eighth request at the last load)")
+        center, qb, off, list(kept), "served queries (the middle one of every sixteenth request at the last load)")
     cpu = None
     if not args.no_cpu:
         from oracle_lib import REF_SO, Oracle, RefLib
@@ -607,10 +628,22 @@ def serve_bench(args):
            "dtype": "int8", "data": "synthetic (seeded, csrc/synth.c config 5)",
            "config": {"workload": "config 5: serving, center 29903 bp resident, requests of U{1..4096} queries "
                                   "of U{1..30000} bp at 1% divergence", "requests": R, "queries": need,
-                      "workers": args.workers, "max_batch_queries": args.batch_queries},
-           "p50_ms": hi.get("p50_ms"), "p99_ms": hi.get("p99_ms"), "load_points": points, "verified": verified,
+                      "distinct_queries": P, "workers": args.workers, "max_batch_queries": args.batch_queries},
+           "p50_ms": hi.get("p50_ms"), "p99_ms": hi.get("p99_ms"), "load_points": points, "memory": memory,
+           "verified": verified,
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def hip_mem_info():
+    """(free, total) device bytes from the HIP runtime libhalign_hip.so links."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    if hip.hipMemGetInfo(C.byref(f), C.byref(t)) != 0:
+        return None, None
+    return f.value, t.value
 
 
 class DeviceBuffer:

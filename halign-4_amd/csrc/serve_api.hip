@@ -2,15 +2,21 @@
 // (RefAligner::globalAlign with one resident center, src/align/ref_aligner.cpp:
 // 91-125), the C ABI halign_server_*.
 //
-// Each worker owns a context (its own stream and buffers) and a copy of the
-// center with its minimizer index resident in HBM. Requests (groups of
-// queries) wait in one FIFO; a free worker takes the requests at the head of
-// it, as many as fit its batch limits (query count and an HBM footprint
-// estimate), runs them as one batch and hands each request its own results.
-// With two or more workers, batches formed while another is on the GPU run
-// beside it. A query's result does not depend on which batch carries it.
+// Each worker owns a context (its own streams, buffers and traceback pool) and
+// a copy of the center with its minimizer index resident in HBM. A request
+// (a group of queries) is cut into chunks of at most chunk_queries queries. A
+// worker that becomes free forms its next batch from the waiting chunks,
+// taking one chunk from each waiting request in arrival order and going round
+// again until the batch is full (max_batch_queries, or the batch's HBM
+// footprint estimate reaches its share of the budget): a request that arrives
+// while others are on the GPU is admitted at the next batch any worker forms,
+// and a small request is not queued behind all of a large one. A request is
+// done when its last chunk is. A query's result does not depend on which
+// batch carries it. The worker measures its device memory in use after each
+// batch (halign_server_stats) next to the footprint estimate of that batch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -33,15 +39,23 @@ struct Request {
     int32_t n = 0;
     std::vector<char> seqs;
     std::vector<int64_t> off;   // n + 1 offsets into seqs
-    int64_t est_bytes = 0;
+    std::vector<int64_t> est;   // footprint estimate per query
     Clock::time_point t_submit, t_start, t_done;
-    bool done = false;
+    bool started = false, done = false;
     int status = HALIGN_OK;
     std::string err;
-    std::vector<uint32_t> cigar;
+    int32_t next = 0;           // first query not yet handed to a worker
+    int32_t chunks_left = 0;    // chunks handed out or waiting, not finished
+    std::vector<std::vector<uint32_t>> qcig;   // CIGAR per query
+    std::vector<int32_t> stats;                // 4 per query
+    int32_t batch_queries = 0;  // largest batch that carried a chunk of it
+    std::vector<uint32_t> cigar;   // packed at completion
     std::vector<int64_t> coff;
-    std::vector<int32_t> stats;
-    int32_t batch_queries = 0;
+};
+
+struct Chunk {
+    std::shared_ptr<Request> r;
+    int32_t a = 0, b = 0;   // queries [a, b) of the request
 };
 
 }  // namespace
@@ -49,7 +63,10 @@ struct Request {
 struct halign_server {
     int32_t rlen = 0;
     halign_server_config cfg{};
+    int32_t chunk_queries = 256;
     int64_t batch_budget = 0;   // HBM footprint estimate allowed per batch
+    int device = 0;
+    size_t free_at_start = 0;   // free device memory before the workers' contexts
     struct Worker {
         halign_ctx *ctx = nullptr;
         halign_center *center = nullptr;
@@ -58,11 +75,14 @@ struct halign_server {
     std::vector<Worker> workers;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<std::shared_ptr<Request>> pending;
+    std::deque<std::shared_ptr<Request>> pending;   // requests with queries not yet handed out
     std::unordered_map<int64_t, std::shared_ptr<Request>> live;
     int64_t next_id = 1;
     bool stop = false;
     std::string err;
+    // measurements: batches run, the largest footprint estimate of a batch,
+    // and the most device memory in use seen after a batch
+    int64_t batches = 0, max_est = 0, max_used = 0;
 };
 
 namespace {
@@ -74,41 +94,72 @@ namespace {
 // context's own pool).
 int64_t query_footprint(int64_t L, int64_t R) { return 4096 + 200 * L + 12 * (R + L); }
 
+// the next batch: one chunk from each waiting request in arrival order, round
+// and round, until the batch is full (s->mu held)
+std::vector<Chunk> form_batch(halign_server *s)
+{
+    std::vector<Chunk> batch;
+    int64_t q = 0, bytes = 0;
+    bool progress = true;
+    while (progress && !s->pending.empty()) {
+        progress = false;
+        for (auto it = s->pending.begin(); it != s->pending.end();) {
+            auto &r = *it;
+            const int32_t b = std::min<int32_t>(r->n, r->next + s->chunk_queries);
+            int64_t e = 0;
+            for (int32_t k = r->next; k < b; ++k) e += r->est[(size_t)k];
+            if (!batch.empty() && (q + (b - r->next) > s->cfg.max_batch_queries || bytes + e > s->batch_budget))
+                return batch;
+            batch.push_back({r, r->next, b});
+            q += b - r->next;
+            bytes += e;
+            r->next = b;
+            progress = true;
+            if (r->next >= r->n) it = s->pending.erase(it);
+            else ++it;
+        }
+    }
+    return batch;
+}
+
 void worker_loop(halign_server *s, int wi)
 {
     halign_server::Worker &w = s->workers[wi];
+    (void)hipSetDevice(s->device);   // hipMemGetInfo below reads the thread's current device
     for (;;) {
-        std::vector<std::shared_ptr<Request>> batch;
+        std::vector<Chunk> batch;
+        int64_t est = 0;
+        bool more = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
             if (s->pending.empty()) return;   // stop requested and nothing left
-            int64_t q = 0, bytes = 0;
-            while (!s->pending.empty()) {
-                const auto &r = s->pending.front();
-                if (!batch.empty() &&
-                    (q + r->n > s->cfg.max_batch_queries || bytes + r->est_bytes > s->batch_budget))
-                    break;
-                q += r->n;
-                bytes += r->est_bytes;
-                batch.push_back(r);
-                s->pending.pop_front();
+            batch = form_batch(s);
+            const auto now = Clock::now();
+            for (auto &c : batch) {
+                if (!c.r->started) {
+                    c.r->started = true;
+                    c.r->t_start = now;
+                }
+                for (int32_t k = c.a; k < c.b; ++k) est += c.r->est[(size_t)k];
             }
+            more = !s->pending.empty();
         }
-        const auto t_start = Clock::now();
+        if (more) s->cv_work.notify_one();   // more work for another free worker
         int64_t n = 0, bases = 0;
-        for (auto &r : batch) {
-            n += r->n;
-            bases += r->off[r->n];
+        for (auto &c : batch) {
+            n += c.b - c.a;
+            bases += c.r->off[c.b] - c.r->off[c.a];
         }
-        std::vector<char> blob((size_t)bases);
+        std::vector<char> blob((size_t)std::max<int64_t>(bases, 1));
         std::vector<int64_t> off((size_t)n + 1);
         {
             int64_t qi = 0, b0 = 0;
-            for (auto &r : batch) {
-                if (r->off[r->n]) memcpy(blob.data() + b0, r->seqs.data(), (size_t)r->off[r->n]);
-                for (int32_t k = 0; k < r->n; ++k) off[qi++] = b0 + r->off[k];
-                b0 += r->off[r->n];
+            for (auto &c : batch) {
+                const int64_t base = c.r->off[c.a], len = c.r->off[c.b] - base;
+                if (len) memcpy(blob.data() + b0, c.r->seqs.data() + base, (size_t)len);
+                for (int32_t k = c.a; k < c.b; ++k) off[qi++] = b0 + c.r->off[k] - base;
+                b0 += len;
             }
             off[n] = b0;
         }
@@ -122,28 +173,50 @@ void worker_loop(halign_server *s, int wi)
                                      stats.data(), nullptr);
         if (n == 0) coff[0] = 0;
         const std::string e = rc == HALIGN_OK ? std::string() : std::string(halign_last_error(w.ctx));
+        size_t free_now = 0, total = 0;
+        (void)hipMemGetInfo(&free_now, &total);
+        std::vector<std::shared_ptr<Request>> finished;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             const auto t_done = Clock::now();
+            s->batches += 1;
+            s->max_est = std::max(s->max_est, est);
+            if (s->free_at_start > free_now) s->max_used = std::max(s->max_used, (int64_t)(s->free_at_start - free_now));
             int64_t qi = 0;
-            for (auto &r : batch) {
-                r->t_start = t_start;
-                r->t_done = t_done;
-                r->status = rc;
-                r->err = e;
-                r->batch_queries = (int32_t)n;
-                if (rc == HALIGN_OK) {
-                    const int64_t c0 = coff[qi], c1 = coff[qi + r->n];
-                    r->cigar.assign(cig.get() + c0, cig.get() + c1);
-                    r->coff.resize((size_t)r->n + 1);
-                    for (int32_t k = 0; k <= r->n; ++k) r->coff[k] = coff[qi + k] - c0;
-                    r->stats.assign(stats.begin() + 4 * qi, stats.begin() + 4 * (qi + r->n));
+            for (auto &c : batch) {
+                Request &r = *c.r;
+                if (rc != HALIGN_OK && r.status == HALIGN_OK) {
+                    r.status = rc;
+                    r.err = e;
                 }
-                qi += r->n;
-                r->done = true;
+                r.batch_queries = std::max<int32_t>(r.batch_queries, (int32_t)n);
+                for (int32_t k = c.a; k < c.b; ++k, ++qi) {
+                    if (rc != HALIGN_OK) continue;
+                    r.qcig[(size_t)k].assign(cig.get() + coff[qi], cig.get() + coff[qi + 1]);
+                    for (int t = 0; t < 4; ++t) r.stats[(size_t)k * 4 + t] = stats[(size_t)qi * 4 + t];
+                }
+                if (--r.chunks_left == 0) {
+                    r.t_done = t_done;
+                    finished.push_back(c.r);
+                }
+            }
+            for (auto &rp : finished) {
+                Request &r = *rp;
+                if (r.status == HALIGN_OK) {
+                    size_t tot = 0;
+                    for (auto &v : r.qcig) tot += v.size();
+                    r.cigar.reserve(tot);
+                    r.coff.assign((size_t)r.n + 1, 0);
+                    for (int32_t k = 0; k < r.n; ++k) {
+                        r.cigar.insert(r.cigar.end(), r.qcig[(size_t)k].begin(), r.qcig[(size_t)k].end());
+                        r.coff[(size_t)k + 1] = (int64_t)r.cigar.size();
+                    }
+                }
+                std::vector<std::vector<uint32_t>>().swap(r.qcig);
+                r.done = true;
             }
         }
-        s->cv_done.notify_all();
+        if (!finished.empty()) s->cv_done.notify_all();
     }
 }
 
@@ -172,19 +245,24 @@ int halign_server_create(int device, const char *center, int32_t len, const hali
 {
     if (!center || len <= 0 || !out) return HALIGN_E_ARG;
     *out = nullptr;
-    halign_server_config c{2, 4096, 0};
+    halign_server_config c{2, 4096, 0, 256, 0};
     if (cfg) c = *cfg;
-    if (c.workers <= 0 || c.workers > 8 || c.max_batch_queries <= 0 || c.hbm_budget < 0) return HALIGN_E_ARG;
+    if (c.chunk_queries == 0) c.chunk_queries = 256;
+    if (c.workers <= 0 || c.workers > 8 || c.max_batch_queries <= 0 || c.hbm_budget < 0 || c.chunk_queries < 0)
+        return HALIGN_E_ARG;
     halign_server *s = new (std::nothrow) halign_server();
     if (!s) return HALIGN_E_NOMEM;
     s->rlen = len;
     s->cfg = c;
+    s->chunk_queries = std::min(c.chunk_queries, c.max_batch_queries);   // a chunk fits in a batch
+    s->device = device;
     if (hipSetDevice(device) != hipSuccess) {
         delete s;
         return HALIGN_E_HIP;
     }
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
+    s->free_at_start = free_b;
     // default: half of free HBM for the batches' working sets, 35% for the
     // traceback pools, split between the workers
     const int64_t budget = c.hbm_budget > 0 ? c.hbm_budget : (int64_t)(0.85 * (double)free_b);
@@ -212,28 +290,37 @@ int halign_server_submit(halign_server *s, int32_t n, const char *queries, const
     auto r = std::make_shared<Request>();
     r->n = n;
     r->off.resize((size_t)n + 1);
-    int64_t est = 0;
+    r->est.resize((size_t)n);
     if (n > 0) {
         for (int32_t k = 0; k <= n; ++k) r->off[k] = offsets[k] - offsets[0];
         for (int32_t k = 0; k < n; ++k) {
             if (r->off[k + 1] < r->off[k]) return HALIGN_E_ARG;
-            est += query_footprint(r->off[k + 1] - r->off[k], s->rlen);
+            r->est[(size_t)k] = query_footprint(r->off[k + 1] - r->off[k], s->rlen);
         }
         r->seqs.assign(queries + offsets[0], queries + offsets[n]);
     } else {
         r->off[0] = 0;
     }
-    r->est_bytes = est;
+    r->qcig.resize((size_t)n);
+    r->stats.assign((size_t)n * 4, 0);
+    r->chunks_left = n > 0 ? (n + s->chunk_queries - 1) / s->chunk_queries : 0;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop) return HALIGN_E_ARG;
         r->id = s->next_id++;
         r->t_submit = Clock::now();
         s->live[r->id] = r;
-        s->pending.push_back(r);
         *request_id = r->id;
+        if (n == 0) {   // nothing to align
+            r->t_start = r->t_done = r->t_submit;
+            r->coff.assign(1, 0);
+            r->done = true;
+        } else {
+            s->pending.push_back(r);
+        }
     }
-    s->cv_work.notify_one();
+    if (n == 0) s->cv_done.notify_all();
+    else s->cv_work.notify_one();
     return HALIGN_OK;
 }
 
@@ -278,6 +365,16 @@ int halign_server_result(halign_server *s, int64_t request_id, uint32_t *cigar_o
         for (int32_t k = 0; k <= r.n; ++k) cigar_offsets[k] = r.coff.empty() ? 0 : r.coff[k];
     if (stats && r.n > 0) memcpy(stats, r.stats.data(), (size_t)r.n * 4 * sizeof(int32_t));
     s->live.erase(it);
+    return HALIGN_OK;
+}
+
+int halign_server_stats(halign_server *s, int64_t *batches, int64_t *max_batch_estimate, int64_t *max_device_used)
+{
+    if (!s) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (batches) *batches = s->batches;
+    if (max_batch_estimate) *max_batch_estimate = s->max_est;
+    if (max_device_used) *max_device_used = s->max_used;
     return HALIGN_OK;
 }
 

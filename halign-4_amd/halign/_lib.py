@@ -70,7 +70,8 @@ def free_extz(ez):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("workers", C.c_int32), ("max_batch_queries", C.c_int32), ("hbm_budget", C.c_int64)]
+    _fields_ = [("workers", C.c_int32), ("max_batch_queries", C.c_int32), ("hbm_budget", C.c_int64),
+                ("chunk_queries", C.c_int32), ("pad_", C.c_int32)]
 
 
 def _load():
@@ -154,6 +155,7 @@ def _load():
     lib.halign_server_wait.argtypes = [_P, C.c_int64, C.c_int32]
     lib.halign_server_result.argtypes = [_P, C.c_int64, _P, C.c_int64, _P, _P, _P,
                                          C.POINTER(C.c_int64)]
+    lib.halign_server_stats.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.halign_server_last_error.argtypes = [_P]
     lib.halign_server_last_error.restype = C.c_char_p
     lib.halign_server_free.argtypes = [_P]
@@ -604,10 +606,10 @@ class Server:
     """Continuous-batching server over a resident center (halign_server_*):
     submit() queues a request of queries, wait()/result() collect it."""
 
-    def __init__(self, center, device=0, workers=2, max_batch_queries=4096, hbm_budget=0):
+    def __init__(self, center, device=0, workers=2, max_batch_queries=4096, hbm_budget=0, chunk_queries=0):
         self.lib = lib()
         center = center.encode() if isinstance(center, str) else bytes(center)
-        cfg = ServerConfig(workers, max_batch_queries, hbm_budget)
+        cfg = ServerConfig(workers, max_batch_queries, hbm_budget, chunk_queries, 0)
         h = _P()
         rc = self.lib.halign_server_create(device, center, len(center), C.byref(cfg), C.byref(h))
         if rc != HALIGN_OK:
@@ -630,6 +632,12 @@ class Server:
         if rc != HALIGN_OK:
             msg = self.lib.halign_server_last_error(self.h).decode()
             raise HalignError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def stats(self):
+        """(batches run, largest batch footprint estimate, most device memory in use after a batch)."""
+        b, e, u = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        self._check(self.lib.halign_server_stats(self.h, C.byref(b), C.byref(e), C.byref(u)), "halign_server_stats")
+        return b.value, e.value, u.value
 
     def submit_flat(self, blob, off):
         """Queue queries blob[off[i]:off[i+1]] (bytes or uint8 array, int64 offsets)."""

@@ -432,18 +432,24 @@ int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes);
 /* Continuous-batching server for RefAligner::globalAlign against one center
  * (src/align/ref_aligner.cpp:91-125; the reference's own request loop is the
  * OpenMP chunk loop of alignQueryToRef, ref_aligner.cpp:348-386). `workers`
- * contexts each keep the center and its minimizer index resident; requests
- * (groups of ASCII queries) queue in arrival order and a free worker takes the
- * requests at the head of the queue that fit max_batch_queries and the HBM
- * footprint budget (hbm_budget bytes over all workers, 0 = 85% of free HBM) as
- * one batch, so requests that arrive while others are on the GPU are admitted
- * as workers and memory free up and their batches overlap on the device. A
- * query's CIGAR is the same whichever batch carries it. Thread-safe. */
+ * contexts each keep the center and its minimizer index resident. Requests
+ * (groups of ASCII queries) are cut into chunks of chunk_queries queries; a
+ * worker that becomes free forms its next batch from one chunk of each
+ * waiting request in arrival order, round after round, until max_batch_queries
+ * or its share of the HBM footprint budget (hbm_budget bytes over all
+ * workers, 0 = 85% of free HBM) is reached. So a request that arrives while
+ * others are on the GPU is admitted at the next batch a worker forms, batches
+ * overlap on the device, and a small request does not wait for the whole of a
+ * large one. A request is done when its last chunk is. A query's CIGAR is the
+ * same whichever batch carries it. Thread-safe. */
 typedef struct halign_server halign_server;
 typedef struct {
     int32_t workers;            /* concurrent batches (1..8), default 2 */
     int32_t max_batch_queries;  /* queries per batch, default 4096 */
     int64_t hbm_budget;         /* bytes, 0 = 85% of free HBM */
+    int32_t chunk_queries;      /* queries per chunk of a request, 0 = 256;
+                                   at most max_batch_queries is used */
+    int32_t pad_;
 } halign_server_config;
 int halign_server_create(int device, const char *center, int32_t len, const halign_server_config *cfg,
                          halign_server **out);
@@ -455,12 +461,18 @@ int halign_server_submit(halign_server *s, int32_t n, const char *queries, const
  * passes first; timeout_ms < 0 waits without a limit. */
 int halign_server_wait(halign_server *s, int64_t request_id, int32_t timeout_ms);
 /* Results of a done request: CIGARs as halign_center_align, stats (4 per
- * query), timing[3] = {ms from submit to batch start, ms from submit to done,
- * queries in its batch}, n_cigar = total CIGAR words. cigar_out NULL only
+ * query), timing[3] = {ms from submit to the start of its first batch, ms
+ * from submit to done, queries in the largest batch that carried a chunk of
+ * it}, n_cigar = total CIGAR words. cigar_out NULL only
  * reports sizes and keeps the request; otherwise the request is released. A
  * failed request returns its error code (and is released). */
 int halign_server_result(halign_server *s, int64_t request_id, uint32_t *cigar_out, int64_t cap,
                          int64_t *cigar_offsets, int32_t *stats, double *timing, int64_t *n_cigar);
+/* Batches run so far, the largest footprint estimate of a batch, and the most
+ * device memory in use (free memory at halign_server_create minus free memory
+ * after a batch, all workers together) seen after a batch. No reference
+ * counterpart. */
+int halign_server_stats(halign_server *s, int64_t *batches, int64_t *max_batch_estimate, int64_t *max_device_used);
 const char *halign_server_last_error(const halign_server *s);
 void halign_server_free(halign_server *s);
 

@@ -148,6 +148,38 @@ def test_serving_workload5_requests():
     assert not bad, f"served CIGARs of queries {[int(f['idx'][k]) for k in bad[:10]]} differ"
 
 
+def test_serving_small_request_not_queued_behind_large():
+    """Chunked round-robin admission on the serving workload: one worker,
+    batches of 256 queries, chunks of 64. A request of 1,400 queries goes in
+    first, one of 20 right after; the small one rides the next batch the
+    worker forms, so it is done while the large one still has batches to
+    go. Both match the reference's CIGARs, and the server's footprint
+    estimate and measured device use are reported."""
+    import halign
+    from halign import synth
+    f = fixture(5)
+    q, off = f["queries"], f["qoff"]
+    srv = halign.Server(synth.center(5), workers=1, max_batch_queries=256, chunk_queries=64)
+    try:
+        big = srv.submit_flat(q[off[0]:off[1400]], off[:1401] - off[0])
+        small = srv.submit_flat(q[off[1400]:off[1420]], off[1400:1421] - off[1400])
+        cs, ofs, _, ts = srv.result(small)
+        big_pending = not srv.wait(big, 0)
+        cb, ofb, _, tb = srv.result(big)
+        batches, est, used = srv.stats()
+    finally:
+        srv.close()
+    print(f"serving chunks: small done at {ts[1]:.1f} ms, large at {tb[1]:.1f} ms, {batches} batches, "
+          f"largest batch estimate {est / 2**20:.1f} MiB, device use {used / 2**20:.1f} MiB",
+          file=sys.stderr, flush=True)
+    assert big_pending, "the small request waited for the whole large one"
+    assert batches >= 1400 // 256
+    assert ts[2] <= 256 and tb[2] <= 256
+    bad = [k for k in range(1400) if not np.array_equal(cb[ofb[k]:ofb[k + 1]], expected_cigar(f, k))]
+    bad += [1400 + j for j in range(20) if not np.array_equal(cs[ofs[j]:ofs[j + 1]], expected_cigar(f, 1400 + j))]
+    assert not bad, f"served CIGARs of queries {[int(f['idx'][k]) for k in bad[:10]]} differ"
+
+
 def test_band_overflow_segments_vs_oracle(ctx):
     """Queries of the divergent workload's center with 1-3 stretches of
     600-2,000 bases replaced by random ones of 50-80% of that length: the
