@@ -595,12 +595,12 @@ def serve_bench(args):
             points.append({"load": f, "offered_req_per_s": round(lam, 3), "seqs_per_s": round(need / sp, 1),
                            "p50_ms": round(float(np.percentile(lat, 50)), 1),
                            "p99_ms": round(float(np.percentile(lat, 99)), 1)})
-        batches, est, used = srv.stats()
-        free_b, _ = hip_mem_info()
-        memory = {"batches": batches, "largest_batch_estimate_bytes": est, "device_used_bytes": used,
-                  "device_free_bytes_after": free_b}
-        log(f"serve memory: {batches} batches, largest batch estimate {est / 2**30:.2f} GiB, "
-            f"device in use {used / 2**30:.2f} GiB")
+        memory = srv.stats()
+        memory["device_free_after"], _ = hip_mem_info()
+        log(f"serve memory: {memory['batches']} batches; resident {memory['resident'] / 2**30:.2f} GiB, "
+            f"traceback reserved {memory['traceback_reserved'] / 2**30:.2f} GiB; working buffers: estimate "
+            f"{memory['sum_worker_estimates'] / 2**30:.2f} GiB (largest batch per worker, summed), measured "
+            f"{memory['working_measured'] / 2**30:.2f} GiB")
     finally:
         srv.close()
     verified = None if args.no_verify else verify_queries(

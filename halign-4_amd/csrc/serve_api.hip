@@ -12,8 +12,16 @@
 // while others are on the GPU is admitted at the next batch any worker forms,
 // and a small request is not queued behind all of a large one. A request is
 // done when its last chunk is. A query's result does not depend on which
-// batch carries it. The worker measures its device memory in use after each
-// batch (halign_server_stats) next to the footprint estimate of that batch.
+// batch carries it.
+//
+// Device memory: each worker's traceback pool is allocated whole when the
+// server is created (its share of the budget), so a batch's whole-pair
+// fallbacks (queries with no anchors, aligned as one unbanded pair) always
+// find their traceback slots, however the other buffers have grown; the DP
+// launches fit their grids to the pool. The workers measure device memory in
+// use after each batch, and halign_server_stats reports it beside the
+// footprint estimates: resident (contexts, centers, indexes), the reserved
+// pools, and what the batches' working buffers took.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,7 +36,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/halign_hip.h"
+#include "host_common.h"
 
 namespace {
 
@@ -73,6 +81,9 @@ struct halign_server {
         std::thread th;
     };
     std::vector<Worker> workers;
+    std::vector<int64_t> worker_max_est;   // largest footprint estimate of a batch, per worker
+    int64_t resident = 0;       // device bytes of the contexts and centers, measured after creation
+    int64_t tb_reserved = 0;    // device bytes of the reserved traceback pools
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Request>> pending;   // requests with queries not yet handed out
@@ -181,6 +192,7 @@ void worker_loop(halign_server *s, int wi)
             const auto t_done = Clock::now();
             s->batches += 1;
             s->max_est = std::max(s->max_est, est);
+            s->worker_max_est[(size_t)wi] = std::max(s->worker_max_est[(size_t)wi], est);
             if (s->free_at_start > free_now) s->max_used = std::max(s->max_used, (int64_t)(s->free_at_start - free_now));
             int64_t qi = 0;
             for (auto &c : batch) {
@@ -269,6 +281,7 @@ int halign_server_create(int device, const char *center, int32_t len, const hali
     s->batch_budget = (int64_t)(budget * (0.50 / 0.85)) / c.workers;
     const int64_t tb = (int64_t)(budget * (0.35 / 0.85)) / c.workers;
     s->workers.resize((size_t)c.workers);
+    s->worker_max_est.assign((size_t)c.workers, 0);
     for (auto &w : s->workers) {
         int rc = halign_init(device, &w.ctx);
         if (rc == HALIGN_OK) rc = halign_set_traceback_budget(w.ctx, tb);
@@ -277,6 +290,21 @@ int halign_server_create(int device, const char *center, int32_t len, const hali
             destroy(s);
             return rc;
         }
+    }
+    (void)hipDeviceSynchronize();
+    size_t free_c = 0;
+    (void)hipMemGetInfo(&free_c, &total_b);
+    s->resident = free_b > free_c ? (int64_t)(free_b - free_c) : 0;
+    // the reserved traceback pools (no batch on the device yet: the pools
+    // are not in use)
+    for (auto &w : s->workers) {
+        std::lock_guard<std::mutex> lk(w.ctx->mu);
+        if (w.ctx->tb.ensure((size_t)tb, (size_t)tb) != hipSuccess) {
+            (void)hipGetLastError();
+            destroy(s);
+            return HALIGN_E_NOMEM;
+        }
+        s->tb_reserved += (int64_t)w.ctx->tb.cap;
     }
     for (int i = 0; i < c.workers; ++i) s->workers[i].th = std::thread(worker_loop, s, i);
     *out = s;
@@ -368,13 +396,19 @@ int halign_server_result(halign_server *s, int64_t request_id, uint32_t *cigar_o
     return HALIGN_OK;
 }
 
-int halign_server_stats(halign_server *s, int64_t *batches, int64_t *max_batch_estimate, int64_t *max_device_used)
+int halign_server_stats(halign_server *s, halign_server_memory *out)
 {
-    if (!s) return HALIGN_E_ARG;
+    if (!s || !out) return HALIGN_E_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
-    if (batches) *batches = s->batches;
-    if (max_batch_estimate) *max_batch_estimate = s->max_est;
-    if (max_device_used) *max_device_used = s->max_used;
+    out->batches = s->batches;
+    out->max_batch_estimate = s->max_est;
+    out->sum_worker_estimates = 0;
+    for (int64_t e : s->worker_max_est) out->sum_worker_estimates += e;
+    out->batch_budget = s->batch_budget;
+    out->resident = s->resident;
+    out->traceback_reserved = s->tb_reserved;
+    out->device_used = s->max_used;
+    out->working_measured = std::max<int64_t>(0, s->max_used - s->resident - s->tb_reserved);
     return HALIGN_OK;
 }
 

@@ -69,6 +69,11 @@ def free_extz(ez):
     ez.m_cigar = ez.n_cigar = 0
 
 
+class ServerMemory(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("batches", "max_batch_estimate", "sum_worker_estimates", "batch_budget",
+                                         "resident", "traceback_reserved", "device_used", "working_measured")]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("workers", C.c_int32), ("max_batch_queries", C.c_int32), ("hbm_budget", C.c_int64),
                 ("chunk_queries", C.c_int32), ("pad_", C.c_int32)]
@@ -155,7 +160,7 @@ def _load():
     lib.halign_server_wait.argtypes = [_P, C.c_int64, C.c_int32]
     lib.halign_server_result.argtypes = [_P, C.c_int64, _P, C.c_int64, _P, _P, _P,
                                          C.POINTER(C.c_int64)]
-    lib.halign_server_stats.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.halign_server_stats.argtypes = [_P, C.POINTER(ServerMemory)]
     lib.halign_server_last_error.argtypes = [_P]
     lib.halign_server_last_error.restype = C.c_char_p
     lib.halign_server_free.argtypes = [_P]
@@ -634,10 +639,11 @@ class Server:
             raise HalignError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
 
     def stats(self):
-        """(batches run, largest batch footprint estimate, most device memory in use after a batch)."""
-        b, e, u = C.c_int64(0), C.c_int64(0), C.c_int64(0)
-        self._check(self.lib.halign_server_stats(self.h, C.byref(b), C.byref(e), C.byref(u)), "halign_server_stats")
-        return b.value, e.value, u.value
+        """Device memory of the server, estimated and measured: a dict of the
+        halign_server_memory fields (bytes; batches run)."""
+        m = ServerMemory()
+        self._check(self.lib.halign_server_stats(self.h, C.byref(m)), "halign_server_stats")
+        return {k: getattr(m, k) for k, _ in ServerMemory._fields_}
 
     def submit_flat(self, blob, off):
         """Queue queries blob[off[i]:off[i+1]] (bytes or uint8 array, int64 offsets)."""

@@ -441,7 +441,10 @@ int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes);
  * others are on the GPU is admitted at the next batch a worker forms, batches
  * overlap on the device, and a small request does not wait for the whole of a
  * large one. A request is done when its last chunk is. A query's CIGAR is the
- * same whichever batch carries it. Thread-safe. */
+ * same whichever batch carries it. Each worker's traceback pool (35/85 of the
+ * budget over the workers) is allocated whole at create, a reserved pool for
+ * the DP of every batch, whole-pair fallbacks included; the batches' other
+ * buffers are admitted against the remaining 50/85. Thread-safe. */
 typedef struct halign_server halign_server;
 typedef struct {
     int32_t workers;            /* concurrent batches (1..8), default 2 */
@@ -468,11 +471,21 @@ int halign_server_wait(halign_server *s, int64_t request_id, int32_t timeout_ms)
  * failed request returns its error code (and is released). */
 int halign_server_result(halign_server *s, int64_t request_id, uint32_t *cigar_out, int64_t cap,
                          int64_t *cigar_offsets, int32_t *stats, double *timing, int64_t *n_cigar);
-/* Batches run so far, the largest footprint estimate of a batch, and the most
- * device memory in use (free memory at halign_server_create minus free memory
- * after a batch, all workers together) seen after a batch. No reference
- * counterpart. */
-int halign_server_stats(halign_server *s, int64_t *batches, int64_t *max_batch_estimate, int64_t *max_device_used);
+/* Device memory of a server, estimated and measured (bytes). Measured use is
+ * free device memory at halign_server_create minus free memory after a
+ * batch, all workers together, the most seen so far. */
+typedef struct {
+    int64_t batches;               /* batches run so far */
+    int64_t max_batch_estimate;    /* largest footprint estimate of one batch */
+    int64_t sum_worker_estimates;  /* sum over workers of the largest estimate of a batch it ran */
+    int64_t batch_budget;          /* estimate allowed per batch (per worker) */
+    int64_t resident;              /* contexts, centers and indexes (measured after create) */
+    int64_t traceback_reserved;    /* traceback pools, allocated whole at create */
+    int64_t device_used;           /* most measured device use after a batch */
+    int64_t working_measured;      /* device_used - resident - traceback_reserved */
+} halign_server_memory;
+/* No reference counterpart. */
+int halign_server_stats(halign_server *s, halign_server_memory *out);
 const char *halign_server_last_error(const halign_server *s);
 void halign_server_free(halign_server *s);
 

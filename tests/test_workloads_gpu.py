@@ -166,14 +166,16 @@ def test_serving_small_request_not_queued_behind_large():
         cs, ofs, _, ts = srv.result(small)
         big_pending = not srv.wait(big, 0)
         cb, ofb, _, tb = srv.result(big)
-        batches, est, used = srv.stats()
+        m = srv.stats()
     finally:
         srv.close()
-    print(f"serving chunks: small done at {ts[1]:.1f} ms, large at {tb[1]:.1f} ms, {batches} batches, "
-          f"largest batch estimate {est / 2**20:.1f} MiB, device use {used / 2**20:.1f} MiB",
+    print(f"serving chunks: small done at {ts[1]:.1f} ms, large at {tb[1]:.1f} ms, {m['batches']} batches, "
+          f"largest batch estimate {m['max_batch_estimate'] / 2**20:.1f} MiB, working buffers measured "
+          f"{m['working_measured'] / 2**20:.1f} MiB, traceback reserved {m['traceback_reserved'] / 2**30:.1f} GiB",
           file=sys.stderr, flush=True)
     assert big_pending, "the small request waited for the whole large one"
-    assert batches >= 1400 // 256
+    assert m["batches"] >= 1400 // 256
+    assert m["traceback_reserved"] > 0 and m["device_used"] >= m["traceback_reserved"]
     assert ts[2] <= 256 and tb[2] <= 256
     bad = [k for k in range(1400) if not np.array_equal(cb[ofb[k]:ofb[k + 1]], expected_cigar(f, k))]
     bad += [1400 + j for j in range(20) if not np.array_equal(cs[ofs[j]:ofs[j + 1]], expected_cigar(f, 1400 + j))]
