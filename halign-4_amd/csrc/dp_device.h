@@ -49,6 +49,34 @@ __device__ __forceinline__ uint32_t pk_ashr15(uint32_t a)
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(s16x2, a) >> (s16x2){15, 15});
 }
 __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { return (a & m) | (b & ~m); }
+// v in a scalar register the compiler cannot see through: a chain of bfi with
+// constant masks is otherwise rewritten as and / or3 (five operations per
+// three merges instead of three)
+__device__ __forceinline__ uint32_t sreg(uint32_t v)
+{
+    asm("" : "+s"(v));
+    return v;
+}
+// Traceback nibbles of the four cells of registers (2m, 2m+1): the flag words
+// of register 2m in f?0 and of 2m+1 in f?1, each flag in bit 15 of its half.
+// v_perm selectors 8-11 replicate bit 15 / 31 of the second source and 15 / 31
+// of the first into a whole byte, so byte i of each gathered word is 0x00 or
+// 0xFF for the cell (2m, 2m+8, 2m+1, 2m+9)[i], and one bfi per flag places it
+// (the masks are set where they are used: held across the loop they cost
+// scalar registers the loop needs).
+// HI: flags (a, b, x, y) in bits (7, 6, 5, 4) of each byte; else in (3, 2, 1, 0).
+// The other nibble of each byte is filler for the caller's final bfi.
+template <bool HI>
+__device__ __forceinline__ uint32_t tb_nibbles(uint32_t fa0, uint32_t fa1, uint32_t fb0, uint32_t fb1,
+                                               uint32_t fx0, uint32_t fx1, uint32_t fy0, uint32_t fy1)
+{
+    const uint32_t ga = __builtin_amdgcn_perm(fa1, fa0, 0x0B0A0908u);
+    const uint32_t gb = __builtin_amdgcn_perm(fb1, fb0, 0x0B0A0908u);
+    const uint32_t gx = __builtin_amdgcn_perm(fx1, fx0, 0x0B0A0908u);
+    const uint32_t gy = __builtin_amdgcn_perm(fy1, fy0, 0x0B0A0908u);
+    constexpr uint32_t s = HI ? 4u : 0u;
+    return bfi(sreg(0x08080808u << s), ga, bfi(sreg(0x04040404u << s), gb, bfi(sreg(0x02020202u << s), gx, gy)));
+}
 // cells (2j-1, 2j) from pairs hi=(2j,2j+1), lo=(2j-2,2j-1)
 __device__ __forceinline__ uint32_t cat16(uint32_t hi, uint32_t lo) { return __builtin_amdgcn_alignbit(hi, lo, 16); }
 

@@ -296,21 +296,16 @@ __device__ void dp_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restri
                         fx[h] = RT ? a2 : pk_subsat(0u, a2);
                         fy[h] = RT ? b2 : pk_subsat(0u, b2);
                     }
-                    // gather the sign bits of 4 cells, then merge the 4 flags into the high nibble
-                    const uint32_t ga = __builtin_amdgcn_perm(fa[1], fa[0], 0x07050301u);
-                    const uint32_t gb = __builtin_amdgcn_perm(fb[1], fb[0], 0x07050301u);
-                    const uint32_t gx = __builtin_amdgcn_perm(fx[1], fx[0], 0x07050301u);
-                    const uint32_t gy = __builtin_amdgcn_perm(fy[1], fy[0], 0x07050301u);
-                    uint32_t c = bfi(0x80808080u, ga, gb >> 1);
-                    c = bfi(0xC0C0C0C0u, c, gx >> 2);
-                    C[m] = bfi(0xE0E0E0E0u, c, gy >> 3);
+                    // 4 flags of cells (2m, 2m+8, 2m+1, 2m+9): low nibbles for even m, high for odd
+                    C[m] = (m & 1) ? tb_nibbles<true>(fa[0], fa[1], fb[0], fb[1], fx[0], fx[1], fy[0], fy[1])
+                                   : tb_nibbles<false>(fa[0], fa[1], fb[0], fb[1], fx[0], fx[1], fy[0], fy[1]);
                 }
             };
             if (right) cells_rl(std::integral_constant<bool, true>{});
             else cells_rl(std::integral_constant<bool, false>{});
             uint2 o;
-            o.x = bfi(0xF0F0F0F0u, C[1], C[0] >> 4);
-            o.y = bfi(0xF0F0F0F0u, C[3], C[2] >> 4);
+            o.x = bfi(0xF0F0F0F0u, C[1], C[0]);
+            o.y = bfi(0xF0F0F0F0u, C[3], C[2]);
             *(uint2 *)(tb + (size_t)r * rowb + (size_t)(b - bs) * 8) = o;
 
             // H bookkeeping
@@ -616,20 +611,15 @@ __device__ void dp_job_wide(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                             fx[h] = RT ? a2 : pk_subsat(0u, a2);
                             fy[h] = RT ? b2 : pk_subsat(0u, b2);
                         }
-                        const uint32_t ga = __builtin_amdgcn_perm(fa[1], fa[0], 0x07050301u);
-                        const uint32_t gb = __builtin_amdgcn_perm(fb[1], fb[0], 0x07050301u);
-                        const uint32_t gx = __builtin_amdgcn_perm(fx[1], fx[0], 0x07050301u);
-                        const uint32_t gy = __builtin_amdgcn_perm(fy[1], fy[0], 0x07050301u);
-                        uint32_t c = bfi(0x80808080u, ga, gb >> 1);
-                        c = bfi(0xC0C0C0C0u, c, gx >> 2);
-                        C[m] = bfi(0xE0E0E0E0u, c, gy >> 3);
+                        C[m] = (m & 1) ? tb_nibbles<true>(fa[0], fa[1], fb[0], fb[1], fx[0], fx[1], fy[0], fy[1])
+                                       : tb_nibbles<false>(fa[0], fa[1], fb[0], fb[1], fx[0], fx[1], fy[0], fy[1]);
                     }
                 };
                 if (right) cells_rl(std::integral_constant<bool, true>{});
                 else cells_rl(std::integral_constant<bool, false>{});
                 uint2 o;
-                o.x = bfi(0xF0F0F0F0u, C[1], C[0] >> 4);
-                o.y = bfi(0xF0F0F0F0u, C[3], C[2] >> 4);
+                o.x = bfi(0xF0F0F0F0u, C[1], C[0]);
+                o.y = bfi(0xF0F0F0F0u, C[3], C[2]);
                 *(uint2 *)(tb + (size_t)r * rowb + (size_t)(b - bs) * 8) = o;
 
                 if (b == (en0 >> 4)) sm.logu[r & 63] = (uint8_t)byte_of(U, en0 & 15);

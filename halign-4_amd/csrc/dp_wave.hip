@@ -515,18 +515,13 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                     fx[h] = RT ? a2 : pk_subsat(0u, a2);
                     fy[h] = RT ? b2 : pk_subsat(0u, b2);
                 }
-                // sign bytes of cells (2m, 2m+8, 2m+1, 2m+9), then 4 flags per byte
-                const uint32_t ga = __builtin_amdgcn_perm(fa[1], fa[0], 0x07050301u);
-                const uint32_t gb = __builtin_amdgcn_perm(fb[1], fb[0], 0x07050301u);
-                const uint32_t gx = __builtin_amdgcn_perm(fx[1], fx[0], 0x07050301u);
-                const uint32_t gy = __builtin_amdgcn_perm(fy[1], fy[0], 0x07050301u);
-                uint32_t c = bfi(0x80808080u, ga, gb >> 1);
-                c = bfi(0xC0C0C0C0u, c, gx >> 2);
-                C[m] = bfi(0xE0E0E0E0u, c, gy >> 3);
+                // 4 flags of cells (2m, 2m+8, 2m+1, 2m+9): low nibbles for even m, high for odd
+                C[m] = (m & 1) ? tb_nibbles<true>(fa[0], fa[1], fb[0], fb[1], fx[0], fx[1], fy[0], fy[1])
+                               : tb_nibbles<false>(fa[0], fa[1], fb[0], fb[1], fx[0], fx[1], fy[0], fy[1]);
             }
             uint2 o;
-            o.x = bfi(0xF0F0F0F0u, C[1], C[0] >> 4);
-            o.y = bfi(0xF0F0F0F0u, C[3], C[2] >> 4);
+            o.x = bfi(0xF0F0F0F0u, C[1], C[0]);
+            o.y = bfi(0xF0F0F0F0u, C[3], C[2]);
             *(uint2 *)(tb + (size_t)r * rowb + (size_t)(b - bs) * 8) = o;
             if constexpr (ROW) {   // per-group bounds: the owning lane's own copy
                 if (k == ke) ue_v = (uint32_t)cell_byte(U[k], en0 & 15);
