@@ -321,8 +321,8 @@ def main():
 
 
 E2E = {1: ("plain method, small: center 16569 bp, 1000 queries at 1% sub / 0.2% indel", 1000, 1000),
-       3: ("uneven batch: center 20000 bp, 1000000 lognormal-length queries (median 1.5 kb)", 16384, 1000000),
-       4: ("divergent stress: center 10000 bp, 200000 queries at 5% sub / 1% indel + long indels", 8192, 200000)}
+       3: ("uneven batch: center 20000 bp, 1000000 lognormal-length queries (median 1.5 kb)", 65536, 1000000),
+       4: ("divergent stress: center 10000 bp, 200000 queries at 5% sub / 1% indel + long indels", 32768, 200000)}
 
 
 def e2e_bench(args):
@@ -356,6 +356,8 @@ def e2e_bench(args):
     nq = sum(x[0] for x in steps)
     nb = sum(x[1] for x in steps)
     tw = sum(x[2] for x in steps)
+    # one more step (not timed) with every DP launch timed on its own
+    buckets = dp_bucket_report(ctx, lambda: c.align_flat(qb, off[0:min(per, need) + 1]))
     verified = None
     if not args.no_verify:
         a, b, cig, coff = last
@@ -385,8 +387,40 @@ def e2e_bench(args):
            "config": {"workload": f"config {cfg}: {label}", "queries_per_step": per},
            "verified": verified,
            "bases_per_s": round(nb / tw, 1), "device_ms_per_step": round(sum(x[3] for x in steps) / len(steps), 3),
-           "cpu_baseline": cpu}
+           "dp_buckets": buckets, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def dp_bucket_report(ctx, run):
+    """Run one step with per-launch DP timing (halign_set_bucket_timing) and
+    report each DP launch: shape, jobs, cells, kernel ms alone on its stream,
+    GCUPS and the fraction of the 78.6 T packed-16-bit VALU peak at 24 int
+    ops per cell. Buckets on different streams overlap, so their times are
+    each one's own span, not shares of the step."""
+    import halign
+    shapes = halign.ring_shapes()
+    seen, names = set(), []
+    for w, l, r in shapes:
+        u = (w, l, r) in seen
+        seen.add((w, l, r))
+        names.append(f"{w}x{r // l}" if w else f"sub{l}x{r // l}{'u' if u else ''}")
+    ctx.set_bucket_timing(True)
+    try:
+        run()
+        rows = ctx.last_buckets()
+    finally:
+        ctx.set_bucket_timing(False)
+    out = []
+    for b in rows:
+        fam = ("planned", "small segments", "multi-wave")[b["family"]]
+        name = names[b["shape"]] if b["family"] < 2 else f"multiwave{b['shape']}"
+        g = b["cells"] / (b["kernel_ms"] * 1e6) if b["kernel_ms"] > 0 else 0.0
+        out.append({"kind": fam, "shape": name, "right": b["right"], "stream": b["stream"], "jobs": b["jobs"],
+                    "cells": b["cells"], "ms": round(b["kernel_ms"], 3), "est_ms": round(b["est_ms"], 3),
+                    "gcups": round(g, 1), "roofline_frac": round(g * 24 / 78.6e3, 4)})
+        log(f"  dp {fam:14s} {name:10s} rt {b['right']} stream {b['stream']}: {b['jobs']:7d} jobs "
+            f"{b['cells'] / 1e9:7.2f} Gcells {b['kernel_ms']:8.2f} ms (est {b['est_ms']:7.2f}) {g:7.1f} GCUPS")
+    return out
 
 
 def msa_bench(args):

@@ -94,7 +94,6 @@ int dp_wave_jobs_per_group(int idx);
 int dp_wave_shape_lanes(int idx);
 int dp_wave_shape_uniform(int idx);   // 1: a sub-wave shape's uniform-ticket form
 int dp_wave_uniform_of(int idx);      // the uniform-ticket form of a sub-wave shape (else idx)
-void dp_wave_shape_cost(int idx, int *issue, int *valu);
 int dp_wave_max_groups(int idx, bool rt, int num_cu);
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
                           uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream,
@@ -109,5 +108,12 @@ inline int dp_wave_find_shape(int waves, int lanes, int slots)
     return -1;
 }
 void bytes_target(const uint8_t *codes, int len, uint8_t *out);
+
+// Planner cost table (tune_api.hip): the table measured when the library was
+// built, and a bucket's estimate on ring shape wi (ns): crit = its longest
+// job, tp = its share of the chip. false: no measurement for wi.
+void tune_load_default(halign_ctx *ctx);
+bool tune_cost(const halign_ctx *ctx, int wi, double max_nc, double max_nd, double sum_nd, double njobs,
+               double &crit, double &tp);
 
 }  // namespace hal

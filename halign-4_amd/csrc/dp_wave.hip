@@ -768,24 +768,6 @@ int dp_wave_uniform_of(int idx)
     return idx;
 }
 
-// Instructions per anti-diagonal of one wave with every slot active, counted
-// in the gfx950 code of each kernel's diagonal loop (hipcc -O3): all issued
-// instructions (one job's critical path: a wave issues one instruction at a
-// time) and VALU instructions (the SIMD throughput cost when waves share it).
-// 0: not counted (shapes for bands wider than a wave).
-void dp_wave_shape_cost(int idx, int *issue, int *valu)
-{
-    const ShapeDesc &d = kShapes[idx];
-    int is = 0, va = 0;
-    if (d.waves == 1 && d.slots == 1) { is = 695; va = 382; }
-    else if (d.waves == 1 && d.slots == 2) { is = 1250; va = 786; }
-    else if (d.waves == 0 && d.slots == 1) { is = 747; va = 580; }
-    else if (d.waves == 0 && d.slots == 2) { is = 1297; va = 1027; }
-    else if (d.waves == 0 && d.slots == 3) { is = 1779; va = 1441; }
-    *issue = is;
-    *valu = va;
-}
-
 int dp_wave_max_groups(int idx, bool rt, int num_cu)
 {
     int per_cu = 0;

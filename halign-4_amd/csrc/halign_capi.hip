@@ -95,13 +95,10 @@ int pick_shape(int ncol, bool stats, int policy)
         const int nw = dp_wave_num_shapes();
         if (policy >= 2 && policy - 2 < nw && ncol <= dp_wave_shape_ring(policy - 2) - 2)
             return kWaveBase + policy - 2;
-        // Measured preference (config-2 band, n_col_ = 293, kernel GCUPS: 3x2 909,
-        // 4x2 913, 5x1 861, 8x1 802, 6x1 786, 16x1 508): two slots per lane pay
-        // for the per-diagonal work better than a tighter ring, one slot only for
-        // bands of a wave's width. Narrow bands first go to the sub-wave shapes
-        // (waves 0: groups of 4 or 16 lanes, 16 or 4 jobs per wave); run_specs
-        // revisits that choice per batch (pick_bucket_shape). First shape of this
-        // list, {waves, lanes per job, slots}, that holds the band.
+        // A job's first shape, which sorts it into a bucket: the first of this
+        // list, {waves, lanes per job, slots}, that holds the band. With the
+        // automatic policy run_specs then gives each bucket the shape the
+        // measured cost table prefers (plan_bucket_shapes).
         static const int kAuto[][3] = {{0, 4, 1},  {0, 4, 2},  {0, 16, 1},  {0, 16, 2}, {0, 16, 3},
                                        {1, 64, 1}, {1, 64, 2}, {2, 128, 2}, {3, 192, 2}, {4, 256, 2},
                                        {8, 512, 2}};
@@ -121,42 +118,36 @@ int pick_shape(int ncol, bool stats, int policy)
     return -1;
 }
 
-// Batch-level shape choice for the narrow-band CIGAR buckets (automatic
-// policy). A bucket on shape s takes about crit = its longest job's
-// anti-diagonals x instructions per anti-diagonal of s (a wave issues one
-// instruction at a time), and adds tp = all its anti-diagonals x VALU
-// instructions per anti-diagonal / (jobs per wave x SIMDs) of chip time. The
-// buckets run side by side, so the batch takes about max(max crit, sum tp):
-// start each bucket on its cheapest shape in tp (sub-wave groups: many jobs
-// per instruction), then move the bucket with the longest crit to a shape with
-// a shorter one (a one-wave ring) while that lowers the estimate.
+// Batch-level shape choice for the ring-kernel CIGAR buckets (automatic
+// policy), from the measured cost table (tune_api.hip). A bucket on shape s
+// takes about crit = its longest job's time on s, and uses tp = all its jobs'
+// time / s's jobs in flight of the chip. The buckets run side by side, so the
+// batch takes about max(max crit, sum tp): start each bucket on its cheapest
+// shape in tp, then move the bucket with the longest crit to a shape with a
+// shorter one while that lowers the estimate.
 struct BucketCost {
     int bi;
-    double max_nd, sum_nd;
+    double max_nd, sum_nd, n;
     int max_nc;
-    std::vector<int> shapes;   // candidate shape indices (ring shapes), by falling crit
+    std::vector<int> shapes;   // candidate ring shapes, by rising tp
     int cur;                   // position in shapes
 };
 
-static void cost_of(const BucketCost &b, int idx, double nsimd, double &crit, double &tp)
+static void cost_of(const halign_ctx *ctx, const BucketCost &b, int wi, double &crit, double &tp)
 {
-    int is = 0, va = 0;
-    dp_wave_shape_cost(idx, &is, &va);
-    crit = b.max_nd * is;
-    tp = b.sum_nd * va / (dp_wave_jobs_per_group(idx) * nsimd);
+    if (!tune_cost(ctx, wi, b.max_nc, b.max_nd, b.sum_nd, b.n, crit, tp)) crit = tp = 1e300;
 }
 
-static void plan_bucket_shapes(const std::vector<Spec> &specs, const std::vector<std::vector<int>> &buckets,
-                               int num_cu, std::vector<int> &shape_of_bucket)
+static void plan_bucket_shapes(const halign_ctx *ctx, const std::vector<Spec> &specs,
+                               const std::vector<std::vector<int>> &buckets, std::vector<int> &shape_of_bucket)
 {
-    const double nsimd = 4.0 * num_cu;
     std::vector<BucketCost> bc;
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         const auto &b = buckets[bi];
         const int shape = (int)bi / 4;
         shape_of_bucket[bi] = shape;
-        if (b.empty() || shape < kWaveBase || (bi & 2) || dp_wave_shape_waves(shape - kWaveBase) > 1) continue;
-        BucketCost c{(int)bi, 0, 0, 0, {}, 0};
+        if (b.empty() || shape < kWaveBase || (bi & 2)) continue;
+        BucketCost c{(int)bi, 0, 0, (double)b.size(), 0, {}, 0};
         for (int i : b) {
             const Spec &s = specs[i];
             const double nd = (double)s.qlen + s.tlen - 1;
@@ -165,17 +156,17 @@ static void plan_bucket_shapes(const std::vector<Spec> &specs, const std::vector
             c.sum_nd += nd;
         }
         for (int i = 0; i < dp_wave_num_shapes(); ++i) {
-            int is = 0, va = 0;
-            dp_wave_shape_cost(i, &is, &va);
-            if (is && c.max_nc <= dp_wave_shape_ring(i) - 2) c.shapes.push_back(i);
+            double cr, tp;
+            if (!dp_wave_shape_uniform(i) && c.max_nc <= dp_wave_shape_ring(i) - 2 &&
+                tune_cost(ctx, i, c.max_nc, c.max_nd, c.sum_nd, c.n, cr, tp))
+                c.shapes.push_back(i);
         }
         if (c.shapes.empty()) continue;
-        // by rising tp; the walk below only takes steps that shorten crit
         std::sort(c.shapes.begin(), c.shapes.end(), [&](int x, int y) {
             double cx, tx, cy, ty;
-            cost_of(c, x, nsimd, cx, tx);
-            cost_of(c, y, nsimd, cy, ty);
-            return tx < ty;
+            cost_of(ctx, c, x, cx, tx);
+            cost_of(ctx, c, y, cy, ty);
+            return tx < ty || (tx == ty && x < y);
         });
         bc.push_back(c);
     }
@@ -183,7 +174,7 @@ static void plan_bucket_shapes(const std::vector<Spec> &specs, const std::vector
         double mc = 0, st = 0;
         for (auto &c : bc) {
             double cr, tp;
-            cost_of(c, c.shapes[c.cur], nsimd, cr, tp);
+            cost_of(ctx, c, c.shapes[c.cur], cr, tp);
             mc = std::max(mc, cr);
             st += tp;
         }
@@ -195,24 +186,39 @@ static void plan_bucket_shapes(const std::vector<Spec> &specs, const std::vector
         double wc = -1;
         for (auto &c : bc) {
             double cr, tp;
-            cost_of(c, c.shapes[c.cur], nsimd, cr, tp);
+            cost_of(ctx, c, c.shapes[c.cur], cr, tp);
             if (cr > wc) { wc = cr; w = &c; }
         }
         if (!w) break;
         const double before = estimate();
-        // next shape (by tp) with a shorter crit
+        // the shape (after the current one by tp) with the shortest crit
         int nxt = -1;
+        double nc = wc;
         for (size_t k = w->cur + 1; k < w->shapes.size(); ++k) {
             double cr, tp;
-            cost_of(*w, w->shapes[k], nsimd, cr, tp);
-            if (cr < wc) { nxt = (int)k; break; }
+            cost_of(ctx, *w, w->shapes[k], cr, tp);
+            if (cr < nc) { nc = cr; nxt = (int)k; }
         }
         if (nxt < 0) break;
         const int old = w->cur;
         w->cur = nxt;
         if (estimate() >= before) { w->cur = old; break; }
     }
-    for (auto &c : bc) shape_of_bucket[c.bi] = kWaveBase + c.shapes[c.cur];
+    for (auto &c : bc) {
+        shape_of_bucket[c.bi] = kWaveBase + c.shapes[c.cur];
+        if (std::getenv("HALIGN_TRACE")) {
+            fprintf(stderr, "[halign plan] bucket %d: %.0f jobs, n_col <= %d, longest %.0f anti-diagonals:", c.bi,
+                    c.n, c.max_nc, c.max_nd);
+            for (size_t k = 0; k < c.shapes.size(); ++k) {
+                double cr, tp;
+                cost_of(ctx, c, c.shapes[k], cr, tp);
+                const int i = c.shapes[k];
+                fprintf(stderr, " %s%dx%d/%d crit %.2f tp %.2f ms", (int)k == c.cur ? "*" : "",
+                        dp_wave_shape_waves(i), dp_wave_shape_lanes(i), dp_wave_shape_ring(i), cr * 1e-6, tp * 1e-6);
+            }
+            fprintf(stderr, "\n");
+        }
+    }
 }
 
 // Uniform tickets: a sub-wave bucket whose jobs share (qlen, tlen, band) in
@@ -221,7 +227,7 @@ static void plan_bucket_shapes(const std::vector<Spec> &specs, const std::vector
 // the bucket becomes its keys in falling anti-diagonal count, each padded with
 // repeats of its last job. The uniform form keeps the band bookkeeping in
 // scalars, about half the VALU work per anti-diagonal of the per-group form.
-static void form_uniform_tickets(std::vector<Spec> &specs, std::vector<std::vector<int>> &buckets)
+static void form_uniform_tickets(std::vector<Spec> &specs, std::vector<std::vector<int>> &buckets, bool keep)
 {
     const int nb = (int)buckets.size();
     std::vector<std::vector<int>> moved(nb);
@@ -233,7 +239,7 @@ static void form_uniform_tickets(std::vector<Spec> &specs, std::vector<std::vect
         const int G = dp_wave_jobs_per_group(wi);
         const int ui = dp_wave_uniform_of(wi);
         const bool forced = dp_wave_shape_uniform(wi) != 0;   // a uniform shape must get its tickets
-        if (G <= 1 || (ui == wi && !forced)) continue;
+        if (G <= 1 || (ui == wi && !forced) || (keep && !forced)) continue;
         auto key = [&](int i) {
             const Spec &x = specs[i];
             return std::make_tuple(x.qlen + x.tlen, x.qlen, x.tlen, eff_w(x.w, x.qlen, x.tlen));
@@ -282,7 +288,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             .push_back((int)i);
     if (ctx->policy == 0 && !plain) {
         std::vector<int> to_shape(nb);
-        plan_bucket_shapes(specs, buckets, ctx->num_cu, to_shape);
+        plan_bucket_shapes(ctx, specs, buckets, to_shape);
         std::vector<std::vector<int>> moved(nb);
         for (int bi = 0; bi < nb; ++bi) {
             auto &b = buckets[bi];
@@ -295,7 +301,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         buckets.swap(moved);
     }
 
-    if (!plain) form_uniform_tickets(specs, buckets);
+    if (!plain) form_uniform_tickets(specs, buckets, ctx->tuning);
     std::vector<int> order;
     order.reserve(n);
     for (int bi = 0; bi < nb; ++bi) {
@@ -343,6 +349,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     if (need_slot > ctx->tb_budget) return fail(ctx, HALIGN_E_NOMEM, "traceback of one job exceeds the device budget");
 
     HCHECK(hipEventRecord(ctx->ev0, ctx->stream), "event");
+    // launches bracketed by bucket-timing events, in event-pair order
+    struct Timed { int family, shape, stream, right; size_t pos, njobs; double est; };
+    std::vector<Timed> timed;
     DpJob *djobs = ctx->jobs.as<DpJob>();
     DpOut *douts = ctx->outs.as<DpOut>();
     int *ctr = ctx->counters.as<int>();
@@ -379,7 +388,6 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         // jobs' anti-diagonal chains, not by the chip, and overlap well.
         struct Launch { int bi; size_t pos, slot, jpg, off; int groups; double est; int stream; };
         std::vector<Launch> ls;
-        const double nsimd = 4.0 * ctx->num_cu;
         size_t pos = 0;
         for (int bi = 0; bi < nb; ++bi) {
             auto &b = buckets[bi];
@@ -402,18 +410,23 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             int groups = wave ? dp_wave_max_groups(wi, rt, ctx->num_cu) : dp_shape_max_groups(shape, stats, ctx->num_cu);
             if (groups <= 0) return fail(ctx, HALIGN_E_HIP, "occupancy query failed");
             L.groups = (int)std::min<size_t>(groups, (b.size() + L.jpg - 1) / L.jpg);
-            // estimated time: the cost model where it has counts, else cells
-            int is = 0, va = 0;
-            if (wave) dp_wave_shape_cost(wi, &is, &va);
+            // estimated time (ns): the measured table for ring shapes; for the
+            // multi-wave kernels, cells at about 650 cells per ns over the chip
             double max_nd = 0, sum_nd = 0, blocks = 0;
+            int max_nc = 0;
             for (int i : b) {
                 const Spec &s = specs[i];
                 const double nd = (double)s.qlen + s.tlen - 1;
+                const int nc = n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen));
                 max_nd = std::max(max_nd, nd);
+                max_nc = std::max(max_nc, nc);
                 sum_nd += nd;
-                blocks += nd * n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen));
+                blocks += nd * nc;
             }
-            L.est = is ? std::max(max_nd * is, sum_nd * va / ((double)L.jpg * nsimd)) : blocks * 400.0 / nsimd;
+            double crit = 0, tp = 0;
+            L.est = wave && tune_cost(ctx, wi, max_nc, max_nd, sum_nd, (double)b.size(), crit, tp)
+                        ? std::max(crit, tp)
+                        : blocks * 16.0 / 650.0;
             ls.push_back(L);
             pos += b.size();
         }
@@ -474,8 +487,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                                  (int64_t)(douts + L.pos), L.stream};
             sig.insert(sig.end(), v, v + 9);
         }
+        const bool use_graphs = ctx->graphs && !ctx->bucket_timing;
         int hit = -1;
-        for (size_t k = 0; ctx->graphs && k < ctx->gcache.size(); ++k)
+        for (size_t k = 0; use_graphs && k < ctx->gcache.size(); ++k)
             if (ctx->gcache[k].sig == sig) hit = (int)k;
         if (hit >= 0) {
             auto cg = ctx->gcache[hit];
@@ -484,11 +498,11 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             HCHECK(hipGraphLaunch(cg.exec, ctx->stream), "graph launch");
             ++ctx->graph_replays;
             ls.clear();   // nothing to launch directly
-        } else if (ctx->graphs) {
+        } else if (use_graphs) {
             // relaxed mode: other threads may allocate while this stream captures
             HCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "begin capture");
         }
-        const bool capturing = ctx->graphs && !ls.empty();
+        const bool capturing = use_graphs && !ls.empty();
         auto end_capture = [&]() {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(ctx->stream, &g);
@@ -514,19 +528,26 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             const bool wave = shape >= kWaveBase;
             const int wi = shape - kWaveBase;
             if (clk.on)
-                fprintf(stderr, "[halign run_specs] bucket: %s %d x%d lanes/job %d ring %d, %zu jobs, %d groups, stream %d, est %.3g\n",
+                fprintf(stderr, "[halign run_specs] bucket: %s %d x%d lanes/job %d ring %d, %zu jobs, %d groups, stream %d, est %.3g ms\n",
                         wave ? (dp_wave_shape_uniform(wi) ? "ring(uniform)" : "ring") : "multiwave",
                         wave ? dp_wave_shape_waves(wi) : shape,
                         wave ? dp_wave_jobs_per_group(wi) : 1, wave ? dp_wave_shape_lanes(wi) : 0,
-                        wave ? dp_wave_shape_ring(wi) : 0, buckets[L.bi].size(), L.groups, L.stream, L.est);
+                        wave ? dp_wave_shape_ring(wi) : 0, buckets[L.bi].size(), L.groups, L.stream, L.est * 1e-6);
             hipStream_t st = L.stream ? ctx->aux[L.stream - 1] : ctx->stream;
             uint8_t *tb = ctx->tb.as<uint8_t>() + L.off;
             const int nj = (int)buckets[L.bi].size();
+            hipEvent_t ta = nullptr, tz = nullptr;
+            if (ctx->bucket_timing) {
+                if (!bucket_events(ctx, timed.size(), ta, tz)) return fail(ctx, HALIGN_E_HIP, "bucket events");
+                HCHECK(hipEventRecord(ta, st), "event");
+                timed.push_back({wave ? 0 : 2, wave ? wi : shape, L.stream, rt ? 1 : 0, L.pos, (size_t)nj, L.est});
+            }
             hipError_t e = wave ? dp_wave_launch(wi, rt, L.groups, djobs + L.pos, nj, ctr + L.bi, tb, L.slot,
                                                  douts + L.pos, st)
                                 : dp_launch(shape, stats, L.groups, djobs + L.pos, nj, ctr + L.bi, tb, L.slot,
                                             douts + L.pos, st);
             HCHECK_CAP(e, "launch dp");
+            if (tz) HCHECK(hipEventRecord(tz, st), "event");
         }
         for (int q = 1; q < kDpStreams && !ls.empty(); ++q) {
             if (!used[q]) continue;
@@ -561,6 +582,14 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     HCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event time");
     if (kernel_ms) *kernel_ms = ms;
     for (size_t k = 0; k < nj; ++k) outs[order[k]] = tmp[k];
+    for (size_t k = 0; k < timed.size(); ++k) {
+        const Timed &t = timed[k];
+        float bms = 0.f;
+        HCHECK(hipEventElapsedTime(&bms, ctx->bev[2 * k], ctx->bev[2 * k + 1]), "bucket event time");
+        halign_bucket_stat b{t.family, t.shape, t.stream, t.right, (int64_t)t.njobs, 0, bms, t.est * 1e-6};
+        for (size_t j = t.pos; j < t.pos + t.njobs; ++j) b.cells += tmp[j].cells;
+        ctx->last_buckets.push_back(b);
+    }
     for (size_t i = 0; i < n; ++i) {
         const int st = outs[i].status;
         if (st != kStOk) {
@@ -599,6 +628,8 @@ void hal::ctx_destroy(halign_ctx *ctx)
                            &ctx->m_tmp, &ctx->m_sel})
         b->release();
     for (auto &cg : ctx->gcache) (void)hipGraphExecDestroy(cg.exec);
+    for (hipEvent_t e : ctx->bev) (void)hipEventDestroy(e);
+    ctx->bev.clear();
     for (auto &b : ctx->msa_out) b.release();
     ctx->msa_tmp.release();
     for (int k = 0; k < 2; ++k) {
@@ -673,6 +704,7 @@ int halign_init(int device, halign_ctx **out)
     // 139 MB slot and the ring kernels keep ~1,024 workgroups resident, so the
     // pool may take 70% of free HBM (sequences, jobs and CIGARs use the rest).
     ctx->tb_budget = (size_t)((double)free_b * 0.70);
+    tune_load_default(ctx);
     *out = ctx;
     return HALIGN_OK;
 }
@@ -743,6 +775,47 @@ int halign_graph_counts(halign_ctx *ctx, int64_t *captures, int64_t *replays)
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (captures) *captures = ctx->graph_captures;
     if (replays) *replays = ctx->graph_replays;
+    return HALIGN_OK;
+}
+
+int halign_plan_shape(halign_ctx *ctx, int32_t n, int32_t qlen, int32_t tlen, int32_t w, int32_t *ring_shape)
+{
+    if (!ctx || !ring_shape || n <= 0 || qlen <= 0 || tlen <= 0) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *ring_shape = -1;
+    Spec s{};
+    s.qlen = qlen;
+    s.tlen = tlen;
+    s.w = w;
+    if (!fill_spec_shape(s, false, ctx->policy)) return HALIGN_OK;
+    if (s.shape < kWaveBase) return HALIGN_OK;
+    std::vector<Spec> specs((size_t)n, s);
+    std::vector<std::vector<int>> buckets(4 * kNumShapeIds);
+    for (int i = 0; i < n; ++i) buckets[(size_t)s.shape * 4].push_back(i);
+    std::vector<int> to(buckets.size());
+    if (ctx->policy == 0) plan_bucket_shapes(ctx, specs, buckets, to);
+    else to[(size_t)s.shape * 4] = s.shape;
+    *ring_shape = to[(size_t)s.shape * 4] - kWaveBase;
+    return HALIGN_OK;
+}
+
+int halign_set_bucket_timing(halign_ctx *ctx, int32_t on)
+{
+    if (!ctx) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->bucket_timing = on != 0;
+    ctx->last_buckets.clear();
+    return HALIGN_OK;
+}
+
+int halign_last_buckets(halign_ctx *ctx, halign_bucket_stat *out, int32_t cap, int32_t *n)
+{
+    if (!ctx || !n || cap < 0 || (cap > 0 && !out)) return HALIGN_E_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *n = (int32_t)ctx->last_buckets.size();
+    if (!out) return HALIGN_OK;   // size query: the records stay
+    for (int32_t k = 0; k < *n && k < cap; ++k) out[k] = ctx->last_buckets[(size_t)k];
+    ctx->last_buckets.clear();
     return HALIGN_OK;
 }
 

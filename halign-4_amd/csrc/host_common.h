@@ -128,9 +128,29 @@ struct halign_ctx {
     bool closing = false;
     // this device's address of the static scratch of halign_consensus_noalloc
     void *noalloc_scratch = nullptr;
+    // planner cost table, one entry per ring shape index (ring == 0: none)
+    std::vector<halign_tune_entry> tune;
+    bool tuning = false;   // halign_tune's launches: forced shapes run as they are
+    // per-bucket timing (halign_set_bucket_timing): an event pair per DP launch
+    bool bucket_timing = false;
+    std::vector<halign_bucket_stat> last_buckets;
+    std::vector<hipEvent_t> bev;   // created on first use, 2 per launch
 };
 
 namespace hal {
+
+// Event pair k of the bucket-timing pool (created as needed); false on failure.
+inline bool bucket_events(halign_ctx *ctx, size_t k, hipEvent_t &a, hipEvent_t &b)
+{
+    while (ctx->bev.size() < 2 * (k + 1)) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return false;
+        ctx->bev.push_back(e);
+    }
+    a = ctx->bev[2 * k];
+    b = ctx->bev[2 * k + 1];
+    return true;
+}
 
 // Registers a new center or pair set on ctx (ctx->mu held by the caller).
 inline void ctx_attach(halign_ctx *ctx) { ++ctx->children; }

@@ -431,13 +431,43 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                             ctx->cig.as<uint32_t>(), ls + kLsCnt, ls + kLsRange, ls + kLsFill, c->ljobs.as<DpJob>(),
                             c->ljseg.as<int64_t>(), nullptr, nullptr, nullptr, nullptr, side),
            "launch small-segment jobs");
+    // bucket timing: pairs from kSmallEv on of the context's event pool (the
+    // batch runner's launches take the pairs from 0)
+    constexpr size_t kSmallEv = 200;
     for (int k = 0; k < kLaneShapes; ++k)
         for (int rt = 0; rt < 2; ++rt) {
             const int bk = 2 * k + rt;
+            hipEvent_t ta = nullptr, tz = nullptr;
+            if (ctx->bucket_timing) {
+                if (!bucket_events(ctx, kSmallEv + bk, ta, tz)) return fail(ctx, HALIGN_E_HIP, "bucket events");
+                HCHECK(hipEventRecord(ta, side), "event");
+            }
             HCHECK(dp_wave_launch(lshape[k], rt != 0, lgroups[k][rt], c->ljobs.as<DpJob>(), 0, ls + kLsTicket + bk,
                                   c->ltb.as<uint8_t>(), lslot, c->louts.as<DpOut>(), side, ls + kLsRange + 2 * bk),
                    "launch small-segment DP");
+            if (tz) HCHECK(hipEventRecord(tz, side), "event");
         }
+    if (ctx->bucket_timing) {   // diagnostic mode: wait here and record the launches
+        int hls[kLsInts];
+        HCHECK(hipMemcpyAsync(hls, ls, sizeof hls, hipMemcpyDeviceToHost, side), "download ranges");
+        HCHECK(hipStreamSynchronize(side), "small-segment DPs");
+        int njobs = 0;
+        for (int bk = 0; bk < 2 * kLaneShapes; ++bk) njobs = std::max(njobs, hls[kLsRange + 2 * bk] + hls[kLsRange + 2 * bk + 1]);
+        std::vector<DpOut> lo((size_t)std::max(njobs, 1));
+        if (njobs > 0)
+            HCHECK(hipMemcpy(lo.data(), c->louts.p, (size_t)njobs * sizeof(DpOut), hipMemcpyDeviceToHost), "download");
+        for (int k = 0; k < kLaneShapes; ++k)
+            for (int rt = 0; rt < 2; ++rt) {
+                const int bk = 2 * k + rt;
+                float bms = 0.f;
+                HCHECK(hipEventElapsedTime(&bms, ctx->bev[2 * (kSmallEv + bk)], ctx->bev[2 * (kSmallEv + bk) + 1]),
+                       "bucket event time");
+                const int first = hls[kLsRange + 2 * bk], cnt = hls[kLsRange + 2 * bk + 1];
+                halign_bucket_stat b{1, lshape[k], 3, rt, cnt, 0, bms, 0.0};
+                for (int j = first; j < first + cnt; ++j) b.cells += lo[(size_t)j].cells;
+                if (cnt > 0) ctx->last_buckets.push_back(b);
+            }
+    }
     HCHECK(lane_jobs_launch(1, S, c->segs.as<SegRec>(), c->llist.as<int64_t>(), lcnt, lb, cbytes,
                             ctx->seqb.as<uint8_t>(), c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(),
                             ctx->cig.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, c->ljseg.as<int64_t>(),

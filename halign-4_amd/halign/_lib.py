@@ -69,6 +69,17 @@ def free_extz(ez):
     ez.m_cigar = ez.n_cigar = 0
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("waves", C.c_int32), ("lanes", C.c_int32), ("ring", C.c_int32), ("uniform", C.c_int32),
+                ("jobs_in_flight", C.c_int32), ("npoints", C.c_int32), ("ncol", C.c_int32 * 4),
+                ("ns_diag", C.c_double * 4), ("ns_job", C.c_double)]
+
+
+class BucketStat(C.Structure):
+    _fields_ = [("family", C.c_int32), ("shape", C.c_int32), ("stream", C.c_int32), ("right", C.c_int32),
+                ("jobs", C.c_int64), ("cells", C.c_int64), ("kernel_ms", C.c_double), ("est_ms", C.c_double)]
+
+
 class ServerMemory(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("batches", "max_batch_estimate", "sum_worker_estimates", "batch_budget",
                                          "resident", "traceback_reserved", "device_used", "working_measured")]
@@ -152,6 +163,11 @@ def _load():
     lib.halign_pairset_free.argtypes = [_P]
     lib.halign_pairset_free.restype = None
     lib.halign_set_traceback_budget.argtypes = [_P, C.c_int64]
+    lib.halign_tune.argtypes = [_P]
+    lib.halign_set_bucket_timing.argtypes = [_P, C.c_int32]
+    lib.halign_last_buckets.argtypes = [_P, C.POINTER(BucketStat), C.c_int32, C.POINTER(C.c_int32)]
+    lib.halign_tune_table.argtypes = [_P, C.POINTER(TuneEntry), C.c_int32, C.POINTER(C.c_int32)]
+    lib.halign_plan_shape.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.halign_set_graph_mode.argtypes = [_P, C.c_int32]
     lib.halign_graph_counts.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.halign_server_create.argtypes = [C.c_int, C.c_char_p, C.c_int32, C.POINTER(ServerConfig),
@@ -241,6 +257,42 @@ class Context:
         if rc != HALIGN_OK:
             msg = self.lib.halign_last_error(self.h).decode()
             raise HalignError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def tune(self):
+        """Measure every ring shape on this device (halign_tune) and plan with it."""
+        self._check(self.lib.halign_tune(self.h), "halign_tune")
+
+    def tune_table(self):
+        """The planner's cost table: a list of dicts (halign_tune_entry fields)."""
+        n = C.c_int32(0)
+        self._check(self.lib.halign_tune_table(self.h, None, 0, C.byref(n)), "halign_tune_table")
+        arr = (TuneEntry * max(1, n.value))()
+        self._check(self.lib.halign_tune_table(self.h, arr, n.value, C.byref(n)), "halign_tune_table")
+        out = []
+        for i in range(n.value):
+            e = arr[i]
+            out.append({"waves": e.waves, "lanes": e.lanes, "ring": e.ring, "uniform": e.uniform,
+                        "jobs_in_flight": e.jobs_in_flight, "ncol": list(e.ncol[:e.npoints]),
+                        "ns_diag": list(e.ns_diag[:e.npoints]), "ns_job": e.ns_job})
+        return out
+
+    def set_bucket_timing(self, on):
+        """Diagnostic mode: time every DP launch on its own (halign_set_bucket_timing)."""
+        self._check(self.lib.halign_set_bucket_timing(self.h, 1 if on else 0), "halign_set_bucket_timing")
+
+    def last_buckets(self):
+        """DP launches timed since the last call: a list of dicts (halign_bucket_stat)."""
+        n = C.c_int32(0)
+        self._check(self.lib.halign_last_buckets(self.h, None, 0, C.byref(n)), "halign_last_buckets")
+        arr = (BucketStat * max(1, n.value))()
+        self._check(self.lib.halign_last_buckets(self.h, arr, n.value, C.byref(n)), "halign_last_buckets")
+        return [{k: getattr(arr[i], k) for k, _ in BucketStat._fields_} for i in range(n.value)]
+
+    def plan_shape(self, n, qlen, tlen, w):
+        """Ring shape index the planner gives n equal jobs (-1: another kernel family)."""
+        out = C.c_int32(-1)
+        self._check(self.lib.halign_plan_shape(self.h, n, qlen, tlen, w, C.byref(out)), "halign_plan_shape")
+        return out.value
 
     def set_kernel_policy(self, policy):
         """0 = automatic, 1 = multi-wave kernels only, 2 + i = ring shape i (see ring_policy)."""

@@ -425,6 +425,57 @@ int halign_set_graph_mode(halign_ctx *ctx, int32_t on);
  * counterpart. */
 int halign_graph_counts(halign_ctx *ctx, int64_t *captures, int64_t *replays);
 
+/* Measured cost table of the DP batch planner (the autotuner). For each ring
+ * shape (halign_ring_shapes), one job's time on a full chip is about
+ * anti-diagonals x ns_diag + ns_job, with ns_diag piecewise linear in the
+ * band's 16-cell column count n_col_ between the measured bands (constant
+ * beyond the first and last); the planner (automatic kernel policy) gives
+ * each bucket of jobs the shape whose estimate max(longest job, all jobs /
+ * jobs_in_flight) is lowest, and spreads the buckets over the streams by the
+ * same estimates. The band itself stays the reference's fixed rule
+ * (include/align.h:159-172). No reference counterpart. */
+typedef struct {
+    int32_t waves, lanes, ring, uniform;  /* the shape, as halign_ring_shapes lists it */
+    int32_t jobs_in_flight;               /* jobs resident at once at full occupancy */
+    int32_t npoints;                      /* bands measured (1..4) */
+    int32_t ncol[4];                      /* the bands, in 16-cell columns, rising */
+    double ns_diag[4];                    /* ns per anti-diagonal of one job at each band */
+    double ns_job;                        /* fixed ns per job (ticket, set-up, backtrack start) */
+} halign_tune_entry;
+/* Measures every ring shape on this context's device (equal-length
+ * synthetic jobs filling the chip once, two lengths at up to four bands per
+ * shape, about a minute) and makes the result the context's planner table. A new
+ * context starts from the table measured when the library was built
+ * (halign.tune writes it). Not to be called while other calls use the
+ * context. */
+int halign_tune(halign_ctx *ctx);
+/* The context's planner table: returns HALIGN_OK, fills up to cap entries
+ * and sets *n to the number of entries. */
+int halign_tune_table(halign_ctx *ctx, halign_tune_entry *out, int32_t cap, int32_t *n);
+/* The ring shape (halign_ring_shapes index) the planner gives a batch of n
+ * equal jobs (qlen, tlen, band w; w < 0: unbanded), or -1 when the job goes
+ * to another kernel family (the multi-wave or wide-band kernels). */
+int halign_plan_shape(halign_ctx *ctx, int32_t n, int32_t qlen, int32_t tlen, int32_t w, int32_t *ring_shape);
+
+/* Per-bucket DP statistics (a diagnostic mode): with bucket timing on, every
+ * DP launch of the context (the planned buckets of the batch runner and the
+ * per-query path's small-segment launches) is bracketed by HIP events on its
+ * stream, and graph replay is skipped; halign_last_buckets returns the launches
+ * recorded since its previous call and forgets them (out NULL: only *n, the
+ * records stay). family: 0 a planned
+ * ring-kernel bucket, 1 a small-segment launch of the per-query path, 2 a
+ * multi-wave or wide-band bucket; shape: the ring shape index
+ * (halign_ring_shapes) for 0 and 1. cells: in-band cells computed. No
+ * reference counterpart. */
+typedef struct {
+    int32_t family, shape, stream, right;
+    int64_t jobs, cells;
+    double kernel_ms;   /* this launch alone, from its events */
+    double est_ms;      /* the planner's estimate (0 where it has none) */
+} halign_bucket_stat;
+int halign_set_bucket_timing(halign_ctx *ctx, int32_t on);
+int halign_last_buckets(halign_ctx *ctx, halign_bucket_stat *out, int32_t cap, int32_t *n);
+
 /* Device memory the context may take for traceback slots (default 70% of the
  * HBM free at halign_init). No reference counterpart. */
 int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes);

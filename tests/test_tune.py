@@ -1,0 +1,149 @@
+"""The DP planner's measured cost table (halign_tune, csrc/dp_tune_table.h).
+
+CPU: the table compiled into the library covers every ring shape the kernel
+has, with positive, finite costs. GPU: a fresh measurement fills the same
+table, and the shape the planner gives a batch of equal jobs is the one its
+greedy rule picks from the measured figures (restated here), for the bands
+of the headline pairwise workload (n_col_ 293), a whole pair of a short query
+against a 20 kb center, and short banded segments."""
+import math
+import os
+import re
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "..", "halign-4_amd", "csrc", "dp_tune_table.h")
+
+
+def table_rows():
+    rows = []
+    for line in open(TABLE):
+        m = re.match(r"\s*\{(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), \{([^}]*)\}, \{([^}]*)\}, ([0-9.]+)\},", line)
+        if m:
+            n = int(m.group(6))
+            rows.append({"waves": int(m.group(1)), "lanes": int(m.group(2)), "ring": int(m.group(3)),
+                         "uniform": int(m.group(4)), "jobs_in_flight": int(m.group(5)),
+                         "ncol": [int(x) for x in m.group(7).split(",")][:n],
+                         "ns_diag": [float(x) for x in m.group(8).split(",")][:n], "ns_job": float(m.group(9))})
+    return rows
+
+
+def check_rows(rows, shapes):
+    seen = {(r["waves"], r["lanes"], r["ring"], r["uniform"]) for r in rows}
+    # ring_shapes lists each sub-wave shape twice: per-group form, then uniform form
+    want = set()
+    for w, l, ring in shapes:
+        u = 1 if (w, l, ring, 0) in want else 0
+        want.add((w, l, ring, u))
+    assert want <= seen, f"shapes without a measurement: {sorted(want - seen)}"
+    for r in rows:
+        assert r["jobs_in_flight"] > 0 and 1 <= len(r["ncol"]) <= 4
+        assert r["ncol"] == sorted(r["ncol"]) and r["ncol"][-1] <= r["ring"] - 2
+        assert all(0 < t < 1e6 and math.isfinite(t) for t in r["ns_diag"]) and r["ns_job"] >= 0
+
+
+def test_builtin_table_covers_every_shape():
+    from halign import _lib
+    rows = table_rows()
+    assert rows, "no measured entries in dp_tune_table.h"
+    check_rows(rows, _lib.ring_shapes())
+
+
+def td_at(r, nc):
+    td = r["ns_diag"][0]
+    for k in range(1, len(r["ncol"])):
+        if nc <= r["ncol"][k - 1]:
+            break
+        f = min(1.0, (nc - r["ncol"][k - 1]) / (r["ncol"][k] - r["ncol"][k - 1]))
+        td = r["ns_diag"][k - 1] + f * (r["ns_diag"][k] - r["ns_diag"][k - 1])
+    return td
+
+
+def greedy_pick(table, shapes, n, qlen, tlen, w):
+    """The planner's choice for one bucket of n equal jobs: the cheapest shape
+    in chip time (tp), then, while the job's own time (crit) binds, the
+    shape after it (by tp) with the shortest crit if that lowers
+    max(crit, tp)."""
+    we = max(qlen, tlen) if w < 0 else w
+    nc = (min(qlen, tlen, we + 1) + 15) // 16 + 1
+    nd = qlen + tlen - 1
+    by_key = {(r["waves"], r["lanes"], r["ring"], r["uniform"]): r for r in table}
+    cand = []
+    seen = set()
+    for i, (wv, l, ring) in enumerate(shapes):
+        key = (wv, l, ring, 1 if (wv, l, ring) in seen else 0)
+        seen.add((wv, l, ring))
+        if key[3] or nc > ring - 2 or key not in by_key:
+            continue
+        r = by_key[key]
+        td = td_at(r, nc)
+        cand.append((i, nd * td + r["ns_job"], (n * nd * td + n * r["ns_job"]) / r["jobs_in_flight"]))
+    cand.sort(key=lambda c: (c[2], c[0]))
+    cur = 0
+    while True:
+        crit, tp = cand[cur][1], cand[cur][2]
+        nxt, best = None, crit
+        for k in range(cur + 1, len(cand)):
+            if cand[k][1] < best:
+                best, nxt = cand[k][1], k
+        if nxt is None or max(cand[nxt][1], cand[nxt][2]) >= max(crit, tp):
+            return cand[cur][0]
+        cur = nxt
+
+
+@pytest.mark.gpu
+def test_tune_measures_and_plans():
+    from halign import _lib
+    ctx = _lib.Context(0)
+    try:
+        shapes = _lib.ring_shapes()
+        builtin = ctx.tune_table()
+        check_rows(builtin, shapes)
+        print("tuning every ring shape", file=sys.stderr, flush=True)
+        ctx.tune()
+        fresh = ctx.tune_table()
+        check_rows(fresh, shapes)
+        for n, q, t, w in [(4096, 29500, 29903, 4665), (4096, 400, 20000, -1), (8192, 300, 320, 230),
+                           (2000, 1500, 1500, 100)]:
+            got = ctx.plan_shape(n, q, t, w)
+            want = greedy_pick(fresh, shapes, n, q, t, w)
+            print(f"plan {n} x ({q}, {t}, w {w}): shape {got} {shapes[got]}", file=sys.stderr, flush=True)
+            assert got == want, f"({q}, {t}, {w}): planner shape {got}, measured-best rule {want}"
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_bucket_timing_same_cigars():
+    """Per-launch DP timing (a diagnostic mode that skips graph replay and
+    brackets each launch with events) leaves every CIGAR as it is, and
+    records launches whose jobs and cells add up."""
+    import numpy as np
+    import halign
+    from seqgen import codes_to_ascii, mutate_codes, rand_codes
+    rng = np.random.default_rng(21)
+    center = rand_codes(rng, 8000)
+    qs = []
+    for i in range(300):
+        if i % 10 == 0:
+            qs.append(codes_to_ascii(rand_codes(rng, int(rng.integers(50, 400)))))   # anchor-less: whole pair
+        else:
+            a = int(rng.integers(0, 4000))
+            qs.append(codes_to_ascii(mutate_codes(rng, center[a:a + int(rng.integers(500, 4000))], 0.02, 0.004)))
+    ctx = halign.Context(0)
+    try:
+        c = halign.Center(ctx, codes_to_ascii(center))
+        base, _, _ = c.align(qs)
+        ctx.set_bucket_timing(True)
+        timed, _, _ = c.align(qs)
+        rows = ctx.last_buckets()
+        ctx.set_bucket_timing(False)
+        assert ctx.last_buckets() == []
+    finally:
+        ctx.close()
+    assert all(np.array_equal(a, b) for a, b in zip(base, timed))
+    print(f"bucket timing: {len(rows)} launches", file=sys.stderr, flush=True)
+    assert rows and all(r["jobs"] > 0 and r["cells"] > 0 and r["kernel_ms"] > 0 for r in rows)
+    assert any(r["family"] == 0 for r in rows) and any(r["family"] == 1 for r in rows)
