@@ -119,23 +119,52 @@ int pick_shape(int ncol, bool stats, int policy)
 }
 
 // Batch-level shape choice for the ring-kernel CIGAR buckets (automatic
-// policy), from the measured cost table (tune_api.hip). A bucket on shape s
-// takes about crit = its longest job's time on s, and uses tp = all its jobs'
-// time / s's jobs in flight of the chip. The buckets run side by side, so the
-// batch takes about max(max crit, sum tp): start each bucket on its cheapest
-// shape in tp, then move the bucket with the longest crit to a shape with a
-// shorter one while that lowers the estimate.
+// policy), from the measured cost table (tune_api.hip). Buckets given the same
+// shape (and gap direction) run as one launch. A launch on shape s takes about
+// max(crit, tp): crit = its longest job's time on s, tp = all its jobs' time /
+// s's jobs in flight. The launches share the main and the three side streams
+// (one hardware queue each), longest first on the least-loaded stream, and the
+// chip: the batch takes about max(stream makespan, sum of tp). Start each
+// bucket on its cheapest shape in tp, then move one bucket at a time to the
+// shape (possibly one another bucket already has: a merge) that lowers that
+// estimate most, while any move does.
 struct BucketCost {
-    int bi;
+    int bi, rt;
     double max_nd, sum_nd, n;
     int max_nc;
     std::vector<int> shapes;   // candidate ring shapes, by rising tp
-    int cur;                   // position in shapes
+    int cur;                   // the bucket's shape (a ring shape index)
 };
 
-static void cost_of(const halign_ctx *ctx, const BucketCost &b, int wi, double &crit, double &tp)
+static double plan_estimate(const halign_ctx *ctx, const std::vector<BucketCost> &bc)
 {
-    if (!tune_cost(ctx, wi, b.max_nc, b.max_nd, b.sum_nd, b.n, crit, tp)) crit = tp = 1e300;
+    struct Group { int shape, rt, max_nc; double max_nd, sum_nd, n; };
+    std::vector<Group> gs;
+    for (const auto &c : bc) {
+        Group *g = nullptr;
+        for (auto &x : gs)
+            if (x.shape == c.cur && x.rt == c.rt) g = &x;
+        if (!g) {
+            gs.push_back({c.cur, c.rt, 0, 0, 0, 0});
+            g = &gs.back();
+        }
+        g->max_nc = std::max(g->max_nc, c.max_nc);
+        g->max_nd = std::max(g->max_nd, c.max_nd);
+        g->sum_nd += c.sum_nd;
+        g->n += c.n;
+    }
+    std::vector<double> t;
+    double sum_tp = 0;
+    for (const auto &g : gs) {
+        double cr, tp;
+        if (!tune_cost(ctx, g.shape, g.max_nc, g.max_nd, g.sum_nd, g.n, cr, tp)) return 1e300;
+        t.push_back(std::max(cr, tp));
+        sum_tp += tp;
+    }
+    std::sort(t.begin(), t.end(), std::greater<double>());
+    double load[kDpStreams] = {};
+    for (double x : t) *std::min_element(load, load + kDpStreams) += x;
+    return std::max(*std::max_element(load, load + kDpStreams), sum_tp);
 }
 
 static void plan_bucket_shapes(const halign_ctx *ctx, const std::vector<Spec> &specs,
@@ -147,7 +176,7 @@ static void plan_bucket_shapes(const halign_ctx *ctx, const std::vector<Spec> &s
         const int shape = (int)bi / 4;
         shape_of_bucket[bi] = shape;
         if (b.empty() || shape < kWaveBase || (bi & 2)) continue;
-        BucketCost c{(int)bi, 0, 0, (double)b.size(), 0, {}, 0};
+        BucketCost c{(int)bi, (int)(bi & 1), 0, 0, (double)b.size(), 0, {}, 0};
         for (int i : b) {
             const Spec &s = specs[i];
             const double nd = (double)s.qlen + s.tlen - 1;
@@ -155,79 +184,65 @@ static void plan_bucket_shapes(const halign_ctx *ctx, const std::vector<Spec> &s
             c.max_nd = std::max(c.max_nd, nd);
             c.sum_nd += nd;
         }
+        std::vector<std::pair<double, int>> cand;
         for (int i = 0; i < dp_wave_num_shapes(); ++i) {
             double cr, tp;
             if (!dp_wave_shape_uniform(i) && c.max_nc <= dp_wave_shape_ring(i) - 2 &&
                 tune_cost(ctx, i, c.max_nc, c.max_nd, c.sum_nd, c.n, cr, tp))
-                c.shapes.push_back(i);
+                cand.push_back({tp, i});
         }
-        if (c.shapes.empty()) continue;
-        std::sort(c.shapes.begin(), c.shapes.end(), [&](int x, int y) {
-            double cx, tx, cy, ty;
-            cost_of(ctx, c, x, cx, tx);
-            cost_of(ctx, c, y, cy, ty);
-            return tx < ty || (tx == ty && x < y);
-        });
+        if (cand.empty()) continue;
+        std::sort(cand.begin(), cand.end());
+        for (auto &x : cand) c.shapes.push_back(x.second);
+        c.cur = c.shapes[0];
         bc.push_back(c);
     }
-    auto estimate = [&]() {
-        double mc = 0, st = 0;
-        for (auto &c : bc) {
-            double cr, tp;
-            cost_of(ctx, c, c.shapes[c.cur], cr, tp);
-            mc = std::max(mc, cr);
-            st += tp;
+    double est = plan_estimate(ctx, bc);
+    for (int guard = 0; guard < 64 && !bc.empty(); ++guard) {
+        double best = est;
+        int bb = -1, bs = -1;
+        for (size_t k = 0; k < bc.size(); ++k) {
+            const int old = bc[k].cur;
+            for (int s : bc[k].shapes) {
+                if (s == old) continue;
+                bc[k].cur = s;
+                const double e = plan_estimate(ctx, bc);
+                if (e < best * (1.0 - 1e-9)) { best = e; bb = (int)k; bs = s; }
+            }
+            bc[k].cur = old;
         }
-        return std::max(mc, st);
-    };
-    for (int guard = 0; guard < 64; ++guard) {
-        // bucket with the longest crit
-        BucketCost *w = nullptr;
-        double wc = -1;
-        for (auto &c : bc) {
-            double cr, tp;
-            cost_of(ctx, c, c.shapes[c.cur], cr, tp);
-            if (cr > wc) { wc = cr; w = &c; }
-        }
-        if (!w) break;
-        const double before = estimate();
-        // the shape (after the current one by tp) with the shortest crit
-        int nxt = -1;
-        double nc = wc;
-        for (size_t k = w->cur + 1; k < w->shapes.size(); ++k) {
-            double cr, tp;
-            cost_of(ctx, *w, w->shapes[k], cr, tp);
-            if (cr < nc) { nc = cr; nxt = (int)k; }
-        }
-        if (nxt < 0) break;
-        const int old = w->cur;
-        w->cur = nxt;
-        if (estimate() >= before) { w->cur = old; break; }
+        if (bb < 0) break;
+        bc[(size_t)bb].cur = bs;
+        est = best;
     }
     for (auto &c : bc) {
-        shape_of_bucket[c.bi] = kWaveBase + c.shapes[c.cur];
+        shape_of_bucket[c.bi] = kWaveBase + c.cur;
         if (std::getenv("HALIGN_TRACE")) {
             fprintf(stderr, "[halign plan] bucket %d: %.0f jobs, n_col <= %d, longest %.0f anti-diagonals:", c.bi,
                     c.n, c.max_nc, c.max_nd);
-            for (size_t k = 0; k < c.shapes.size(); ++k) {
+            for (int i : c.shapes) {
                 double cr, tp;
-                cost_of(ctx, c, c.shapes[k], cr, tp);
-                const int i = c.shapes[k];
-                fprintf(stderr, " %s%dx%d/%d crit %.2f tp %.2f ms", (int)k == c.cur ? "*" : "",
-                        dp_wave_shape_waves(i), dp_wave_shape_lanes(i), dp_wave_shape_ring(i), cr * 1e-6, tp * 1e-6);
+                if (!tune_cost(ctx, i, c.max_nc, c.max_nd, c.sum_nd, c.n, cr, tp)) continue;
+                fprintf(stderr, " %s%dx%d/%d crit %.2f tp %.2f ms", i == c.cur ? "*" : "", dp_wave_shape_waves(i),
+                        dp_wave_shape_lanes(i), dp_wave_shape_ring(i), cr * 1e-6, tp * 1e-6);
             }
             fprintf(stderr, "\n");
         }
     }
+    if (std::getenv("HALIGN_TRACE") && !bc.empty())
+        fprintf(stderr, "[halign plan] estimate %.2f ms\n", est * 1e-6);
 }
 
 // Uniform tickets: a sub-wave bucket whose jobs share (qlen, tlen, band) in
-// groups runs on the uniform form of its shape when padding every such key to
-// a whole number of tickets (jobs per wave) adds at most a quarter more jobs:
-// the bucket becomes its keys in falling anti-diagonal count, each padded with
-// repeats of its last job. The uniform form keeps the band bookkeeping in
-// scalars, about half the VALU work per anti-diagonal of the per-group form.
-static void form_uniform_tickets(std::vector<Spec> &specs, std::vector<std::vector<int>> &buckets, bool keep)
+// groups can run on the uniform form of its shape: the bucket becomes its keys
+// in falling anti-diagonal count, each padded with repeats of its last job to
+// a whole number of tickets (jobs per wave). The uniform form keeps the band
+// bookkeeping in scalars. The bucket takes that form when the measured table
+// puts it (padding included) below the per-group form (without table entries:
+// when padding adds at most a quarter more jobs); a uniform shape forced by
+// the kernel policy always does.
+static void form_uniform_tickets(const halign_ctx *ctx, std::vector<Spec> &specs,
+                                 std::vector<std::vector<int>> &buckets, bool keep)
 {
     const int nb = (int)buckets.size();
     std::vector<std::vector<int>> moved(nb);
@@ -255,7 +270,24 @@ static void form_uniform_tickets(std::vector<Spec> &specs, std::vector<std::vect
             for (size_t c = e - a; c % G; ++c) padded.push_back(srt[e - 1]);
             a = e;
         }
-        if (!forced && padded.size() * 4 > b.size() * 5) continue;
+        if (!forced) {
+            double max_nd = 0, sum_nd = 0, sum_ndp = 0;
+            int max_nc = 0;
+            for (int i : b) {
+                const Spec &x = specs[i];
+                max_nd = std::max(max_nd, (double)x.qlen + x.tlen - 1);
+                max_nc = std::max(max_nc, n_col(x.qlen, x.tlen, eff_w(x.w, x.qlen, x.tlen)));
+                sum_nd += (double)x.qlen + x.tlen - 1;
+            }
+            for (int i : padded) sum_ndp += (double)specs[i].qlen + specs[i].tlen - 1;
+            double cg, tg, cu, tu;
+            if (tune_cost(ctx, wi, max_nc, max_nd, sum_nd, (double)b.size(), cg, tg) &&
+                tune_cost(ctx, ui, max_nc, max_nd, sum_ndp, (double)padded.size(), cu, tu)) {
+                if (std::max(cu, tu) >= std::max(cg, tg)) continue;
+            } else if (padded.size() * 4 > b.size() * 5) {
+                continue;
+            }
+        }
         for (int i : b) specs[i].shape = kWaveBase + ui;
         auto &dst = moved[(kWaveBase + ui) * 4 + (bi & 3)];
         dst.insert(dst.end(), padded.begin(), padded.end());
@@ -301,7 +333,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         buckets.swap(moved);
     }
 
-    if (!plain) form_uniform_tickets(specs, buckets, ctx->tuning);
+    if (!plain) form_uniform_tickets(ctx, specs, buckets, ctx->tuning);
     std::vector<int> order;
     order.reserve(n);
     for (int bi = 0; bi < nb; ++bi) {

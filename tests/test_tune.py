@@ -62,10 +62,10 @@ def td_at(r, nc):
 
 
 def greedy_pick(table, shapes, n, qlen, tlen, w):
-    """The planner's choice for one bucket of n equal jobs: the cheapest shape
-    in chip time (tp), then, while the job's own time (crit) binds, the
-    shape after it (by tp) with the shortest crit if that lowers
-    max(crit, tp)."""
+    """The planner's choice for one bucket of n equal jobs: candidates in
+    rising chip time (tp); it starts on the first and moves to the shape with
+    the lowest max(crit, tp) when that is lower (crit: one job's time), the
+    first such shape in that order."""
     we = max(qlen, tlen) if w < 0 else w
     nc = (min(qlen, tlen, we + 1) + 15) // 16 + 1
     nd = qlen + tlen - 1
@@ -79,18 +79,14 @@ def greedy_pick(table, shapes, n, qlen, tlen, w):
             continue
         r = by_key[key]
         td = td_at(r, nc)
-        cand.append((i, nd * td + r["ns_job"], (n * nd * td + n * r["ns_job"]) / r["jobs_in_flight"]))
-    cand.sort(key=lambda c: (c[2], c[0]))
-    cur = 0
-    while True:
-        crit, tp = cand[cur][1], cand[cur][2]
-        nxt, best = None, crit
-        for k in range(cur + 1, len(cand)):
-            if cand[k][1] < best:
-                best, nxt = cand[k][1], k
-        if nxt is None or max(cand[nxt][1], cand[nxt][2]) >= max(crit, tp):
-            return cand[cur][0]
-        cur = nxt
+        crit = nd * td + r["ns_job"]
+        tp = (n * nd * td + n * r["ns_job"]) / r["jobs_in_flight"]
+        cand.append((tp, i, max(crit, tp)))
+    cand.sort()
+    best = min(c[2] for c in cand)
+    if best >= cand[0][2] * (1 - 1e-9):
+        return cand[0][1]
+    return next(c[1] for c in cand if c[2] <= best * (1 + 1e-12))
 
 
 @pytest.mark.gpu
