@@ -97,6 +97,10 @@ bool tune_cost(const halign_ctx *ctx, int wi, double max_nc, double max_nd, doub
     if (wi < 0 || (size_t)wi >= ctx->tune.size()) return false;
     const halign_tune_entry &e = ctx->tune[(size_t)wi];
     if (e.ring == 0 || e.jobs_in_flight <= 0 || e.npoints < 1 || e.npoints > 4) return false;
+    // no estimate below the bands measured (a quarter of the widest the ring
+    // holds): a narrower band leaves most of the ring idle in ways the table
+    // does not see
+    if (max_nc < e.ncol[0]) return false;
     double td = e.ns_diag[0];
     for (int k = 1; k < e.npoints; ++k) {
         if (max_nc <= e.ncol[k - 1]) break;

@@ -429,7 +429,8 @@ int halign_graph_counts(halign_ctx *ctx, int64_t *captures, int64_t *replays);
  * shape (halign_ring_shapes), one job's time on a full chip is about
  * anti-diagonals x ns_diag + ns_job, with ns_diag piecewise linear in the
  * band's 16-cell column count n_col_ between the measured bands (constant
- * beyond the first and last); the planner (automatic kernel policy) gives
+ * beyond the last; a shape is no candidate for bands narrower than the
+ * first); the planner (automatic kernel policy) gives
  * each bucket of jobs the shape whose estimate max(longest job, all jobs /
  * jobs_in_flight) is lowest, and spreads the buckets over the streams by the
  * same estimates. The band itself stays the reference's fixed rule

@@ -512,3 +512,34 @@ def test_oracle_matches_reference_program_on_workloads(O, cfg):
         assert bool(f["has_insertion"][k]) == bool(np.any((exp & 0xF) == 1)), k
         got = O.ref_global_align(center, q[off[j]:off[j + 1]])[0]
         assert np.array_equal(got, exp), (cfg, int(f["idx"][k]))
+
+
+def test_oracle_cigar_to_string_matches_reference_sam():
+    """or_cigar_to_string (cigarToString, src/align/cigar.cpp:82-97) against
+    the CIGAR fields of the reference program's own SAM files (sam_cases.npz):
+    every field parsed with the BAM op numbering and written back must give
+    the reference's text."""
+    import ctypes as C
+    import os
+    import numpy as np
+    from oracle_lib import Oracle, parse_cigar_string
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sam_cases.npz")
+    L = Oracle().lib
+    L.or_cigar_to_string.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+    n = 0
+    with np.load(gold, allow_pickle=False) as z:
+        for key in ("sam", "sam_ins"):
+            blob, off = z[key].tobytes(), z[key + "_off"]
+            for k in range(len(off) - 1):
+                for line in blob[off[k]:off[k + 1]].split(b"\n"):
+                    if not line or line.startswith(b"@"):
+                        continue
+                    field = line.split(b"\t")[5]
+                    if field == b"*":
+                        continue
+                    cig = parse_cigar_string(field)
+                    buf = C.create_string_buffer(len(field) + 8)
+                    got = L.or_cigar_to_string(cig.ctypes.data_as(C.c_void_p), len(cig), buf, len(buf))
+                    assert got == len(field) and buf.value == field
+                    n += 1
+    assert n > 100

@@ -75,7 +75,7 @@ def greedy_pick(table, shapes, n, qlen, tlen, w):
     for i, (wv, l, ring) in enumerate(shapes):
         key = (wv, l, ring, 1 if (wv, l, ring) in seen else 0)
         seen.add((wv, l, ring))
-        if key[3] or nc > ring - 2 or key not in by_key:
+        if key[3] or nc > ring - 2 or key not in by_key or nc < by_key[key]["ncol"][0]:
             continue
         r = by_key[key]
         td = td_at(r, nc)
