@@ -325,9 +325,28 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
         if ((bd1.st >> 4) == (bd.st >> 4)) fe_next = load_byte_uniform(qr + (qlen - 2 + (bd1.st & ~15)));
     }
 
+    // Ring bookkeeping kept across anti-diagonals: the band's first block bs
+    // only grows, by at most one block per anti-diagonal (st = max(0, r-qlen+1,
+    // (r-w+1)>>1)), so blk[k] (this lane's block in slot k, within [bs, bs+R))
+    // changes only when bs does, and only for the block that left the band: it
+    // comes back R blocks on. bsm = bs mod R, the first block's ring position.
+    int blk[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) blk[k] = NL * k + L;
+    int cur_bs = 0, bsm = 0;
+
     for (int r = 0; r < ndiag; ++r) {
         const int st0 = bd.st0, en0 = bd.en0, st = bd.st, en = bd.en;
-        const int bs = st >> 4, be = en >> 4, bsm = bs % R;
+        const int bs = st >> 4, be = en >> 4;
+        if (bs != cur_bs) {
+#pragma unroll
+            for (int k = 0; k < S; ++k) blk[k] += blk[k] < bs ? R : 0;
+            bsm = bsm + (bs - cur_bs) < R ? bsm + (bs - cur_bs) : bsm + (bs - cur_bs) - R;
+            cur_bs = bs;
+        }
+        // ring positions of the last block (be) and of the block before it
+        const int bem = bsm + (be - bs) < R ? bsm + (be - bs) : bsm + (be - bs) - R;
+        const int bem1 = bem > 0 ? bem - 1 : R - 1;
         cells += en0 - st0 + 1;
         // band-edge masks (uniform): cells < st0 of the first block, cells <= en0 of the last
         uint32_t leftm[8], lastm[8];
@@ -348,15 +367,9 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 lastm[j] = kHalfMask.m[(en0 & 15) + 1][j];
             }
         }
-        int blk[S];
         bool act[S];
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-            int d = NL * k + L - bsm;
-            if (d < 0) d += R;
-            blk[k] = bs + d;
-            act[k] = blk[k] <= be;
-        }
+        for (int k = 0; k < S; ++k) act[k] = blk[k] <= be;
 
         // neighbour values of the previous anti-diagonal: x, v of cell 15 of the left
         // block (high half of register 7) and byte 15 of its query window
@@ -426,16 +439,31 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 Q4[k][0] = __builtin_amdgcn_alignbyte(Q4[k][0], in, 3);
             }
         }
-        // left boundary of the rounded region (:118-122)
-        if (L == bsm % NL) {
+        // left boundary of the rounded region (:118-122): the first block's left
+        // neighbour x, v are those of (r-1, st-1) when that cell was computed,
+        // else 0 (v = q on the top row, st = 0 and r > 0). One wave: the lane
+        // holding the first block replaces them in that block's slot with
+        // selects; several waves: behind a branch on that lane (the form each
+        // measured faster in).
+        {
             const int ks = bsm / NL;
             const bool keep = st > 0 && st - 1 >= last_st && st - 1 <= last_en;
             const uint32_t v1 = (st == 0 && r) ? (uint32_t)kQ << 24 : 0u;
+            if constexpr (W == 1) {
+                const bool own = !keep && L == bsm - ks * NL;
 #pragma unroll
-            for (int k = 0; k < S; ++k) {
-                const uint32_t m = keep ? 0u : sel_mask(k, ks);
-                nx[k] = nx[k] & ~m;
-                nv[k] = (nv[k] & ~m) | (v1 & m);
+                for (int k = 0; k < S; ++k) {
+                    const bool mk = own && k == ks;
+                    nx[k] = mk ? 0u : nx[k];
+                    nv[k] = mk ? v1 : nv[k];
+                }
+            } else if (L == bsm - ks * NL) {
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    const uint32_t m = keep ? 0u : sel_mask(k, ks);
+                    nx[k] = nx[k] & ~m;
+                    nv[k] = (nv[k] & ~m) | (v1 & m);
+                }
             }
         }
         // top boundary y[r]=0, u[r]=q (:123)
@@ -450,9 +478,11 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
             }
         }
 
-        const int ke = ((en0 >> 4) % R) / NL, le = ((en0 >> 4) % R) % NL;
+        // en0 is in block be; cell tv = en0 - 1 in be or, at a block start, be - 1
+        const int ke = bem / NL, le = bem - ke * NL;
         const int tv = en0 > 0 ? en0 - 1 : 0;
-        const int kv = ((tv >> 4) % R) / NL, lv = ((tv >> 4) % R) % NL;
+        const int tvm = (tv >> 4) == be ? bem : bem1;
+        const int kv = tvm / NL, lv = tvm - kv * NL;
         uint32_t ue_v = 0, vl_v = 0;
 
 #pragma unroll
@@ -474,12 +504,13 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                     z[j] = __builtin_amdgcn_perm(zb[(j >> 2) + 2], zb[j >> 2],
                                                  0x040C000Cu + (uint32_t)(j & 3) * 0x01000100u);
                 // right of the band: the never-written score 0 (s+16 = 16)
-                if (b == be) {
+                // (several waves: only slot bem / NL can hold block be)
+                if ((W == 1 || k == bem / NL) && b == be) {
 #pragma unroll
                     for (int j = 0; j < 8; ++j) z[j] = bfi(lastm[j], z[j], 0x10001000u);
                 }
                 // left of the band (first block): the frozen last in-band score
-                if (b == bs) {
+                if ((W == 1 || k == bsm / NL) && b == bs) {
 #pragma unroll
                     for (int j = 0; j < 8; ++j) z[j] = bfi(leftm[j], FZ[j], z[j]);
 #pragma unroll
