@@ -494,9 +494,10 @@ __device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__r
                 // fresh s+2(q+e) per byte: table[t^q] = [21,12,12,12,16,16,16,16]
                 uint32_t zb[4];
 #pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    zb[d] = __builtin_amdgcn_perm(0x10101010u, 0x0C0C0C15u, T4[k][d] ^ Q4[k][d]);
-                    if (hasn) zb[d] -= ((T4[k][d] & Q4[k][d] & 0x04040404u) >> 2) * 5u;   // N:N scores 0
+                for (int d = 0; d < 4; ++d) zb[d] = __builtin_amdgcn_perm(0x10101010u, 0x0C0C0C15u, T4[k][d] ^ Q4[k][d]);
+                if (hasn) {   // one branch per block (N:N scores 0)
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) zb[d] -= ((T4[k][d] & Q4[k][d] & 0x04040404u) >> 2) * 5u;
                 }
                 // spread: register j <- cells j and j+8, high byte of each half
 #pragma unroll
