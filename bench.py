@@ -401,12 +401,7 @@ def dp_bucket_report(ctx, run):
     ops per cell. Buckets on different streams overlap, so their times are
     each one's own span, not shares of the step."""
     import halign
-    shapes = halign.ring_shapes()
-    seen, names = set(), []
-    for w, l, r in shapes:
-        u = (w, l, r) in seen
-        seen.add((w, l, r))
-        names.append(f"{w}x{r // l}" if w else f"sub{l}x{r // l}{'u' if u else ''}")
+    names = halign.shape_names()
     ctx.set_bucket_timing(True)
     try:
         run()

@@ -87,6 +87,26 @@ __device__ __forceinline__ uint32_t wave_ror1(uint32_t v)
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x13C, 0xF, 0xF, false);
 }
 
+// 16 bytes starting at an arbitrary byte address (aligned dword loads + v_alignbyte)
+__device__ __forceinline__ void load16(const uint8_t *p, uint32_t (&o)[4])
+{
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t *d = (const uint32_t *)(a & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(a & 3);
+    uint32_t wv[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) wv[i] = d[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = __builtin_amdgcn_alignbyte(wv[i + 1], wv[i], sh);
+}
+
+__device__ __forceinline__ uint32_t load_byte_uniform(const uint8_t *p)
+{
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t wd = *(const uint32_t *)(a & ~(uintptr_t)3);
+    return (wd >> (8 * (uint32_t)(a & 3))) & 0xffu;
+}
+
 struct Bounds { int st0, en0, st, en; };
 
 // Band of anti-diagonal r: src/align/ksw2_extz2_sse.c:107-116
@@ -113,6 +133,19 @@ __device__ __forceinline__ uint32_t pick8(const uint32_t (&a)[8], int j)
     for (int i = 0; i < 8; ++i) v |= a[i] & sel_mask(j, i);
     return v;
 }
+// cell c (0..15) of a block held as registers j = c & 7, half c >> 3 (x<<8
+// form) set to v16 in every register-half that holds it (c wave-uniform)
+__device__ __forceinline__ void set_cell(uint32_t (&a)[8], int c, uint32_t v16)
+{
+    const uint32_t hm = (c >> 3) ? 0xffff0000u : 0x0000ffffu;
+    const uint32_t vv = v16 | (v16 << 16);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t m = hm & sel_mask(c & 7, i);
+        a[i] = (a[i] & ~m) | (vv & m);
+    }
+}
+
 __device__ __forceinline__ int32_t pick16(const int32_t (&a)[16], int j)
 {
     uint32_t v = 0;

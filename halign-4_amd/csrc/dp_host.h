@@ -94,10 +94,15 @@ int dp_wave_jobs_per_group(int idx);
 int dp_wave_shape_lanes(int idx);
 int dp_wave_shape_uniform(int idx);   // 1: a sub-wave shape's uniform-ticket form
 int dp_wave_uniform_of(int idx);      // the uniform-ticket form of a sub-wave shape (else idx)
+// query-fixed shapes (dp_qfix.hip) follow the ring shapes: shape index
+// dp_wave_qfix_index(q) for q < dp_wave_num_qfix(); their "ring" is G*S blocks
+int dp_wave_num_qfix();
+int dp_wave_qfix_index(int q);
+int dp_wave_shape_qfix(int idx);
 int dp_wave_max_groups(int idx, bool rt, int num_cu);
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
                           uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream,
-                          const int *range = nullptr);
+                          const int *range = nullptr, bool prio = false);
 // index of the ring shape with these waves (0: sub-wave), lanes per job and slots; -1 if none
 inline int dp_wave_find_shape(int waves, int lanes, int slots)
 {

@@ -24,27 +24,6 @@
 
 namespace hal {
 
-
-// 16 bytes starting at an arbitrary byte address (aligned dword loads + v_alignbyte)
-__device__ __forceinline__ void load16(const uint8_t *p, uint32_t (&o)[4])
-{
-    const uintptr_t a = (uintptr_t)p;
-    const uint32_t *d = (const uint32_t *)(a & ~(uintptr_t)3);
-    const uint32_t sh = (uint32_t)(a & 3);
-    uint32_t wv[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) wv[i] = d[i];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) o[i] = __builtin_amdgcn_alignbyte(wv[i + 1], wv[i], sh);
-}
-
-__device__ __forceinline__ uint32_t load_byte_uniform(const uint8_t *p)
-{
-    const uintptr_t a = (uintptr_t)p;
-    const uint32_t wd = *(const uint32_t *)(a & ~(uintptr_t)3);
-    return (wd >> (8 * (uint32_t)(a & 3))) & 0xffu;
-}
-
 // Mask of the cells c < k of a block for each register j (cells j and j+8 in
 // the low and high halves), k = 0..16: wave-uniform band-edge masks.
 struct HalfMaskTable { uint32_t m[17][8]; };
@@ -63,17 +42,6 @@ __device__ __forceinline__ int cell_byte(const uint32_t (&a)[8], int c)
 {
     return (int)((pick8(a, c & 7) >> ((c >> 3) * 16 + 8)) & 0xffu);
 }
-__device__ __forceinline__ void set_cell(uint32_t (&a)[8], int c, uint32_t v16)
-{
-    const uint32_t hm = (c >> 3) ? 0xffff0000u : 0x0000ffffu;
-    const uint32_t vv = v16 | (v16 << 16);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint32_t m = hm & sel_mask(c & 7, i);
-        a[i] = (a[i] & ~m) | (vv & m);
-    }
-}
-
 // Byte of cell c (0..15) of slot k of lane `from`, all three wave-uniform: one
 // v_readlane of the register that holds the cell, picked by a uniform switch
 // (no per-lane select over the block's registers).
@@ -115,10 +83,10 @@ __device__ __forceinline__ uint32_t group_ror1(uint32_t v)
 // Cell layout 1 of backtrack_to_cigar; ops go to J.cigar last op first, written
 // by every lane of the group with the same value.
 template <int SUB>
-__device__ inline int group_backtrack(const DpJob &J, const uint8_t *__restrict__ tb, size_t rowb, int w,
+__device__ __forceinline__ int group_backtrack(const DpJob &J, const uint8_t *__restrict__ tb, size_t rowb, int w,
                                       bool right, int i0, int j0, uint32_t (*chunk_)[4])
 {
-    volatile uint32_t(*chunk)[4] = chunk_;
+    uint32_t(*chunk)[4] = chunk_;   // LDS reads and writes once inlined (see backtrack_to_cigar)
     const int t = threadIdx.x & (SUB - 1);
     const int qlen = J.qlen, tlen = J.tlen;
     int nout = 0, i = i0, j = j0, state = 0, rc = -1, bb0 = 0, cur_op = -1, cur_len = 0;
@@ -139,28 +107,29 @@ __device__ inline int group_backtrack(const DpJob &J, const uint8_t *__restrict_
         if (rc < 0 || r < rc - 15) {
             rc = r;
             bb0 = (i - 15 > 0 ? i - 15 : 0) >> 4;
+            // unconditional loads (row and block clamped into the band, as in
+            // backtrack_to_cigar) issued together, then the LDS writes
+            uint32_t wv[16 / SUB][4];
 #pragma unroll
             for (int m = 0; m < 16 / SUB; ++m) {
-                const int d = t + SUB * m, rr = rc - d;
-                uint32_t wv[4] = {0, 0, 0, 0};
-                Bounds br;
-                if (rr >= 0 && diag_bounds(rr, qlen, tlen, w, br)) {
-                    const int sb = br.st >> 4, eb = br.en >> 4;
-                    const uint8_t *row = tb + (size_t)rr * rowb;
+                const int rr = rc - t - SUB * m > 0 ? rc - t - SUB * m : 0;
+                Bounds br{0, 0, 0, 0};
+                diag_bounds(rr, qlen, tlen, w, br);
+                const int sb = br.st >> 4, eb = br.en >> 4;
+                const uint8_t *row = tb + (size_t)rr * rowb;
 #pragma unroll
-                    for (int q2 = 0; q2 < 2; ++q2) {
-                        const int bb = bb0 + q2;
-                        if (bb >= sb && bb <= eb) {
-                            const uint64_t v = __hip_atomic_load((const uint64_t *)(row + (size_t)(bb - sb) * 8),
-                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            wv[2 * q2] = (uint32_t)v;
-                            wv[2 * q2 + 1] = (uint32_t)(v >> 32);
-                        }
-                    }
+                for (int q2 = 0; q2 < 2; ++q2) {
+                    const int bb = min(max(bb0 + q2, sb), eb);
+                    const uint64_t v = __hip_atomic_load((const uint64_t *)(row + (size_t)(bb - sb) * 8),
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    wv[m][2 * q2] = (uint32_t)v;
+                    wv[m][2 * q2 + 1] = (uint32_t)(v >> 32);
                 }
-#pragma unroll
-                for (int q2 = 0; q2 < 4; ++q2) chunk[d][q2] = wv[q2];
             }
+#pragma unroll
+            for (int m = 0; m < 16 / SUB; ++m)
+#pragma unroll
+                for (int q2 = 0; q2 < 4; ++q2) chunk[t + SUB * m][q2] = wv[m][q2];
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }
@@ -198,6 +167,12 @@ struct WaveSmem {
     union {
         uint32_t chunk[64][10];       // backtrack staging (one job per workgroup)
         uint32_t group[16][16][4];    // backtrack staging per job (sub-wave mode)
+        // one-wave shapes of four or more slots (the widest bands: traceback
+        // slots of ~140 MB, so HBM, not the chip, caps the jobs in flight):
+        // more than 32 KB of LDS keeps at most four workgroups on a CU, one
+        // per SIMD; their registers would allow two per SIMD, and the
+        // dispatcher then stacks pairs on some SIMDs while others idle
+        uint8_t spread[(W == 1 && S >= 4) ? 33 * 1024 : 4];
     };
     uint8_t logu[2], logv[2];   // H-edge bytes of the previous anti-diagonal (W > 1)
     int job;
@@ -214,7 +189,7 @@ struct WaveSmem {
 // wave-uniform (scalar) as in the one-wave ring; only the sequences, the
 // H-row edge and the backtrack stay per group.
 template <int W, int S, bool RT, int SUB = 0, bool UNI = false>
-__device__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restrict__ out,
+__device__ __forceinline__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restrict__ out,
                             WaveSmem<W, S> &sm)
 {
     constexpr bool ROW = SUB != 0;
@@ -652,8 +627,10 @@ template <int W, int S, bool RT>
 __global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__restrict__ jobs, int njobs,
                                                              int *__restrict__ counter,
                                                              uint8_t *__restrict__ tb_pool, size_t tb_slot_bytes,
-                                                             DpOut *__restrict__ outs, const int *__restrict__ range)
+                                                             DpOut *__restrict__ outs, const int *__restrict__ range,
+                                                             int prio)
 {
+    if (prio) __builtin_amdgcn_s_setprio(3);
     if (range) {
         jobs += range[0];
         outs += range[0];
@@ -695,8 +672,9 @@ template <int SUB, int S, bool RT>
 __global__ __launch_bounds__(64) void dp_sub_persistent(const DpJob *__restrict__ jobs, int njobs,
                                                         int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
                                                         size_t tb_slot_bytes, DpOut *__restrict__ outs,
-                                                        const int *__restrict__ range)
+                                                        const int *__restrict__ range, int prio)
 {
+    if (prio) __builtin_amdgcn_s_setprio(3);
     constexpr int G = 64 / SUB;
     if (range) {
         jobs += range[0];
@@ -723,8 +701,9 @@ template <int SUB, int S, bool RT>
 __global__ __launch_bounds__(64) void dp_usub_persistent(const DpJob *__restrict__ jobs, int njobs,
                                                          int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
                                                          size_t tb_slot_bytes, DpOut *__restrict__ outs,
-                                                         const int *__restrict__ range)
+                                                         const int *__restrict__ range, int prio)
 {
+    if (prio) __builtin_amdgcn_s_setprio(3);
     constexpr int G = 64 / SUB;
     if (range) {
         jobs += range[0];
@@ -745,13 +724,19 @@ __global__ __launch_bounds__(64) void dp_usub_persistent(const DpJob *__restrict
 
 // Ring shapes: (W waves, S slots per lane) serve n_col_ <= 64*W*S - 2;
 // sub-wave shapes (SUB lanes x S slots per job, 64/SUB jobs per wave) serve
-// n_col_ <= SUB*S - 2.
+// n_col_ <= SUB*S - 2; query-fixed shapes (G lanes x S slots per job, 64/G
+// jobs per wave, dp_qfix.hip) serve full-band jobs of at most 16*G*S query
+// bases. The query-fixed shapes follow the ring shapes in the index space.
 #define HAL_WAVE_SHAPES(X) \
     X(1, 1) X(1, 2) X(2, 1) X(1, 3) X(3, 1) X(2, 2) X(4, 1) X(5, 1) X(3, 2) X(6, 1) X(4, 2) X(8, 1) X(8, 2) X(16, 1) \
     X(2, 3) X(1, 5) X(2, 4)
 #define HAL_SUB_SHAPES(X) X(16, 1) X(16, 2) X(16, 3) X(4, 1) X(4, 2)
+#define HAL_QFIX_SHAPES(X) X(1, 1) X(4, 1) X(4, 2) X(16, 1) X(16, 2) X(16, 3) X(64, 1) X(64, 2)
 
-// waves = 0: sub-wave shape with groups of sub lanes; uni: its uniform-ticket form
+const void *dp_qfix_kernel(int g, int s, bool rt);   // dp_qfix.hip
+
+// waves = 0: sub-wave shape with groups of sub lanes; uni: its uniform-ticket
+// form; waves = -1: query-fixed shape with groups of sub lanes
 struct ShapeDesc { int waves, sub, slots, uni; };
 static const ShapeDesc kShapes[] = {
 #define X(w, s) {w, 0, s, 0},
@@ -763,8 +748,17 @@ static const ShapeDesc kShapes[] = {
 #define X(g, s) {0, g, s, 1},
     HAL_SUB_SHAPES(X)
 #undef X
+#define X(g, s) {-1, g, s, 0},
+    HAL_QFIX_SHAPES(X)
+#undef X
 };
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+constexpr int kNumQfix = 0
+#define X(g, s) +1
+    HAL_QFIX_SHAPES(X)
+#undef X
+    ;
+constexpr int kNumRing = kNumShapes - kNumQfix;
 
 static const void *wave_kernel(int idx, bool rt)
 {
@@ -779,16 +773,20 @@ static const void *wave_kernel(int idx, bool rt)
         HAL_SUB_SHAPES(X)
 #undef X
     };
+    if (idx >= kNumRing) return dp_qfix_kernel(kShapes[idx].sub, kShapes[idx].slots, rt);
     return tab[idx][rt ? 1 : 0];
 }
 
-static int block_threads(int idx) { return kShapes[idx].waves ? 64 * kShapes[idx].waves : 64; }
+static int block_threads(int idx) { return kShapes[idx].waves > 0 ? 64 * kShapes[idx].waves : 64; }
 
-int dp_wave_num_shapes() { return kNumShapes; }
+int dp_wave_num_shapes() { return kNumRing; }
+int dp_wave_num_qfix() { return kNumQfix; }
+int dp_wave_qfix_index(int q) { return kNumRing + q; }
+int dp_wave_shape_qfix(int idx) { return kShapes[idx].waves < 0; }
 int dp_wave_shape_waves(int idx) { return kShapes[idx].waves; }
-int dp_wave_shape_lanes(int idx) { return kShapes[idx].waves ? 64 * kShapes[idx].waves : kShapes[idx].sub; }
+int dp_wave_shape_lanes(int idx) { return kShapes[idx].waves > 0 ? 64 * kShapes[idx].waves : kShapes[idx].sub; }
 int dp_wave_shape_ring(int idx) { return dp_wave_shape_lanes(idx) * kShapes[idx].slots; }
-int dp_wave_jobs_per_group(int idx) { return kShapes[idx].waves ? 1 : 64 / kShapes[idx].sub; }
+int dp_wave_jobs_per_group(int idx) { return kShapes[idx].waves > 0 ? 1 : 64 / kShapes[idx].sub; }
 int dp_wave_shape_uniform(int idx) { return kShapes[idx].uni; }
 // the uniform-ticket form of sub-wave shape idx (itself if it has none)
 int dp_wave_uniform_of(int idx)
@@ -810,12 +808,16 @@ int dp_wave_max_groups(int idx, bool rt, int num_cu)
 }
 
 // tb_pool holds groups * dp_wave_jobs_per_group(idx) slots of tb_slot_bytes;
-// range (device, optional) overrides jobs/outs offset and njobs
+// range (device, optional) overrides jobs/outs offset and njobs; prio: the
+// waves raise their issue priority (s_setprio), for a launch bound by its
+// jobs' anti-diagonal chains that shares the chip with other launches
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
-                          uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream, const int *range)
+                          uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream, const int *range,
+                          bool prio)
 {
+    int pr = prio ? 1 : 0;
     void *args[] = {(void *)&jobs, (void *)&njobs, (void *)&counter, (void *)&tb_pool,
-                    (void *)&tb_slot_bytes, (void *)&outs, (void *)&range};
+                    (void *)&tb_slot_bytes, (void *)&outs, (void *)&range, (void *)&pr};
     return hipLaunchKernel(wave_kernel(idx, rt), dim3(groups), dim3(block_threads(idx)), args, 0, stream);
 }
 

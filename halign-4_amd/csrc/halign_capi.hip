@@ -83,16 +83,41 @@ void bytes_query_rev(const uint8_t *codes, int len, uint8_t *out)
 }
 
 // Shape ids: [0, dp_num_shapes()) multi-wave kernels (any mode), then
-// kWaveBase + i for ring shape i (CIGAR mode).
+// kWaveBase + i for ring shape i and, after the ring shapes, query-fixed
+// shape i (CIGAR mode).
 constexpr int kWaveBase = 16;
-constexpr int kNumShapeIds = kWaveBase + 32;
+constexpr int kNumShapeIds = kWaveBase + 48;
+
+// Query-fixed shape (dp_qfix.hip) for a CIGAR-mode job, -1 if none: the band
+// is the whole matrix (w >= max(qlen, tlen) - 1), the query is the short side
+// and fits the shape. forced >= 0: only that query-fixed shape; else the
+// smallest that holds the query (the most jobs per wave).
+static int qfix_pick(int qlen, int tlen, int w, int forced)
+{
+    if (qlen <= 0 || qlen > tlen || w < std::max(qlen, tlen) - 1) return -1;
+    const int nb = (qlen + 15) / 16;
+    const int nq = dp_wave_num_qfix();
+    if (forced >= 0) return forced < nq && nb <= dp_wave_shape_ring(dp_wave_qfix_index(forced)) ? forced : -1;
+    int best = -1;
+    for (int q = 0; q < nq; ++q) {
+        const int cap = dp_wave_shape_ring(dp_wave_qfix_index(q));
+        if (nb <= cap && (best < 0 || cap < dp_wave_shape_ring(dp_wave_qfix_index(best)))) best = q;
+    }
+    return best;
+}
 constexpr int kDpStreams = 1 + halign_ctx::kAux;   // main + side streams for DP buckets
 
-int pick_shape(int ncol, bool stats, int policy)
+int pick_shape(int ncol, bool stats, int policy, int qlen, int tlen, int w)
 {
     if (policy == 2 + dp_wave_num_shapes()) return dp_num_shapes() - 1;   // the wide-band kernel only
     if (!stats && policy != 1) {
         const int nw = dp_wave_num_shapes();
+        // query-fixed shapes: automatically for the jobs they hold, or forced
+        // (policy 3 + nw + q) where the job allows
+        if (policy == 0 || policy >= 3 + nw) {
+            const int q = qfix_pick(qlen, tlen, w, policy == 0 ? -1 : policy - 3 - nw);
+            if (q >= 0) return kWaveBase + dp_wave_qfix_index(q);
+        }
         if (policy >= 2 && policy - 2 < nw && ncol <= dp_wave_shape_ring(policy - 2) - 2)
             return kWaveBase + policy - 2;
         // A job's first shape, which sorts it into a bucket: the first of this
@@ -176,20 +201,25 @@ static void plan_bucket_shapes(const halign_ctx *ctx, const std::vector<Spec> &s
         const int shape = (int)bi / 4;
         shape_of_bucket[bi] = shape;
         if (b.empty() || shape < kWaveBase || (bi & 2)) continue;
+        // a query-fixed bucket (full band, short query) chooses among the
+        // query-fixed shapes that hold its longest query (max_nc: query
+        // blocks), a ring bucket among the ring shapes that hold its band
+        const bool qf = dp_wave_shape_qfix(shape - kWaveBase) != 0;
         BucketCost c{(int)bi, (int)(bi & 1), 0, 0, (double)b.size(), 0, {}, 0};
         for (int i : b) {
             const Spec &s = specs[i];
             const double nd = (double)s.qlen + s.tlen - 1;
-            c.max_nc = std::max(c.max_nc, n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen)));
+            c.max_nc = std::max(c.max_nc, qf ? (s.qlen + 15) / 16 : n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen)));
             c.max_nd = std::max(c.max_nd, nd);
             c.sum_nd += nd;
         }
         std::vector<std::pair<double, int>> cand;
-        for (int i = 0; i < dp_wave_num_shapes(); ++i) {
+        for (int i = 0; i < dp_wave_num_shapes() + dp_wave_num_qfix(); ++i) {
             double cr, tp;
-            if (!dp_wave_shape_uniform(i) && c.max_nc <= dp_wave_shape_ring(i) - 2 &&
-                tune_cost(ctx, i, c.max_nc, c.max_nd, c.sum_nd, c.n, cr, tp))
-                cand.push_back({tp, i});
+            const bool fits = qf ? dp_wave_shape_qfix(i) && c.max_nc <= dp_wave_shape_ring(i)
+                                 : !dp_wave_shape_qfix(i) && !dp_wave_shape_uniform(i) &&
+                                       c.max_nc <= dp_wave_shape_ring(i) - 2;
+            if (fits && tune_cost(ctx, i, c.max_nc, c.max_nd, c.sum_nd, c.n, cr, tp)) cand.push_back({tp, i});
         }
         if (cand.empty()) continue;
         std::sort(cand.begin(), cand.end());
@@ -313,6 +343,8 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
 
     // bucket by (shape, stats, gap direction)
     const int ns = kNumShapeIds;
+    if (ns - kWaveBase < dp_wave_num_shapes() + dp_wave_num_qfix())
+        return fail(ctx, HALIGN_E_KERNEL, "shape id space smaller than the shape table");
     const int nb = 4 * ns;
     std::vector<std::vector<int>> buckets(nb);
     for (size_t i = 0; i < n; ++i)
@@ -326,7 +358,13 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             auto &b = buckets[bi];
             if (b.empty()) continue;
             const int to = to_shape[bi];
-            for (int i : b) specs[i].shape = to;
+            for (int i : b) {
+                specs[i].shape = to;
+                // a query-fixed shape's rows hold all its blocks
+                if (to >= kWaveBase && dp_wave_shape_qfix(to - kWaveBase))
+                    specs[i].tb_bytes = (size_t)(specs[i].qlen + specs[i].tlen - 1) *
+                                        (size_t)dp_wave_shape_ring(to - kWaveBase) * 8;
+            }
             auto &dst = moved[to * 4 + (bi & 3)];
             dst.insert(dst.end(), b.begin(), b.end());
         }
@@ -341,7 +379,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         // row shapes run several jobs in one wave: neighbours of similar length
         // (longest first) keep the rows of a wave busy for the same time
         // (uniform tickets are already ordered and padded)
-        if (bi / 4 >= kWaveBase && dp_wave_jobs_per_group(bi / 4 - kWaveBase) > 1 &&
+        if (!b.empty() && bi / 4 >= kWaveBase && dp_wave_jobs_per_group(bi / 4 - kWaveBase) > 1 &&
             !dp_wave_shape_uniform(bi / 4 - kWaveBase))
             std::stable_sort(b.begin(), b.end(), [&](int x, int y) {
                 return specs[x].qlen + specs[x].tlen > specs[y].qlen + specs[y].tlen;
@@ -418,7 +456,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         // runtime's default is four per process), longest estimate first on
         // the least loaded stream: narrow-band buckets are bound by their
         // jobs' anti-diagonal chains, not by the chip, and overlap well.
-        struct Launch { int bi; size_t pos, slot, jpg, off; int groups; double est; int stream; };
+        struct Launch { int bi; size_t pos, slot, jpg, off; int groups; double est; int stream; bool prio; };
         std::vector<Launch> ls;
         size_t pos = 0;
         for (int bi = 0; bi < nb; ++bi) {
@@ -427,7 +465,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             const int shape = bi / 4;
             const bool stats = (bi & 2) != 0;
             const bool rt = (bi & 1) != 0;
-            Launch L{bi, pos, 0, 1, 0, 0, 0.0, 0};
+            Launch L{bi, pos, 0, 1, 0, 0, 0.0, 0, false};
             for (int i : b) L.slot = std::max(L.slot, specs[i].tb_bytes);
             // rounded up to 1/32 of its power of two (at least 256 B), so that
             // batches of similar jobs share a plan (and its graph)
@@ -444,6 +482,8 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             L.groups = (int)std::min<size_t>(groups, (b.size() + L.jpg - 1) / L.jpg);
             // estimated time (ns): the measured table for ring shapes; for the
             // multi-wave kernels, cells at about 650 cells per ns over the chip
+            // (a query-fixed shape's table is by query blocks)
+            const bool qf = wave && dp_wave_shape_qfix(wi);
             double max_nd = 0, sum_nd = 0, blocks = 0;
             int max_nc = 0;
             for (int i : b) {
@@ -451,14 +491,17 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                 const double nd = (double)s.qlen + s.tlen - 1;
                 const int nc = n_col(s.qlen, s.tlen, eff_w(s.w, s.qlen, s.tlen));
                 max_nd = std::max(max_nd, nd);
-                max_nc = std::max(max_nc, nc);
+                max_nc = std::max(max_nc, qf ? (s.qlen + 15) / 16 : nc);
                 sum_nd += nd;
                 blocks += nd * nc;
             }
             double crit = 0, tp = 0;
-            L.est = wave && tune_cost(ctx, wi, max_nc, max_nd, sum_nd, (double)b.size(), crit, tp)
-                        ? std::max(crit, tp)
-                        : blocks * 16.0 / 650.0;
+            const bool tab = wave && tune_cost(ctx, wi, max_nc, max_nd, sum_nd, (double)b.size(), crit, tp);
+            L.est = tab ? std::max(crit, tp) : blocks * 16.0 / 650.0;
+            // bound by its longest job's anti-diagonal chain rather than by the
+            // chip: its waves issue first on the SIMDs they share with other
+            // launches' waves (which fill the issue slots they leave)
+            L.prio = tab && crit > tp;
             ls.push_back(L);
             pos += b.size();
         }
@@ -470,6 +513,18 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         };
         size_t min_bytes = 0;
         for (auto &L : ls) min_bytes += L.slot * L.jpg;
+        // Tickets (jpg jobs each) are taken one at a time, so n tickets on g
+        // groups run in ceil(n / g) rounds; when the traceback budget has cut
+        // g below the chip's resident count, the fewest groups with as many
+        // rounds leave no partial last round (4,096 config-2 pairs on 1,152
+        // groups of a shape that fits two per SIMD: 1,024 groups, one per
+        // SIMD, four rounds).
+        auto balance = [&](Launch &L) {
+            const size_t tickets = (buckets[L.bi].size() + L.jpg - 1) / L.jpg;
+            if (L.groups <= 0 || tickets == 0) return;
+            const size_t rounds = (tickets + (size_t)L.groups - 1) / (size_t)L.groups;
+            L.groups = (int)((tickets + rounds - 1) / rounds);
+        };
         const bool side = min_bytes <= ctx->tb_budget;   // else one launch at a time in one region
         if (side) {
             while (pool_bytes() > ctx->tb_budget) {
@@ -480,6 +535,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                 if (!big) break;
                 big->groups = std::max(1, (int)(big->groups * 0.75));
             }
+            for (auto &L : ls) balance(L);
             size_t off = 0;
             for (auto &L : ls) {
                 L.off = off;
@@ -502,6 +558,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         } else {
             for (auto &L : ls) {
                 L.groups = (int)std::min<size_t>(L.groups, std::max<size_t>(1, ctx->tb_budget / (L.slot * L.jpg)));
+                balance(L);
                 L.off = 0;
                 L.stream = 0;
             }
@@ -516,8 +573,8 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         for (auto &L : ls) {
             const int64_t v[] = {L.bi, L.groups, (int64_t)(djobs + L.pos), (int64_t)buckets[L.bi].size(),
                                  (int64_t)(ctr + L.bi), (int64_t)(ctx->tb.as<uint8_t>() + L.off), (int64_t)L.slot,
-                                 (int64_t)(douts + L.pos), L.stream};
-            sig.insert(sig.end(), v, v + 9);
+                                 (int64_t)(douts + L.pos), L.stream, L.prio};
+            sig.insert(sig.end(), v, v + 10);
         }
         const bool use_graphs = ctx->graphs && !ctx->bucket_timing;
         int hit = -1;
@@ -575,7 +632,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                 timed.push_back({wave ? 0 : 2, wave ? wi : shape, L.stream, rt ? 1 : 0, L.pos, (size_t)nj, L.est});
             }
             hipError_t e = wave ? dp_wave_launch(wi, rt, L.groups, djobs + L.pos, nj, ctr + L.bi, tb, L.slot,
-                                                 douts + L.pos, st)
+                                                 douts + L.pos, st, nullptr, L.prio)
                                 : dp_launch(shape, stats, L.groups, djobs + L.pos, nj, ctr + L.bi, tb, L.slot,
                                             douts + L.pos, st);
             HCHECK_CAP(e, "launch dp");
@@ -638,10 +695,14 @@ bool fill_spec_shape(Spec &s, bool full_stats, int policy)
     const int w = eff_w(s.w, s.qlen, s.tlen);
     const int nc = n_col(s.qlen, s.tlen, w);
     s.stats = full_stats || predict_zdrop(s.qlen, s.tlen, w);
-    s.shape = pick_shape(nc, s.stats, policy);
+    s.shape = pick_shape(nc, s.stats, policy, s.qlen, s.tlen, w);
     if (s.shape == dp_num_shapes() - 1) s.stats = true;   // the wide-band kernel computes every field
+    // query-fixed shapes keep every block of the shape in each anti-diagonal's row
+    const int rowblk = s.shape >= kWaveBase && dp_wave_shape_qfix(s.shape - kWaveBase)
+                           ? dp_wave_shape_ring(s.shape - kWaveBase)
+                           : nc;
     s.tb_bytes = s.shape >= 0 && s.shape < kWaveBase ? dp_shape_slot_bytes(s.shape, s.qlen, s.tlen, nc)
-                                                     : (size_t)(s.qlen + s.tlen - 1) * (size_t)nc * 8;
+                                                     : (size_t)(s.qlen + s.tlen - 1) * (size_t)rowblk * 8;
     return s.shape >= 0;
 }
 
@@ -793,6 +854,17 @@ int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t ca
     return n;
 }
 
+int halign_qfix_shapes(int32_t *lanes, int32_t *slots, int32_t cap)
+{
+    const int n = dp_wave_num_qfix();
+    for (int q = 0; q < n && q < cap; ++q) {
+        const int idx = dp_wave_qfix_index(q);
+        if (lanes) lanes[q] = dp_wave_shape_lanes(idx);
+        if (slots) slots[q] = dp_wave_shape_ring(idx) / dp_wave_shape_lanes(idx);
+    }
+    return n;
+}
+
 int halign_set_graph_mode(halign_ctx *ctx, int32_t on)
 {
     if (!ctx) return HALIGN_E_ARG;
@@ -861,7 +933,7 @@ int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes)
 
 int halign_set_kernel_policy(halign_ctx *ctx, int policy)
 {
-    if (!ctx || policy < 0 || policy > 2 + dp_wave_num_shapes()) return HALIGN_E_ARG;
+    if (!ctx || policy < 0 || policy > 2 + dp_wave_num_shapes() + dp_wave_num_qfix()) return HALIGN_E_ARG;
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->policy = policy;
     return HALIGN_OK;

@@ -102,6 +102,7 @@ def _load():
     lib.halign_version.restype = C.c_char_p
     lib.halign_set_kernel_policy.argtypes = [_P, C.c_int]
     lib.halign_ring_shapes.argtypes = [_P, _P, _P, C.c_int32]
+    lib.halign_qfix_shapes.argtypes = [_P, _P, C.c_int32]
     lib.halign_ksw_extz2.argtypes = [_P, _P, C.c_int, _P, C.c_int, _P, C.c_int8, _P, C.c_int8, C.c_int8,
                                      C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(ExtzT)]
     lib.halign_dp_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, _P,
@@ -208,6 +209,38 @@ def ring_shapes():
     r = np.zeros(n, np.int32)
     lib().halign_ring_shapes(_ptr(w), _ptr(g), _ptr(r), n)
     return [(int(a), int(b), int(c)) for a, b, c in zip(w, g, r)]
+
+
+def qfix_shapes():
+    """[(lanes per job, slots per lane)] of the query-fixed kernel, by index:
+    full-band jobs whose query is the short side, 16 * lanes * slots query
+    bases at most."""
+    n = lib().halign_qfix_shapes(None, None, 0)
+    g = np.zeros(n, np.int32)
+    s = np.zeros(n, np.int32)
+    lib().halign_qfix_shapes(_ptr(g), _ptr(s), n)
+    return [(int(a), int(b)) for a, b in zip(g, s)]
+
+
+def qfix_policy(lanes, slots):
+    """Kernel policy that forces the query-fixed shape (lanes, slots) on the
+    jobs it holds (the automatic choice for the others)."""
+    hits = [q for q, (g, s) in enumerate(qfix_shapes()) if g == lanes and s == slots]
+    if not hits:
+        raise ValueError(f"no query-fixed shape with {lanes} lanes and {slots} slots")
+    return 3 + len(ring_shapes()) + hits[0]
+
+
+def shape_names():
+    """Names of the ring shapes (WxS, subLxS, subLxSu for the uniform-ticket
+    form) followed by the query-fixed shapes (qLxS), by the index the planner
+    and the bucket statistics use."""
+    seen, names = set(), []
+    for w, l, r in ring_shapes():
+        u = (w, l, r) in seen
+        seen.add((w, l, r))
+        names.append(f"{w}x{r // l}" if w else f"sub{l}x{r // l}{'u' if u else ''}")
+    return names + [f"q{g}x{s}" for g, s in qfix_shapes()]
 
 
 def wide_policy():

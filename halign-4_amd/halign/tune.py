@@ -4,7 +4,8 @@ library's built-in table.
     python -m halign.tune [--header PATH] [--summary PATH]
 
 Runs halign_tune (every ring shape: equal-length synthetic jobs filling the
-chip, two bands and two lengths per shape) on device 0, prints the table and
+chip, two bands and two lengths per shape; every query-fixed shape: unbanded
+jobs whose query fills 1..S slots, two target lengths) on device 0, prints the table and
 writes it as csrc/dp_tune_table.h (rebuild the library afterwards). Run it
 again after any change to the DP kernels: the table is what the automatic
 kernel policy plans with.
@@ -23,9 +24,10 @@ DEFAULT_HEADER = os.path.join(HERE, "..", "csrc", "dp_tune_table.h")
 def header_text(rows, device_name):
     lines = [
         "// dp_tune_table.h -- planner cost table measured by halign.tune (halign_tune",
-        f"// over every ring shape) on {device_name}. Fields: waves, lanes, ring,",
-        "// uniform, jobs in flight, bands measured, the bands (16-cell columns), ns",
-        "// per anti-diagonal of one job at each band, ns per job.",
+        f"// over every ring and query-fixed shape) on {device_name}. Fields: waves",
+        "// (-1: query-fixed), lanes, ring (query-fixed: blocks), uniform, jobs in",
+        "// flight, bands measured, the bands (16-cell columns; query-fixed: query",
+        "// blocks), ns per anti-diagonal of one job at each band, ns per job.",
         "static const halign_tune_entry kTuneDefault[] = {",
     ]
     for r in rows:
@@ -41,6 +43,8 @@ def header_text(rows, device_name):
 
 def shape_name(r):
     s = r["ring"] // r["lanes"]
+    if r["waves"] < 0:
+        return f"q{r['lanes']}x{s}"
     return f"{r['waves']}x{s}" if r["waves"] else f"sub{r['lanes']}x{s}{'u' if r['uniform'] else ''}"
 
 

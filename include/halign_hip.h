@@ -68,11 +68,14 @@ void halign_free(halign_ctx *ctx);
 const char *halign_last_error(const halign_ctx *ctx);
 /* library version string */
 const char *halign_version(void);
-/* Kernel family choice: 0 = automatic (ring-form kernel where the band fits,
- * multi-wave otherwise, the wide-band kernel beyond the widest multi-wave
- * ring), 1 = multi-wave kernels only, 2 + i = ring shape i wherever the band
- * fits it (automatic otherwise), 2 + (number of ring shapes) = the wide-band
- * kernel for every job. Results are identical;
+/* Kernel family choice: 0 = automatic (a query-fixed shape for full-band
+ * jobs whose query is the short side, else the ring-form kernel where the band
+ * fits, multi-wave otherwise, the wide-band kernel beyond the widest
+ * multi-wave ring), 1 = multi-wave kernels only, 2 + i = ring shape i
+ * wherever the band fits it (automatic otherwise), 2 + R (R = number of ring
+ * shapes) = the wide-band kernel for every job, 3 + R + q = query-fixed shape
+ * q (halign_qfix_shapes) for the jobs it holds (automatic otherwise). Results
+ * are identical;
  * used to test every family and to tune the shape choice. No reference
  * counterpart (the reference has one CPU kernel). */
 int halign_set_kernel_policy(halign_ctx *ctx, int policy);
@@ -91,6 +94,13 @@ int halign_band_info(int32_t qlen, int32_t tlen, int32_t w, int32_t *ncol, int32
  * jobs of a wave share qlen, tlen and band; the library groups the jobs and
  * pads each group with repeats). No reference counterpart. */
 int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t cap);
+/* Query-fixed shapes: returns their number and fills up to cap entries of
+ * lanes per job (1, 4, 16 or 64; a wave runs 64/lanes jobs) and slots per
+ * lane. Shape q serves CIGAR-mode jobs whose band is the whole matrix (w < 0
+ * or w >= max(qlen, tlen) - 1) and whose query is the shorter side with at
+ * most 16 * lanes * slots bases: the anchor-less whole pairs and the long
+ * end segments of the per-query path. No reference counterpart. */
+int halign_qfix_shapes(int32_t *lanes, int32_t *slots, int32_t cap);
 
 /* Drop-in for ksw_extz2_sse (reference include/ksw2.h:64-65, called at
  * src/align/align.cpp:49-55): the plain one-pair-per-launch GPU form with every
@@ -454,8 +464,9 @@ int halign_tune(halign_ctx *ctx);
  * and sets *n to the number of entries. */
 int halign_tune_table(halign_ctx *ctx, halign_tune_entry *out, int32_t cap, int32_t *n);
 /* The ring shape (halign_ring_shapes index) the planner gives a batch of n
- * equal jobs (qlen, tlen, band w; w < 0: unbanded), or -1 when the job goes
- * to another kernel family (the multi-wave or wide-band kernels). */
+ * equal jobs (qlen, tlen, band w; w < 0: unbanded), R + q for query-fixed
+ * shape q (R = number of ring shapes), or -1 when the job goes to another
+ * kernel family (the multi-wave or wide-band kernels). */
 int halign_plan_shape(halign_ctx *ctx, int32_t n, int32_t qlen, int32_t tlen, int32_t w, int32_t *ring_shape);
 
 /* Per-bucket DP statistics (a diagnostic mode): with bucket timing on, every
@@ -466,7 +477,7 @@ int halign_plan_shape(halign_ctx *ctx, int32_t n, int32_t qlen, int32_t tlen, in
  * records stay). family: 0 a planned
  * ring-kernel bucket, 1 a small-segment launch of the per-query path, 2 a
  * multi-wave or wide-band bucket; shape: the ring shape index
- * (halign_ring_shapes) for 0 and 1. cells: in-band cells computed. No
+ * (halign_ring_shapes, R + q for query-fixed shape q) for 0 and 1. cells: in-band cells computed. No
  * reference counterpart. */
 typedef struct {
     int32_t family, shape, stream, right;

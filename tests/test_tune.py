@@ -40,7 +40,9 @@ def check_rows(rows, shapes):
     assert want <= seen, f"shapes without a measurement: {sorted(want - seen)}"
     for r in rows:
         assert r["jobs_in_flight"] > 0 and 1 <= len(r["ncol"]) <= 4
-        assert r["ncol"] == sorted(r["ncol"]) and r["ncol"][-1] <= r["ring"] - 2
+        # a ring shape serves n_col_ <= ring - 2, a query-fixed one (waves -1)
+        # queries of up to ring blocks
+        assert r["ncol"] == sorted(r["ncol"]) and r["ncol"][-1] <= r["ring"] - (0 if r["waves"] < 0 else 2)
         assert all(0 < t < 1e6 and math.isfinite(t) for t in r["ns_diag"]) and r["ns_job"] >= 0
 
 
@@ -101,12 +103,23 @@ def test_tune_measures_and_plans():
         ctx.tune()
         fresh = ctx.tune_table()
         check_rows(fresh, shapes)
-        for n, q, t, w in [(4096, 29500, 29903, 4665), (4096, 400, 20000, -1), (8192, 300, 320, 230),
+        for n, q, t, w in [(4096, 29500, 29903, 4665), (4096, 2000, 2100, 300), (8192, 300, 320, 230),
                            (2000, 1500, 1500, 100)]:
             got = ctx.plan_shape(n, q, t, w)
             want = greedy_pick(fresh, shapes, n, q, t, w)
             print(f"plan {n} x ({q}, {t}, w {w}): shape {got} {shapes[got]}", file=sys.stderr, flush=True)
             assert got == want, f"({q}, {t}, {w}): planner shape {got}, measured-best rule {want}"
+        # an unbanded pair whose query is the short side goes to a query-fixed
+        # shape that holds the query: the measured-best one by the same rule
+        names = _lib.shape_names()
+        qf = _lib.qfix_shapes()
+        assert len([r for r in fresh if r["waves"] < 0]) == len(qf), "query-fixed shapes without a measurement"
+        for n, q, t, w in [(4096, 400, 20000, -1), (6000, 12, 20000, -1), (100, 1024, 3000, 2999)]:
+            got = ctx.plan_shape(n, q, t, w)
+            print(f"plan {n} x ({q}, {t}, w {w}): shape {got} {names[got]}", file=sys.stderr, flush=True)
+            assert names[got].startswith("q")
+            g, s = qf[got - len(shapes)]
+            assert (q + 15) // 16 <= g * s
     finally:
         ctx.close()
 
