@@ -410,8 +410,9 @@ def dp_bucket_report(ctx, run):
         ctx.set_bucket_timing(False)
     out = []
     for b in rows:
-        fam = ("planned", "small segments", "multi-wave")[b["family"]]
-        name = names[b["shape"]] if b["family"] < 2 else f"multiwave{b['shape']}"
+        fam = ("planned", "small segments", "multi-wave", "combined")[b["family"]]
+        name = (names[b["shape"]] if b["family"] < 2 else f"multiwave{b['shape']}" if b["family"] == 2
+                else "q-parts")
         g = b["cells"] / (b["kernel_ms"] * 1e6) if b["kernel_ms"] > 0 else 0.0
         out.append({"kind": fam, "shape": name, "right": b["right"], "stream": b["stream"], "jobs": b["jobs"],
                     "cells": b["cells"], "ms": round(b["kernel_ms"], 3), "est_ms": round(b["est_ms"], 3),

@@ -50,4 +50,17 @@ struct DpOut {
     int64_t cells;   // sum over computed anti-diagonals of (en0 - st0 + 1)
 };
 
+// One part of a combined query-fixed launch: a bucket that runs beside the
+// others of the launch in one kernel (workgroups [first, first + groups) of
+// the grid serve it). shape: query-fixed shape index * 2 + gap direction
+// (1 = RIGHT).
+struct QfixPart {
+    const DpJob *jobs;
+    DpOut *outs;
+    int *counter;
+    uint8_t *tb_pool;
+    uint64_t tb_slot_bytes;
+    int32_t njobs, shape, first, pad_;
+};
+
 }  // namespace hal

@@ -103,6 +103,8 @@ int dp_wave_max_groups(int idx, bool rt, int num_cu);
 hipError_t dp_wave_launch(int idx, bool rt, int groups, const DpJob *jobs, int njobs, int *counter,
                           uint8_t *tb_pool, size_t tb_slot_bytes, DpOut *outs, hipStream_t stream,
                           const int *range = nullptr, bool prio = false);
+// several query-fixed buckets in one launch (dp_qfix.hip); parts in device memory
+hipError_t dp_qfix_combined_launch(const QfixPart *parts, int nparts, int groups, hipStream_t stream);
 // index of the ring shape with these waves (0: sub-wave), lanes per job and slots; -1 if none
 inline int dp_wave_find_shape(int waves, int lanes, int slots)
 {

@@ -393,27 +393,19 @@ __device__ __forceinline__ void qfix_job(const DpJob &J, uint8_t *__restrict__ t
     out->cells = (int64_t)qlen * tlen;   // every cell of the matrix is in band
 }
 
-// Persistent kernel: one wave per workgroup, 64/G jobs per ticket (group g of
-// G lanes takes job (64/G)*t + g; groups past the last job repeat it, writing
-// the same values to the same outputs). Workgroup b owns the 64/G traceback
-// slots from (64/G)*b on as one region with the groups' rows interleaved. All
-// jobs of one launch share the gap direction.
+// Persistent loop of one workgroup: 64/G jobs per ticket (group g of G lanes
+// takes job (64/G)*t + g; groups past the last job repeat it, writing the
+// same values to the same outputs). Workgroup wg owns the 64/G traceback slots
+// from (64/G)*wg on as one region with the groups' rows interleaved. All jobs
+// share the gap direction.
 template <int G, int S, bool RT>
-__global__ __launch_bounds__(64) void dp_qfix_persistent(const DpJob *__restrict__ jobs, int njobs,
-                                                         int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
-                                                         size_t tb_slot_bytes, DpOut *__restrict__ outs,
-                                                         const int *__restrict__ range, int prio)
+__device__ __forceinline__ void qfix_run(const DpJob *__restrict__ jobs, int njobs, int *__restrict__ counter,
+                                         uint8_t *__restrict__ tb_pool, size_t tb_slot_bytes,
+                                         DpOut *__restrict__ outs, int wg, QfixSmem<G> &sm)
 {
-    if (prio) __builtin_amdgcn_s_setprio(3);
     constexpr int NG = 64 / G;
-    if (range) {
-        jobs += range[0];
-        outs += range[0];
-        njobs = range[1];
-    }
-    __shared__ QfixSmem<G> sm;
     const int g = (int)(threadIdx.x / G);
-    uint8_t *tb = tb_pool + (size_t)blockIdx.x * NG * tb_slot_bytes + (size_t)g * (G * S * 8);
+    uint8_t *tb = tb_pool + (size_t)wg * NG * tb_slot_bytes + (size_t)g * (G * S * 8);
     for (;;) {
         // ticket as in dp_wave_persistent: all lanes run the atomic, lane 0 adds 1
         const int t = __builtin_amdgcn_readfirstlane(atomicAdd(counter, threadIdx.x == 0 ? 1 : 0));
@@ -422,6 +414,59 @@ __global__ __launch_bounds__(64) void dp_qfix_persistent(const DpJob *__restrict
         const DpJob J = jobs[jid];
         qfix_job<G, S, RT>(J, tb, &outs[jid], sm);
     }
+}
+
+template <int G, int S, bool RT>
+__global__ __launch_bounds__(64) void dp_qfix_persistent(const DpJob *__restrict__ jobs, int njobs,
+                                                         int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
+                                                         size_t tb_slot_bytes, DpOut *__restrict__ outs,
+                                                         const int *__restrict__ range, int prio)
+{
+    if (prio) __builtin_amdgcn_s_setprio(3);
+    if (range) {
+        jobs += range[0];
+        outs += range[0];
+        njobs = range[1];
+    }
+    __shared__ QfixSmem<G> sm;
+    qfix_run<G, S, RT>(jobs, njobs, counter, tb_pool, tb_slot_bytes, outs, (int)blockIdx.x, sm);
+}
+
+// Several query-fixed buckets bound by their jobs' anti-diagonal chains in one
+// launch: each workgroup serves the part whose workgroup range holds it, with
+// that part's shape, so the chains run side by side instead of one after
+// another on a stream (a process has four hardware queues). The waves raise
+// their issue priority as the chain-bound single launches do.
+__global__ __launch_bounds__(64) void dp_qfix_combined(const QfixPart *__restrict__ parts, int nparts)
+{
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ union {
+        QfixSmem<1> g1;
+        QfixSmem<4> g4;
+        QfixSmem<16> g16;
+        QfixSmem<64> g64;
+    } sm;
+    int p = 0;
+    for (int k = 1; k < nparts; ++k)
+        if ((int)blockIdx.x >= parts[k].first) p = k;
+    const QfixPart P = parts[p];
+    const int wg = (int)blockIdx.x - P.first;
+#define HAL_QP(idx, gg, ss, mem)                                                                            \
+    case 2 * (idx): qfix_run<gg, ss, false>(P.jobs, P.njobs, P.counter, P.tb_pool, P.tb_slot_bytes, P.outs, wg, sm.mem); break; \
+    case 2 * (idx) + 1: qfix_run<gg, ss, true>(P.jobs, P.njobs, P.counter, P.tb_pool, P.tb_slot_bytes, P.outs, wg, sm.mem); break;
+    switch (P.shape) {
+        HAL_QP(0, 1, 1, g1) HAL_QP(1, 4, 1, g4) HAL_QP(2, 4, 2, g4) HAL_QP(3, 16, 1, g16) HAL_QP(4, 16, 2, g16)
+        HAL_QP(5, 16, 3, g16) HAL_QP(6, 64, 1, g64) HAL_QP(7, 64, 2, g64)
+    default:
+        break;
+    }
+#undef HAL_QP
+}
+
+hipError_t dp_qfix_combined_launch(const QfixPart *parts, int nparts, int groups, hipStream_t stream)
+{
+    void *args[] = {(void *)&parts, (void *)&nparts};
+    return hipLaunchKernel((const void *)&dp_qfix_combined, dim3(groups), dim3(64), args, 0, stream);
 }
 
 const void *dp_qfix_kernel(int g, int s, bool rt)

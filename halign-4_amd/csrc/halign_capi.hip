@@ -422,6 +422,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     // launches bracketed by bucket-timing events, in event-pair order
     struct Timed { int family, shape, stream, right; size_t pos, njobs; double est; };
     std::vector<Timed> timed;
+    std::vector<std::pair<size_t, size_t>> qranges;   // job ranges of the combined query-fixed launch
     DpJob *djobs = ctx->jobs.as<DpJob>();
     DpOut *douts = ctx->outs.as<DpOut>();
     int *ctr = ctx->counters.as<int>();
@@ -457,6 +458,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         // the least loaded stream: narrow-band buckets are bound by their
         // jobs' anti-diagonal chains, not by the chip, and overlap well.
         struct Launch { int bi; size_t pos, slot, jpg, off; int groups; double est; int stream; bool prio; };
+        std::vector<QfixPart> qparts;   // parts of the combined query-fixed launch (bi = -1)
         std::vector<Launch> ls;
         size_t pos = 0;
         for (int bi = 0; bi < nb; ++bi) {
@@ -544,6 +546,41 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             clk.mark("launch plan");
             HCHECK(ctx->tb.ensure(std::max<size_t>(off, 1), ctx->tb_budget), "alloc traceback pool");
             clk.mark("traceback pool");
+            // Query-fixed launches bound by their jobs' chains run as the parts
+            // of one combined launch (dp_qfix_combined): side by side rather
+            // than one after another on the four streams. Each part keeps its
+            // jobs, counter and traceback region; the combined estimate is its
+            // longest part's.
+            if (!plain) {
+                std::vector<Launch> keep, merged;
+                for (auto &L : ls) {
+                    const int sh = L.bi / 4;
+                    if (sh >= kWaveBase && dp_wave_shape_qfix(sh - kWaveBase) && L.prio && !(L.bi & 2))
+                        merged.push_back(L);
+                    else
+                        keep.push_back(L);
+                }
+                if (merged.size() >= 2) {
+                    Launch C{-1, 0, 0, 1, 0, 0, 0.0, 0, true};
+                    int first = 0;
+                    for (auto &L : merged) {
+                        const int wi = L.bi / 4 - kWaveBase;
+                        qparts.push_back({djobs + L.pos, douts + L.pos, ctr + L.bi, ctx->tb.as<uint8_t>() + L.off,
+                                          (uint64_t)L.slot, (int32_t)buckets[L.bi].size(),
+                                          2 * (wi - dp_wave_qfix_index(0)) + (L.bi & 1), first, 0});
+                        first += L.groups;
+                        C.est = std::max(C.est, L.est);
+                    }
+                    C.groups = first;
+                    keep.push_back(C);
+                    for (auto &L : merged) qranges.push_back({L.pos, buckets[L.bi].size()});
+                    ls.swap(keep);
+                    HCHECK(ctx->qparts.ensure(qparts.size() * sizeof(QfixPart)), "alloc combined parts");
+                    HCHECK(hipMemcpyAsync(ctx->qparts.p, qparts.data(), qparts.size() * sizeof(QfixPart),
+                                          hipMemcpyHostToDevice, ctx->stream),
+                           "upload combined parts");
+                }
+            }
             std::vector<int> ord(ls.size());
             for (size_t k = 0; k < ls.size(); ++k) ord[k] = (int)k;
             std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return ls[x].est > ls[y].est; });
@@ -571,6 +608,16 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         // the plan's signature: a repeat replays the captured graph
         std::vector<int64_t> sig;
         for (auto &L : ls) {
+            if (L.bi < 0) {   // the combined launch: its parts table and where it lives
+                const int64_t v[] = {-1, L.groups, (int64_t)ctx->qparts.p, (int64_t)qparts.size(), L.stream};
+                sig.insert(sig.end(), v, v + 5);
+                for (const QfixPart &q : qparts) {
+                    const int64_t w[] = {(int64_t)q.jobs, (int64_t)q.outs, (int64_t)q.counter, (int64_t)q.tb_pool,
+                                         (int64_t)q.tb_slot_bytes, q.njobs, q.shape, q.first};
+                    sig.insert(sig.end(), w, w + 8);
+                }
+                continue;
+            }
             const int64_t v[] = {L.bi, L.groups, (int64_t)(djobs + L.pos), (int64_t)buckets[L.bi].size(),
                                  (int64_t)(ctr + L.bi), (int64_t)(ctx->tb.as<uint8_t>() + L.off), (int64_t)L.slot,
                                  (int64_t)(douts + L.pos), L.stream, L.prio};
@@ -611,6 +658,21 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         for (int q = 1; q < kDpStreams && !ls.empty(); ++q)
             if (used[q]) HCHECK_CAP(hipStreamWaitEvent(ctx->aux[q - 1], ctx->fork, 0), "stream wait");
         for (auto &L : ls) {
+            if (L.bi < 0) {   // the combined query-fixed launch
+                hipStream_t st = L.stream ? ctx->aux[L.stream - 1] : ctx->stream;
+                hipEvent_t ta = nullptr, tz = nullptr;
+                if (ctx->bucket_timing) {
+                    if (!bucket_events(ctx, timed.size(), ta, tz)) return fail(ctx, HALIGN_E_HIP, "bucket events");
+                    HCHECK(hipEventRecord(ta, st), "event");
+                    size_t nj = 0;
+                    for (const QfixPart &q : qparts) nj += (size_t)q.njobs;
+                    timed.push_back({3, -1, L.stream, -1, 0, nj, L.est});
+                }
+                HCHECK_CAP(dp_qfix_combined_launch(ctx->qparts.as<QfixPart>(), (int)qparts.size(), L.groups, st),
+                           "launch dp (combined)");
+                if (tz) HCHECK(hipEventRecord(tz, st), "event");
+                continue;
+            }
             const int shape = L.bi / 4;
             const bool stats = (L.bi & 2) != 0;
             const bool rt = (L.bi & 1) != 0;
@@ -676,7 +738,12 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         float bms = 0.f;
         HCHECK(hipEventElapsedTime(&bms, ctx->bev[2 * k], ctx->bev[2 * k + 1]), "bucket event time");
         halign_bucket_stat b{t.family, t.shape, t.stream, t.right, (int64_t)t.njobs, 0, bms, t.est * 1e-6};
-        for (size_t j = t.pos; j < t.pos + t.njobs; ++j) b.cells += tmp[j].cells;
+        if (t.family == 3) {   // the combined launch: the cells of every part
+            for (const auto &q : qranges)
+                for (size_t j = q.first; j < q.first + q.second; ++j) b.cells += tmp[j].cells;
+        } else {
+            for (size_t j = t.pos; j < t.pos + t.njobs; ++j) b.cells += tmp[j].cells;
+        }
         ctx->last_buckets.push_back(b);
     }
     for (size_t i = 0; i < n; ++i) {
@@ -713,7 +780,7 @@ void hal::ctx_destroy(halign_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     ctx->seq.release(); ctx->seqb.release(); ctx->jobs.release(); ctx->outs.release();
-    ctx->counters.release(); ctx->tb.release(); ctx->cig.release();
+    ctx->counters.release(); ctx->tb.release(); ctx->cig.release(); ctx->qparts.release();
     ctx->crows.release(); ctx->ccounts.release(); ctx->ccons.release();
     for (hal::DevBuf *b : {&ctx->s_seqs, &ctx->s_off, &ctx->s_woff, &ctx->s_moff, &ctx->s_codes, &ctx->s_nmask,
                            &ctx->s_hoff, &ctx->s_hx, &ctx->s_hy, &ctx->s_cnt, &ctx->s_st, &ctx->s_list,

@@ -90,7 +90,7 @@ struct halign_ctx {
     hipEvent_t fork = nullptr, join[kAux] = {}, pfork = nullptr, pjoin = nullptr;
     std::mutex mu;
     std::string err;
-    hal::DevBuf seq, seqb, jobs, outs, counters, tb, cig;
+    hal::DevBuf seq, seqb, jobs, outs, counters, tb, cig, qparts;
     hal::DevBuf crows, ccounts, ccons;   // consensus rows (host input), counts, characters
     // seeding: sequences, offsets, packed codes and N masks, hits, counts, status, workspace
     hal::DevBuf s_seqs, s_off, s_woff, s_moff, s_codes, s_nmask, s_hoff, s_hx, s_hy, s_cnt, s_st, s_list;

@@ -476,7 +476,9 @@ int halign_plan_shape(halign_ctx *ctx, int32_t n, int32_t qlen, int32_t tlen, in
  * recorded since its previous call and forgets them (out NULL: only *n, the
  * records stay). family: 0 a planned
  * ring-kernel bucket, 1 a small-segment launch of the per-query path, 2 a
- * multi-wave or wide-band bucket; shape: the ring shape index
+ * multi-wave or wide-band bucket, 3 a combined launch of several query-fixed
+ * buckets bound by their anti-diagonal chains (shape -1, right -1; jobs and
+ * cells of all its parts); shape: the ring shape index
  * (halign_ring_shapes, R + q for query-fixed shape q) for 0 and 1. cells: in-band cells computed. No
  * reference counterpart. */
 typedef struct {

@@ -136,3 +136,43 @@ def test_qfix_slots_reused(oracle):
     picks = list(range(0, len(cases), 37)) + [len(cases) - 1]
     sub = [cases[i] for i in picks]
     assert _compare(oracle, sub, res[picks], [cigs[i] for i in picks], False, "qfix-reuse") == 0
+
+
+def test_qfix_combined_launch_vs_oracle(oracle):
+    """Queries of 1-128 bases against 8-12 kb targets, both gap directions:
+    several query-fixed buckets bound by their anti-diagonal chains, which the
+    batch runner merges into one combined launch (bucket statistics family 3)."""
+    import halign
+    rng = np.random.default_rng(31337)
+    cases = []
+    for i in range(96):
+        tl = int(rng.integers(8000, 12001))
+        t = rand_codes(rng, tl)
+        ql = int(rng.integers(1, 17)) if i % 3 == 0 else int(rng.integers(17, 65)) if i % 3 == 1 else \
+            int(rng.integers(65, 129))
+        if i % 2:
+            s0 = int(rng.integers(0, tl - ql))
+            q = mutate_codes(rng, t[s0:s0 + ql], 0.04, 0.01)[:ql]
+            if len(q) == 0:
+                q = rand_codes(rng, 1)
+        else:
+            q = rand_codes(rng, ql)
+        cases.append((q, t, -1, FLAG_RIGHT if (i // 2) % 2 else FLAG_LEFT))
+    ctx = halign.Context(0)
+    try:
+        ctx.set_bucket_timing(True)
+        tasks, qc, tc = _tasks(cases)
+        print("qfix combined: 96 short queries against 8-12 kb targets", file=sys.stderr, flush=True)
+        res, cigs, ms = ctx.dp_batch(tasks, qc, tc, full_stats=False, plain=False)
+        rows = ctx.last_buckets()
+        ctx.set_bucket_timing(False)
+        # the same batch again: the plan's graph is captured and replayed
+        res2, cigs2, _ = ctx.dp_batch(tasks, qc, tc, full_stats=False, plain=False)
+        res3, cigs3, _ = ctx.dp_batch(tasks, qc, tc, full_stats=False, plain=False)
+    finally:
+        ctx.close()
+    comb = [b for b in rows if b["family"] == 3]
+    assert len(comb) == 1 and comb[0]["jobs"] >= 2, rows
+    assert _compare(oracle, cases, res, cigs, False, "qfix-combined") == 0
+    for r2, c2 in ((res2, cigs2), (res3, cigs3)):
+        assert np.array_equal(r2["score"], res["score"]) and all(np.array_equal(a, b) for a, b in zip(c2, cigs))
