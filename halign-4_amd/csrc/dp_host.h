@@ -36,6 +36,16 @@ inline int auto_band(int ref_len, int query_len)
 
 inline int eff_w(int w, int qlen, int tlen) { return w < 0 ? std::max(qlen, tlen) : w; }
 
+// A launch's traceback slot: its largest job's bytes rounded up to 1/32 of
+// their power of two (at least 256 B), so that batches of similar jobs share a
+// plan (and its graph)
+inline size_t round_slot(size_t bytes)
+{
+    size_t g = 256;
+    while (g * 64 <= bytes) g <<= 1;
+    return (bytes + g - 1) / g * g;
+}
+
 // ksw2's n_col_ (reference src/align/ksw2_extz2_sse.c:75-76)
 inline int n_col(int qlen, int tlen, int w)
 {
@@ -94,6 +104,9 @@ int dp_wave_jobs_per_group(int idx);
 int dp_wave_shape_lanes(int idx);
 int dp_wave_shape_uniform(int idx);   // 1: a sub-wave shape's uniform-ticket form
 int dp_wave_uniform_of(int idx);      // the uniform-ticket form of a sub-wave shape (else idx)
+int dp_wave_shape_dense(int idx);     // 1: the two-per-SIMD form of a one-wave shape
+int dp_wave_shape_form(int idx);      // 0 plain, 1 uniform-ticket form, 2 two-per-SIMD form
+int dp_wave_sparse_of(int idx);       // the one-per-SIMD form of a dense shape (else idx)
 // query-fixed shapes (dp_qfix.hip) follow the ring shapes: shape index
 // dp_wave_qfix_index(q) for q < dp_wave_num_qfix(); their "ring" is G*S blocks
 int dp_wave_num_qfix();

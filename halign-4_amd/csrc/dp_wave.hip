@@ -161,18 +161,19 @@ __device__ __forceinline__ int group_backtrack(const DpJob &J, const uint8_t *__
     return nout;
 }
 
-template <int W, int S>
+// PAD (one-wave shapes of four or more slots, the widest bands: traceback
+// slots of ~140 MB, so HBM, not the chip, caps the jobs in flight): more than
+// 32 KB of LDS keeps at most four workgroups on a CU, one per SIMD. When their
+// registers allow two per SIMD and fewer workgroups than that are resident,
+// the dispatcher stacks pairs on some SIMDs while others idle. The
+// two-per-SIMD form of these shapes (dp_wave_dense) has no pad.
+template <int W, int S, bool PAD = (W == 1 && S >= 4)>
 struct WaveSmem {
     uint4 xch[2][W][S];   // last lane of each wave: x, v of cell 15, query window dword 3 (w unused)
     union {
         uint32_t chunk[64][10];       // backtrack staging (one job per workgroup)
         uint32_t group[16][16][4];    // backtrack staging per job (sub-wave mode)
-        // one-wave shapes of four or more slots (the widest bands: traceback
-        // slots of ~140 MB, so HBM, not the chip, caps the jobs in flight):
-        // more than 32 KB of LDS keeps at most four workgroups on a CU, one
-        // per SIMD; their registers would allow two per SIMD, and the
-        // dispatcher then stacks pairs on some SIMDs while others idle
-        uint8_t spread[(W == 1 && S >= 4) ? 33 * 1024 : 4];
+        uint8_t spread[PAD ? 33 * 1024 : 4];
     };
     uint8_t logu[2], logv[2];   // H-edge bytes of the previous anti-diagonal (W > 1)
     int job;
@@ -188,9 +189,9 @@ struct WaveSmem {
 // qlen, tlen and band, so the band bounds, masks and ring positions are
 // wave-uniform (scalar) as in the one-wave ring; only the sequences, the
 // H-row edge and the backtrack stay per group.
-template <int W, int S, bool RT, int SUB = 0, bool UNI = false>
+template <int W, int S, bool RT, int SUB = 0, bool UNI = false, bool PAD>
 __device__ __forceinline__ void dp_wave_job(const DpJob &J, uint8_t *__restrict__ tb, DpOut *__restrict__ out,
-                            WaveSmem<W, S> &sm)
+                            WaveSmem<W, S, PAD> &sm)
 {
     constexpr bool ROW = SUB != 0;
     constexpr bool ROWB = ROW && !UNI;   // per-group band bounds
@@ -619,16 +620,14 @@ __device__ __forceinline__ void dp_wave_job(const DpJob &J, uint8_t *__restrict_
     }
 }
 
-// Persistent kernel: each workgroup pulls jobs from a queue and reuses its own
+// Persistent loop: each workgroup pulls jobs from a queue and reuses its own
 // traceback slot; all jobs of one launch share the gap direction.
 // range (optional, on the device): {first job, job count} within jobs/outs,
 // for job lists built on the device.
-template <int W, int S, bool RT>
-__global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__restrict__ jobs, int njobs,
-                                                             int *__restrict__ counter,
-                                                             uint8_t *__restrict__ tb_pool, size_t tb_slot_bytes,
-                                                             DpOut *__restrict__ outs, const int *__restrict__ range,
-                                                             int prio)
+template <int W, int S, bool RT, bool PAD>
+__device__ __forceinline__ void wave_persistent(const DpJob *__restrict__ jobs, int njobs, int *__restrict__ counter,
+                                                uint8_t *__restrict__ tb_pool, size_t tb_slot_bytes,
+                                                DpOut *__restrict__ outs, const int *__restrict__ range, int prio)
 {
     if (prio) __builtin_amdgcn_s_setprio(3);
     if (range) {
@@ -636,7 +635,7 @@ __global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__rest
         outs += range[0];
         njobs = range[1];
     }
-    __shared__ WaveSmem<W, S> sm;
+    __shared__ WaveSmem<W, S, PAD> sm;
     uint8_t *tb = tb_pool + (size_t)blockIdx.x * tb_slot_bytes;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
@@ -662,6 +661,32 @@ __global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__rest
         const DpJob J = jobs[jid];
         dp_wave_job<W, S, RT>(J, tb, &outs[jid], sm);
     }
+}
+
+template <int W, int S, bool RT>
+__global__ __launch_bounds__(64 * W) void dp_wave_persistent(const DpJob *__restrict__ jobs, int njobs,
+                                                             int *__restrict__ counter,
+                                                             uint8_t *__restrict__ tb_pool, size_t tb_slot_bytes,
+                                                             DpOut *__restrict__ outs, const int *__restrict__ range,
+                                                             int prio)
+{
+    wave_persistent<W, S, RT, (W == 1 && S >= 4)>(jobs, njobs, counter, tb_pool, tb_slot_bytes, outs, range, prio);
+}
+
+// Two-per-SIMD form of a one-wave shape: registers capped at 256 so that two
+// workgroups share a SIMD, and no LDS pad. One wave per SIMD issues one
+// instruction at a time, so the scalar bookkeeping and branches of an
+// anti-diagonal stall its VALU; a second wave fills those issue slots (1x5 on
+// jobs of band 4,665: 1.45x the cells per second of the one-per-SIMD form with
+// every SIMD holding two). It pays only when the traceback budget holds two
+// slots per SIMD: with fewer workgroups the dispatcher stacks pairs on some
+// SIMDs while others idle.
+template <int W, int S, bool RT>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void dp_wave_dense(
+    const DpJob *__restrict__ jobs, int njobs, int *__restrict__ counter, uint8_t *__restrict__ tb_pool,
+    size_t tb_slot_bytes, DpOut *__restrict__ outs, const int *__restrict__ range, int prio)
+{
+    wave_persistent<W, S, RT, false>(jobs, njobs, counter, tb_pool, tb_slot_bytes, outs, range, prio);
 }
 
 // Sub-wave persistent kernel: one wave, G = 64/SUB jobs per ticket (group g
@@ -727,28 +752,35 @@ __global__ __launch_bounds__(64) void dp_usub_persistent(const DpJob *__restrict
 // n_col_ <= SUB*S - 2; query-fixed shapes (G lanes x S slots per job, 64/G
 // jobs per wave, dp_qfix.hip) serve full-band jobs of at most 16*G*S query
 // bases. The query-fixed shapes follow the ring shapes in the index space.
+// Dense shapes: the two-per-SIMD form (dp_wave_dense) of a ring shape, listed
+// after the ring shapes with the same (waves, slots).
 #define HAL_WAVE_SHAPES(X) \
     X(1, 1) X(1, 2) X(2, 1) X(1, 3) X(3, 1) X(2, 2) X(4, 1) X(5, 1) X(3, 2) X(6, 1) X(4, 2) X(8, 1) X(8, 2) X(16, 1) \
     X(2, 3) X(1, 5) X(2, 4)
+#define HAL_DENSE_SHAPES(X) X(1, 5)
 #define HAL_SUB_SHAPES(X) X(16, 1) X(16, 2) X(16, 3) X(4, 1) X(4, 2)
 #define HAL_QFIX_SHAPES(X) X(1, 1) X(4, 1) X(4, 2) X(16, 1) X(16, 2) X(16, 3) X(64, 1) X(64, 2)
 
 const void *dp_qfix_kernel(int g, int s, bool rt);   // dp_qfix.hip
 
 // waves = 0: sub-wave shape with groups of sub lanes; uni: its uniform-ticket
-// form; waves = -1: query-fixed shape with groups of sub lanes
-struct ShapeDesc { int waves, sub, slots, uni; };
+// form; dense: the two-per-SIMD form of a ring shape; waves = -1: query-fixed
+// shape with groups of sub lanes
+struct ShapeDesc { int waves, sub, slots, uni, dense; };
 static const ShapeDesc kShapes[] = {
-#define X(w, s) {w, 0, s, 0},
+#define X(w, s) {w, 0, s, 0, 0},
     HAL_WAVE_SHAPES(X)
 #undef X
-#define X(g, s) {0, g, s, 0},
+#define X(w, s) {w, 0, s, 0, 1},
+    HAL_DENSE_SHAPES(X)
+#undef X
+#define X(g, s) {0, g, s, 0, 0},
     HAL_SUB_SHAPES(X)
 #undef X
-#define X(g, s) {0, g, s, 1},
+#define X(g, s) {0, g, s, 1, 0},
     HAL_SUB_SHAPES(X)
 #undef X
-#define X(g, s) {-1, g, s, 0},
+#define X(g, s) {-1, g, s, 0, 0},
     HAL_QFIX_SHAPES(X)
 #undef X
 };
@@ -765,6 +797,9 @@ static const void *wave_kernel(int idx, bool rt)
     static const void *const tab[][2] = {
 #define X(w, s) {(const void *)&dp_wave_persistent<w, s, false>, (const void *)&dp_wave_persistent<w, s, true>},
         HAL_WAVE_SHAPES(X)
+#undef X
+#define X(w, s) {(const void *)&dp_wave_dense<w, s, false>, (const void *)&dp_wave_dense<w, s, true>},
+        HAL_DENSE_SHAPES(X)
 #undef X
 #define X(g, s) {(const void *)&dp_sub_persistent<g, s, false>, (const void *)&dp_sub_persistent<g, s, true>},
         HAL_SUB_SHAPES(X)
@@ -788,6 +823,18 @@ int dp_wave_shape_lanes(int idx) { return kShapes[idx].waves > 0 ? 64 * kShapes[
 int dp_wave_shape_ring(int idx) { return dp_wave_shape_lanes(idx) * kShapes[idx].slots; }
 int dp_wave_jobs_per_group(int idx) { return kShapes[idx].waves > 0 ? 1 : 64 / kShapes[idx].sub; }
 int dp_wave_shape_uniform(int idx) { return kShapes[idx].uni; }
+int dp_wave_shape_dense(int idx) { return kShapes[idx].dense; }
+// 0 plain, 1 uniform-ticket form, 2 two-per-SIMD form
+int dp_wave_shape_form(int idx) { return kShapes[idx].uni ? 1 : kShapes[idx].dense ? 2 : 0; }
+// the one-per-SIMD form of dense shape idx (itself for any other shape)
+int dp_wave_sparse_of(int idx)
+{
+    const ShapeDesc &d = kShapes[idx];
+    if (!d.dense) return idx;
+    for (int i = 0; i < kNumRing; ++i)
+        if (kShapes[i].waves == d.waves && !kShapes[i].dense && !kShapes[i].uni && kShapes[i].slots == d.slots) return i;
+    return idx;
+}
 // the uniform-ticket form of sub-wave shape idx (itself if it has none)
 int dp_wave_uniform_of(int idx)
 {

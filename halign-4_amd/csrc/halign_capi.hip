@@ -457,7 +457,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         // runtime's default is four per process), longest estimate first on
         // the least loaded stream: narrow-band buckets are bound by their
         // jobs' anti-diagonal chains, not by the chip, and overlap well.
-        struct Launch { int bi; size_t pos, slot, jpg, off; int groups; double est; int stream; bool prio; };
+        // wi: the ring shape launched (the bucket's, or the one-per-SIMD form
+        // of a two-per-SIMD shape the budget cannot fill)
+        struct Launch { int bi; size_t pos, slot, jpg, off; int groups; double est; int stream; bool prio; int wi; };
         std::vector<QfixPart> qparts;   // parts of the combined query-fixed launch (bi = -1)
         std::vector<Launch> ls;
         size_t pos = 0;
@@ -467,15 +469,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             const int shape = bi / 4;
             const bool stats = (bi & 2) != 0;
             const bool rt = (bi & 1) != 0;
-            Launch L{bi, pos, 0, 1, 0, 0, 0.0, 0, false};
+            Launch L{bi, pos, 0, 1, 0, 0, 0.0, 0, false, shape - kWaveBase};
             for (int i : b) L.slot = std::max(L.slot, specs[i].tb_bytes);
-            // rounded up to 1/32 of its power of two (at least 256 B), so that
-            // batches of similar jobs share a plan (and its graph)
-            {
-                size_t g = 256;
-                while (g * 64 <= L.slot) g <<= 1;
-                L.slot = (L.slot + g - 1) / g * g;
-            }
+            L.slot = round_slot(L.slot);
             const bool wave = shape >= kWaveBase;
             const int wi = shape - kWaveBase;
             L.jpg = wave ? (size_t)dp_wave_jobs_per_group(wi) : 1;
@@ -521,9 +517,19 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         // rounds leave no partial last round (4,096 config-2 pairs on 1,152
         // groups of a shape that fits two per SIMD: 1,024 groups, one per
         // SIMD, four rounds).
+        // A two-per-SIMD launch keeps every SIMD's pair (its rounds are not
+        // rebalanced); under the automatic policy one the budget has cut below
+        // that runs in the one-per-SIMD form.
         auto balance = [&](Launch &L) {
             const size_t tickets = (buckets[L.bi].size() + L.jpg - 1) / L.jpg;
             if (L.groups <= 0 || tickets == 0) return;
+            const bool wv = L.bi / 4 >= kWaveBase;
+            if (wv && dp_wave_shape_dense(L.wi)) {
+                const int full = dp_wave_max_groups(L.wi, (L.bi & 1) != 0, ctx->num_cu);
+                if (ctx->policy != 0 || L.groups >= full || (size_t)L.groups >= tickets) return;
+                L.wi = dp_wave_sparse_of(L.wi);
+                L.groups = std::min(L.groups, dp_wave_max_groups(L.wi, (L.bi & 1) != 0, ctx->num_cu));
+            }
             const size_t rounds = (tickets + (size_t)L.groups - 1) / (size_t)L.groups;
             L.groups = (int)((tickets + rounds - 1) / rounds);
         };
@@ -620,8 +626,8 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             }
             const int64_t v[] = {L.bi, L.groups, (int64_t)(djobs + L.pos), (int64_t)buckets[L.bi].size(),
                                  (int64_t)(ctr + L.bi), (int64_t)(ctx->tb.as<uint8_t>() + L.off), (int64_t)L.slot,
-                                 (int64_t)(douts + L.pos), L.stream, L.prio};
-            sig.insert(sig.end(), v, v + 10);
+                                 (int64_t)(douts + L.pos), L.stream, L.prio, L.wi};
+            sig.insert(sig.end(), v, v + 11);
         }
         const bool use_graphs = ctx->graphs && !ctx->bucket_timing;
         int hit = -1;
@@ -677,10 +683,11 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             const bool stats = (L.bi & 2) != 0;
             const bool rt = (L.bi & 1) != 0;
             const bool wave = shape >= kWaveBase;
-            const int wi = shape - kWaveBase;
+            const int wi = L.wi;
             if (clk.on)
                 fprintf(stderr, "[halign run_specs] bucket: %s %d x%d lanes/job %d ring %d, %zu jobs, %d groups, stream %d, est %.3g ms\n",
-                        wave ? (dp_wave_shape_uniform(wi) ? "ring(uniform)" : "ring") : "multiwave",
+                        wave ? (dp_wave_shape_uniform(wi) ? "ring(uniform)" : dp_wave_shape_dense(wi) ? "ring(dense)" : "ring")
+                             : "multiwave",
                         wave ? dp_wave_shape_waves(wi) : shape,
                         wave ? dp_wave_jobs_per_group(wi) : 1, wave ? dp_wave_shape_lanes(wi) : 0,
                         wave ? dp_wave_shape_ring(wi) : 0, buckets[L.bi].size(), L.groups, L.stream, L.est * 1e-6);
@@ -860,10 +867,16 @@ int halign_init(int device, halign_ctx **out)
     }
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
-    // Traceback pool: one slot per resident workgroup. A config-2 pair needs a
-    // 139 MB slot and the ring kernels keep ~1,024 workgroups resident, so the
-    // pool may take 70% of free HBM (sequences, jobs and CIGARs use the rest).
-    ctx->tb_budget = (size_t)((double)free_b * 0.70);
+    // Traceback pool: one slot per resident workgroup. A pair of ~30 kb at
+    // band 4,665 needs a 140 MB slot, and the two-per-SIMD form of its shape
+    // keeps 2,048 workgroups resident (287 GB), so the pool may take all of
+    // the free HBM but a reserve of 4% (at least 8 GiB) for sequences, jobs,
+    // CIGARs and the per-query path's buffers. It is allocated as launches
+    // need it, not up front.
+    {
+        const size_t reserve = std::max<size_t>(free_b / 25, (size_t)8 << 30);
+        ctx->tb_budget = free_b > 2 * reserve ? free_b - reserve : free_b / 2;
+    }
     tune_load_default(ctx);
     *out = ctx;
     return HALIGN_OK;

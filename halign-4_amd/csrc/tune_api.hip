@@ -44,7 +44,7 @@ struct Lcg {
 bool same_shape(const halign_tune_entry &e, int wi)
 {
     return e.waves == hal::dp_wave_shape_waves(wi) && e.lanes == hal::dp_wave_shape_lanes(wi) &&
-           e.ring == hal::dp_wave_shape_ring(wi) && e.uniform == hal::dp_wave_shape_uniform(wi);
+           e.ring == hal::dp_wave_shape_ring(wi) && e.uniform == hal::dp_wave_shape_form(wi);
 }
 
 // One timed launch of n equal jobs (qlen = tlen = len, band w) forced onto ring
@@ -150,6 +150,14 @@ bool tune_cost(const halign_ctx *ctx, int wi, double max_nc, double max_nd, doub
     // holds): a narrower band leaves most of the ring idle in ways the table
     // does not see
     if (max_nc < e.ncol[0]) return false;
+    // a two-per-SIMD form is measured with every SIMD holding two workgroups:
+    // no estimate for a bucket that cannot fill them (fewer jobs, or a
+    // traceback budget without a slot for each; a slot is n_col_ x 8 bytes per
+    // anti-diagonal of the longest job, rounded as the launch plan rounds it)
+    if (dp_wave_shape_dense(wi) &&
+        (njobs < e.jobs_in_flight ||
+         (double)e.jobs_in_flight * (double)round_slot((size_t)(max_nd * max_nc * 8.0)) > (double)ctx->tb_budget))
+        return false;
     double td = e.ns_diag[0];
     for (int k = 1; k < e.npoints; ++k) {
         if (max_nc <= e.ncol[k - 1]) break;
@@ -189,7 +197,7 @@ int halign_tune(halign_ctx *ctx)
         e.waves = dp_wave_shape_waves(wi);
         e.lanes = dp_wave_shape_lanes(wi);
         e.ring = ring;
-        e.uniform = dp_wave_shape_uniform(wi);
+        e.uniform = dp_wave_shape_form(wi);
         e.jobs_in_flight = conc;
         int cols[4], np = 0;
         double t0sum = 0;
@@ -255,7 +263,7 @@ int halign_tune(halign_ctx *ctx)
         e.ns_job = t0sum / np;
         if (clk.on) {
             fprintf(stderr, "[halign tune] shape %2d waves %d lanes %3d ring %4d%s: %6d jobs in flight, ns/diag", wi,
-                    e.waves, e.lanes, ring, e.uniform ? "u" : " ", conc);
+                    e.waves, e.lanes, ring, e.uniform == 1 ? "u" : e.uniform == 2 ? "d" : " ", conc);
             for (int c = 0; c < np; ++c) fprintf(stderr, " %.1f (n_col %d)", e.ns_diag[c], e.ncol[c]);
             fprintf(stderr, ", ns/job %.0f\n", e.ns_job);
         }

@@ -232,14 +232,14 @@ def qfix_policy(lanes, slots):
 
 
 def shape_names():
-    """Names of the ring shapes (WxS, subLxS, subLxSu for the uniform-ticket
-    form) followed by the query-fixed shapes (qLxS), by the index the planner
-    and the bucket statistics use."""
+    """Names of the ring shapes (WxS, WxSd for the two-per-SIMD form, subLxS,
+    subLxSu for the uniform-ticket form) followed by the query-fixed shapes
+    (qLxS), by the index the planner and the bucket statistics use."""
     seen, names = set(), []
     for w, l, r in ring_shapes():
         u = (w, l, r) in seen
         seen.add((w, l, r))
-        names.append(f"{w}x{r // l}" if w else f"sub{l}x{r // l}{'u' if u else ''}")
+        names.append(f"{w}x{r // l}{'d' if u else ''}" if w else f"sub{l}x{r // l}{'u' if u else ''}")
     return names + [f"q{g}x{s}" for g, s in qfix_shapes()]
 
 
@@ -248,16 +248,18 @@ def wide_policy():
     return 2 + len(ring_shapes())
 
 
-def ring_policy(waves, slots, lanes=None, uniform=False):
+def ring_policy(waves, slots, lanes=None, uniform=False, dense=False):
     """Kernel policy that forces the ring shape (waves, slots per lane); waves 0
     selects the sub-wave shape with groups of `lanes` lanes (default 16), in its
-    per-group form or (uniform=True) its uniform-ticket form, listed second."""
+    per-group form or (uniform=True) its uniform-ticket form, listed second;
+    dense=True selects a ring shape's two-per-SIMD form, listed second."""
     lanes = 64 * waves if waves else (lanes or 16)
     hits = [2 + i for i, (w, g, r) in enumerate(ring_shapes()) if w == waves and g == lanes and r == lanes * slots]
-    want = 1 if (uniform and not waves) else 0
+    want = 1 if ((uniform and not waves) or (dense and waves)) else 0
     if len(hits) > want:
         return hits[want]
-    raise ValueError(f"no ring shape with {waves} waves, {lanes} lanes and {slots} slots (uniform={uniform})")
+    raise ValueError(f"no ring shape with {waves} waves, {lanes} lanes and {slots} slots "
+                     f"(uniform={uniform}, dense={dense})")
 
 
 class Context:

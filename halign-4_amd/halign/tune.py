@@ -25,9 +25,10 @@ def header_text(rows, device_name):
     lines = [
         "// dp_tune_table.h -- planner cost table measured by halign.tune (halign_tune",
         f"// over every ring and query-fixed shape) on {device_name}. Fields: waves",
-        "// (-1: query-fixed), lanes, ring (query-fixed: blocks), uniform, jobs in",
-        "// flight, bands measured, the bands (16-cell columns; query-fixed: query",
-        "// blocks), ns per anti-diagonal of one job at each band, ns per job.",
+        "// (-1: query-fixed), lanes, ring (query-fixed: blocks), form (1: uniform",
+        "// tickets, 2: two workgroups per SIMD), jobs in flight, bands measured, the",
+        "// bands (16-cell columns; query-fixed: query blocks), ns per anti-diagonal",
+        "// of one job at each band, ns per job.",
         "static const halign_tune_entry kTuneDefault[] = {",
     ]
     for r in rows:
@@ -45,7 +46,9 @@ def shape_name(r):
     s = r["ring"] // r["lanes"]
     if r["waves"] < 0:
         return f"q{r['lanes']}x{s}"
-    return f"{r['waves']}x{s}" if r["waves"] else f"sub{r['lanes']}x{s}{'u' if r['uniform'] else ''}"
+    if r["waves"]:
+        return f"{r['waves']}x{s}{'d' if r['uniform'] == 2 else ''}"
+    return f"sub{r['lanes']}x{s}{'u' if r['uniform'] else ''}"
 
 
 def table_text(rows):

@@ -92,7 +92,11 @@ int halign_band_info(int32_t qlen, int32_t tlen, int32_t w, int32_t *ncol, int32
  * shape serves bands of n_col_ <= ring - 2 blocks). Each sub-wave shape is
  * listed twice: first its per-group form, then its uniform-ticket form (all
  * jobs of a wave share qlen, tlen and band; the library groups the jobs and
- * pads each group with repeats). No reference counterpart. */
+ * pads each group with repeats). A ring shape listed a second time with the
+ * same waves, lanes and ring is its two-per-SIMD form (registers for two
+ * workgroups per SIMD; the planner takes it only for buckets that fill every
+ * SIMD with two and whose slots the traceback budget holds). No reference
+ * counterpart. */
 int halign_ring_shapes(int32_t *waves, int32_t *lanes, int32_t *ring, int32_t cap);
 /* Query-fixed shapes: returns their number and fills up to cap entries of
  * lanes per job (1, 4, 16 or 64; a wave runs 64/lanes jobs) and slots per
@@ -444,7 +448,9 @@ int halign_graph_counts(halign_ctx *ctx, int64_t *captures, int64_t *replays);
  * each bucket of jobs the shape whose estimate max(longest job, all jobs /
  * jobs_in_flight) is lowest, and spreads the buckets over the streams by the
  * same estimates. The band itself stays the reference's fixed rule
- * (include/align.h:159-172). No reference counterpart. */
+ * (include/align.h:159-172). Field uniform: 1 for the uniform-ticket form of
+ * a sub-wave shape, 2 for the two-per-SIMD form of a ring shape. No reference
+ * counterpart. */
 typedef struct {
     int32_t waves, lanes, ring, uniform;  /* the shape, as halign_ring_shapes lists it */
     int32_t jobs_in_flight;               /* jobs resident at once at full occupancy */
@@ -490,8 +496,9 @@ typedef struct {
 int halign_set_bucket_timing(halign_ctx *ctx, int32_t on);
 int halign_last_buckets(halign_ctx *ctx, halign_bucket_stat *out, int32_t cap, int32_t *n);
 
-/* Device memory the context may take for traceback slots (default 70% of the
- * HBM free at halign_init). No reference counterpart. */
+/* Device memory the context may take for traceback slots (default: the HBM
+ * free at halign_init but a reserve of 4%, at least 8 GiB; the pool is
+ * allocated as launches need it). No reference counterpart. */
 int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes);
 
 /* Continuous-batching server for RefAligner::globalAlign against one center

@@ -19,8 +19,10 @@ FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "
 
 # auto, multi-wave only, and forced ring shapes (waves, slots per lane) so that
 # small cases also run through the multi-wave LDS exchange of the ring kernel;
-# waves 0: sub-wave shapes (slots, lanes per job), forced where the band fits them
-POLICIES = ["auto", "multiwave", "wide", (2, 1), (3, 2), (5, 1), (0, 1, 16), (0, 3, 16), (0, 1, 4), (0, 2, 4)]
+# waves 0: sub-wave shapes (slots, lanes per job), forced where the band fits them;
+# dense1x5: the two-per-SIMD form of the one-wave ring of five slots
+POLICIES = ["auto", "multiwave", "wide", "dense1x5", (2, 1), (3, 2), (5, 1), (1, 5), (0, 1, 16), (0, 3, 16), (0, 1, 4),
+            (0, 2, 4)]
 
 
 @pytest.fixture(scope="module", params=POLICIES, ids=lambda p: p if isinstance(p, str) else (f"ring{p[0]}x{p[1]}" if p[0] else f"sub{p[2]}x{p[1]}"))
@@ -28,7 +30,8 @@ def ctx(request):
     import halign
     c = halign.Context(0)
     p = request.param
-    pol = {"auto": 0, "multiwave": 1, "wide": halign.wide_policy()}[p] if isinstance(p, str) else halign.ring_policy(*p)
+    named = {"auto": 0, "multiwave": 1, "wide": halign.wide_policy(), "dense1x5": halign.ring_policy(1, 5, dense=True)}
+    pol = named[p] if isinstance(p, str) else halign.ring_policy(*p)
     c.set_kernel_policy(pol)
     print(f"kernel policy {p} -> {pol}", file=sys.stderr, flush=True)
     yield c

@@ -32,10 +32,11 @@ def table_rows():
 
 def check_rows(rows, shapes):
     seen = {(r["waves"], r["lanes"], r["ring"], r["uniform"]) for r in rows}
-    # ring_shapes lists each sub-wave shape twice: per-group form, then uniform form
+    # ring_shapes lists each sub-wave shape twice (per-group form, then uniform
+    # form: 1) and a ring shape with a two-per-SIMD form twice (then form 2)
     want = set()
     for w, l, ring in shapes:
-        u = 1 if (w, l, ring, 0) in want else 0
+        u = (2 if w else 1) if (w, l, ring, 0) in want else 0
         want.add((w, l, ring, u))
     assert want <= seen, f"shapes without a measurement: {sorted(want - seen)}"
     for r in rows:
@@ -63,11 +64,21 @@ def td_at(r, nc):
     return td
 
 
-def greedy_pick(table, shapes, n, qlen, tlen, w):
+def round_slot(b):
+    g = 256
+    while g * 64 <= b:
+        g <<= 1
+    return (b + g - 1) // g * g
+
+
+def greedy_pick(table, shapes, n, qlen, tlen, w, budget):
     """The planner's choice for one bucket of n equal jobs: candidates in
     rising chip time (tp); it starts on the first and moves to the shape with
     the lowest max(crit, tp) when that is lower (crit: one job's time), the
-    first such shape in that order."""
+    first such shape in that order. Uniform-ticket forms are not candidates;
+    a two-per-SIMD form is one only for at least its jobs in flight whose
+    traceback slots (n_col_ x 8 bytes per anti-diagonal, rounded up to 1/32 of
+    their power of two) fit the budget."""
     we = max(qlen, tlen) if w < 0 else w
     nc = (min(qlen, tlen, we + 1) + 15) // 16 + 1
     nd = qlen + tlen - 1
@@ -75,11 +86,14 @@ def greedy_pick(table, shapes, n, qlen, tlen, w):
     cand = []
     seen = set()
     for i, (wv, l, ring) in enumerate(shapes):
-        key = (wv, l, ring, 1 if (wv, l, ring) in seen else 0)
+        form = (2 if wv else 1) if (wv, l, ring) in seen else 0
+        key = (wv, l, ring, form)
         seen.add((wv, l, ring))
-        if key[3] or nc > ring - 2 or key not in by_key or nc < by_key[key]["ncol"][0]:
+        if form == 1 or nc > ring - 2 or key not in by_key or nc < by_key[key]["ncol"][0]:
             continue
         r = by_key[key]
+        if form == 2 and (n < r["jobs_in_flight"] or r["jobs_in_flight"] * round_slot(nd * nc * 8) > budget):
+            continue
         td = td_at(r, nc)
         crit = nd * td + r["ns_job"]
         tp = (n * nd * td + n * r["ns_job"]) / r["jobs_in_flight"]
@@ -103,15 +117,22 @@ def test_tune_measures_and_plans():
         ctx.tune()
         fresh = ctx.tune_table()
         check_rows(fresh, shapes)
-        for n, q, t, w in [(4096, 29500, 29903, 4665), (4096, 2000, 2100, 300), (8192, 300, 320, 230),
-                           (2000, 1500, 1500, 100)]:
-            got = ctx.plan_shape(n, q, t, w)
-            want = greedy_pick(fresh, shapes, n, q, t, w)
-            print(f"plan {n} x ({q}, {t}, w {w}): shape {got} {shapes[got]}", file=sys.stderr, flush=True)
-            assert got == want, f"({q}, {t}, {w}): planner shape {got}, measured-best rule {want}"
+        names = _lib.shape_names()
+        # planning allocates nothing: budgets that hold the config-2 launch's
+        # 2,048 slots of 140 MB for the two-per-SIMD form, and that do not
+        for budget in (300e9, 150e9):
+            assert ctx.lib.halign_set_traceback_budget(ctx.h, int(budget)) == 0
+            for n, q, t, w in [(4096, 29500, 29903, 4665), (4096, 2000, 2100, 300), (8192, 300, 320, 230),
+                               (2000, 1500, 1500, 100)]:
+                got = ctx.plan_shape(n, q, t, w)
+                want = greedy_pick(fresh, shapes, n, q, t, w, budget)
+                print(f"plan {n} x ({q}, {t}, w {w}), budget {budget / 1e9:.0f} GB: shape {got} {names[got]}",
+                      file=sys.stderr, flush=True)
+                assert got == want, f"({q}, {t}, {w}): planner shape {got}, measured-best rule {want}"
+            if budget < 200e9:   # 287 GB of slots do not fit: the one-per-SIMD form
+                assert not names[ctx.plan_shape(4096, 29500, 29903, 4665)].endswith("d")
         # an unbanded pair whose query is the short side goes to a query-fixed
         # shape that holds the query: the measured-best one by the same rule
-        names = _lib.shape_names()
         qf = _lib.qfix_shapes()
         assert len([r for r in fresh if r["waves"] < 0]) == len(qf), "query-fixed shapes without a measurement"
         for n, q, t, w in [(4096, 400, 20000, -1), (6000, 12, 20000, -1), (100, 1024, 3000, 2999)]:
