@@ -195,7 +195,8 @@ def main():
     ap.add_argument("--msa-batch", type=int, default=0,
                     help="msa: queries per chunk (0 = the library default, alignQueryToRef's 25600)")
     ap.add_argument("--kernel", default="auto",
-                    help="auto | multiwave | WxS (ring kernel with W waves and S slots per lane)")
+                    help="auto | multiwave | WxS (ring kernel with W waves and S slots per lane; "
+                         "WxSd: its two-per-SIMD form)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -240,8 +241,9 @@ def main():
     elif args.kernel == "multiwave":
         policy = 1
     else:
-        wv, sl = (int(x) for x in args.kernel.lower().split("x"))
-        policy = halign.ring_policy(wv, sl)
+        k = args.kernel.lower()
+        wv, sl = (int(x) for x in k.rstrip("d").split("x"))
+        policy = halign.ring_policy(wv, sl, dense=k.endswith("d"))
     ctx.set_kernel_policy(policy)
     ps = halign.PairSet(ctx, center, qbytes, off)
 

@@ -253,8 +253,9 @@ static void plan_bucket_shapes(const halign_ctx *ctx, const std::vector<Spec> &s
             for (int i : c.shapes) {
                 double cr, tp;
                 if (!tune_cost(ctx, i, c.max_nc, c.max_nd, c.sum_nd, c.n, cr, tp)) continue;
-                fprintf(stderr, " %s%dx%d/%d crit %.2f tp %.2f ms", i == c.cur ? "*" : "", dp_wave_shape_waves(i),
-                        dp_wave_shape_lanes(i), dp_wave_shape_ring(i), cr * 1e-6, tp * 1e-6);
+                fprintf(stderr, " %s%dx%d/%d%s crit %.2f tp %.2f ms", i == c.cur ? "*" : "", dp_wave_shape_waves(i),
+                        dp_wave_shape_lanes(i), dp_wave_shape_ring(i), dp_wave_shape_dense(i) ? "d" : "", cr * 1e-6,
+                        tp * 1e-6);
             }
             fprintf(stderr, "\n");
         }
