@@ -28,6 +28,7 @@
 // Traceback: 4 bits per cell (two state-compare bits and the two gap
 // continuation bits), 8 bytes per block per anti-diagonal, rows of
 // n_col_*8 bytes (n_col_ exactly as the reference sizes its matrix).
+#include <atomic>
 #include <climits>
 
 #include "dp_device.h"
@@ -806,12 +807,18 @@ static const void *any_kernel(int idx, bool stats)
     return idx == kNumShapes ? (const void *)&dp_wide_persistent<kWideWaves> : shape_kernel(idx, stats);
 }
 
+// resident workgroups per CU of each kernel, asked once (0 = not asked yet)
 int dp_shape_max_groups(int idx, bool stats, int num_cu)
 {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, any_kernel(idx, stats), dp_shape_threads(idx), 0) !=
-        hipSuccess)
-        return 0;
+    static std::atomic<int> cache[kNumShapes + 1][2];
+    int per_cu = cache[idx][stats ? 1 : 0].load(std::memory_order_relaxed);
+    if (per_cu <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, any_kernel(idx, stats), dp_shape_threads(idx), 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            return 0;
+        cache[idx][stats ? 1 : 0].store(per_cu, std::memory_order_relaxed);
+    }
     return per_cu * num_cu;
 }
 

@@ -20,6 +20,8 @@
 //    wave) or by two LDS bytes read one diagonal later (several waves);
 //  * the gap direction (RIGHT/LEFT) is a template parameter, so the cell loop
 //    has no runtime branch and no register state merges at a join.
+#include <atomic>
+
 #include "dp_device.h"
 
 namespace hal {
@@ -845,12 +847,20 @@ int dp_wave_uniform_of(int idx)
     return idx;
 }
 
+// Resident workgroups per CU of each kernel: the occupancy query costs
+// microseconds and every planned launch asks, so each answer is kept (the
+// device is gfx950 throughout; 0 = not asked yet).
 int dp_wave_max_groups(int idx, bool rt, int num_cu)
 {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(idx, rt), block_threads(idx), 0) !=
-        hipSuccess)
-        return 0;
+    static std::atomic<int> cache[kNumShapes][2];
+    int per_cu = cache[idx][rt ? 1 : 0].load(std::memory_order_relaxed);
+    if (per_cu <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(idx, rt), block_threads(idx), 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            return 0;
+        cache[idx][rt ? 1 : 0].store(per_cu, std::memory_order_relaxed);
+    }
     return per_cu * num_cu;
 }
 

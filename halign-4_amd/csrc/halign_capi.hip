@@ -88,6 +88,46 @@ void bytes_query_rev(const uint8_t *codes, int len, uint8_t *out)
 constexpr int kWaveBase = 16;
 constexpr int kNumShapeIds = kWaveBase + 48;
 
+// Shape-table lookups the per-job shape choice makes, computed once: the
+// query-fixed shapes' capacities (query blocks) and the ring shape of each
+// entry of the automatic first-shape list (-1 where the table has none).
+namespace {
+// A job's first shape, which sorts it into a bucket: the first of this list,
+// {waves, lanes per job, slots}, that holds the band. With the automatic
+// policy run_specs then gives each bucket the shape the measured cost table
+// prefers (plan_bucket_shapes).
+constexpr int kAuto[][3] = {{0, 4, 1},  {0, 4, 2},  {0, 16, 1},  {0, 16, 2}, {0, 16, 3}, {1, 64, 1},
+                            {1, 64, 2}, {2, 128, 2}, {3, 192, 2}, {4, 256, 2}, {8, 512, 2}};
+constexpr int kNumAuto = (int)(sizeof(kAuto) / sizeof(kAuto[0]));
+struct ShapeLookup {
+    int nq = 0, nw = 0;
+    int qcap[32] = {};       // query-fixed shape q holds queries of at most qcap[q] blocks
+    int auto_idx[kNumAuto];  // ring shape of kAuto[k]
+    int auto_ring[kNumAuto];
+};
+const ShapeLookup &shape_lookup()
+{
+    static const ShapeLookup t = [] {
+        ShapeLookup x;
+        x.nw = dp_wave_num_shapes();
+        x.nq = std::min(dp_wave_num_qfix(), 32);
+        for (int q = 0; q < x.nq; ++q) x.qcap[q] = dp_wave_shape_ring(dp_wave_qfix_index(q));
+        for (int k = 0; k < kNumAuto; ++k) {
+            x.auto_idx[k] = -1;
+            x.auto_ring[k] = 0;
+            for (int i = 0; i < x.nw && x.auto_idx[k] < 0; ++i)
+                if (dp_wave_shape_waves(i) == kAuto[k][0] && dp_wave_shape_lanes(i) == kAuto[k][1] &&
+                    dp_wave_shape_ring(i) == kAuto[k][1] * kAuto[k][2]) {
+                    x.auto_idx[k] = i;
+                    x.auto_ring[k] = dp_wave_shape_ring(i);
+                }
+        }
+        return x;
+    }();
+    return t;
+}
+}  // namespace
+
 // Query-fixed shape (dp_qfix.hip) for a CIGAR-mode job, -1 if none: the band
 // is the whole matrix (w >= max(qlen, tlen) - 1), the query is the short side
 // and fits the shape. forced >= 0: only that query-fixed shape; else the
@@ -96,22 +136,21 @@ static int qfix_pick(int qlen, int tlen, int w, int forced)
 {
     if (qlen <= 0 || qlen > tlen || w < std::max(qlen, tlen) - 1) return -1;
     const int nb = (qlen + 15) / 16;
-    const int nq = dp_wave_num_qfix();
-    if (forced >= 0) return forced < nq && nb <= dp_wave_shape_ring(dp_wave_qfix_index(forced)) ? forced : -1;
+    const ShapeLookup &t = shape_lookup();
+    if (forced >= 0) return forced < t.nq && nb <= t.qcap[forced] ? forced : -1;
     int best = -1;
-    for (int q = 0; q < nq; ++q) {
-        const int cap = dp_wave_shape_ring(dp_wave_qfix_index(q));
-        if (nb <= cap && (best < 0 || cap < dp_wave_shape_ring(dp_wave_qfix_index(best)))) best = q;
-    }
+    for (int q = 0; q < t.nq; ++q)
+        if (nb <= t.qcap[q] && (best < 0 || t.qcap[q] < t.qcap[best])) best = q;
     return best;
 }
 constexpr int kDpStreams = 1 + halign_ctx::kAux;   // main + side streams for DP buckets
 
 int pick_shape(int ncol, bool stats, int policy, int qlen, int tlen, int w)
 {
-    if (policy == 2 + dp_wave_num_shapes()) return dp_num_shapes() - 1;   // the wide-band kernel only
+    const ShapeLookup &t = shape_lookup();
+    const int nw = t.nw;
+    if (policy == 2 + nw) return dp_num_shapes() - 1;   // the wide-band kernel only
     if (!stats && policy != 1) {
-        const int nw = dp_wave_num_shapes();
         // query-fixed shapes: automatically for the jobs they hold, or forced
         // (policy 3 + nw + q) where the job allows
         if (policy == 0 || policy >= 3 + nw) {
@@ -120,22 +159,8 @@ int pick_shape(int ncol, bool stats, int policy, int qlen, int tlen, int w)
         }
         if (policy >= 2 && policy - 2 < nw && ncol <= dp_wave_shape_ring(policy - 2) - 2)
             return kWaveBase + policy - 2;
-        // A job's first shape, which sorts it into a bucket: the first of this
-        // list, {waves, lanes per job, slots}, that holds the band. With the
-        // automatic policy run_specs then gives each bucket the shape the
-        // measured cost table prefers (plan_bucket_shapes).
-        static const int kAuto[][3] = {{0, 4, 1},  {0, 4, 2},  {0, 16, 1},  {0, 16, 2}, {0, 16, 3},
-                                       {1, 64, 1}, {1, 64, 2}, {2, 128, 2}, {3, 192, 2}, {4, 256, 2},
-                                       {8, 512, 2}};
-        int best = -1;
-        for (const auto &ws : kAuto) {
-            for (int i = 0; i < nw && best < 0; ++i)
-                if (dp_wave_shape_waves(i) == ws[0] && dp_wave_shape_lanes(i) == ws[1] &&
-                    dp_wave_shape_ring(i) == ws[1] * ws[2] && ncol <= dp_wave_shape_ring(i) - 2)
-                    best = i;
-            if (best >= 0) break;
-        }
-        if (best >= 0) return kWaveBase + best;
+        for (int k = 0; k < kNumAuto; ++k)
+            if (t.auto_idx[k] >= 0 && ncol <= t.auto_ring[k] - 2) return kWaveBase + t.auto_idx[k];
     }
     const int ns = dp_num_shapes();
     for (int i = 0; i < ns; ++i)
@@ -264,6 +289,30 @@ static void plan_bucket_shapes(const halign_ctx *ctx, const std::vector<Spec> &s
         fprintf(stderr, "[halign plan] estimate %.2f ms\n", est * 1e-6);
 }
 
+// Positions 0..n-1 ordered by falling key, equal keys in position order: a
+// counting sort when the keys span little more than n values (lengths), else
+// one sort of packed words.
+static void falling_order(const std::vector<uint32_t> &key, std::vector<uint32_t> &pos)
+{
+    const size_t n = key.size();
+    pos.resize(n);
+    if (n == 0) return;
+    const uint32_t mx = *std::max_element(key.begin(), key.end());
+    const uint32_t mn = *std::min_element(key.begin(), key.end());
+    const size_t span = (size_t)(mx - mn) + 1;
+    if (span <= 4 * n + 65536) {
+        std::vector<uint32_t> cnt(span + 1, 0);
+        for (uint32_t k : key) ++cnt[mx - k + 1];
+        for (size_t v = 1; v <= span; ++v) cnt[v] += cnt[v - 1];
+        for (size_t i = 0; i < n; ++i) pos[cnt[mx - key[i]]++] = (uint32_t)i;
+        return;
+    }
+    std::vector<uint64_t> pk(n);
+    for (size_t i = 0; i < n; ++i) pk[i] = (uint64_t)(mx - key[i]) << 32 | (uint32_t)i;
+    std::sort(pk.begin(), pk.end());
+    for (size_t i = 0; i < n; ++i) pos[i] = (uint32_t)pk[i];
+}
+
 // Uniform tickets: a sub-wave bucket whose jobs share (qlen, tlen, band) in
 // groups can run on the uniform form of its shape: the bucket becomes its keys
 // in falling anti-diagonal count, each padded with repeats of its last job to
@@ -286,17 +335,35 @@ static void form_uniform_tickets(const halign_ctx *ctx, std::vector<Spec> &specs
         const int ui = dp_wave_uniform_of(wi);
         const bool forced = dp_wave_shape_uniform(wi) != 0;   // a uniform shape must get its tickets
         if (G <= 1 || (ui == wi && !forced) || (keep && !forced)) continue;
-        auto key = [&](int i) {
-            const Spec &x = specs[i];
-            return std::make_tuple(x.qlen + x.tlen, x.qlen, x.tlen, eff_w(x.w, x.qlen, x.tlen));
-        };
-        std::vector<int> srt(b);
-        std::stable_sort(srt.begin(), srt.end(), [&](int x, int y) { return key(x) > key(y); });
+        // keys (qlen + tlen, qlen, tlen, band), falling; equal keys keep the
+        // bucket's order: a counting sort on qlen + tlen, then each run of
+        // equal qlen + tlen (already in bucket order) sorted stably by the rest
+        std::vector<uint32_t> k1(b.size()), pos;
+        for (size_t k = 0; k < b.size(); ++k) k1[k] = (uint32_t)(specs[b[k]].qlen + specs[b[k]].tlen);
+        falling_order(k1, pos);
+        struct Key { uint64_t a, b; int idx; };
+        std::vector<Key> keys(b.size());
+        for (size_t k = 0; k < b.size(); ++k) {
+            const Spec &x = specs[b[pos[k]]];
+            keys[k] = {(uint64_t)(uint32_t)(x.qlen + x.tlen) << 32 | (uint32_t)x.qlen,
+                       (uint64_t)(uint32_t)x.tlen << 32 | (uint32_t)eff_w(x.w, x.qlen, x.tlen), b[pos[k]]};
+        }
+        for (size_t a = 0; a < keys.size();) {
+            size_t e = a + 1;
+            while (e < keys.size() && keys[e].a >> 32 == keys[a].a >> 32) ++e;
+            if (e - a > 1)
+                std::stable_sort(keys.begin() + a, keys.begin() + e, [](const Key &x, const Key &y) {
+                    return x.a != y.a ? x.a > y.a : x.b > y.b;
+                });
+            a = e;
+        }
+        std::vector<int> srt(b.size());
+        for (size_t k = 0; k < b.size(); ++k) srt[k] = keys[k].idx;
         std::vector<int> padded;
         padded.reserve(srt.size() + srt.size() / 2);
         for (size_t a = 0; a < srt.size();) {
             size_t e = a;
-            while (e < srt.size() && key(srt[e]) == key(srt[a])) ++e;
+            while (e < srt.size() && keys[e].a == keys[a].a && keys[e].b == keys[a].b) ++e;
             padded.insert(padded.end(), srt.begin() + a, srt.begin() + e);
             for (size_t c = e - a; c % G; ++c) padded.push_back(srt[e - 1]);
             a = e;
@@ -351,6 +418,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     for (size_t i = 0; i < n; ++i)
         buckets[specs[i].shape * 4 + (specs[i].stats ? 2 : 0) + ((specs[i].flags & kFlagRight) ? 1 : 0)]
             .push_back((int)i);
+    clk.mark("bucketing");
     if (ctx->policy == 0 && !plain) {
         std::vector<int> to_shape(nb);
         plan_bucket_shapes(ctx, specs, buckets, to_shape);
@@ -372,7 +440,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         buckets.swap(moved);
     }
 
+    clk.mark("shape plan");
     if (!plain) form_uniform_tickets(ctx, specs, buckets, ctx->tuning);
+    clk.mark("uniform tickets");
     std::vector<int> order;
     order.reserve(n);
     for (int bi = 0; bi < nb; ++bi) {
@@ -381,19 +451,26 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         // (longest first) keep the rows of a wave busy for the same time
         // (uniform tickets are already ordered and padded)
         if (!b.empty() && bi / 4 >= kWaveBase && dp_wave_jobs_per_group(bi / 4 - kWaveBase) > 1 &&
-            !dp_wave_shape_uniform(bi / 4 - kWaveBase))
-            std::stable_sort(b.begin(), b.end(), [&](int x, int y) {
-                return specs[x].qlen + specs[x].tlen > specs[y].qlen + specs[y].tlen;
-            });
-        for (int i : b) order.push_back(i);
+            !dp_wave_shape_uniform(bi / 4 - kWaveBase)) {
+            // falling qlen + tlen, equal lengths in bucket order
+            std::vector<uint32_t> key(b.size()), pos;
+            for (size_t k = 0; k < b.size(); ++k) key[k] = (uint32_t)(specs[b[k]].qlen + specs[b[k]].tlen);
+            falling_order(key, pos);
+            std::vector<int> srt(b.size());
+            for (size_t k = 0; k < b.size(); ++k) srt[k] = b[pos[k]];
+            b.swap(srt);
+        }
+        order.insert(order.end(), b.begin(), b.end());
     }
+    clk.mark("job order");
     // with uniform tickets a job may appear more than once (its repeats
     // compute and write the same results)
     const size_t nj = order.size();
     HCHECK(ctx->jobs.ensure(nj * sizeof(DpJob)), "alloc jobs");
     HCHECK(ctx->outs.ensure(nj * sizeof(DpOut)), "alloc outs");
     HCHECK(ctx->counters.ensure((size_t)(plain ? nj : nb) * sizeof(int)), "alloc counters");
-    std::vector<DpJob> hjobs(nj);
+    if (ctx->hjobs.size() < nj) ctx->hjobs.resize(nj);   // kept: no zero fill of a fresh array per batch
+    DpJob *hjobs = ctx->hjobs.data();
     for (size_t k = 0; k < nj; ++k) {
         const Spec &s = specs[order[k]];
         DpJob &j = hjobs[k];
@@ -408,10 +485,11 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
         j.cig_cap = s.cig_cap;
         j.pad_ = 0;
     }
-    clk.mark("buckets and jobs");
-    HCHECK(hipMemcpyAsync(ctx->jobs.p, hjobs.data(), nj * sizeof(DpJob), hipMemcpyHostToDevice, ctx->stream),
+    clk.mark("jobs");
+    HCHECK(hipMemcpyAsync(ctx->jobs.p, hjobs, nj * sizeof(DpJob), hipMemcpyHostToDevice, ctx->stream),
            "upload jobs");
     HCHECK(hipMemsetAsync(ctx->counters.p, 0, ctx->counters.cap, ctx->stream), "clear counters");
+    clk.mark("job upload");
 
     // traceback pool: one slot per resident workgroup (per job group of a
     // sub-wave workgroup)
@@ -732,8 +810,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     }
     clk.mark("launches");
     HCHECK(hipEventRecord(ctx->ev1, ctx->stream), "event");
-    std::vector<DpOut> tmp(nj);
-    HCHECK(hipMemcpyAsync(tmp.data(), ctx->outs.p, nj * sizeof(DpOut), hipMemcpyDeviceToHost, ctx->stream),
+    if (!ctx->out_pin.ensure(std::max<size_t>(nj, 1) * sizeof(DpOut))) return fail(ctx, HALIGN_E_NOMEM, "pinned buffer");
+    const DpOut *tmp = reinterpret_cast<const DpOut *>(ctx->out_pin.p);
+    HCHECK(hipMemcpyAsync(ctx->out_pin.p, ctx->outs.p, nj * sizeof(DpOut), hipMemcpyDeviceToHost, ctx->stream),
            "download outs");
     HCHECK(hipStreamSynchronize(ctx->stream), "dp kernels");
     clk.mark("kernels");
@@ -808,6 +887,7 @@ void hal::ctx_destroy(halign_ctx *ctx)
         ctx->up_pin[k].release();
     }
     ctx->dn_pin.release();
+    ctx->out_pin.release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     for (int k = 0; k < halign_ctx::kAux; ++k) {

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/halign_hip.h"
+#include "dp_common.h"
 
 namespace hal {
 
@@ -91,6 +92,7 @@ struct halign_ctx {
     std::mutex mu;
     std::string err;
     hal::DevBuf seq, seqb, jobs, outs, counters, tb, cig, qparts;
+    std::vector<hal::DpJob> hjobs;   // host copy of the last batch's jobs (kept between batches)
     hal::DevBuf crows, ccounts, ccons;   // consensus rows (host input), counts, characters
     // seeding: sequences, offsets, packed codes and N masks, hits, counts, status, workspace
     hal::DevBuf s_seqs, s_off, s_woff, s_moff, s_codes, s_nmask, s_hoff, s_hx, s_hy, s_cnt, s_st, s_list;
@@ -120,6 +122,7 @@ struct halign_ctx {
     // reused after the event recorded behind its last copy) and one for
     // downloads of the large-segment records
     hal::HostPinned up_pin[2], dn_pin;
+    hal::HostPinned out_pin;   // the batch runner's download of its DpOuts
     hipEvent_t up_ev[2] = {};
     // Centers and pair sets live on the context's streams and buffers. They
     // are counted here (under mu); halign_free with some still live only

@@ -86,6 +86,10 @@ struct halign_center {
     DevBuf cmap;         // column of each center position in its aligned record (cwidth > 0)
     int64_t cwidth = 0;
     DevBuf lsmall, ljobs, ljseg, louts, ltb;   // small segments as ring jobs: counters, jobs, outputs, traceback
+    // host lists of the large segments' DP jobs and results, kept between
+    // batches (a fresh list per batch paid its page faults every time)
+    std::vector<Spec> hspecs;
+    std::vector<DpOut> houts;
 };
 
 namespace {
@@ -500,7 +504,8 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
         const SegRec *hs = reinterpret_cast<const SegRec *>(ctx->dn_pin.p);
         clk.mark("large segment download");
         // keep the device list order (atomic order) paired with the records
-        std::vector<Spec> specs(nh);
+        std::vector<Spec> &specs = c->hspecs;
+        specs.resize(nh);
         for (size_t k = 0; k < nh; ++k) {
             const SegRec &sg = hs[k];
             Spec &s = specs[k];
@@ -520,7 +525,7 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
             s.cig_cap = s.tlen + s.qlen + 1;
             if (!fill_spec_shape(s, false, ctx->policy)) return fail(ctx, HALIGN_E_UNSUPPORTED, "segment band too wide");
         }
-        std::vector<DpOut> outs;
+        std::vector<DpOut> &outs = c->houts;
         double ms = 0;
         clk.mark("large segment specs");
         rc = run_specs(ctx, specs, false, outs, &ms);
