@@ -8,8 +8,10 @@
 //    or band overflow. A query without a chain gets one whole-pair segment
 //    whose CIGAR is taken as it is (align.cpp:204-206).
 //  * seg_split: small segments that are exact matches get nM directly; the
-//    others become DpJobs built on the device (lane_jobs_*) for the sub-wave
-//    ring kernels (dp_wave.hip), each one globalAlignKSW2 (align.cpp:22-65);
+//    others become DpJobs built on the device (lane_jobs_*), longest first in
+//    each bucket, for the sub-wave ring kernels (dp_wave.hip) or, when the
+//    band is the whole matrix and the query the short side, the query-fixed
+//    kernels (dp_qfix.hip), each one globalAlignKSW2 (align.cpp:22-65);
 //    large segments and band overflows go to the host's batch runner.
 //  * mm2_assemble: one wave per query appends the segment CIGARs with
 //    appendCigar's merge (src/align/cigar.cpp:302-329; the ring and
@@ -266,78 +268,123 @@ __global__ __launch_bounds__(256) void seg_split(SegRec *__restrict__ segs, int6
     if (which == 1) host_list[lbase[1] + rank] = i;
 }
 
-// ---- small segments through the ring kernels (jobs built on the device) ----
+// ---- small segments as DP jobs built on the device ----
 
-__device__ __forceinline__ int lane_bucket(const SegRec &sg, const LaneShapeBounds &b)
+// bucket (shape * 2 + gap direction) and anti-diagonal count of a small
+// segment: a query-fixed shape when the band is the whole matrix and the query
+// the short side (as the batch runner's qfix_pick), else the first sub-wave
+// ring shape whose n_col_ bound holds the band
+__device__ __forceinline__ int lane_bucket(const SegRec &sg, const LaneShapeBounds &b, int &nd)
 {
     const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
+    nd = srl + sql - 1;
+    const int rt = (sg.flags & kSegRight) ? 1 : 0;
     int w = auto_band_dev(srl, sql);
     if (w < 0) w = srl > sql ? srl : sql;
+    if (sql <= srl && w >= srl - 1) {
+        for (int q = 0; q < kLaneQfix; ++q)
+            if (sql <= b.qcap[q]) return (kLaneRing + q) * 2 + rt;
+    }
     int nc = srl < sql ? srl : sql;
     nc = ((nc < w + 1 ? nc : w + 1) + 15) / 16 + 1;
     int sh = 0;
-    while (sh < kLaneShapes - 1 && nc > b.ncol_max[sh]) ++sh;
-    return sh * 2 + ((sg.flags & kSegRight) ? 1 : 0);
+    while (sh < kLaneRing - 1 && nc > b.ncol_max[sh]) ++sh;
+    return sh * 2 + rt;
 }
 
-// per-bucket job counts (bcnt[2 * kLaneShapes], cleared before): counted in
-// LDS, one global atomic per bucket and workgroup
+// bin of a small segment: its bucket's kLaneNd bins by anti-diagonal count
+// (a small segment has 1..kLaneNd anti-diagonals: its traceback is at most
+// kLaneMaxTb bytes at 16 per block and n_col_ >= 2)
+constexpr int kLaneBins = kLaneBuckets * kLaneNd;
+__device__ __forceinline__ int lane_bin(const SegRec &sg, const LaneShapeBounds &b)
+{
+    int nd = 0;
+    const int bk = lane_bucket(sg, b, nd);
+    nd = nd < 1 ? 1 : nd > kLaneNd ? kLaneNd : nd;
+    return bk * kLaneNd + nd - 1;
+}
+
+// per-bin job counts (hist[kLaneBins], cleared before): counted in LDS, one
+// global atomic per bin and workgroup (by the thread that opened the bin)
 __global__ __launch_bounds__(256) void lane_jobs_count(const SegRec *__restrict__ segs,
                                                        const int64_t *__restrict__ list,
                                                        const unsigned long long *__restrict__ count,
-                                                       LaneShapeBounds b, int *__restrict__ bcnt)
+                                                       LaneShapeBounds b, int *__restrict__ hist)
 {
-    __shared__ int lc[2 * kLaneShapes];
-    if (threadIdx.x < 2 * kLaneShapes) lc[threadIdx.x] = 0;
+    __shared__ int lc[kLaneBins];
+    for (int k = (int)threadIdx.x; k < kLaneBins; k += 256) lc[k] = 0;
     __syncthreads();
     const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (li < (int64_t)*count) atomicAdd(&lc[lane_bucket(segs[list[li]], b)], 1);
+    int bin = -1, rank = 0;
+    if (li < (int64_t)*count) {
+        bin = lane_bin(segs[list[li]], b);
+        rank = atomicAdd(&lc[bin], 1);
+    }
     __syncthreads();
-    if (threadIdx.x < 2 * kLaneShapes && lc[threadIdx.x]) atomicAdd(&bcnt[threadIdx.x], lc[threadIdx.x]);
+    if (bin >= 0 && rank == 0) atomicAdd(&hist[bin], lc[bin]);
 }
 
-// range[2k] = first job of bucket k, range[2k+1] = its job count; bfill cleared
-__global__ void lane_jobs_offsets(const int *__restrict__ bcnt, int *__restrict__ range, int *__restrict__ bfill)
+// range[2k] = first job of bucket k, range[2k+1] = its job count; cursor[bin]
+// = the first job of the bin: the bins of a bucket from the most
+// anti-diagonals down (one workgroup)
+__global__ __launch_bounds__(64) void lane_jobs_offsets(const int *__restrict__ hist, int *__restrict__ range,
+                                                        int *__restrict__ cursor)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int at = 0;
-    for (int k = 0; k < 2 * kLaneShapes; ++k) {
-        range[2 * k] = at;
-        range[2 * k + 1] = bcnt[k];
-        bfill[k] = 0;
-        at += bcnt[k];
+    __shared__ int tot[kLaneBuckets], first[kLaneBuckets];
+    const int t = (int)threadIdx.x;
+    if (t < kLaneBuckets) {
+        int sum = 0;
+        for (int d = 0; d < kLaneNd; ++d) sum += hist[t * kLaneNd + d];
+        tot[t] = sum;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int at = 0;
+        for (int k = 0; k < kLaneBuckets; ++k) {
+            first[k] = at;
+            at += tot[k];
+        }
+    }
+    __syncthreads();
+    if (t < kLaneBuckets) {
+        range[2 * t] = first[t];
+        range[2 * t + 1] = tot[t];
+        int at = first[t];
+        for (int d = kLaneNd - 1; d >= 0; --d) {
+            cursor[t * kLaneNd + d] = at;
+            at += hist[t * kLaneNd + d];
+        }
     }
 }
 
-// one DpJob per small segment, in its bucket's range (byte-code windows of the
-// center and of the reversed query, as the host builds them for large segments)
+// one DpJob per small segment at its bin's cursor (byte-code windows of the
+// center and of the reversed query, as the host builds them for large
+// segments); ranks within the workgroup in LDS, one global atomic per bin
 __global__ __launch_bounds__(256) void lane_jobs_place(
     const SegRec *__restrict__ segs, const int64_t *__restrict__ list, const unsigned long long *__restrict__ count,
     LaneShapeBounds b, const uint8_t *__restrict__ center, const uint8_t *__restrict__ qrev,
     const int64_t *__restrict__ qrev_off, const int64_t *__restrict__ qlens, uint32_t *__restrict__ cigs,
-    const int *__restrict__ range, int *__restrict__ bfill, DpJob *__restrict__ jobs, int64_t *__restrict__ job_seg)
+    int *__restrict__ cursor, DpJob *__restrict__ jobs, int64_t *__restrict__ job_seg)
 {
-    // ranks within the workgroup in LDS, one global atomic per bucket and workgroup
-    __shared__ int lc[2 * kLaneShapes], lbase[2 * kLaneShapes];
-    if (threadIdx.x < 2 * kLaneShapes) lc[threadIdx.x] = 0;
+    __shared__ int lc[kLaneBins], lbase[kLaneBins];
+    for (int k = (int)threadIdx.x; k < kLaneBins; k += 256) lc[k] = 0;
     __syncthreads();
     const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = li < (int64_t)*count;
     int64_t i = 0;
     SegRec sg{};
-    int k = 0, rank = 0;
+    int bin = 0, rank = 0;
     if (valid) {
         i = list[li];
         sg = segs[i];
-        k = lane_bucket(sg, b);
-        rank = atomicAdd(&lc[k], 1);
+        bin = lane_bin(sg, b);
+        rank = atomicAdd(&lc[bin], 1);
     }
     __syncthreads();
-    if (threadIdx.x < 2 * kLaneShapes && lc[threadIdx.x])
-        lbase[threadIdx.x] = atomicAdd(&bfill[threadIdx.x], lc[threadIdx.x]);
+    if (valid && rank == 0) lbase[bin] = atomicAdd(&cursor[bin], lc[bin]);
     __syncthreads();
     if (!valid) return;
-    const int pos = range[2 * k] + lbase[k] + rank;
+    const int pos = lbase[bin] + rank;
     const int srl = (int)(sg.re - sg.rs), sql = (int)(sg.qe - sg.qs);
     DpJob j;
     j.tgt_even = j.tgt_odd = j.qr_even = j.qr_odd = nullptr;   // spread copies: multi-wave kernel only
@@ -369,19 +416,21 @@ __global__ void lane_jobs_finish(const DpOut *__restrict__ outs, const int64_t *
     segs[i].cls = kSegRing;
 }
 
+// phase 0: jobs (hist: kLaneBins counters, cleared before; cursor:
+// kLaneBins; range: 2 * kLaneBuckets); phase 1: results back to the segments
 hipError_t lane_jobs_launch(int phase, int64_t nmax, const SegRec *segs, const int64_t *list,
                             const unsigned long long *count, const LaneShapeBounds &b, const uint8_t *center,
                             const uint8_t *qrev, const int64_t *qrev_off, const int64_t *qlens, uint32_t *cigs,
-                            int *bcnt, int *range, int *bfill, DpJob *jobs, int64_t *job_seg, const DpOut *outs,
+                            int *hist, int *range, int *cursor, DpJob *jobs, int64_t *job_seg, const DpOut *outs,
                             SegRec *wsegs, int32_t *ncig, int *err, hipStream_t stream)
 {
     if (nmax <= 0) return hipSuccess;
     const dim3 grid((unsigned)((nmax + 255) / 256)), blk(256);
     if (phase == 0) {
-        hipLaunchKernelGGL(lane_jobs_count, grid, blk, 0, stream, segs, list, count, b, bcnt);
-        hipLaunchKernelGGL(lane_jobs_offsets, dim3(1), dim3(64), 0, stream, bcnt, range, bfill);
+        hipLaunchKernelGGL(lane_jobs_count, grid, blk, 0, stream, segs, list, count, b, hist);
+        hipLaunchKernelGGL(lane_jobs_offsets, dim3(1), dim3(64), 0, stream, hist, range, cursor);
         hipLaunchKernelGGL(lane_jobs_place, grid, blk, 0, stream, segs, list, count, b, center, qrev, qrev_off, qlens,
-                           cigs, range, bfill, jobs, job_seg);
+                           cigs, cursor, jobs, job_seg);
     } else {
         hipLaunchKernelGGL(lane_jobs_finish, grid, blk, 0, stream, outs, job_seg, count, wsegs, ncig, err);
     }

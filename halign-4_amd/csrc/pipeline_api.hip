@@ -49,7 +49,7 @@ hipError_t seg_split_launch(SegRec *segs, int64_t n, const uint8_t *center, cons
 hipError_t lane_jobs_launch(int phase, int64_t nmax, const SegRec *segs, const int64_t *list,
                             const unsigned long long *count, const LaneShapeBounds &b, const uint8_t *center,
                             const uint8_t *qrev, const int64_t *qrev_off, const int64_t *qlens, uint32_t *cigs,
-                            int *bcnt, int *range, int *bfill, DpJob *jobs, int64_t *job_seg, const DpOut *outs,
+                            int *hist, int *range, int *cursor, DpJob *jobs, int64_t *job_seg, const DpOut *outs,
                             SegRec *wsegs, int32_t *ncig, int *err, hipStream_t stream);
 hipError_t project_rows_launch(const uint32_t *cig, const int64_t *coff, const char *qseq, const int64_t *qoff,
                                int nq, char *rows, int64_t stride, int64_t *row_len, int64_t *ws, uint8_t *has_ins,
@@ -130,11 +130,16 @@ __global__ void scatter_ncig(const int64_t *__restrict__ list, const int32_t *__
 
 constexpr int kK = 15, kW = 10;   // RefAligner's k-mer size and window (include/config.hpp defaults)
 
-// small segments (kSegLane) through the sub-wave ring kernels: groups of 4
-// lanes with 1 and 2 slots, of 16 lanes with 1 and 2 slots
-constexpr int kLaneShapeDesc[kLaneShapes][2] = {{4, 1}, {4, 2}, {16, 1}, {16, 2}};
-// small-segment counters: bucket counts, ranges, fill, tickets, error flag
-constexpr int kLsCnt = 0, kLsRange = 8, kLsFill = 24, kLsTicket = 32, kLsErr = 40, kLsInts = 48;
+// small segments (kSegLane): the sub-wave ring shapes (lanes per job, slots)
+// groups of 4 lanes with 1 and 2 slots, of 16 lanes with 1 and 2 slots; the
+// query-fixed shapes (lanes per job, slots) for queries of up to 16, 64 and
+// 128 bases
+constexpr int kLaneShapeDesc[kLaneShapes][2] = {{4, 1}, {4, 2}, {16, 1}, {16, 2}, {1, 1}, {4, 1}, {4, 2}};
+// small-segment counters: bucket ranges, tickets, error flag (the head, read
+// back in the diagnostic mode), then the per-bin histogram and cursors
+constexpr int kLsRange = 0, kLsTicket = 2 * kLaneBuckets, kLsErr = 3 * kLaneBuckets, kLsHead = 64;
+constexpr int kLsHist = kLsHead, kLsCursor = kLsHist + kLaneBuckets * kLaneNd, kLsInts = kLsCursor + kLaneBuckets * kLaneNd;
+static_assert(kLsErr < kLsHead, "small-segment counter head");
 
 template <class T>
 hipError_t up(DevBuf &b, const std::vector<T> &v, hipStream_t st)
@@ -404,9 +409,11 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                             c->hlist.as<int64_t>(), hcnt, st),
            "launch seg_split");
     // small segments: DpJobs built on the device, bucketed by shape and gap
-    // direction, run by the sub-wave ring kernels with their job ranges read on
-    // the device (the buckets run one after another and share one traceback
-    // region: a small segment's traceback is at most kLaneMaxTb / 2 bytes)
+    // direction (longest first within a bucket), run by the sub-wave ring and
+    // query-fixed kernels with their job ranges read on the device (the
+    // buckets run one after another and share one traceback region: a small
+    // segment's traceback is at most kLaneMaxTb / 2 bytes on a ring shape and
+    // kLaneNd rows of the shape's blocks on a query-fixed one)
     hipStream_t side = ctx->aux[2];
     HCHECK(hipEventRecord(ctx->pfork, st), "event");
     HCHECK(hipStreamWaitEvent(side, ctx->pfork, 0), "stream wait");
@@ -418,21 +425,33 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
     int *ls = c->lsmall.as<int>();
     LaneShapeBounds lb;
     int lshape[kLaneShapes], lgroups[kLaneShapes][2];
-    size_t lslot = ((size_t)kLaneMaxTb / 2 + 255) & ~(size_t)255, ltb = 0;
+    size_t lslot[kLaneShapes], ltb = 0;
     for (int k = 0; k < kLaneShapes; ++k) {
-        lshape[k] = dp_wave_find_shape(0, kLaneShapeDesc[k][0], kLaneShapeDesc[k][1]);
+        const int g = kLaneShapeDesc[k][0], sl = kLaneShapeDesc[k][1];
+        lshape[k] = -1;
+        if (k < kLaneRing) {
+            lshape[k] = dp_wave_find_shape(0, g, sl);
+            if (lshape[k] >= 0) lb.ncol_max[k] = dp_wave_shape_ring(lshape[k]) - 2;
+            lslot[k] = ((size_t)kLaneMaxTb / 2 + 255) & ~(size_t)255;
+        } else {
+            for (int q = 0; q < dp_wave_num_qfix(); ++q) {
+                const int idx = dp_wave_qfix_index(q);
+                if (dp_wave_shape_lanes(idx) == g && dp_wave_shape_ring(idx) == g * sl) lshape[k] = idx;
+            }
+            lb.qcap[k - kLaneRing] = 16 * g * sl;
+            lslot[k] = (size_t)kLaneNd * (size_t)(g * sl) * 8;   // kLaneNd rows of the shape's blocks
+        }
         if (lshape[k] < 0) return fail(ctx, HALIGN_E_HIP, "small-segment shape missing");
-        lb.ncol_max[k] = dp_wave_shape_ring(lshape[k]) - 2;
         for (int rt = 0; rt < 2; ++rt) {
             lgroups[k][rt] = std::max(1, dp_wave_max_groups(lshape[k], rt != 0, ctx->num_cu));
-            ltb = std::max(ltb, lslot * (size_t)dp_wave_jobs_per_group(lshape[k]) * (size_t)lgroups[k][rt]);
+            ltb = std::max(ltb, lslot[k] * (size_t)dp_wave_jobs_per_group(lshape[k]) * (size_t)lgroups[k][rt]);
         }
     }
     HCHECK(c->ltb.ensure(ltb), "alloc small-segment traceback");
     const uint8_t *cbytes = ctx->seqb.as<uint8_t>() + kBytePad;
     HCHECK(lane_jobs_launch(0, S, c->segs.as<SegRec>(), c->llist.as<int64_t>(), lcnt, lb, cbytes,
                             ctx->seqb.as<uint8_t>(), c->qrev_off.as<int64_t>(), c->qlens.as<int64_t>(),
-                            ctx->cig.as<uint32_t>(), ls + kLsCnt, ls + kLsRange, ls + kLsFill, c->ljobs.as<DpJob>(),
+                            ctx->cig.as<uint32_t>(), ls + kLsHist, ls + kLsRange, ls + kLsCursor, c->ljobs.as<DpJob>(),
                             c->ljseg.as<int64_t>(), nullptr, nullptr, nullptr, nullptr, side),
            "launch small-segment jobs");
     // bucket timing: pairs from kSmallEv on of the context's event pool (the
@@ -447,12 +466,12 @@ int run_batch(halign_center *c, int n, const char *queries, const int64_t *offse
                 HCHECK(hipEventRecord(ta, side), "event");
             }
             HCHECK(dp_wave_launch(lshape[k], rt != 0, lgroups[k][rt], c->ljobs.as<DpJob>(), 0, ls + kLsTicket + bk,
-                                  c->ltb.as<uint8_t>(), lslot, c->louts.as<DpOut>(), side, ls + kLsRange + 2 * bk),
+                                  c->ltb.as<uint8_t>(), lslot[k], c->louts.as<DpOut>(), side, ls + kLsRange + 2 * bk),
                    "launch small-segment DP");
             if (tz) HCHECK(hipEventRecord(tz, side), "event");
         }
     if (ctx->bucket_timing) {   // diagnostic mode: wait here and record the launches
-        int hls[kLsInts];
+        int hls[kLsHead];
         HCHECK(hipMemcpyAsync(hls, ls, sizeof hls, hipMemcpyDeviceToHost, side), "download ranges");
         HCHECK(hipStreamSynchronize(side), "small-segment DPs");
         int njobs = 0;

@@ -62,10 +62,20 @@ constexpr uint32_t kSegHost = 3;      // large DP or band overflow: ring / full-
 constexpr uint32_t kSegExact = 4;     // small, equal lengths, identical ACGT bases: CIGAR nM without a DP
 constexpr uint32_t kSegRing = 5;      // small DP done by a ring / sub-wave kernel (CIGAR stored last op first)
 
-// small segments as ring-kernel jobs: bucket = shape * 2 + gap direction, the
-// shape the first whose n_col_ bound holds the band
-constexpr int kLaneShapes = 4;
-struct LaneShapeBounds { int ncol_max[kLaneShapes]; };
+// small segments as DP jobs built on the device: bucket = shape * 2 + gap
+// direction. Shapes 0 .. kLaneRing-1 are sub-wave ring shapes (the first whose
+// n_col_ bound holds the band); shapes kLaneRing .. are query-fixed shapes,
+// which take the segments whose band is the whole matrix and whose query is
+// the short side (the first whose query capacity holds the query). Within a
+// bucket the jobs run longest first (a wave runs to its longest job).
+constexpr int kLaneRing = 4;
+constexpr int kLaneQfix = 3;
+constexpr int kLaneShapes = kLaneRing + kLaneQfix;
+constexpr int kLaneBuckets = 2 * kLaneShapes;
+struct LaneShapeBounds {
+    int ncol_max[kLaneRing];   // ring shape k serves n_col_ <= ncol_max[k]
+    int qcap[kLaneQfix];       // query-fixed shape q serves queries of <= qcap[q] bases
+};
 
 // block-wide inclusive scan of one value per thread (256 threads); *total
 // receives the sum over the block
@@ -111,5 +121,8 @@ __device__ __forceinline__ int32_t wave_scan_incl(int32_t v, int32_t id, Op op)
 // anti-diagonal; the ring kernels store half of that)
 constexpr int kLaneMaxT = 512;
 constexpr int kLaneMaxTb = 16384;
+// anti-diagonals of a small segment: at most kLaneMaxTb / 16 / n_col_, and
+// n_col_ >= 2
+constexpr int kLaneNd = kLaneMaxTb / 32;
 
 }  // namespace hal
