@@ -203,11 +203,10 @@ int halign_tune(halign_ctx *ctx)
         double t0sum = 0;
         if (wi >= nring) {
             // query-fixed: queries filling 1..S slots (at most four points),
-            // unbanded, targets of len1 and len1 + 1024 bases; at most 16,384
-            // jobs (the one-lane shape keeps far more in flight; its jobs are
-            // short, and the estimate scales by jobs in flight)
+            // unbanded, targets of len1 and len1 + 1024 bases, as many jobs as
+            // the chip keeps in flight (the estimate divides by that count)
             const int lanes = dp_wave_shape_lanes(wi), slots = ring / lanes;
-            const int n = std::min(conc, 16384);
+            const int n = conc;
             for (int k = std::max(1, slots - 3); k <= slots; ++k) cols[np++] = k * lanes;
             for (int c = 0; c < np; ++c) {
                 const int qlen = 16 * cols[c];

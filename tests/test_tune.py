@@ -20,7 +20,7 @@ TABLE = os.path.join(HERE, "..", "halign-4_amd", "csrc", "dp_tune_table.h")
 def table_rows():
     rows = []
     for line in open(TABLE):
-        m = re.match(r"\s*\{(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), \{([^}]*)\}, \{([^}]*)\}, ([0-9.]+)\},", line)
+        m = re.match(r"\s*\{(-?\d+), (\d+), (\d+), (\d+), (\d+), (\d+), \{([^}]*)\}, \{([^}]*)\}, ([0-9.]+)\},", line)
         if m:
             n = int(m.group(6))
             rows.append({"waves": int(m.group(1)), "lanes": int(m.group(2)), "ring": int(m.group(3)),
