@@ -17,7 +17,9 @@
 //  * minimizers_seq: one lane per sequence, the reference's queue step by step
 //    (a ring of `win` slots indexed as the reference indexes it, in a larger
 //    per-lane LDS workspace so that the reference's out-of-range slots are
-//    modelled instead of faulting).
+//    modelled instead of faulting); minimizers_seqw, the same steps with one
+//    wave per sequence for windows of up to 14 (hashes of 64 positions at a
+//    time by the lanes, the workspace one entry per lane).
 //
 // The queue differs from a plain windowed minimum in one case: when it holds
 // `win` entries (the last win hashes strictly increasing) and a larger hash
@@ -356,6 +358,155 @@ __global__ void minimizers_seq(const char *__restrict__ seqs, const int64_t *__r
     status[s] = st;
 }
 
+// The same loop with one wave per listed sequence, for queue workspaces of at
+// most 64 entries (w <= 14): for 64 positions at a time the lanes classify
+// the base, build the k-mer codes from the bases staged in LDS and hash them;
+// then the whole wave steps the reference's queue through those positions, the
+// workspace held one entry per lane (entry e in lane e, read and written with
+// readlane / writelane at wave-uniform indices), so the dependent chain is only
+// the queue's own steps.
+__global__ __launch_bounds__(64) void minimizers_seqw(const char *__restrict__ seqs, const int64_t *__restrict__ off,
+                                                      const int32_t *__restrict__ list, int nlist, int k_, int w_,
+                                                      int non_canonical, const int64_t *__restrict__ hit_off,
+                                                      int ws_cap, uint64_t *__restrict__ hx, uint64_t *__restrict__ hy,
+                                                      int64_t *__restrict__ counts, int32_t *__restrict__ status)
+{
+    __shared__ uint8_t code[128];   // base classes of positions b-64 .. b+63
+    const int t = (int)blockIdx.x;
+    if (t >= nlist) return;
+    const int lane = (int)threadIdx.x;
+    const int s = list[t];
+    const char *seq = seqs + off[s];
+    const uint32_t n = (uint32_t)(off[s + 1] - off[s]), k = (uint32_t)k_, w = (uint32_t)w_;
+    if (k == 0 || w == 0 || n < k || k > 31 || w >= 256) {
+        if (lane == 0) {
+            counts[s] = 0;
+            status[s] = 0;
+        }
+        return;
+    }
+    const uint32_t total = n - k + 1;
+    const uint32_t win = w < total ? w : total;
+    const uint64_t mask = ((uint64_t)1 << (2 * k)) - 1;
+    const uint32_t shift = 2 * (k - 1);
+    const uint32_t big = 4 * win + 8 <= (uint32_t)ws_cap ? 4 * win + 8 : (uint32_t)ws_cap;   // <= 64
+    uint64_t *ox = hx + hit_off[s], *oy = hy + hit_off[s];
+    uint32_t qlo = 0, qhi = 0, qpos = 0;   // queue workspace entry `lane`: hash, position
+    uint32_t head = 0, size = 0, last_p = 0;
+    uint64_t last_h = 0;
+    bool has_last = false, over = false;
+    int st = 0;
+    int64_t cnt = 0;
+    auto idx = [&](uint32_t o) {   // the reference's index: one wrap by `win`
+        uint32_t i = head + o;
+        if (i >= win) i -= win;
+        return i;
+    };
+    auto check = [&](uint32_t i) {
+        if (i >= win) st |= kSeedUb;
+        if (i >= big) over = true;
+    };
+    auto rl64 = [](uint32_t lo, uint32_t hi, uint32_t l) {
+        return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, (int)l) << 32 |
+               (uint32_t)__builtin_amdgcn_readlane((int)lo, (int)l);
+    };
+    code[64 + lane] = 4;
+    for (uint32_t b = 0; b < n && !over; b += 64) {
+        // classes of positions b-64 .. b+63 (4: not ACGTU, or before the sequence)
+        __syncthreads();
+        code[lane] = code[64 + lane];
+        const uint32_t i = b + (uint32_t)lane;
+        code[64 + lane] = (uint8_t)(i < n ? nt4((uint8_t)seq[i]) : 4u);
+        __syncthreads();
+        // 0: a base outside ACGTU (the queue restarts), 1: no k-mer ends here,
+        // 2: the k-mer of bases i-k+1 .. i, with its hash and strand
+        uint32_t cls = code[64 + lane] >= 4 ? 0u : 1u, strand = 0, hlo = 0, hhi = 0;
+        if (cls && i + 1 >= k) {
+            uint64_t f = 0, r = 0;
+            bool ok = true;
+#pragma unroll
+            for (int d = 0; d < 31; ++d) {
+                if (d < (int)k) {
+                    const uint32_t c = code[64 + lane - (int)k + 1 + d];
+                    ok = ok && c < 4;
+                    f = ((f << 2) | (c & 3)) & mask;
+                    r = (r >> 2) | ((uint64_t)(3u ^ (c & 3)) << shift);
+                }
+            }
+            if (ok) {
+                cls = 2;
+                const uint64_t h56 = splitmix64(non_canonical ? f : (f < r ? f : r)) >> 8;
+                hlo = (uint32_t)h56;
+                hhi = (uint32_t)(h56 >> 32);
+                strand = (non_canonical || f <= r) ? 1u : 0u;
+            }
+        }
+        const uint32_t jn = n - b < 64 ? n - b : 64;
+        for (uint32_t j = 0; j < jn; ++j) {
+            const uint32_t cj = (uint32_t)__builtin_amdgcn_readlane((int)cls, (int)j);
+            if (cj == 0) {
+                head = size = 0;
+                has_last = false;
+                continue;
+            }
+            if (cj == 1) continue;
+            const uint64_t h = rl64(hlo, hhi, j);
+            const uint32_t pos = b + j + 1 - k;
+            while (size) {   // pop_back while back.h >= h
+                const uint32_t bi = idx(size - 1);
+                check(bi);
+                if (over) break;
+                if (rl64(qlo, qhi, bi) >= h) --size;
+                else break;
+            }
+            if (over) break;
+            {
+                const uint32_t wi = idx(size);
+                check(wi);
+                if (over) break;
+                qlo = lane == (int)wi ? (uint32_t)h : qlo;
+                qhi = lane == (int)wi ? (uint32_t)(h >> 32) : qhi;
+                qpos = lane == (int)wi ? pos : qpos;
+                ++size;
+            }
+            if (pos + 1 < win) continue;
+            const uint32_t win_start = pos + 1 - win;
+            while (size) {   // popExpired
+                check(head);
+                if (over) break;
+                if ((uint32_t)__builtin_amdgcn_readlane((int)qpos, (int)head) < win_start) {
+                    head = idx(1);
+                    --size;
+                } else {
+                    break;
+                }
+            }
+            if (over) break;
+            if (size) {
+                check(head);
+                if (over) break;
+                const uint64_t ch = rl64(qlo, qhi, head);
+                const uint32_t cp = (uint32_t)__builtin_amdgcn_readlane((int)qpos, (int)head);
+                if (!has_last || ch != last_h || cp != last_p) {
+                    const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)strand, (int)j);
+                    if (lane == 0) {
+                        ox[cnt] = (ch << 8) | (uint64_t)(uint8_t)k;
+                        oy[cnt] = ((uint64_t)(sj ? 0x80000000u : 0u) << 32) | (uint64_t)cp;
+                    }
+                    ++cnt;
+                    last_h = ch;
+                    last_p = cp;
+                    has_last = true;
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        counts[s] = cnt;
+        status[s] = st | (over ? kSeedOverflow : 0);
+    }
+}
+
 hipError_t seed_pack(const char *seqs, const int64_t *off, const int64_t *woff, const int64_t *moff, int n,
                      int64_t max_words, uint64_t *codes, uint64_t *nmask, hipStream_t stream)
 {
@@ -383,6 +534,11 @@ hipError_t seed_seq(const char *seqs, const int64_t *off, const int32_t *list, i
                     int64_t *counts, int32_t *status, hipStream_t stream)
 {
     if (nlist <= 0) return hipSuccess;
+    if (ws_cap <= 64) {   // the workspace fits one entry per lane: one wave per sequence
+        hipLaunchKernelGGL(minimizers_seqw, dim3(nlist), dim3(64), 0, stream, seqs, off, list, nlist, k, w,
+                           non_canonical, hit_off, ws_cap, hx, hy, counts, status);
+        return hipGetLastError();
+    }
     // 12 bytes of queue per entry and lane; lanes per workgroup so that a
     // workgroup takes at most 48 KiB of LDS (64 lanes for the path's w = 10)
     const size_t per_lane = (size_t)ws_cap * 12;
