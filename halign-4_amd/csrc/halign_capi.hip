@@ -496,6 +496,19 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
     size_t need_slot = 0;
     for (auto &s : specs) need_slot = std::max(need_slot, s.tb_bytes);
     if (need_slot > ctx->tb_budget) return fail(ctx, HALIGN_E_NOMEM, "traceback of one job exceeds the device budget");
+    // the pool is planned within the budget and within what the device can
+    // still give it (the pool it holds plus the free HBM, less 1 GiB for the
+    // rest of the call), so contexts that share a device shrink their launches
+    // instead of failing an allocation
+    size_t budget = ctx->tb_budget;
+    {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+            const size_t keep = (size_t)1 << 30;
+            budget = std::min(budget, ctx->tb.cap + (fr > keep ? fr - keep : 0));
+            budget = std::max(budget, need_slot);
+        }
+    }
 
     HCHECK(hipEventRecord(ctx->ev0, ctx->stream), "event");
     // launches bracketed by bucket-timing events, in event-pair order
@@ -612,9 +625,9 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             const size_t rounds = (tickets + (size_t)L.groups - 1) / (size_t)L.groups;
             L.groups = (int)((tickets + rounds - 1) / rounds);
         };
-        const bool side = min_bytes <= ctx->tb_budget;   // else one launch at a time in one region
+        const bool side = min_bytes <= budget;   // else one launch at a time in one region
         if (side) {
-            while (pool_bytes() > ctx->tb_budget) {
+            while (pool_bytes() > budget) {
                 Launch *big = nullptr;
                 for (auto &L : ls)
                     if (L.groups > 1 && (!big || L.slot * L.jpg * L.groups > big->slot * big->jpg * big->groups))
@@ -629,7 +642,7 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
                 off += L.slot * L.jpg * (size_t)L.groups;
             }
             clk.mark("launch plan");
-            HCHECK(ctx->tb.ensure(std::max<size_t>(off, 1), ctx->tb_budget), "alloc traceback pool");
+            HCHECK(ctx->tb.ensure(std::max<size_t>(off, 1), budget), "alloc traceback pool");
             clk.mark("traceback pool");
             // Query-fixed launches bound by their jobs' chains run as the parts
             // of one combined launch (dp_qfix_combined): side by side rather
@@ -679,14 +692,14 @@ int run_specs(halign_ctx *ctx, std::vector<Spec> &specs, bool plain, std::vector
             }
         } else {
             for (auto &L : ls) {
-                L.groups = (int)std::min<size_t>(L.groups, std::max<size_t>(1, ctx->tb_budget / (L.slot * L.jpg)));
+                L.groups = (int)std::min<size_t>(L.groups, std::max<size_t>(1, budget / (L.slot * L.jpg)));
                 balance(L);
                 L.off = 0;
                 L.stream = 0;
             }
             size_t mx = 0;
             for (auto &L : ls) mx = std::max(mx, L.slot * L.jpg * (size_t)L.groups);
-            HCHECK(ctx->tb.ensure(std::max<size_t>(mx, 1), ctx->tb_budget), "alloc traceback pool");
+            HCHECK(ctx->tb.ensure(std::max<size_t>(mx, 1), budget), "alloc traceback pool");
         }
         bool used[kDpStreams] = {true};
         for (auto &L : ls) used[L.stream] = true;
