@@ -516,8 +516,10 @@ __device__ __forceinline__ void dp_wave_job(const DpJob &J, uint8_t *__restrict_
                     fb[h] = RT ? pk_subsat(bb, zz) : pk_subsat(zz, bb);
                     zz = pk_maxu(zz, bb);
                     zz = pk_minu(zz, 0x15001500u);
-                    U[k][j] = pk_sub(zz, vs);
+                    // v before u: the old u (ut) dies before the new one is
+                    // written, so both update in place (no register copies)
                     V[k][j] = pk_sub(zz, ut);
+                    U[k][j] = pk_sub(zz, vs);
                     zz = pk_sub(zz, 0x06000600u);
                     const uint32_t a2 = pk_sub(a, zz), b2 = pk_sub(bb, zz);
                     X[k][j] = pk_maxi(a2, 0u);
