@@ -534,7 +534,9 @@ __device__ __forceinline__ void dp_wave_job(const DpJob &J, uint8_t *__restrict_
             uint2 o;
             o.x = bfi(0xF0F0F0F0u, C[1], C[0]);
             o.y = bfi(0xF0F0F0F0u, C[3], C[2]);
-            *(uint2 *)(tb + (size_t)r * rowb + (size_t)(b - bs) * 8) = o;
+            // row base uniform, block offset a 32-bit lane value (a slot is < 4 GB):
+            // the store takes the scalar base and the 32-bit offset as they are
+            *(uint2 *)(tb + (size_t)r * rowb + (uint32_t)((b - bs) * 8)) = o;
             if constexpr (ROW) {   // per-group bounds: the owning lane's own copy
                 if (k == ke) ue_v = (uint32_t)cell_byte(U[k], en0 & 15);
                 if (k == kv) vl_v = (uint32_t)cell_byte(V[k], tv & 15);
