@@ -5,7 +5,12 @@ a context created before other allocations (another context's pool, a
 framework's tensors) would plan past the memory that is left. The batch
 runner therefore plans each call within min(budget, its pool + free HBM - 1
 GiB): under that pressure the launches shrink (fewer workgroups, more rounds
-of tickets) and the results stay those of an unconstrained run."""
+of tickets) and the results stay those of an unconstrained run.
+
+The memory held elsewhere comes from the HIP runtime the library links
+(libamdhip64.so.7, already loaded with it), not from a framework: a second
+runtime of the same soname loaded after the library's would bind to it."""
+import ctypes as C
 import sys
 
 import numpy as np
@@ -31,8 +36,31 @@ def _jobs(n, tlen, seed):
     return cases
 
 
+class Held:
+    """Device memory held through the library's own HIP runtime."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.ptr = C.c_void_p()
+
+    def free_bytes(self):
+        free, total = C.c_size_t(0), C.c_size_t(0)
+        assert self.hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+        return free.value
+
+    def hold(self, nbytes):
+        assert self.hip.hipMalloc(C.byref(self.ptr), nbytes) == 0, nbytes
+
+    def release(self):
+        if self.ptr:
+            assert self.hip.hipFree(self.ptr) == 0
+            self.ptr = C.c_void_p()
+
+
 def test_pool_shrinks_under_memory_pressure():
-    import torch
     import halign
     cases = _jobs(2048, 8000, 99)
     tasks, qc, tc = _tasks(cases)
@@ -42,22 +70,20 @@ def test_pool_shrinks_under_memory_pressure():
         r0, c0, _ = ref.dp_batch(tasks, qc, tc, full_stats=False, plain=False)
     finally:
         ref.close()
-    torch.cuda.empty_cache()
     ctx = halign.Context(0)   # budget taken now, with nearly all HBM free
-    hog = None
+    hog = Held()
     try:
-        free, _ = torch.cuda.mem_get_info(0)
+        free = hog.free_bytes()
         left = 24 << 30
-        hog = torch.empty(int(free - left), dtype=torch.uint8, device="cuda:0")
-        print(f"holding {hog.numel() / 2**30:.0f} GiB elsewhere, {left >> 30} GiB left", file=sys.stderr, flush=True)
+        hog.hold(free - left)
+        print(f"holding {(free - left) / 2**30:.0f} GiB elsewhere, {left >> 30} GiB left", file=sys.stderr, flush=True)
         ctx.set_bucket_timing(True)
         r1, c1, _ = ctx.dp_batch(tasks, qc, tc, full_stats=False, plain=False)
         rows = ctx.last_buckets()
         ctx.set_bucket_timing(False)
     finally:
         ctx.close()
-        del hog
-        torch.cuda.empty_cache()
+        hog.release()
     assert rows and all(b["jobs"] == 2048 for b in rows if b["family"] == 0), rows
     assert np.array_equal(r1["score"], r0["score"]) and np.array_equal(r1["n_cigar"], r0["n_cigar"])
     assert all(np.array_equal(a, b) for a, b in zip(c1, c0))
