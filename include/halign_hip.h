@@ -498,7 +498,10 @@ int halign_last_buckets(halign_ctx *ctx, halign_bucket_stat *out, int32_t cap, i
 
 /* Device memory the context may take for traceback slots (default: the HBM
  * free at halign_init but a reserve of 4%, at least 8 GiB; the pool is
- * allocated as launches need it). No reference counterpart. */
+ * allocated as launches need it). Each batch plans within the smaller of this
+ * budget and the pool already held plus the HBM free then, less 1 GiB, so
+ * memory taken elsewhere after halign_init shrinks the launches instead of
+ * failing them. No reference counterpart. */
 int halign_set_traceback_budget(halign_ctx *ctx, int64_t bytes);
 
 /* Continuous-batching server for RefAligner::globalAlign against one center
